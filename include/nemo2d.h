@@ -1,0 +1,149 @@
+/*
+ * nemo2d -- C-ABI of the MI355X (gfx950) NEMOLite2D time-stepper.
+ *
+ * Drop-in for the reference CUDA port's extern "C" boundary
+ * (cuda/cuda_nemolite2d.cu:234-270, bound from Fortran in
+ * cuda/nemolite2d.f90:62-127).  Differences from that boundary, all in the
+ * direction of a safer C-ABI:
+ *   - an opaque handle instead of process-global device state
+ *     (cuda/cuda_nemolite2d.cu:272-280), so several models can coexist;
+ *   - every entry point returns an int status (0 = success) instead of
+ *     throwing through Fortran frames (cuda/cuda_utils.cuh:9-24);
+ *     nemo2d_last_error() gives the message;
+ *   - the Fortran driver owns the grid, mask and initial state and uploads
+ *     them (the reference recomputes the grid on the device,
+ *     cuda/cuda_nemolite2d.cu:647-771);
+ *   - the reference's cuda_momentum_ / cuda_boundary_conditions_ are split
+ *     into the nine stage calls of the serial model's step
+ *     (original/nemolite2d.f90:375-769).
+ *
+ * Host arrays use the reference's Fortran layout: column-major, first index
+ * fastest, with these shapes (jpi, jpj = interior T-grid size, i.e. the
+ * namelist's jpiglo-2, jpjglo-2, original/nemolite2d.f90:133-137):
+ *   T  (1:jpi, 1:jpj)      e1t e2t e12t ht sshn ssha
+ *   U  (0:jpi, 1:jpj)      e1u e2u e12u gphiu hu sshn_u un ua
+ *   V  (1:jpi, 0:jpj)      e1v e2v e12v gphiv hv sshn_v vn va
+ *   pt (0:jpi+1, 0:jpj+1)  int: 1 water, 0 land, -1 open boundary
+ * All arithmetic is IEEE double; results are bitwise those of the serial
+ * model built without fast-math (same operation order, no FMA contraction,
+ * sin/sqrt evaluated on the host with the C library).
+ *
+ * Threading: one host thread per model; all device work of a model is
+ * issued on one HIP stream in call order.  Calls that return data
+ * (checksums, retrieve) synchronise that stream.
+ */
+#ifndef NEMO2D_H
+#define NEMO2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nemo2d_model nemo2d_model;
+
+/* status codes */
+#define NEMO2D_OK 0
+#define NEMO2D_ERR_ARG 1      /* bad argument (null pointer, size, range)  */
+#define NEMO2D_ERR_HIP 2      /* HIP runtime error                        */
+#define NEMO2D_ERR_MASK 3     /* mask the serial model would index outside */
+#define NEMO2D_ERR_STATE 4    /* call out of order (no grid / no state)   */
+#define NEMO2D_ERR_NONFINITE 5 /* a checksum is NaN or Inf                 */
+
+/* Message for the last failing call on this thread (never NULL). */
+const char *nemo2d_last_error(void);
+
+/* Library version string. */
+const char *nemo2d_version(void);
+
+/* Create a model on the current HIP device.  Replaces
+ * cuda_setup_model_params_ (cuda/cuda_nemolite2d.cu:236-246, all scalars by
+ * value as there) plus the allocation half of cuda_initialise_grid_
+ * (:248).  jpi, jpj are the interior sizes.  The grid must then be given
+ * with nemo2d_set_grid and the state with nemo2d_set_state. */
+int nemo2d_create(nemo2d_model **model, int jpi, int jpj, double dx, double dy,
+                  double dep_const, int nit000, int nitend, int irecord,
+                  double rdt, double cbfr, double visc);
+
+/* Upload the grid: mask, metrics, latitudes and depths, in the host layout
+ * above.  Replaces the grid half of cuda_initialise_grid_
+ * (cuda/cuda_nemolite2d.cu:282-416), which rebuilt them on the device.
+ * Static coefficients (Coriolis 2*omega*sin(phi*d2r), Flather sqrt(g/h))
+ * are evaluated here on the host.  Uniform metrics and depths are detected
+ * and passed to the kernels as scalars. */
+int nemo2d_set_grid(nemo2d_model *m, const int *pt,
+                    const double *e1t, const double *e2t,
+                    const double *e1u, const double *e2u,
+                    const double *e1v, const double *e2v,
+                    const double *e12t, const double *e12u, const double *e12v,
+                    const double *gphiu, const double *gphiv,
+                    const double *ht, const double *hu, const double *hv);
+
+/* Upload the now-state (sshn T, sshn_u U, sshn_v V, un U, vn V).  The
+ * after-state ua/va starts at zero, as after the serial model's allocation
+ * (original/nemolite2d.f90:310-318). */
+int nemo2d_set_state(nemo2d_model *m, const double *sshn, const double *sshn_u,
+                     const double *sshn_v, const double *un, const double *vn);
+
+/* --- the nine stages of one step, in the serial model's order ----------
+ * Each replaces one block of original/nemolite2d.f90; together, called in
+ * this order, they are one step.  cuda_continuity_
+ * (cuda/cuda_nemolite2d.cu:249), cuda_momentum_ (:250, empty there) and
+ * cuda_boundary_conditions_ (:251) are what they replace at the C-ABI. */
+int nemo2d_continuity(nemo2d_model *m);              /* :390-410 */
+int nemo2d_momentum_u(nemo2d_model *m);              /* :430-517 */
+int nemo2d_momentum_v(nemo2d_model *m);              /* :519-609 */
+int nemo2d_bc_ssh(nemo2d_model *m, double rtime);    /* :628-645 */
+int nemo2d_bc_solid_u(nemo2d_model *m);              /* :648-652 */
+int nemo2d_bc_solid_v(nemo2d_model *m);              /* :656-660 */
+int nemo2d_bc_flather_u(nemo2d_model *m);            /* :669-680 */
+int nemo2d_bc_flather_v(nemo2d_model *m);            /* :684-695 */
+int nemo2d_next(nemo2d_model *m);                    /* :704-769; replaces cuda_next_ (:252) */
+
+/* One whole step (original/nemolite2d.f90:375-383, rtime = istp*rdt) as one
+ * fused pass over HBM plus a boundary epilogue.  Bitwise equal to the nine
+ * stage calls above. */
+int nemo2d_step(nemo2d_model *m, int istp);
+
+/* Steps nit000..nitend back to back (nemo2d_step each), recording the
+ * per-step checksums on the device when checksum recording is on. */
+int nemo2d_run(nemo2d_model *m, int nit000, int nitend);
+
+/* Per-step checksums.  When recording is on (capacity > 0), every
+ * nemo2d_step / nemo2d_next stores sum|sshn|, sum|un|, sum|vn| over
+ * (1:jpi,1:jpj) of the new now-state into a device log (slot istp-nit000
+ * for nemo2d_step; the next free slot for nemo2d_next).  capacity = 0 turns
+ * recording off. */
+int nemo2d_record_checksums(nemo2d_model *m, int capacity);
+/* Copies up to max_steps*3 logged values (step-major: ssh, u, v) to out and
+ * sets *nsteps to how many steps are logged.  Synchronises. */
+int nemo2d_checksum_log(nemo2d_model *m, double *out, int max_steps, int *nsteps);
+
+/* sum|sshn|, sum|un|, sum|vn| over (1:jpi,1:jpj) of the current now-state
+ * (the field_checksum of original/nemolite2d.f90:82-85 and
+ * shared/dl_esm_inf/finite_difference/src/field_mod.f90:935-956, with a
+ * compensated, order-fixed reduction).  Synchronises.  Returns
+ * NEMO2D_ERR_NONFINITE (after filling out) when any sum is not finite. */
+int nemo2d_checksums(nemo2d_model *m, double out[3]);
+
+/* Copy the state back to host arrays (host layout).  Replaces
+ * cuda_retrieve_results_ (cuda/cuda_nemolite2d.cu:259-265): sshn T, un U,
+ * vn V, ua U, va V.  Any pointer may be NULL to skip that array. */
+int nemo2d_retrieve_results(nemo2d_model *m, double *sshn, double *un, double *vn,
+                            double *ua, double *va);
+/* Copy sshn_u (U), sshn_v (V) and ssha (T) back; any may be NULL. */
+int nemo2d_retrieve_aux(nemo2d_model *m, double *sshn_u, double *sshn_v, double *ssha);
+
+/* Device time in milliseconds of the work issued on the model's stream
+ * between nemo2d_timer_start and nemo2d_timer_stop (HIP events on that
+ * stream).  timer_stop synchronises. */
+int nemo2d_timer_start(nemo2d_model *m);
+int nemo2d_timer_stop(nemo2d_model *m, double *ms);
+
+/* Free the model's device memory and stream.  Replaces cuda_finalise_
+ * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */
+int nemo2d_destroy(nemo2d_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEMO2D_H */

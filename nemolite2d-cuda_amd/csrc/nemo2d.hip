@@ -1,0 +1,979 @@
+// nemo2d: MI355X (gfx950) NEMOLite2D FP64 time-stepper behind the C-ABI of
+// include/nemo2d.h.
+//
+// Two device paths share the point functions of nemo2d_core.h:
+//   * stage kernels, one per call of the serial model's step
+//     (original/nemolite2d.f90:375-769), each over its Fortran index range,
+//     writing the after-arrays ssha/ua/va like the serial model;
+//   * a fused step: one HBM pass that reads the now-state once and writes
+//     the new now-state (ping-pong buffers), with the boundary stages folded
+//     in as per-point classes and the per-step checksums reduced on the fly,
+//     followed by a one-block kernel that finishes the checksum reduction.
+// The O(N^2) mask scans of the serial model's boundary code become per-point
+// flag bits and per-line lists built once on the host.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "nemo2d.h"
+#include "nemo2d_core.h"
+
+using namespace nemo2d;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(NEMO2D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(what)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(NEMO2D_ERR_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum { F_SSHN = 0, F_SSHN_U, F_SSHN_V, F_UN, F_VN, NF };
+enum {
+    S_E1T = 0, S_E2T, S_E1U, S_E2U, S_E1V, S_E2V, S_E12T, S_E12U, S_E12V,
+    S_HT, S_HU, S_HV, S_FCU, S_FCV, NS
+};
+
+constexpr int TBX = 64, TBY = 4;   // 2-D tiles: one wave per row segment
+constexpr int RED_THREADS = 1024;  // checksum finishing kernel
+
+}  // namespace
+
+struct nemo2d_model {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    Layout L{};
+    Params prm{};
+    int nit000 = 1, nitend = 1, irecord = 1;
+    double dx = 0, dy = 0, dep_const = 0;
+
+    double* S[2][NF] = {};  // now-state ping-pong
+    int cur = 0;
+    double *ssha = nullptr, *ua = nullptr, *va = nullptr;  // stage path after-arrays
+    bool a_current = false;  // ua/va hold the last step's after-values
+
+    uint16_t* flags = nullptr;
+    double *flu = nullptr, *flv = nullptr;  // Flather sqrt(g/h) at Flather points
+    double* st[NS] = {};
+    bool uniform = false;
+    Scal scal{};
+    bool fused_ok = false;
+
+    // Flather point lists in the serial model's loop order:
+    //   u: rows jj, entries sorted by ji;  v: columns ji, entries sorted by jj.
+    int* flu_ptr = nullptr;   // jpj+1 row pointers (rows 1..jpj)
+    long* flu_pts = nullptr;
+    int* flv_ptr = nullptr;   // jpi+1 column pointers (columns 1..jpi)
+    long* flv_pts = nullptr;
+    int nflu = 0, nflv = 0;
+
+    // checksum reduction
+    double* partials = nullptr;  // 6 doubles per block
+    int npart_max = 0;
+    double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
+    double* cs_log = nullptr;    // 3 doubles per logged step
+    int cs_cap = 0;
+    int cs_next = 0;             // next free slot for stage-path logging
+    int cs_n = 0;                // steps logged (max slot + 1)
+    double* h_pinned = nullptr;  // 3 doubles of pinned host memory
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool grid_set = false, state_set = false;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+struct AOut {
+    double *sshn, *sshn_u, *sshn_v, *un, *vn;
+};
+
+__device__ __forceinline__ bool in_rect(int i, int j, int i0, int i1, int j0, int j1) {
+    return i >= i0 && i <= i1 && j >= j0 && j <= j1;
+}
+
+// Pre-Flather value of the after-velocity at U point o, for a point that is
+// the inner neighbour of a Flather point (never itself a Flather point when
+// the fused path is used): momentum if computed, else 0 (solid points are
+// zeroed, untouched points keep the zero of the allocation).
+template <class S>
+__device__ __forceinline__ double u_pre_flather(const S& st, const NState& n, const uint16_t* fl,
+                                                long o, long P, const Params& p) {
+    const unsigned f = fl[o];
+    return ucls(f) == CLS_COMP ? momentum_u_at(st, n, o, P, f, p) : 0.0;
+}
+template <class S>
+__device__ __forceinline__ double v_pre_flather(const S& st, const NState& n, const uint16_t* fl,
+                                                long o, long P, const Params& p) {
+    const unsigned f = fl[o];
+    return vcls(f) == CLS_COMP ? momentum_v_at(st, n, o, P, f, p) : 0.0;
+}
+
+// ssha at T point o after bc_ssh: the tide value where clamped.
+template <class S>
+__device__ __forceinline__ double ssha_at(const S& st, const NState& n, const uint16_t* fl, long o,
+                                          long P, double rdt, double tide) {
+    return (fl[o] & T_CLAMP) ? tide : continuity_at(st, n, o, P, rdt);
+}
+
+// ---- fused step ------------------------------------------------------------
+// One thread per cell (i, j), i in [0, jpi], j in [0, jpj]: it produces the
+// T point (i, j), the U point (i, j) (east face), the V point (i, j) (north
+// face) and sshn_u/sshn_v there.  The new ssh east and north of the cell,
+// needed by the interpolation of next, is recomputed from the now-state.
+template <class S>
+__global__ void __launch_bounds__(TBX * TBY)
+k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
+             const double* __restrict__ flu, const double* __restrict__ flv, Layout L,
+             Params p, double tide, double* __restrict__ partials) {
+    const int i = blockIdx.x * TBX + threadIdx.x;
+    const int j = blockIdx.y * TBY + threadIdx.y;
+    const long P = L.pitch;
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (i <= L.jpi && j <= L.jpj) {
+        const long o = L.at(i, j);
+        const unsigned f = fl[o];
+        double s_c = 0.0;
+        if (i >= 1 && j >= 1) {
+            s_c = (f & T_CLAMP) ? tide : continuity_at(st, n, o, P, p.rdt);
+            a.sshn[o] = s_c;
+            dd_add(ct, fabs(s_c));
+        }
+        if (j >= 1) {
+            // U point (i, j)
+            double u;
+            const unsigned c = ucls(f);
+            if (c == CLS_COMP) {
+                u = momentum_u_at(st, n, o, P, f, p);
+            } else if (c == CLS_FLATHER) {
+                const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
+                u = u_pre_flather(st, n, fl, oi, P, p) + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
+            } else {
+                u = 0.0;
+            }
+            a.un[o] = u;
+            if (i >= 1) dd_add(cu, fabs(u));
+            const unsigned su = sucls(f);
+            double s_hi = 0.0;
+            if (su == NX_AVG || su == NX_HI) s_hi = ssha_at(st, n, fl, o + 1, P, p.rdt, tide);
+            a.sshn_u[o] = next_su(st, su, o, s_c, s_hi, n.sshn_u[o]);
+        }
+        if (i >= 1) {
+            // V point (i, j)
+            double v;
+            const unsigned c = vcls(f);
+            if (c == CLS_COMP) {
+                v = momentum_v_at(st, n, o, P, f, p);
+            } else if (c == CLS_FLATHER) {
+                const long oi = (f & V_FL_NORTH) ? o + P : o - P;
+                v = v_pre_flather(st, n, fl, oi, P, p) + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
+            } else {
+                v = 0.0;
+            }
+            a.vn[o] = v;
+            if (j >= 1) dd_add(cv, fabs(v));
+            const unsigned sv = svcls(f);
+            double s_hi = 0.0;
+            if (sv == NX_AVG || sv == NX_HI) s_hi = ssha_at(st, n, fl, o + P, P, p.rdt, tide);
+            a.sshn_v[o] = next_sv(st, sv, o, P, s_c, s_hi, n.sshn_v[o]);
+        }
+    }
+    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// ---- checksum ----------------------------------------------------------------
+// sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of the given state.
+__global__ void __launch_bounds__(TBX * TBY)
+k_checksum_partials(const double* __restrict__ sshn, const double* __restrict__ un,
+                    const double* __restrict__ vn, Layout L, double* __restrict__ partials) {
+    const int i = blockIdx.x * TBX + threadIdx.x;
+    const int j = blockIdx.y * TBY + threadIdx.y;
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (in_rect(i, j, 1, L.jpi, 1, L.jpj)) {
+        const long o = L.at(i, j);
+        dd_add(ct, fabs(sshn[o]));
+        dd_add(cu, fabs(un[o]));
+        dd_add(cv, fabs(vn[o]));
+    }
+    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// Finish: reduce nparts block partials in a fixed order into out[0..2].
+__global__ void __launch_bounds__(RED_THREADS)
+k_checksum_finish(const double* __restrict__ partials, int nparts, double* __restrict__ out) {
+    DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += RED_THREADS) {
+        const double* q = partials + 6 * (long)k;
+        a0 = dd_merge(a0, DD{q[0], q[1]});
+        a1 = dd_merge(a1, DD{q[2], q[3]});
+        a2 = dd_merge(a2, DD{q[4], q[5]});
+    }
+    __shared__ double r[6];
+    block_reduce3(a0, a1, a2, r);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = r[0] + r[1];
+        out[1] = r[2] + r[3];
+        out[2] = r[4] + r[5];
+    }
+}
+
+// ---- stage kernels (one thread per point of the stage's Fortran range) ------
+template <class S>
+__global__ void k_continuity(S st, NState n, double* __restrict__ ssha, Layout L, double rdt) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 1, L.jpi, 1, L.jpj)) return;
+    const long o = L.at(i, j);
+    ssha[o] = continuity_at(st, n, o, L.pitch, rdt);
+}
+
+template <class S>
+__global__ void k_momentum_u(S st, NState n, const uint16_t* __restrict__ fl,
+                             double* __restrict__ ua, Layout L, Params p) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 1, L.jpi - 1, 1, L.jpj)) return;
+    const long o = L.at(i, j);
+    const unsigned f = fl[o];
+    if (ucls(f) == CLS_COMP) ua[o] = momentum_u_at(st, n, o, L.pitch, f, p);
+}
+
+template <class S>
+__global__ void k_momentum_v(S st, NState n, const uint16_t* __restrict__ fl,
+                             double* __restrict__ va, Layout L, Params p) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 1, L.jpi, 1, L.jpj - 1)) return;
+    const long o = L.at(i, j);
+    const unsigned f = fl[o];
+    if (vcls(f) == CLS_COMP) va[o] = momentum_v_at(st, n, o, L.pitch, f, p);
+}
+
+__global__ void k_bc_ssh(const uint16_t* __restrict__ fl, double* __restrict__ ssha, Layout L,
+                         double tide) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 1, L.jpi, 1, L.jpj)) return;
+    const long o = L.at(i, j);
+    if (fl[o] & T_CLAMP) ssha[o] = tide;
+}
+
+__global__ void k_bc_solid_u(const uint16_t* __restrict__ fl, double* __restrict__ ua, Layout L) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 0, L.jpi, 1, L.jpj)) return;
+    const long o = L.at(i, j);
+    if (ucls(fl[o]) == CLS_SOLID) ua[o] = 0.0;
+}
+
+__global__ void k_bc_solid_v(const uint16_t* __restrict__ fl, double* __restrict__ va, Layout L) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (!in_rect(i, j, 1, L.jpi, 0, L.jpj)) return;
+    const long o = L.at(i, j);
+    if (vcls(fl[o]) == CLS_SOLID) va[o] = 0.0;
+}
+
+// Flather: one thread per line walks that line's points in the serial
+// model's loop order, so chained points see exactly the values it would.
+__global__ void k_bc_flather_u(const int* __restrict__ ptr, const long* __restrict__ pts,
+                               int nlines, const uint16_t* __restrict__ fl,
+                               const double* __restrict__ flu, const double* __restrict__ sshn_u,
+                               double* __restrict__ ua) {
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    for (int k = ptr[line]; k < ptr[line + 1]; ++k) {
+        const long o = pts[k];
+        const long oi = (fl[o] & U_FL_EAST) ? o + 1 : o - 1;
+        ua[o] = ua[oi] + flu[o] * (sshn_u[o] - sshn_u[oi]);
+    }
+}
+
+__global__ void k_bc_flather_v(const int* __restrict__ ptr, const long* __restrict__ pts,
+                               int nlines, const uint16_t* __restrict__ fl,
+                               const double* __restrict__ flv, const double* __restrict__ sshn_v,
+                               double* __restrict__ va, long P) {
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    for (int k = ptr[line]; k < ptr[line + 1]; ++k) {
+        const long o = pts[k];
+        const long oi = (fl[o] & V_FL_NORTH) ? o + P : o - P;
+        va[o] = va[oi] + flv[o] * (sshn_v[o] - sshn_v[oi]);
+    }
+}
+
+// next, original/nemolite2d.f90:704-769: un=ua, vn=va, sshn=ssha, then the
+// interpolations from the new ssh (read from ssha, which equals it).
+template <class S>
+__global__ void k_next(S st, const uint16_t* __restrict__ fl, const double* __restrict__ ssha,
+                       const double* __restrict__ ua, const double* __restrict__ va,
+                       double* __restrict__ sshn, double* __restrict__ sshn_u,
+                       double* __restrict__ sshn_v, double* __restrict__ un,
+                       double* __restrict__ vn, Layout L) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j), P = L.pitch;
+    const unsigned f = fl[o];
+    if (i >= 1 && j >= 1) sshn[o] = ssha[o];
+    if (j >= 1) {
+        un[o] = ua[o];
+        const unsigned c = sucls(f);
+        if (c != NX_SKIP) sshn_u[o] = next_su(st, c, o, ssha[o], ssha[o + 1], 0.0);
+    }
+    if (i >= 1) {
+        vn[o] = va[o];
+        const unsigned c = svcls(f);
+        if (c != NX_SKIP) sshn_v[o] = next_sv(st, c, o, P, ssha[o], ssha[o + P], 0.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+dim3 grid2d(int ni, int nj) { return dim3((ni + TBX - 1) / TBX, (nj + TBY - 1) / TBY); }
+
+NState nstate(const nemo2d_model* m, int k) {
+    NState n;
+    n.sshn = m->S[k][F_SSHN];
+    n.sshn_u = m->S[k][F_SSHN_U];
+    n.sshn_v = m->S[k][F_SSHN_V];
+    n.un = m->S[k][F_UN];
+    n.vn = m->S[k][F_VN];
+    return n;
+}
+
+StArrays starrays(const nemo2d_model* m) {
+    StArrays s;
+    s.a_e1t = m->st[S_E1T]; s.a_e2t = m->st[S_E2T];
+    s.a_e1u = m->st[S_E1U]; s.a_e2u = m->st[S_E2U];
+    s.a_e1v = m->st[S_E1V]; s.a_e2v = m->st[S_E2V];
+    s.a_e12t = m->st[S_E12T]; s.a_e12u = m->st[S_E12U]; s.a_e12v = m->st[S_E12V];
+    s.a_ht = m->st[S_HT]; s.a_hu = m->st[S_HU]; s.a_hv = m->st[S_HV];
+    s.a_fcu = m->st[S_FCU]; s.a_fcv = m->st[S_FCV];
+    return s;
+}
+
+// Dispatch a templated launch on the statics policy.
+#define WITH_STATICS(m, STV, BODY)          \
+    do {                                    \
+        if ((m)->uniform) {                 \
+            StUniform STV{(m)->scal};       \
+            BODY;                           \
+        } else {                            \
+            StArrays STV = starrays(m);     \
+            BODY;                           \
+        }                                   \
+    } while (0)
+
+// Tide value of the clamped-ssh boundary, original/nemolite2d.f90:629-635,
+// evaluated with the host C library.
+double tide_value(double rtime) {
+    const double amp_tide = 0.2;
+    const double omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0);
+    return amp_tide * sin(omega_tide * rtime);
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int dalloc(double** p, size_t n) {
+    HIPCHK(hipMalloc((void**)p, n * sizeof(double)));
+    HIPCHK(hipMemset(*p, 0, n * sizeof(double)));
+    return NEMO2D_OK;
+}
+
+// Host Fortran array (lb1:ub1, lb2:ub2) -> device layout.
+int upload(const nemo2d_model* m, double* dst, const double* src, int lb1, int ub1, int lb2,
+           int ub2) {
+    const size_t w = (size_t)(ub1 - lb1 + 1) * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(dst + m->L.at(lb1, lb2), m->L.pitch * sizeof(double), src, w, w,
+                            ub2 - lb2 + 1, hipMemcpyHostToDevice, m->stream));
+    return NEMO2D_OK;
+}
+
+int download(const nemo2d_model* m, double* dst, const double* src, int lb1, int ub1, int lb2,
+             int ub2) {
+    const size_t w = (size_t)(ub1 - lb1 + 1) * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(dst, w, src + m->L.at(lb1, lb2), m->L.pitch * sizeof(double), w,
+                            ub2 - lb2 + 1, hipMemcpyDeviceToHost, m->stream));
+    return NEMO2D_OK;
+}
+
+bool all_equal(const double* a, size_t n) {
+    for (size_t k = 1; k < n; ++k)
+        if (memcmp(&a[k], &a[0], sizeof(double)) != 0) return false;
+    return true;
+}
+
+int checksum_into(nemo2d_model* m, int k, double* out) {
+    const Layout& L = m->L;
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    k_checksum_partials<<<g, dim3(TBX, TBY), 0, m->stream>>>(
+        m->S[k][F_SSHN], m->S[k][F_UN], m->S[k][F_VN], L, m->partials);
+    LAUNCHCHK("checksum_partials");
+    k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y), out);
+    LAUNCHCHK("checksum_finish");
+    return NEMO2D_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* nemo2d_last_error(void) { return g_err.c_str(); }
+
+const char* nemo2d_version(void) { return "nemo2d 0.1 gfx950 fp64"; }
+
+int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, double dep_const,
+                  int nit000, int nitend, int irecord, double rdt, double cbfr, double visc) {
+    if (!out) return fail(NEMO2D_ERR_ARG, "nemo2d_create: null handle pointer");
+    *out = nullptr;
+    if (jpi < 1 || jpj < 1 || jpi > (1 << 20) || jpj > (1 << 20))
+        return fail(NEMO2D_ERR_ARG, "nemo2d_create: grid size must be >= 1 (jpi=" +
+                                        std::to_string(jpi) + ", jpj=" + std::to_string(jpj) + ")");
+    if (!(rdt > 0.0) || !isfinite(cbfr) || !isfinite(visc))
+        return fail(NEMO2D_ERR_ARG, "nemo2d_create: rdt must be > 0 and cbfr, visc finite");
+    nemo2d_model* m = new nemo2d_model();
+    m->prm.rdt = rdt;
+    m->prm.cbfr = cbfr;
+    m->prm.visc = visc;
+    m->prm.fric = 1.0 + cbfr * rdt;
+    m->nit000 = nit000;
+    m->nitend = nitend;
+    m->irecord = irecord;
+    m->dx = dx;
+    m->dy = dy;
+    m->dep_const = dep_const;
+    Layout& L = m->L;
+    L.jpi = jpi;
+    L.jpj = jpj;
+    // columns: guard (i=-1) .. i=jpi+2, with i=0 at COL0; pitch a multiple of
+    // 16 doubles (128 B)
+    L.pitch = ((long)COL0 + jpi + 3 + 15) / 16 * 16;
+    L.rows = (long)jpj + 4;
+    L.base = L.pitch + COL0;
+    int rc = NEMO2D_OK;
+    auto bail = [&](int code) {
+        nemo2d_destroy(m);
+        return code;
+    };
+    if (hipGetDevice(&m->dev) != hipSuccess) return bail(fail(NEMO2D_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(NEMO2D_ERR_HIP, "hipStreamCreate failed"));
+    const size_t n = (size_t)L.size();
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < NF; ++f)
+            if ((rc = dalloc(&m->S[k][f], n))) return bail(rc);
+    if ((rc = dalloc(&m->ssha, n)) || (rc = dalloc(&m->ua, n)) || (rc = dalloc(&m->va, n)) ||
+        (rc = dalloc(&m->flu, n)) || (rc = dalloc(&m->flv, n)))
+        return bail(rc);
+    if (hipMalloc((void**)&m->flags, n * sizeof(uint16_t)) != hipSuccess ||
+        hipMemset(m->flags, 0, n * sizeof(uint16_t)) != hipSuccess)
+        return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
+    dim3 g = grid2d(jpi + 1, jpj + 1);
+    m->npart_max = (int)(g.x * g.y);
+    if ((rc = dalloc(&m->partials, 6 * (size_t)m->npart_max)) || (rc = dalloc(&m->cs_now, 3)))
+        return bail(rc);
+    if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(NEMO2D_ERR_HIP, "pinned allocation failed"));
+    if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
+        return bail(fail(NEMO2D_ERR_HIP, "event creation failed"));
+    *out = m;
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const double* e2t,
+                    const double* e1u, const double* e2u, const double* e1v, const double* e2v,
+                    const double* e12t, const double* e12u, const double* e12v,
+                    const double* gphiu, const double* gphiv, const double* ht, const double* hu,
+                    const double* hv) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null model");
+    const double* arrs[] = {e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, gphiu, gphiv, ht, hu, hv};
+    for (const double* a : arrs)
+        if (!a) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null array");
+    if (!pt) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null mask");
+    const Layout& L = m->L;
+    const int jpi = L.jpi, jpj = L.jpj;
+    const int pw = jpi + 2;
+    auto PT = [&](int i, int j) { return pt[i + (long)j * pw]; };
+    for (long k = 0; k < (long)pw * (jpj + 2); ++k)
+        if (pt[k] < -1 || pt[k] > 1)
+            return fail(NEMO2D_ERR_MASK, "nemo2d_set_grid: mask values must be -1, 0 or 1");
+    const long nu = (long)(jpi + 1) * jpj, nv = (long)jpi * (jpj + 1), nt = (long)jpi * jpj;
+    auto U = [&](const double* a, int i, int j) { return a[i + (long)(j - 1) * (jpi + 1)]; };
+    auto V = [&](const double* a, int i, int j) { return a[(i - 1) + (long)j * jpi]; };
+
+    // ---- per-point classes --------------------------------------------------
+    std::vector<uint16_t> fl((size_t)L.size(), 0);
+    std::vector<double> hflu((size_t)L.size(), 0.0), hflv((size_t)L.size(), 0.0);
+    auto err_oob = [&](const char* what, int i, int j) {
+        return fail(NEMO2D_ERR_MASK, std::string("nemo2d_set_grid: mask makes the serial model read "
+                                                 "outside its arrays (") + what + " at i=" +
+                                         std::to_string(i) + ", j=" + std::to_string(j) + ")");
+    };
+    for (int j = 1; j <= jpj; ++j)
+        for (int i = 0; i <= jpi; ++i) {  // U points
+            unsigned f = 0;
+            const int a = PT(i, j), b = PT(i + 1, j);
+            unsigned cls = CLS_NONE;
+            if (i >= 1 && i <= jpi - 1 && !(a + b <= 0) && !(a <= 0 || b <= 0)) {
+                cls = CLS_COMP;
+                if (PT(i, j - 1) <= 0 || PT(i + 1, j - 1) <= 0) f |= U_WALL_S;
+                else if (j == 1) return err_oob("momentum_u south", i, j);
+                if (PT(i, j + 1) <= 0 || PT(i + 1, j + 1) <= 0) f |= U_WALL_N;
+                else if (j == jpj) return err_oob("momentum_u north", i, j);
+            }
+            if (a * b == 0) cls = CLS_SOLID;
+            if (!(a + b <= -1) && (a < 0 || b < 0)) {
+                cls = CLS_FLATHER;
+                const bool east = a < 0;
+                if (east) f |= U_FL_EAST;
+                const int ii = east ? i + 1 : i - 1;
+                if (ii < 0 || ii > jpi) return err_oob("bc_flather_u", i, j);
+                hflu[L.at(i, j)] = sqrt(GRAV / U(hu, i, j));
+            }
+            f |= cls << U_CLS_SHIFT;
+            unsigned nx = NX_SKIP;
+            if (!(a + b <= 0)) {
+                if (a * b > 0) nx = NX_AVG;
+                else if (a <= 0) nx = NX_HI;
+                else if (b <= 0) nx = NX_LO;
+            }
+            if ((nx == NX_AVG && (i == 0 || i == jpi)) || (nx == NX_HI && i == jpi) ||
+                (nx == NX_LO && i == 0))
+                return err_oob("next sshn_u", i, j);
+            f |= nx << SU_SHIFT;
+            fl[L.at(i, j)] |= (uint16_t)f;
+        }
+    for (int j = 0; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i) {  // V points
+            unsigned f = 0;
+            const int a = PT(i, j), b = PT(i, j + 1);
+            unsigned cls = CLS_NONE;
+            // the first skip test looks east (original/nemolite2d.f90:523)
+            if (j >= 1 && j <= jpj - 1 && !(a + PT(i + 1, j) <= 0) && !(a <= 0 || b <= 0)) {
+                cls = CLS_COMP;
+                if (PT(i - 1, j) <= 0 || PT(i - 1, j + 1) <= 0) f |= V_WALL_W;
+                else if (i == 1) return err_oob("momentum_v west", i, j);
+                if (PT(i + 1, j) <= 0 || PT(i + 1, j + 1) <= 0) f |= V_WALL_E;
+                else if (i == jpi) return err_oob("momentum_v east", i, j);
+            }
+            if (a * b == 0) cls = CLS_SOLID;
+            if (!(a + b <= -1) && (a < 0 || b < 0)) {
+                cls = CLS_FLATHER;
+                const bool north = a < 0;
+                if (north) f |= V_FL_NORTH;
+                const int jj = north ? j + 1 : j - 1;
+                if (jj < 0 || jj > jpj) return err_oob("bc_flather_v", i, j);
+                hflv[L.at(i, j)] = sqrt(GRAV / V(hv, i, j));
+            }
+            f |= cls << V_CLS_SHIFT;
+            unsigned nx = NX_SKIP;
+            if (!(a + b <= 0)) {
+                if (a * b > 0) nx = NX_AVG;
+                else if (a <= 0) nx = NX_HI;
+                else if (b <= 0) nx = NX_LO;
+            }
+            if ((nx == NX_AVG && (j == 0 || j == jpj)) || (nx == NX_HI && j == jpj) ||
+                (nx == NX_LO && j == 0))
+                return err_oob("next sshn_v", i, j);
+            f |= nx << SV_SHIFT;
+            fl[L.at(i, j)] |= (uint16_t)f;
+        }
+    for (int j = 1; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i) {  // T points: bc_ssh
+            if (PT(i, j) <= 0) continue;
+            if (PT(i, j - 1) < 0 || PT(i, j + 1) < 0 || PT(i + 1, j) < 0 || PT(i - 1, j) < 0)
+                fl[L.at(i, j)] |= T_CLAMP;
+        }
+
+    // Flather lines, in loop order; the fused path needs every Flather
+    // point's inner neighbour to be a non-Flather point.
+    std::vector<int> uptr(jpj + 1, 0), vptr(jpi + 1, 0);
+    std::vector<long> upts, vpts;
+    bool chained = false;
+    for (int j = 1; j <= jpj; ++j) {
+        uptr[j - 1] = (int)upts.size();
+        for (int i = 0; i <= jpi; ++i) {
+            const long o = L.at(i, j);
+            if (ucls(fl[o]) == CLS_FLATHER) {
+                upts.push_back(o);
+                const long oi = (fl[o] & U_FL_EAST) ? o + 1 : o - 1;
+                if (ucls(fl[oi]) == CLS_FLATHER) chained = true;
+            }
+        }
+    }
+    uptr[jpj] = (int)upts.size();
+    for (int i = 1; i <= jpi; ++i) {
+        vptr[i - 1] = (int)vpts.size();
+        for (int j = 0; j <= jpj; ++j) {
+            const long o = L.at(i, j);
+            if (vcls(fl[o]) == CLS_FLATHER) {
+                vpts.push_back(o);
+                const long oi = (fl[o] & V_FL_NORTH) ? o + L.pitch : o - L.pitch;
+                if (vcls(fl[oi]) == CLS_FLATHER) chained = true;
+            }
+        }
+    }
+    vptr[jpi] = (int)vpts.size();
+    m->fused_ok = !chained;
+
+    // ---- statics ------------------------------------------------------------
+    std::vector<double> fcu((size_t)nu), fcv((size_t)nv);
+    const double d2r = PI / 180.0;
+    for (long k = 0; k < nu; ++k) fcu[k] = 2.0 * OMEGA * sin(gphiu[k] * d2r);
+    for (long k = 0; k < nv; ++k) fcv[k] = 2.0 * OMEGA * sin(gphiv[k] * d2r);
+    struct {
+        const double* a;
+        int lb1, ub1, lb2, ub2;
+        long n;
+    } sd[NS] = {
+        {e1t, 1, jpi, 1, jpj, nt},   {e2t, 1, jpi, 1, jpj, nt},   {e1u, 0, jpi, 1, jpj, nu},
+        {e2u, 0, jpi, 1, jpj, nu},   {e1v, 1, jpi, 0, jpj, nv},   {e2v, 1, jpi, 0, jpj, nv},
+        {e12t, 1, jpi, 1, jpj, nt},  {e12u, 0, jpi, 1, jpj, nu},  {e12v, 1, jpi, 0, jpj, nv},
+        {ht, 1, jpi, 1, jpj, nt},    {hu, 0, jpi, 1, jpj, nu},    {hv, 1, jpi, 0, jpj, nv},
+        {fcu.data(), 0, jpi, 1, jpj, nu}, {fcv.data(), 1, jpi, 0, jpj, nv}};
+    bool uni = true;
+    for (int s = 0; s < NS; ++s) uni = uni && all_equal(sd[s].a, (size_t)sd[s].n);
+    m->uniform = uni;
+    m->scal = Scal{e1t[0], e2t[0], e1u[0], e2u[0], e1v[0], e2v[0], e12t[0], e12u[0], e12v[0],
+                   ht[0], hu[0], hv[0], fcu[0], fcv[0]};
+    const size_t n = (size_t)L.size();
+    for (int s = 0; s < NS; ++s) {
+        if (!m->st[s]) {
+            int rc = dalloc(&m->st[s], n);
+            if (rc) return rc;
+        }
+        int rc = upload(m, m->st[s], sd[s].a, sd[s].lb1, sd[s].ub1, sd[s].lb2, sd[s].ub2);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(m->flags, fl.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->flu, hflu.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->flv, hflv.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+    dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
+    m->flu_ptr = nullptr; m->flu_pts = nullptr; m->flv_ptr = nullptr; m->flv_pts = nullptr;
+    m->nflu = (int)upts.size();
+    m->nflv = (int)vpts.size();
+    HIPCHK(hipMalloc((void**)&m->flu_ptr, uptr.size() * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&m->flv_ptr, vptr.size() * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&m->flu_pts, (upts.size() + 1) * sizeof(long)));
+    HIPCHK(hipMalloc((void**)&m->flv_pts, (vpts.size() + 1) * sizeof(long)));
+    HIPCHK(hipMemcpyAsync(m->flu_ptr, uptr.data(), uptr.size() * sizeof(int), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->flv_ptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice, m->stream));
+    if (!upts.empty())
+        HIPCHK(hipMemcpyAsync(m->flu_pts, upts.data(), upts.size() * sizeof(long), hipMemcpyHostToDevice, m->stream));
+    if (!vpts.empty())
+        HIPCHK(hipMemcpyAsync(m->flv_pts, vpts.data(), vpts.size() * sizeof(long), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->grid_set = true;
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, const double* sshn_v,
+                     const double* un, const double* vn) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_state: null model");
+    if (!sshn || !sshn_u || !sshn_v || !un || !vn)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_set_state: null array");
+    const int jpi = m->L.jpi, jpj = m->L.jpj;
+    const size_t bytes = (size_t)m->L.size() * sizeof(double);
+    m->cur = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < NF; ++f) HIPCHK(hipMemsetAsync(m->S[k][f], 0, bytes, m->stream));
+    HIPCHK(hipMemsetAsync(m->ssha, 0, bytes, m->stream));
+    HIPCHK(hipMemsetAsync(m->ua, 0, bytes, m->stream));
+    HIPCHK(hipMemsetAsync(m->va, 0, bytes, m->stream));
+    int rc;
+    if ((rc = upload(m, m->S[0][F_SSHN], sshn, 1, jpi, 1, jpj)) ||
+        (rc = upload(m, m->S[0][F_SSHN_U], sshn_u, 0, jpi, 1, jpj)) ||
+        (rc = upload(m, m->S[0][F_SSHN_V], sshn_v, 1, jpi, 0, jpj)) ||
+        (rc = upload(m, m->S[0][F_UN], un, 0, jpi, 1, jpj)) ||
+        (rc = upload(m, m->S[0][F_VN], vn, 1, jpi, 0, jpj)))
+        return rc;
+    // the skip-class points of sshn_u/sshn_v are carried, not recomputed:
+    // give the second buffer the same values
+    for (int f = 0; f < NF; ++f)
+        HIPCHK(hipMemcpyAsync(m->S[1][f], m->S[0][f], bytes, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->a_current = true;
+    m->state_set = true;
+    m->cs_next = 0;
+    m->cs_n = 0;
+    return NEMO2D_OK;
+}
+
+#define READY(m, who)                                                                      \
+    do {                                                                                   \
+        if (!(m)) return fail(NEMO2D_ERR_ARG, std::string(who) + ": null model");          \
+        if (!(m)->grid_set || !(m)->state_set)                                             \
+            return fail(NEMO2D_ERR_STATE, std::string(who) + ": grid and state must be set first"); \
+    } while (0)
+
+int nemo2d_continuity(nemo2d_model* m) {
+    READY(m, "nemo2d_continuity");
+    const Layout& L = m->L;
+    NState n = nstate(m, m->cur);
+    WITH_STATICS(m, st, (k_continuity<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, n, m->ssha, L, m->prm.rdt)));
+    LAUNCHCHK("continuity");
+    return NEMO2D_OK;
+}
+
+int nemo2d_momentum_u(nemo2d_model* m) {
+    READY(m, "nemo2d_momentum_u");
+    const Layout& L = m->L;
+    NState n = nstate(m, m->cur);
+    WITH_STATICS(m, st, (k_momentum_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, n, m->flags, m->ua, L, m->prm)));
+    LAUNCHCHK("momentum_u");
+    m->a_current = true;
+    return NEMO2D_OK;
+}
+
+int nemo2d_momentum_v(nemo2d_model* m) {
+    READY(m, "nemo2d_momentum_v");
+    const Layout& L = m->L;
+    NState n = nstate(m, m->cur);
+    WITH_STATICS(m, st, (k_momentum_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, n, m->flags, m->va, L, m->prm)));
+    LAUNCHCHK("momentum_v");
+    m->a_current = true;
+    return NEMO2D_OK;
+}
+
+int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
+    READY(m, "nemo2d_bc_ssh");
+    const Layout& L = m->L;
+    k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ssha, L,
+                                                                            tide_value(rtime));
+    LAUNCHCHK("bc_ssh");
+    return NEMO2D_OK;
+}
+
+int nemo2d_bc_solid_u(nemo2d_model* m) {
+    READY(m, "nemo2d_bc_solid_u");
+    const Layout& L = m->L;
+    k_bc_solid_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ua, L);
+    LAUNCHCHK("bc_solid_u");
+    return NEMO2D_OK;
+}
+
+int nemo2d_bc_solid_v(nemo2d_model* m) {
+    READY(m, "nemo2d_bc_solid_v");
+    const Layout& L = m->L;
+    k_bc_solid_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->va, L);
+    LAUNCHCHK("bc_solid_v");
+    return NEMO2D_OK;
+}
+
+int nemo2d_bc_flather_u(nemo2d_model* m) {
+    READY(m, "nemo2d_bc_flather_u");
+    if (m->nflu == 0) return NEMO2D_OK;
+    const int lines = m->L.jpj;
+    k_bc_flather_u<<<(lines + 255) / 256, 256, 0, m->stream>>>(
+        m->flu_ptr, m->flu_pts, lines, m->flags, m->flu, m->S[m->cur][F_SSHN_U], m->ua);
+    LAUNCHCHK("bc_flather_u");
+    return NEMO2D_OK;
+}
+
+int nemo2d_bc_flather_v(nemo2d_model* m) {
+    READY(m, "nemo2d_bc_flather_v");
+    if (m->nflv == 0) return NEMO2D_OK;
+    const int lines = m->L.jpi;
+    k_bc_flather_v<<<(lines + 255) / 256, 256, 0, m->stream>>>(
+        m->flv_ptr, m->flv_pts, lines, m->flags, m->flv, m->S[m->cur][F_SSHN_V], m->va, m->L.pitch);
+    LAUNCHCHK("bc_flather_v");
+    return NEMO2D_OK;
+}
+
+int nemo2d_next(nemo2d_model* m) {
+    READY(m, "nemo2d_next");
+    const Layout& L = m->L;
+    double** s = m->S[m->cur];
+    WITH_STATICS(m, st, (k_next<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, m->flags, m->ssha, m->ua, m->va, s[F_SSHN], s[F_SSHN_U],
+                            s[F_SSHN_V], s[F_UN], s[F_VN], L)));
+    LAUNCHCHK("next");
+    m->a_current = true;
+    if (m->cs_cap > 0 && m->cs_next < m->cs_cap) {
+        int rc = checksum_into(m, m->cur, m->cs_log + 3 * (long)m->cs_next);
+        if (rc) return rc;
+        ++m->cs_next;
+        if (m->cs_next > m->cs_n) m->cs_n = m->cs_next;
+    }
+    return NEMO2D_OK;
+}
+
+static int run_stages(nemo2d_model* m, int istp) {
+    int rc;
+    if ((rc = nemo2d_continuity(m)) || (rc = nemo2d_momentum_u(m)) || (rc = nemo2d_momentum_v(m)) ||
+        (rc = nemo2d_bc_ssh(m, (double)istp * m->prm.rdt)) || (rc = nemo2d_bc_solid_u(m)) ||
+        (rc = nemo2d_bc_solid_v(m)) || (rc = nemo2d_bc_flather_u(m)) || (rc = nemo2d_bc_flather_v(m)))
+        return rc;
+    // next logs into the next free slot; keep fused and stage slots aligned
+    const int slot = istp - m->nit000;
+    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) m->cs_next = slot;
+    return nemo2d_next(m);
+}
+
+int nemo2d_step(nemo2d_model* m, int istp) {
+    READY(m, "nemo2d_step");
+    if (!m->fused_ok) return run_stages(m, istp);
+    const Layout& L = m->L;
+    const int k = m->cur;
+    NState n = nstate(m, k);
+    AOut a{m->S[1 - k][F_SSHN], m->S[1 - k][F_SSHN_U], m->S[1 - k][F_SSHN_V], m->S[1 - k][F_UN],
+           m->S[1 - k][F_VN]};
+    const double tide = tide_value((double)istp * m->prm.rdt);
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    WITH_STATICS(m, st, (k_step_fused<<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                            st, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials)));
+    LAUNCHCHK("step_fused");
+    m->cur = 1 - k;
+    m->a_current = false;
+    const int slot = istp - m->nit000;
+    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y),
+                                                            m->cs_log + 3 * (long)slot);
+        LAUNCHCHK("checksum_finish");
+        if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
+        m->cs_next = slot + 1;
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_run(nemo2d_model* m, int nit000, int nitend) {
+    READY(m, "nemo2d_run");
+    for (int istp = nit000; istp <= nitend; ++istp) {
+        int rc = nemo2d_step(m, istp);
+        if (rc) return rc;
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_record_checksums(nemo2d_model* m, int capacity) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_record_checksums: null model");
+    if (capacity < 0) return fail(NEMO2D_ERR_ARG, "nemo2d_record_checksums: negative capacity");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    dfree(m->cs_log);
+    m->cs_log = nullptr;
+    m->cs_cap = 0;
+    m->cs_n = 0;
+    m->cs_next = 0;
+    if (capacity > 0) {
+        int rc = dalloc(&m->cs_log, 3 * (size_t)capacity);
+        if (rc) return rc;
+        m->cs_cap = capacity;
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_checksum_log(nemo2d_model* m, double* out, int max_steps, int* nsteps) {
+    if (!m || (!out && max_steps > 0)) return fail(NEMO2D_ERR_ARG, "nemo2d_checksum_log: bad argument");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    const int n = m->cs_n < max_steps ? m->cs_n : max_steps;
+    if (n > 0) HIPCHK(hipMemcpy(out, m->cs_log, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (nsteps) *nsteps = m->cs_n;
+    return NEMO2D_OK;
+}
+
+int nemo2d_checksums(nemo2d_model* m, double out[3]) {
+    READY(m, "nemo2d_checksums");
+    if (!out) return fail(NEMO2D_ERR_ARG, "nemo2d_checksums: null output");
+    int rc = checksum_into(m, m->cur, m->cs_now);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(m->h_pinned, m->cs_now, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (int k = 0; k < 3; ++k) out[k] = m->h_pinned[k];
+    for (int k = 0; k < 3; ++k)
+        if (!isfinite(out[k])) return fail(NEMO2D_ERR_NONFINITE, "nemo2d_checksums: non-finite checksum");
+    return NEMO2D_OK;
+}
+
+int nemo2d_retrieve_results(nemo2d_model* m, double* sshn, double* un, double* vn, double* ua,
+                            double* va) {
+    READY(m, "nemo2d_retrieve_results");
+    const int jpi = m->L.jpi, jpj = m->L.jpj;
+    double** s = m->S[m->cur];
+    // after a fused step, ua/va equal un/vn (next copied them)
+    const double* dua = m->a_current ? m->ua : s[F_UN];
+    const double* dva = m->a_current ? m->va : s[F_VN];
+    int rc = NEMO2D_OK;
+    if (sshn && (rc = download(m, sshn, s[F_SSHN], 1, jpi, 1, jpj))) return rc;
+    if (un && (rc = download(m, un, s[F_UN], 0, jpi, 1, jpj))) return rc;
+    if (vn && (rc = download(m, vn, s[F_VN], 1, jpi, 0, jpj))) return rc;
+    if (ua && (rc = download(m, ua, dua, 0, jpi, 1, jpj))) return rc;
+    if (va && (rc = download(m, va, dva, 1, jpi, 0, jpj))) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_retrieve_aux(nemo2d_model* m, double* sshn_u, double* sshn_v, double* ssha) {
+    READY(m, "nemo2d_retrieve_aux");
+    const int jpi = m->L.jpi, jpj = m->L.jpj;
+    double** s = m->S[m->cur];
+    int rc = NEMO2D_OK;
+    if (sshn_u && (rc = download(m, sshn_u, s[F_SSHN_U], 0, jpi, 1, jpj))) return rc;
+    if (sshn_v && (rc = download(m, sshn_v, s[F_SSHN_V], 1, jpi, 0, jpj))) return rc;
+    // after a fused step the new ssh is the now-state's sshn
+    if (ssha && (rc = download(m, ssha, m->a_current ? m->ssha : s[F_SSHN], 1, jpi, 1, jpj))) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_timer_start(nemo2d_model* m) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_timer_start: null model");
+    HIPCHK(hipEventRecord(m->ev0, m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_timer_stop(nemo2d_model* m, double* ms) {
+    if (!m || !ms) return fail(NEMO2D_ERR_ARG, "nemo2d_timer_stop: bad argument");
+    HIPCHK(hipEventRecord(m->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, m->ev0, m->ev1));
+    *ms = (double)t;
+    return NEMO2D_OK;
+}
+
+int nemo2d_destroy(nemo2d_model* m) {
+    if (!m) return NEMO2D_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < NF; ++f) dfree(m->S[k][f]);
+    for (int s = 0; s < NS; ++s) dfree(m->st[s]);
+    dfree(m->ssha); dfree(m->ua); dfree(m->va);
+    dfree(m->flags); dfree(m->flu); dfree(m->flv);
+    dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
+    dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log);
+    if (m->h_pinned) (void)hipHostFree(m->h_pinned);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return NEMO2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+!> NEMOLite2D driver for the nemo2d gfx950 library.
+!!
+!! Keeps the serial model's program structure (original/nemolite2d.f90:1-90):
+!! the &namctl namelist read from ./namelist, the grid and initial state
+!! built on the host, the go2d_NNNNN.dat snapshots at istp=0 and every
+!! irecord steps, and the final 'ua checksum' / 'va checksum' lines.  The step
+!! calls the library's nine stage entry points in the serial model's order
+!! (continuity, momentum_u, momentum_v, bc_ssh, bc_solid_u, bc_solid_v,
+!! bc_flather_u, bc_flather_v, next) or, with fused=.true. in the optional
+!! &namhip group, the single fused step.  Per-step checksums sum|ssh|, sum|u|,
+!! sum|v| are written to the file named by csfile.
+program nemolite2d_hip
+    use, intrinsic :: iso_c_binding, only: c_int, c_double, c_ptr, c_null_ptr
+    use nemo2d_api
+    implicit none
+
+    integer, parameter :: wp = c_double
+
+    integer(c_int), allocatable :: pt(:, :)
+    real(wp), allocatable :: e1t(:, :), e2t(:, :), e1u(:, :), e2u(:, :), e1v(:, :), e2v(:, :)
+    real(wp), allocatable :: e12t(:, :), e12u(:, :), e12v(:, :), gphiu(:, :), gphiv(:, :)
+    real(wp), allocatable :: xt(:, :), yt(:, :), ht(:, :), hu(:, :), hv(:, :)
+    real(wp), allocatable :: sshn(:, :), sshn_u(:, :), sshn_v(:, :)
+    real(wp), allocatable :: un(:, :), vn(:, :), ua(:, :), va(:, :)
+    real(wp), allocatable :: cslog(:)
+
+    integer :: jpiglo, jpjglo, jpi, jpj, jphgr_msh, nit000, nitend, irecord
+    real(wp) :: dx, dy, dep_const, rdt, cbfr, visc
+    logical :: fused, write_output
+    character(len=256) :: csfile
+
+    type(c_ptr) :: model = c_null_ptr
+    integer :: istp, nsteps, k
+    integer(c_int) :: nlogged
+    integer(8) :: c0, c1, crate
+    real(wp) :: t_loop, t_dev_ms, cells
+
+    call setup
+    call grid
+    call initialisation
+
+    call nemo2d_check(nemo2d_create(model, jpi, jpj, dx, dy, dep_const, nit000, nitend, irecord, &
+                                    rdt, cbfr, visc), 'nemo2d_create')
+    call nemo2d_check(nemo2d_set_grid(model, pt, e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, &
+                                      gphiu, gphiv, ht, hu, hv), 'nemo2d_set_grid')
+    call nemo2d_check(nemo2d_set_state(model, sshn, sshn_u, sshn_v, un, vn), 'nemo2d_set_state')
+    nsteps = max(nitend - nit000 + 1, 0)
+    call nemo2d_check(nemo2d_record_checksums(model, int(nsteps, c_int)), 'nemo2d_record_checksums')
+
+    istp = 0
+    if (write_output) call output
+
+    call system_clock(c0, crate)
+    call nemo2d_check(nemo2d_timer_start(model), 'nemo2d_timer_start')
+    do istp = nit000, nitend
+        call step
+    end do
+    call nemo2d_check(nemo2d_timer_stop(model, t_dev_ms), 'nemo2d_timer_stop')
+    call system_clock(c1)
+    t_loop = real(c1 - c0, wp)/real(crate, wp)
+
+    call nemo2d_check(nemo2d_retrieve_results(model, sshn, un, vn, ua, va), 'nemo2d_retrieve_results')
+
+    ! per-step checksums, then the serial model's two final lines
+    allocate (cslog(3*max(nsteps, 1)))
+    call nemo2d_check(nemo2d_checksum_log(model, cslog, int(nsteps, c_int), nlogged), &
+                      'nemo2d_checksum_log')
+    open (2, file=trim(csfile), status='replace')
+    write (2, '(A)') '#  istp        sum|ssh|                 sum|u|                   sum|v|'
+    do k = 1, min(int(nlogged), nsteps)
+        write (2, '(I8,3ES25.16E3)') nit000 + k - 1, cslog(3*k - 2), cslog(3*k - 1), cslog(3*k)
+    end do
+    close (2)
+    if (nsteps > 0) then
+        write (*, "('ua checksum = ',E16.8)") cslog(3*nsteps - 1)
+        write (*, "('va checksum = ',E16.8)") cslog(3*nsteps)
+    end if
+
+    cells = real(jpi, wp)*real(jpj, wp)*real(nsteps, wp)
+    write (*, '(A)') '==================================== Timing report ===================================='
+    write (*, '(A)') 'Region                          Counts      Total       Average*   Avg/repeat*'
+    write (*, '(A,I10,3ES13.5)') 'Time-stepping                 ', 1, t_loop, t_loop, &
+        t_loop/real(max(nsteps, 1), wp)
+    write (*, '(A,F12.3,A,ES12.4,A)') 'device time (HIP events): ', t_dev_ms, ' ms;  ', &
+        cells/(t_dev_ms*1.0e-3_wp), ' cell-updates/s'
+    write (*, '(A)') '========================================================================================'
+
+    call nemo2d_check(nemo2d_destroy(model), 'nemo2d_destroy')
+    write (*, *) 'Simulation finished!!'
+
+contains
+
+    !> &namctl as original/nemolite2d.f90:96-141 (same defaults, same -2
+    !! correction), plus the optional &namhip group of this driver.
+    subroutine setup
+        integer :: ios
+        namelist /namctl/ jpiglo, jpjglo, jphgr_msh, dx, dy, dep_const, &
+            nit000, nitend, irecord, rdt, cbfr, visc
+        namelist /namhip/ fused, write_output, csfile
+
+        jpiglo = 258; jpjglo = 258; jphgr_msh = 1
+        dx = 1000._wp; dy = 1000._wp; dep_const = 100._wp
+        nit000 = 1; nitend = 1000; irecord = 1
+        rdt = 10._wp; cbfr = 0.001_wp; visc = 100._wp
+        fused = .false.; write_output = .true.; csfile = 'nemo2d_checksums.txt'
+
+        open (1, file='namelist', status='OLD')
+        rewind (1)
+        read (1, nml=namctl, iostat=ios)
+        if (ios /= 0) stop "err found in reading namelist file"
+        rewind (1)
+        read (1, nml=namhip, iostat=ios)   ! optional group
+        close (1)
+        write (*, nml=namctl)
+        if (jphgr_msh == 0) stop "It is not ready to Read in grid from a file"
+        if (jphgr_msh /= 1) stop "Wrong grid defination type, check your setup !!!!"
+        jpiglo = jpiglo - 2
+        jpjglo = jpjglo - 2
+        jpi = jpiglo
+        jpj = jpjglo
+        if (jpi < 1 .or. jpj < 1) stop "grid too small: jpiglo and jpjglo must be at least 3"
+    end subroutine setup
+
+    !> Mask, metrics, latitudes and depths of the jphgr_msh=1 grid
+    !! (original/nemolite2d.f90:180-279): a land ring with an open south edge.
+    subroutine grid
+        integer :: ji, jj
+        allocate (pt(0:jpi + 1, 0:jpj + 1))
+        allocate (e1t(jpi, jpj), e2t(jpi, jpj), e12t(jpi, jpj), xt(jpi, jpj), yt(jpi, jpj), ht(jpi, jpj))
+        allocate (e1u(0:jpi, jpj), e2u(0:jpi, jpj), e12u(0:jpi, jpj), gphiu(0:jpi, jpj), hu(0:jpi, jpj))
+        allocate (e1v(jpi, 0:jpj), e2v(jpi, 0:jpj), e12v(jpi, 0:jpj), gphiv(jpi, 0:jpj), hv(jpi, 0:jpj))
+        allocate (sshn(jpi, jpj), sshn_u(0:jpi, jpj), sshn_v(jpi, 0:jpj))
+        allocate (un(0:jpi, jpj), vn(jpi, 0:jpj), ua(0:jpi, jpj), va(jpi, 0:jpj))
+
+        pt = 1
+        pt(0, :) = 0
+        pt(jpi + 1, :) = 0
+        pt(:, jpj + 1) = 0
+        pt(1:jpi, 0) = -1
+        pt(0, 0) = 0
+        pt(jpi + 1, 0) = 0
+
+        e1t = dx; e2t = dy; e1u = dx; e2u = dy; e1v = dx; e2v = dy
+        gphiu = 50._wp; gphiv = 50._wp
+        ht = dep_const; hu = dep_const; hv = dep_const
+        do jj = 1, jpj
+            xt(1, jj) = 0.0_wp + 0.5_wp*e1t(1, jj)
+            do ji = 2, jpi
+                xt(ji, jj) = xt(ji - 1, jj) + dx
+            end do
+        end do
+        yt(:, 1) = 0.0_wp + 0.5_wp*e2t(:, 1)
+        do jj = 2, jpj
+            yt(:, jj) = yt(:, jj - 1) + dy
+        end do
+        e12t = e1t*e2t
+        e12u = e1u*e2u
+        e12v = e1v*e2v
+    end subroutine grid
+
+    !> Rest state (original/nemolite2d.f90:324-371).
+    subroutine initialisation
+        integer :: ji, jj, i1, i2
+        sshn = 0.0_wp
+        do ji = 0, jpi
+            do jj = 1, jpj
+                i1 = min(ji + 1, jpi)
+                i2 = max(ji, 1)
+                sshn_u(ji, jj) = 0.5_wp*(e12t(i1, jj)*sshn(i1, jj) + e12t(i2, jj)*sshn(i2, jj))/e12u(ji, jj)
+            end do
+        end do
+        do ji = 1, jpi
+            do jj = 0, jpj
+                i1 = min(jj + 1, jpj)
+                i2 = max(jj, 1)
+                sshn_v(ji, jj) = 0.5_wp*(e12t(ji, i1)*sshn(ji, i1) + e12t(ji, i2)*sshn(ji, i2))/e12v(ji, jj)
+            end do
+        end do
+        un = 0.0_wp
+        vn = 0.0_wp
+        ua = 0.0_wp
+        va = 0.0_wp
+    end subroutine initialisation
+
+    !> One step (original/nemolite2d.f90:375-386) through the library.
+    subroutine step
+        real(wp) :: rtime
+        rtime = real(istp, wp)*rdt
+        if (fused) then
+            call nemo2d_check(nemo2d_step(model, istp), 'nemo2d_step')
+        else
+            call nemo2d_check(nemo2d_continuity(model), 'nemo2d_continuity')
+            call nemo2d_check(nemo2d_momentum_u(model), 'nemo2d_momentum_u')
+            call nemo2d_check(nemo2d_momentum_v(model), 'nemo2d_momentum_v')
+            call nemo2d_check(nemo2d_bc_ssh(model, rtime), 'nemo2d_bc_ssh')
+            call nemo2d_check(nemo2d_bc_solid_u(model), 'nemo2d_bc_solid_u')
+            call nemo2d_check(nemo2d_bc_solid_v(model), 'nemo2d_bc_solid_v')
+            call nemo2d_check(nemo2d_bc_flather_u(model), 'nemo2d_bc_flather_u')
+            call nemo2d_check(nemo2d_bc_flather_v(model), 'nemo2d_bc_flather_v')
+            call nemo2d_check(nemo2d_next(model), 'nemo2d_next')
+        end if
+        if (write_output .and. mod(istp, irecord) == 0) then
+            call nemo2d_check(nemo2d_retrieve_results(model, sshn, un, vn, ua, va), &
+                              'nemo2d_retrieve_results')
+            call output
+        end if
+    end subroutine step
+
+    !> go2d_NNNNN.dat snapshot in the serial model's format
+    !! (original/nemolite2d.f90:773-799).
+    subroutine output
+        character(len=5) :: fname
+        integer :: ji, jj
+        write (fname, '(I5.5)') istp
+        open (1, file='go2d_'//fname//'.dat', status='UNKNOWN')
+        rewind (1)
+        do jj = 1, jpj
+            do ji = 1, jpi
+                write (1, '(6e20.7e4)') xt(ji, jj), yt(ji, jj), ht(ji, jj), sshn(ji, jj), &
+                    0.5_wp*(un(ji - 1, jj) + un(ji, jj)), 0.5_wp*(vn(ji, jj - 1) + vn(ji, jj))
+            end do
+            write (1, *)
+        end do
+        close (1)
+    end subroutine output
+
+end program nemolite2d_hip

@@ -1,0 +1,266 @@
+"""Python host binding of the nemo2d gfx950 library (include/nemo2d.h).
+
+A thin ctypes layer over ``libnemo2d.so``, mirroring the reference's driver
+interface (original/nemolite2d.f90: setup / grid / initialisation / step)
+for scripts and tests.  Host arrays are numpy arrays in the reference's
+Fortran layout: an array with Fortran bounds ``(lb1:ub1, lb2:ub2)`` is a C-
+ordered numpy array of shape ``(ub2-lb2+1, ub1-lb1+1)``, so
+``a[jj - lb2, ji - lb1]`` is the reference's ``a(ji, jj)``.
+
+There is no CPU fallback: loading fails loudly if the library is missing.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnemo2d.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_MASK, ERR_STATE, ERR_NONFINITE = range(6)
+
+# every entry point declared in include/nemo2d.h
+SYMBOLS = (
+    "nemo2d_last_error", "nemo2d_version", "nemo2d_create", "nemo2d_set_grid",
+    "nemo2d_set_state", "nemo2d_continuity", "nemo2d_momentum_u", "nemo2d_momentum_v",
+    "nemo2d_bc_ssh", "nemo2d_bc_solid_u", "nemo2d_bc_solid_v", "nemo2d_bc_flather_u",
+    "nemo2d_bc_flather_v", "nemo2d_next", "nemo2d_step", "nemo2d_run",
+    "nemo2d_record_checksums", "nemo2d_checksum_log", "nemo2d_checksums",
+    "nemo2d_retrieve_results", "nemo2d_retrieve_aux", "nemo2d_timer_start",
+    "nemo2d_timer_stop", "nemo2d_destroy",
+)
+
+
+class Nemo2dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nemo2d error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libnemo2d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("nemo2d: %s not built (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.nemo2d_last_error.restype = ctypes.c_char_p
+        L.nemo2d_version.restype = ctypes.c_char_p
+        L.nemo2d_create.argtypes = [ctypes.POINTER(vp), i, i, d, d, d, i, i, i, d, d, d]
+        L.nemo2d_set_grid.argtypes = [vp, ip] + [dp] * 14
+        L.nemo2d_set_state.argtypes = [vp] + [dp] * 5
+        for n in ("continuity", "momentum_u", "momentum_v", "bc_solid_u", "bc_solid_v",
+                  "bc_flather_u", "bc_flather_v", "next", "timer_start", "destroy"):
+            getattr(L, "nemo2d_" + n).argtypes = [vp]
+        L.nemo2d_bc_ssh.argtypes = [vp, d]
+        L.nemo2d_step.argtypes = [vp, i]
+        L.nemo2d_run.argtypes = [vp, i, i]
+        L.nemo2d_record_checksums.argtypes = [vp, i]
+        L.nemo2d_checksum_log.argtypes = [vp, dp, i, ip]
+        L.nemo2d_checksums.argtypes = [vp, dp]
+        L.nemo2d_retrieve_results.argtypes = [vp] + [dp] * 5
+        L.nemo2d_retrieve_aux.argtypes = [vp] + [dp] * 3
+        L.nemo2d_timer_stop.argtypes = [vp, dp]
+        for s in SYMBOLS:
+            if s not in ("nemo2d_last_error", "nemo2d_version"):
+                getattr(L, s).restype = i
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise Nemo2dError(rc, lib().nemo2d_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def shapes(jpi, jpj):
+    """numpy shapes of the T, U, V and pt arrays."""
+    return {"T": (jpj, jpi), "U": (jpj, jpi + 1), "V": (jpj + 1, jpi), "pt": (jpj + 2, jpi + 2)}
+
+
+def default_grid(jpi, jpj, dx=1000.0, dy=1000.0, dep_const=100.0):
+    """The jphgr_msh=1 grid of original/nemolite2d.f90:180-279: land ring,
+    open south edge, uniform spacing, f-plane at 50 N, constant depth."""
+    s = shapes(jpi, jpj)
+    pt = np.ones(s["pt"], dtype=np.int32)
+    pt[:, 0] = 0
+    pt[:, jpi + 1] = 0
+    pt[jpj + 1, :] = 0
+    pt[0, 1:jpi + 1] = -1
+    g = {"pt": pt}
+    for k, v in (("e1t", dx), ("e2t", dy), ("ht", dep_const)):
+        g[k] = np.full(s["T"], v)
+    for k, v in (("e1u", dx), ("e2u", dy), ("gphiu", 50.0), ("hu", dep_const)):
+        g[k] = np.full(s["U"], v)
+    for k, v in (("e1v", dx), ("e2v", dy), ("gphiv", 50.0), ("hv", dep_const)):
+        g[k] = np.full(s["V"], v)
+    g["e12t"] = g["e1t"] * g["e2t"]
+    g["e12u"] = g["e1u"] * g["e2u"]
+    g["e12v"] = g["e1v"] * g["e2v"]
+    return g
+
+
+def initial_state(grid, sshn=None):
+    """Initial now-state of original/nemolite2d.f90:324-371 (sshn given or 0;
+    sshn_u, sshn_v interpolated with clamped indices; un = vn = 0)."""
+    jpj, jpi = grid["e1t"].shape
+    s = shapes(jpi, jpj)
+    sshn = np.zeros(s["T"]) if sshn is None else np.ascontiguousarray(sshn, dtype=np.float64)
+    e12t = grid["e12t"]
+    sshn_u = np.zeros(s["U"])
+    for ji in range(0, jpi + 1):
+        i1, i2 = min(ji + 1, jpi) - 1, max(ji, 1) - 1
+        sshn_u[:, ji] = 0.5 * (e12t[:, i1] * sshn[:, i1] + e12t[:, i2] * sshn[:, i2]) / grid["e12u"][:, ji]
+    sshn_v = np.zeros(s["V"])
+    for jj in range(0, jpj + 1):
+        j1, j2 = min(jj + 1, jpj) - 1, max(jj, 1) - 1
+        sshn_v[jj, :] = 0.5 * (e12t[j1, :] * sshn[j1, :] + e12t[j2, :] * sshn[j2, :]) / grid["e12v"][jj, :]
+    return {"sshn": sshn, "sshn_u": sshn_u, "sshn_v": sshn_v,
+            "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
+
+
+def tide(rtime):
+    """Clamped-ssh tide of original/nemolite2d.f90:629-635 (host libm)."""
+    omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0)
+    return 0.2 * math.sin(omega_tide * rtime)
+
+
+class Model:
+    """One nemo2d model (device state + stream) on the current HIP device."""
+
+    GRID_KEYS = ("e1t", "e2t", "e1u", "e2u", "e1v", "e2v", "e12t", "e12u", "e12v",
+                 "gphiu", "gphiv", "ht", "hu", "hv")
+    STATE_KEYS = ("sshn", "sshn_u", "sshn_v", "un", "vn")
+
+    def __init__(self, jpi, jpj, dx=1000.0, dy=1000.0, dep_const=100.0, nit000=1, nitend=1,
+                 irecord=1, rdt=20.0, cbfr=0.0015, visc=0.1, grid=None, state=None):
+        self.jpi, self.jpj = int(jpi), int(jpj)
+        self.rdt = float(rdt)
+        self.nit000, self.nitend = int(nit000), int(nitend)
+        h = ctypes.c_void_p()
+        _check(lib().nemo2d_create(ctypes.byref(h), self.jpi, self.jpj, dx, dy, dep_const,
+                                   int(nit000), int(nitend), int(irecord), rdt, cbfr, visc))
+        self._h = h
+        g = default_grid(self.jpi, self.jpj, dx, dy, dep_const) if grid is None else grid
+        self.set_grid(g)
+        self.set_state(initial_state(g) if state is None else state)
+
+    @classmethod
+    def from_namelist(cls, nml, **kw):
+        return cls(nml["jpiglo"] - 2, nml["jpjglo"] - 2, dx=nml["dx"], dy=nml["dy"],
+                   dep_const=nml.get("dep", nml.get("dep_const")), nit000=nml["nit000"],
+                   nitend=nml["nitend"], irecord=nml.get("irecord", 1), rdt=nml["rdt"],
+                   cbfr=nml["cbfr"], visc=nml["visc"], **kw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nemo2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _arr(self, g, k, shape):
+        a = np.ascontiguousarray(g[k], dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError("%s: shape %s, expected %s" % (k, a.shape, shape))
+        return a
+
+    def set_grid(self, g):
+        s = shapes(self.jpi, self.jpj)
+        kinds = {"e1t": "T", "e2t": "T", "e12t": "T", "ht": "T", "e1u": "U", "e2u": "U",
+                 "e12u": "U", "gphiu": "U", "hu": "U", "e1v": "V", "e2v": "V", "e12v": "V",
+                 "gphiv": "V", "hv": "V"}
+        arrs = [self._arr(g, k, s[kinds[k]]) for k in self.GRID_KEYS]
+        pt = np.ascontiguousarray(g["pt"], dtype=np.int32)
+        if pt.shape != s["pt"]:
+            raise ValueError("pt: shape %s, expected %s" % (pt.shape, s["pt"]))
+        self._grid_keep = (pt, arrs)
+        _check(lib().nemo2d_set_grid(self._h, pt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     *[_dp(a) for a in arrs]))
+
+    def set_state(self, st):
+        s = shapes(self.jpi, self.jpj)
+        kinds = {"sshn": "T", "sshn_u": "U", "sshn_v": "V", "un": "U", "vn": "V"}
+        arrs = [self._arr(st, k, s[kinds[k]]) for k in self.STATE_KEYS]
+        _check(lib().nemo2d_set_state(self._h, *[_dp(a) for a in arrs]))
+
+    # the nine stages
+    def continuity(self): _check(lib().nemo2d_continuity(self._h))
+    def momentum_u(self): _check(lib().nemo2d_momentum_u(self._h))
+    def momentum_v(self): _check(lib().nemo2d_momentum_v(self._h))
+    def bc_ssh(self, rtime): _check(lib().nemo2d_bc_ssh(self._h, float(rtime)))
+    def bc_solid_u(self): _check(lib().nemo2d_bc_solid_u(self._h))
+    def bc_solid_v(self): _check(lib().nemo2d_bc_solid_v(self._h))
+    def bc_flather_u(self): _check(lib().nemo2d_bc_flather_u(self._h))
+    def bc_flather_v(self): _check(lib().nemo2d_bc_flather_v(self._h))
+    def next(self): _check(lib().nemo2d_next(self._h))
+
+    def step_stages(self, istp):
+        """One step as the nine stage calls (original/nemolite2d.f90:375-383)."""
+        rtime = float(istp) * self.rdt
+        self.continuity(); self.momentum_u(); self.momentum_v()
+        self.bc_ssh(rtime); self.bc_solid_u(); self.bc_solid_v()
+        self.bc_flather_u(); self.bc_flather_v(); self.next()
+
+    def step(self, istp):
+        """One fused step."""
+        _check(lib().nemo2d_step(self._h, int(istp)))
+
+    def run(self, nit000, nitend):
+        _check(lib().nemo2d_run(self._h, int(nit000), int(nitend)))
+
+    def record_checksums(self, capacity):
+        _check(lib().nemo2d_record_checksums(self._h, int(capacity)))
+
+    def checksum_log(self, max_steps):
+        out = np.zeros((max(int(max_steps), 1), 3))
+        n = ctypes.c_int(0)
+        _check(lib().nemo2d_checksum_log(self._h, _dp(out), int(max_steps), ctypes.byref(n)))
+        return out[:min(n.value, int(max_steps))]
+
+    def checksums(self):
+        out = np.zeros(3)
+        rc = lib().nemo2d_checksums(self._h, _dp(out))
+        if rc not in (OK, ERR_NONFINITE):
+            _check(rc)
+        return out
+
+    def retrieve(self):
+        s = shapes(self.jpi, self.jpj)
+        r = {"sshn": np.zeros(s["T"]), "un": np.zeros(s["U"]), "vn": np.zeros(s["V"]),
+             "ua": np.zeros(s["U"]), "va": np.zeros(s["V"]), "sshn_u": np.zeros(s["U"]),
+             "sshn_v": np.zeros(s["V"]), "ssha": np.zeros(s["T"])}
+        _check(lib().nemo2d_retrieve_results(self._h, _dp(r["sshn"]), _dp(r["un"]), _dp(r["vn"]),
+                                             _dp(r["ua"]), _dp(r["va"])))
+        _check(lib().nemo2d_retrieve_aux(self._h, _dp(r["sshn_u"]), _dp(r["sshn_v"]), _dp(r["ssha"])))
+        return r
+
+    def timer_start(self):
+        _check(lib().nemo2d_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double(0.0)
+        _check(lib().nemo2d_timer_stop(self._h, ctypes.byref(ms)))
+        return ms.value

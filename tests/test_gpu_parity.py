@@ -1,0 +1,256 @@
+"""GPU parity: the HIP library (through its C-ABI) against the C oracle and
+the reference's golden checksums.
+
+Bar: bitwise equality of every field for the same inputs (the library keeps
+the serial model's operation order with FMA contraction off, and evaluates
+sin/sqrt on the host with the same C library as the oracle); per-step
+checksums within 1e-12 relative (both sides use compensated summation, so in
+practice they differ by at most a few ulps)."""
+import os
+
+import numpy as np
+import pytest
+
+import nemo2d
+from golden_util import e16_8_close, load_golden, rel_close, trajectories
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_T = ("sshn",)
+FIELDS = ("sshn", "sshn_u", "sshn_v", "un", "vn")
+
+
+def copy_grid_from_oracle(o):
+    g = {k: np.array(o.arr(k)) for k in nemo2d.Model.GRID_KEYS}
+    g["pt"] = np.array(o.pt)
+    return g
+
+
+def state_from_oracle(o):
+    return {k: np.array(o.arr(k)) for k in FIELDS}
+
+
+def model_like(o, **kw):
+    p = o.params
+    return nemo2d.Model(o.jpi, o.jpj, dx=p["dx"], dy=p["dy"], dep_const=p["dep_const"],
+                        rdt=p["rdt"], cbfr=p["cbfr"], visc=p["visc"],
+                        grid=copy_grid_from_oracle(o), state=state_from_oracle(o), **kw)
+
+
+def assert_fields_equal(m, o, what=""):
+    r = m.retrieve()
+    for k in FIELDS:
+        a, b = r[k], o.arr(k)
+        if not np.array_equal(a.view(np.uint64), np.asarray(b).view(np.uint64)):
+            bad = np.argwhere(a.view(np.uint64) != np.asarray(b).view(np.uint64))
+            jj, ji = bad[0]
+            raise AssertionError("%s %s differs at [%d,%d]: gpu %r oracle %r (%d points)" % (
+                what, k, jj, ji, a[jj, ji], b[jj, ji], len(bad)))
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("jpi,jpj", [(1, 1), (2, 2), (1, 5), (5, 1), (3, 7), (16, 10), (65, 9),
+                                     (128, 128)])
+@pytest.mark.parametrize("path", ["fused", "stages"])
+def test_bitwise_vs_oracle_default_grid(jpi, jpj, path):
+    nsteps = 120 if jpi * jpj < 5000 else 60
+    o = Oracle(jpi, jpj)
+    with model_like(o) as m:
+        m.record_checksums(nsteps)
+        for istp in range(1, nsteps + 1):
+            (m.step if path == "fused" else m.step_stages)(istp)
+            o.step(istp)
+            if istp in (1, 2, 3, 10, nsteps):
+                assert_fields_equal(m, o, "%s step %d" % (path, istp))
+        log = m.checksum_log(nsteps)
+    o2 = Oracle(jpi, jpj)
+    ref, _ = o2.run(1, nsteps)
+    assert log.shape == ref.shape
+    assert np.all(rel_close(log, ref, 1e-12)), np.max(np.abs(log - ref) / np.maximum(np.abs(ref), 1e-300))
+
+
+def test_stage_by_stage_vs_oracle():
+    """Every one of the nine calls, compared after the call, on a state that
+    has been evolving (so every term is non-zero)."""
+    o = Oracle(40, 24, cbfr=0.00015)
+    o.run(1, 150)
+    with model_like(o) as m:
+        for istp in (151, 152):
+            rt = istp * o.params["rdt"]
+            m.continuity(); o.continuity()
+            assert np.array_equal(m.retrieve()["ssha"], o.arr("ssha"))
+            m.momentum_u(); o.momentum_u()
+            assert np.array_equal(m.retrieve()["ua"], o.arr("ua"))
+            m.momentum_v(); o.momentum_v()
+            assert np.array_equal(m.retrieve()["va"], o.arr("va"))
+            m.bc_ssh(rt); o.bc_ssh(rt)
+            assert np.array_equal(m.retrieve()["ssha"], o.arr("ssha"))
+            m.bc_solid_u(); o.bc_solid_u()
+            assert np.array_equal(m.retrieve()["ua"], o.arr("ua"))
+            m.bc_solid_v(); o.bc_solid_v()
+            assert np.array_equal(m.retrieve()["va"], o.arr("va"))
+            m.bc_flather_u(); o.bc_flather_u()
+            assert np.array_equal(m.retrieve()["ua"], o.arr("ua"))
+            m.bc_flather_v(); o.bc_flather_v()
+            assert np.array_equal(m.retrieve()["va"], o.arr("va"))
+            m.next(); o.next()
+            assert_fields_equal(m, o, "next %d" % istp)
+
+
+def test_fused_and_stages_mix():
+    """Fused and stage steps can be interleaved on one model."""
+    o = Oracle(30, 20)
+    with model_like(o) as m:
+        for istp in range(1, 41):
+            (m.step if istp % 3 else m.step_stages)(istp)
+            o.step(istp)
+        assert_fields_equal(m, o, "mixed")
+        r = m.retrieve()
+        assert np.array_equal(r["ua"], r["un"]) or np.array_equal(r["ua"], o.arr("ua"))
+
+
+@pytest.mark.parametrize("traj", trajectories(load_golden(), max_cells=130 * 130),
+                         ids=lambda t: "g%dx%d_n%d-%d_cb%s" % (t[0]["jpiglo"], t[0]["jpjglo"], t[0]["nit000"],
+                                                             t[0]["nitend"], t[0]["cbfr"]))
+def test_reference_golden_checksums(traj):
+    """GPU per-step checksum log against the reference binary's ua/va lines."""
+    nml, pts = traj
+    n = nml["nitend"] - nml["nit000"] + 1
+    with nemo2d.Model.from_namelist(nml) as m:
+        m.record_checksums(n)
+        m.run(nml["nit000"], nml["nitend"])
+        log = m.checksum_log(n)
+    for nitend, (ua, va) in pts.items():
+        k = nitend - nml["nit000"]
+        assert e16_8_close(log[k, 1], ua), (nitend, log[k, 1], ua)
+        assert e16_8_close(log[k, 2], va), (nitend, log[k, 2], va)
+
+
+# ---- general masks and metrics ------------------------------------------------
+def random_grid_oracle(jpi, jpj, seed, islands=True, open_edges="WEN", vary_metrics=True,
+                       chained=False):
+    rng = np.random.default_rng(seed)
+    o = Oracle(jpi, jpj, dx=900.0, dy=1100.0, dep_const=80.0, rdt=15.0, cbfr=0.002, visc=5.0,
+               default_grid=True)
+    pt = o.pt
+    if islands:
+        land = rng.random((jpj, jpi)) < 0.08
+        pt[1:jpj + 1, 1:jpi + 1][land] = 0
+    if "W" in open_edges:
+        pt[2:jpj // 2 + 1, 0] = -1
+    if "E" in open_edges:
+        pt[jpj // 3:jpj, jpi + 1] = -1
+    if "N" in open_edges:
+        pt[jpj + 1, 3:jpi - 1] = -1
+    if chained:
+        # water cell between two interior open cells: chained Flather points
+        jj = jpj // 2
+        pt[jj, 4:7] = [-1, 1, -1]
+    if vary_metrics:
+        for k in ("e1t", "e2t", "e1u", "e2u", "e1v", "e2v"):
+            a = o.arr(k)
+            a[:] = a * (1.0 + 0.2 * rng.random(a.shape))
+        for k in ("ht", "hu", "hv"):
+            a = o.arr(k)
+            a[:] = 40.0 + 60.0 * rng.random(a.shape)
+        for k in ("gphiu", "gphiv"):
+            a = o.arr(k)
+            a[:] = 30.0 + 20.0 * rng.random(a.shape)
+        o.grid_areas()
+    o.arr("sshn")[:] = 0.05 * rng.standard_normal(o.arr("sshn").shape)
+    o.initialise()
+    un, vn = o.arr("un"), o.arr("vn")
+    un[:] = 0.02 * rng.standard_normal(un.shape)
+    vn[:] = 0.02 * rng.standard_normal(vn.shape)
+    return o
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("path", ["fused", "stages"])
+def test_general_mask_and_metrics(seed, path):
+    o = random_grid_oracle(37, 29, seed)
+    with model_like(o) as m:
+        for istp in range(1, 61):
+            (m.step if path == "fused" else m.step_stages)(istp)
+            o.step(istp)
+        assert_fields_equal(m, o, "general seed %d" % seed)
+
+
+def test_chained_flather_falls_back_exactly():
+    o = random_grid_oracle(24, 20, 7, islands=False, vary_metrics=False, chained=True)
+    with model_like(o) as m:
+        for istp in range(1, 41):
+            m.step(istp)
+            o.step(istp)
+        assert_fields_equal(m, o, "chained")
+
+
+def test_mask_the_reference_cannot_run_is_rejected():
+    g = nemo2d.default_grid(8, 6)
+    g["pt"][3, 0] = 1      # water on the west ring: momentum would read un(-1, j)
+    with pytest.raises(nemo2d.Nemo2dError) as e:
+        nemo2d.Model(8, 6, grid=g)
+    assert e.value.code == nemo2d.ERR_MASK
+    g = nemo2d.default_grid(8, 6)
+    g["pt"][2, 3] = 2
+    with pytest.raises(nemo2d.Nemo2dError) as e:
+        nemo2d.Model(8, 6, grid=g)
+    assert e.value.code == nemo2d.ERR_MASK
+
+
+def test_nonfinite_checksum_is_flagged():
+    """A CFL-violating time step blows up; the checksum call reports it."""
+    with nemo2d.Model(32, 32, rdt=400.0, visc=0.0) as m:
+        m.run(1, 400)
+        L = nemo2d.lib()
+        out = np.zeros(3)
+        rc = L.nemo2d_checksums(m._h, nemo2d._dp(out))
+        assert rc == nemo2d.ERR_NONFINITE
+        assert not np.all(np.isfinite(out))
+
+
+def test_deterministic_repeat():
+    a = nemo2d.Model(300, 200, cbfr=0.00015)
+    b = nemo2d.Model(300, 200, cbfr=0.00015)
+    a.record_checksums(200)
+    b.record_checksums(200)
+    a.run(1, 200)
+    b.run(1, 200)
+    la, lb = a.checksum_log(200), b.checksum_log(200)
+    assert np.array_equal(la, lb)
+    ra, rb = a.retrieve(), b.retrieve()
+    for k in FIELDS:
+        assert np.array_equal(ra[k], rb[k])
+
+
+# ---- full size (4096^2) -------------------------------------------------------
+def test_4096_first_steps_bitwise_vs_oracle():
+    o = Oracle(4096, 4096)
+    with nemo2d.Model(4096, 4096) as m:
+        for istp in (1, 2, 3):
+            m.step(istp)
+            o.step(istp)
+        assert_fields_equal(m, o, "4096 step 3")
+
+
+def test_4096_reference_checksums_and_stage_path():
+    runs = [r for r in load_golden() if r["namelist"]["jpiglo"] == 4098]
+    if not runs:
+        pytest.skip("no 4096^2 reference checksums in tests/golden")
+    nml = max((r["namelist"] for r in runs), key=lambda n: n["nitend"])
+    n = nml["nitend"]
+    with nemo2d.Model.from_namelist(nml) as m:
+        m.record_checksums(n)
+        m.run(1, n)
+        log = m.checksum_log(n)
+    for r in runs:
+        k = r["namelist"]["nitend"] - 1
+        assert e16_8_close(log[k, 1], r["ua"]), (k + 1, log[k, 1], r["ua"])
+        assert e16_8_close(log[k, 2], r["va"]), (k + 1, log[k, 2], r["va"])
+    # the stage path gives the same checksums bit for bit over 50 steps
+    with nemo2d.Model.from_namelist(nml) as s:
+        s.record_checksums(50)
+        for istp in range(1, 51):
+            s.step_stages(istp)
+        assert np.array_equal(s.checksum_log(50), log[:50])
