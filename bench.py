@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096, help="interior grid size (jpi = jpj)")
     ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (torch) whose HBM counters "
+                         "calibrate the FETCH_SIZE/WRITE_SIZE readings of the step kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the "
@@ -129,7 +132,7 @@ def main():
         "checksum_last_step": [float(x) for x in log[-1]] if len(log) else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step_fused", "kernel_ms": ms_kernel,
+                     "kernel": "k_step_march", "kernel_ms": ms_kernel,
                      "bytes_per_launch": BYTES_PER_CELL * cells},
     }
     if args.cpu_steps > 0:
@@ -137,6 +140,13 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

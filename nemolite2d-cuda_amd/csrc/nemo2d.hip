@@ -204,6 +204,155 @@ k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
     block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
 }
 
+// ---- fused step, column-marching form (uniform statics) ----------------------
+// Each wave owns MOUT = 63 consecutive columns and marches up a segment of
+// NJ rows, keeping a 3x3 register window of the now-state per column; each
+// new row is one coalesced row of loads.  The new ssh of row j is computed
+// once: the column to the east comes from the next lane (lane 63 is the
+// east halo of the wave and only supplies that value), the row to the north
+// from the next iteration, so next's interpolations need no recomputation.
+constexpr int MW = 4;     // waves per block (side by side in i)
+constexpr int MOUT = 63;  // output columns per wave
+
+struct RegWin {
+    Scal s;
+    double un_[3][3], vn_[3][3], sshn_[3][3], su_[3][3], sv_[3][3];  // [DJ+1][DI+1]
+#define NEMO2D_REGF(F, A) \
+    template <int DI, int DJ> __device__ double F() const { return A[DJ + 1][DI + 1]; }
+    NEMO2D_REGF(un, un_) NEMO2D_REGF(vn, vn_) NEMO2D_REGF(sshn, sshn_) NEMO2D_REGF(sshn_u, su_)
+    NEMO2D_REGF(sshn_v, sv_)
+#undef NEMO2D_REGF
+#define NEMO2D_REGS(F) \
+    template <int DI, int DJ> __device__ double F() const { return s.F; }
+    NEMO2D_REGS(e1t) NEMO2D_REGS(e2t) NEMO2D_REGS(e1u) NEMO2D_REGS(e2u) NEMO2D_REGS(e1v)
+    NEMO2D_REGS(e2v) NEMO2D_REGS(e12t) NEMO2D_REGS(e12u) NEMO2D_REGS(e12v) NEMO2D_REGS(ht)
+    NEMO2D_REGS(hu) NEMO2D_REGS(hv) NEMO2D_REGS(fcu) NEMO2D_REGS(fcv)
+#undef NEMO2D_REGS
+
+    // rows move down one; row 2 is then reloaded
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            un_[0][c] = un_[1][c]; un_[1][c] = un_[2][c];
+            vn_[0][c] = vn_[1][c]; vn_[1][c] = vn_[2][c];
+            sshn_[0][c] = sshn_[1][c]; sshn_[1][c] = sshn_[2][c];
+            su_[0][c] = su_[1][c]; su_[1][c] = su_[2][c];
+            sv_[0][c] = sv_[1][c]; sv_[1][c] = sv_[2][c];
+        }
+    }
+    // load window row r from storage row j, columns i-1..i+1 (rowoff = offset
+    // of (i-1, j))
+    __device__ __forceinline__ void load(int r, const NState& n, long rowoff) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            un_[r][c] = n.un[rowoff + c];
+            vn_[r][c] = n.vn[rowoff + c];
+            sshn_[r][c] = n.sshn[rowoff + c];
+            su_[r][c] = n.sshn_u[rowoff + c];
+            sv_[r][c] = n.sshn_v[rowoff + c];
+        }
+    }
+};
+
+// Block b of a launch goes to XCD b % 8; hand each XCD a contiguous range of
+// tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int per = nb >> 3;  // nb is a multiple of 8
+    return (b & 7) * per + (b >> 3);
+}
+
+template <int NJ>
+__global__ void __launch_bounds__(64 * MW)
+k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
+             const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
+             double tide, double* __restrict__ partials, int nbx, int ntiles) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const long P = L.pitch;
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (t < ntiles) {
+        const int bx = t % nbx, by = t / nbx;
+        const int i = (bx * MW + wave) * MOUT + lane;
+        const int il = i < L.jpi + 1 ? i : L.jpi + 1;  // clamped column for loads
+        const bool own = lane < MOUT && i <= L.jpi;
+        const bool tcol = i >= 1 && i <= L.jpi;        // column holds T points
+        const int jb = by * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        StUniform st{sc_};
+        RegWin w;
+        w.s = sc_;
+        w.load(1, n, L.at(il - 1, jb - 1));
+        w.load(2, n, L.at(il - 1, jb));
+        double s_prev = 0.0;   // new ssh at (i, j-1)
+        double sv_prev = 0.0;  // old sshn_v at (i, j-1)
+        unsigned f_prev = 0;
+        for (int j = jb; j <= je; ++j) {
+            w.shift();
+            w.load(2, n, L.at(il - 1, j + 1));
+            const long o = L.at(il, j);
+            const unsigned f = fl[o];
+            double s_c = 0.0;
+            if (j >= 1 && tcol) s_c = (f & T_CLAMP) ? tide : continuity_w(w, p.rdt);
+            const double s_e = __shfl_down(s_c, 1, 64);
+            if (own) {
+                if (j >= 1) {
+                    if (i >= 1) {
+                        a.sshn[o] = s_c;
+                        dd_add(ct, fabs(s_c));
+                    }
+                    double u;
+                    const unsigned c = ucls(f);
+                    if (c == CLS_COMP) {
+                        u = momentum_u_w(w, f, p);
+                    } else if (c == CLS_FLATHER) {
+                        const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
+                        const unsigned fi = fl[oi];
+                        const double ui = ucls(fi) == CLS_COMP ? momentum_u_at(st, n, oi, P, fi, p) : 0.0;
+                        u = ui + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
+                    } else {
+                        u = 0.0;
+                    }
+                    a.un[o] = u;
+                    if (i >= 1) dd_add(cu, fabs(u));
+                    a.sshn_u[o] = next_su(st, sucls(f), o, s_c, s_e, w.su_[1][1]);
+                }
+                if (i >= 1) {
+                    double v;
+                    const unsigned c = vcls(f);
+                    if (c == CLS_COMP) {
+                        v = momentum_v_w(w, f, p);
+                    } else if (c == CLS_FLATHER) {
+                        const long oi = (f & V_FL_NORTH) ? o + P : o - P;
+                        const unsigned fi = fl[oi];
+                        const double vi = vcls(fi) == CLS_COMP ? momentum_v_at(st, n, oi, P, fi, p) : 0.0;
+                        v = vi + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
+                    } else {
+                        v = 0.0;
+                    }
+                    a.vn[o] = v;
+                    if (j >= 1) dd_add(cv, fabs(v));
+                    if (j > jb) a.sshn_v[o - P] = next_sv(st, svcls(f_prev), o - P, P, s_prev, s_c, sv_prev);
+                }
+            }
+            s_prev = s_c;
+            sv_prev = w.sv_[1][1];
+            f_prev = f;
+        }
+        // the last row's sshn_v needs the new ssh one row up
+        if (own && i >= 1) {
+            const long o = L.at(il, je);
+            const unsigned c = svcls(f_prev);
+            double s_n = 0.0;
+            if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI)) {
+                const long on = o + P;
+                s_n = (fl[on] & T_CLAMP) ? tide : continuity_at(st, n, on, P, p.rdt);
+            }
+            a.sshn_v[o] = next_sv(st, c, o, P, s_prev, s_n, sv_prev);
+        }
+    }
+    block_reduce3(ct, cu, cv, partials + 6 * (long)blockIdx.x);
+}
+
 // ---- checksum ----------------------------------------------------------------
 // sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of the given state.
 __global__ void __launch_bounds__(TBX * TBY)
@@ -350,6 +499,17 @@ __global__ void k_next(S st, const uint16_t* __restrict__ fl, const double* __re
 // ---------------------------------------------------------------------------
 dim3 grid2d(int ni, int nj) { return dim3((ni + TBX - 1) / TBX, (nj + TBY - 1) / TBY); }
 
+constexpr int MJ = 32;  // rows per marching segment
+
+// Tiles of the marching kernel: MW*MOUT columns (i = 0..jpi) by MJ rows
+// (j = 0..jpj); the launch is padded to a multiple of the 8 XCDs.
+void march_grid(const Layout& L, int& nbx, int& ntiles, int& nblocks) {
+    nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
+    const int nby = (L.jpj + 1 + MJ - 1) / MJ;
+    ntiles = nbx * nby;
+    nblocks = (ntiles + 7) / 8 * 8;
+}
+
 NState nstate(const nemo2d_model* m, int k) {
     NState n;
     n.sshn = m->S[k][F_SSHN];
@@ -493,7 +653,9 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         hipMemset(m->flags, 0, n * sizeof(uint16_t)) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
     dim3 g = grid2d(jpi + 1, jpj + 1);
-    m->npart_max = (int)(g.x * g.y);
+    int nbx_, ntiles_, nblocks_;
+    march_grid(L, nbx_, ntiles_, nblocks_);
+    m->npart_max = (int)(g.x * g.y) > nblocks_ ? (int)(g.x * g.y) : nblocks_;
     if ((rc = dalloc(&m->partials, 6 * (size_t)m->npart_max)) || (rc = dalloc(&m->cs_now, 3)))
         return bail(rc);
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
@@ -846,15 +1008,27 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     AOut a{m->S[1 - k][F_SSHN], m->S[1 - k][F_SSHN_U], m->S[1 - k][F_SSHN_V], m->S[1 - k][F_UN],
            m->S[1 - k][F_VN]};
     const double tide = tide_value((double)istp * m->prm.rdt);
-    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
-    WITH_STATICS(m, st, (k_step_fused<<<g, dim3(TBX, TBY), 0, m->stream>>>(
-                            st, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials)));
-    LAUNCHCHK("step_fused");
+    int nparts;
+    if (m->uniform) {
+        int nbx, ntiles, nblocks;
+        march_grid(L, nbx, ntiles, nblocks);
+        k_step_march<MJ><<<nblocks, 64 * MW, 0, m->stream>>>(m->scal, n, a, m->flags, m->flu, m->flv,
+                                                             L, m->prm, tide, m->partials, nbx, ntiles);
+        LAUNCHCHK("step_march");
+        nparts = nblocks;
+    } else {
+        dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+        StArrays st = starrays(m);
+        k_step_fused<<<g, dim3(TBX, TBY), 0, m->stream>>>(st, n, a, m->flags, m->flu, m->flv, L,
+                                                          m->prm, tide, m->partials);
+        LAUNCHCHK("step_fused");
+        nparts = (int)(g.x * g.y);
+    }
     m->cur = 1 - k;
     m->a_current = false;
     const int slot = istp - m->nit000;
     if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
-        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y),
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts,
                                                             m->cs_log + 3 * (long)slot);
         LAUNCHCHK("checksum_finish");
         if (slot + 1 > m->cs_n) m->cs_n = slot + 1;

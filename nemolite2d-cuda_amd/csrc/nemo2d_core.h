@@ -110,110 +110,150 @@ struct NState {
     const double *__restrict__ un, *__restrict__ vn;
 };
 
-// continuity at T point o, original/nemolite2d.f90:399-403.
+// Stencil windows.  The point functions read every operand through a
+// window, w.F<DI,DJ>() = field F at (i+DI, j+DJ), so the same arithmetic runs
+// on values loaded from memory (MemWin) or held in registers by a marching
+// kernel (a register window with the same interface).
 template <class S>
-__device__ __forceinline__ double continuity_at(const S& st, const NState& n, long o, long P,
-                                                double rdt) {
-    const double r1 = (n.sshn_u[o] + st.hu(o)) * n.un[o];
-    const double r2 = (n.sshn_u[o - 1] + st.hu(o - 1)) * n.un[o - 1];
-    const double r3 = (n.sshn_v[o] + st.hv(o)) * n.vn[o];
-    const double r4 = (n.sshn_v[o - P] + st.hv(o - P)) * n.vn[o - P];
-    return n.sshn[o] + (r2 - r1 + r4 - r3) * rdt / st.e12t(o);
+struct MemWin {
+    const S& st;
+    const NState& n;
+    long o, P;
+    template <int DI, int DJ> __device__ long at() const { return o + DI + (long)DJ * P; }
+#define NEMO2D_MEMF(F) \
+    template <int DI, int DJ> __device__ double F() const { return n.F[at<DI, DJ>()]; }
+    NEMO2D_MEMF(un) NEMO2D_MEMF(vn) NEMO2D_MEMF(sshn) NEMO2D_MEMF(sshn_u) NEMO2D_MEMF(sshn_v)
+#undef NEMO2D_MEMF
+#define NEMO2D_MEMS(F) \
+    template <int DI, int DJ> __device__ double F() const { return st.F(at<DI, DJ>()); }
+    NEMO2D_MEMS(e1t) NEMO2D_MEMS(e2t) NEMO2D_MEMS(e1u) NEMO2D_MEMS(e2u) NEMO2D_MEMS(e1v)
+    NEMO2D_MEMS(e2v) NEMO2D_MEMS(e12t) NEMO2D_MEMS(e12u) NEMO2D_MEMS(e12v) NEMO2D_MEMS(ht)
+    NEMO2D_MEMS(hu) NEMO2D_MEMS(hv) NEMO2D_MEMS(fcu) NEMO2D_MEMS(fcv)
+#undef NEMO2D_MEMS
+};
+
+#define W_(F, DI, DJ) w.template F<DI, DJ>()
+
+// continuity at T point (i, j), original/nemolite2d.f90:399-403.
+template <class W>
+__device__ __forceinline__ double continuity_w(const W& w, double rdt) {
+    const double r1 = (W_(sshn_u, 0, 0) + W_(hu, 0, 0)) * W_(un, 0, 0);
+    const double r2 = (W_(sshn_u, -1, 0) + W_(hu, -1, 0)) * W_(un, -1, 0);
+    const double r3 = (W_(sshn_v, 0, 0) + W_(hv, 0, 0)) * W_(vn, 0, 0);
+    const double r4 = (W_(sshn_v, 0, -1) + W_(hv, 0, -1)) * W_(vn, 0, -1);
+    return W_(sshn, 0, 0) + (r2 - r1 + r4 - r3) * rdt / W_(e12t, 0, 0);
 }
 
 // Upwind weight SIGN(0.5, x) with Fortran's signed-zero rule.
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
 
-// momentum_u at U point o (flags f), original/nemolite2d.f90:439-514.
-template <class S>
-__device__ __forceinline__ double momentum_u_at(const S& st, const NState& n, long o, long P,
-                                                unsigned f, const Params& p) {
-    const double un_c = n.un[o], un_e = n.un[o + 1], un_w = n.un[o - 1];
-    const double u_e = 0.5 * (un_c + un_e) * st.e2t(o + 1);
-    const double depe = st.ht(o + 1) + n.sshn[o + 1];
-    const double u_w = 0.5 * (un_c + un_w) * st.e2t(o);
-    const double depw = st.ht(o) + n.sshn[o];
-    const double v_sc = 0.5 * (n.vn[o - P] + n.vn[o - P + 1]);
-    const double v_s = 0.5 * v_sc * (st.e1v(o - P) + st.e1v(o - P + 1));
-    const double deps = 0.5 * (st.hv(o - P) + n.sshn_v[o - P] + st.hv(o - P + 1) + n.sshn_v[o - P + 1]);
-    const double v_nc = 0.5 * (n.vn[o] + n.vn[o + 1]);
-    const double v_n = 0.5 * v_nc * (st.e1v(o) + st.e1v(o + 1));
-    const double depn = 0.5 * (st.hv(o) + n.sshn_v[o] + st.hv(o + 1) + n.sshn_v[o + 1]);
+// momentum_u at U point (i, j) with flags f, original/nemolite2d.f90:439-514.
+template <class W>
+__device__ __forceinline__ double momentum_u_w(const W& w, unsigned f, const Params& p) {
+    const double un_c = W_(un, 0, 0), un_e = W_(un, 1, 0), un_w = W_(un, -1, 0);
+    const double u_e = 0.5 * (un_c + un_e) * W_(e2t, 1, 0);
+    const double depe = W_(ht, 1, 0) + W_(sshn, 1, 0);
+    const double u_w = 0.5 * (un_c + un_w) * W_(e2t, 0, 0);
+    const double depw = W_(ht, 0, 0) + W_(sshn, 0, 0);
+    const double v_sc = 0.5 * (W_(vn, 0, -1) + W_(vn, 1, -1));
+    const double v_s = 0.5 * v_sc * (W_(e1v, 0, -1) + W_(e1v, 1, -1));
+    const double deps = 0.5 * (W_(hv, 0, -1) + W_(sshn_v, 0, -1) + W_(hv, 1, -1) + W_(sshn_v, 1, -1));
+    const double v_nc = 0.5 * (W_(vn, 0, 0) + W_(vn, 1, 0));
+    const double v_n = 0.5 * v_nc * (W_(e1v, 0, 0) + W_(e1v, 1, 0));
+    const double depn = 0.5 * (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
 
     const double uu_w = (0.5 - sgn_half(u_w)) * un_c + (0.5 + sgn_half(u_w)) * un_w;
     const double uu_e = (0.5 + sgn_half(u_e)) * un_c + (0.5 - sgn_half(u_e)) * un_e;
     const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
-    const double un_s = n.un[o - P], un_n = n.un[o + P];
+    const double un_s = W_(un, 0, -1), un_n = W_(un, 0, 1);
     double uu_s = (0.5 - sgn_half(v_s)) * un_c;
     if (!ws) uu_s = uu_s + (0.5 + sgn_half(v_s)) * un_s;
     double uu_n = (0.5 + sgn_half(v_n)) * un_c;
     if (!wn) uu_n = uu_n + (0.5 - sgn_half(v_n)) * un_n;
     const double adv = uu_w * u_w * depw - uu_e * u_e * depe + uu_s * v_s * deps - uu_n * v_n * depn;
 
-    const double dudx_e = (un_e - un_c) / st.e1t(o + 1) * (st.ht(o + 1) + n.sshn[o + 1]);
-    const double dudx_w = (un_c - un_w) / st.e1t(o) * (st.ht(o) + n.sshn[o]);
-    const double hsu = st.hu(o) + n.sshn_u[o];
+    const double dudx_e = (un_e - un_c) / W_(e1t, 1, 0) * (W_(ht, 1, 0) + W_(sshn, 1, 0));
+    const double dudx_w = (un_c - un_w) / W_(e1t, 0, 0) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const double hsu = W_(hu, 0, 0) + W_(sshn_u, 0, 0);
     double dudy_s = 0.0, dudy_n = 0.0;
     if (!ws)
-        dudy_s = (un_c - un_s) / (st.e2u(o) + st.e2u(o - P)) *
-                 (st.hu(o) + n.sshn_u[o] + st.hu(o - P) + n.sshn_u[o - P]);
+        dudy_s = (un_c - un_s) / (W_(e2u, 0, 0) + W_(e2u, 0, -1)) *
+                 (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, -1) + W_(sshn_u, 0, -1));
     if (!wn)
-        dudy_n = (un_n - un_c) / (st.e2u(o) + st.e2u(o + P)) *
-                 (st.hu(o) + n.sshn_u[o] + st.hu(o + P) + n.sshn_u[o + P]);
-    double vis = (dudx_e - dudx_w) * st.e2u(o) + (dudy_n - dudy_s) * st.e1u(o) * 0.5;
+        dudy_n = (un_n - un_c) / (W_(e2u, 0, 0) + W_(e2u, 0, 1)) *
+                 (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
+    double vis = (dudx_e - dudx_w) * W_(e2u, 0, 0) + (dudy_n - dudy_s) * W_(e1u, 0, 0) * 0.5;
     vis = p.visc * vis;
 
-    const double cor = 0.5 * (st.fcu(o) * (v_sc + v_nc)) * st.e12u(o) * hsu;
-    const double hpg = -GRAV * hsu * st.e2u(o) * (n.sshn[o + 1] - n.sshn[o]);
+    const double cor = 0.5 * (W_(fcu, 0, 0) * (v_sc + v_nc)) * W_(e12u, 0, 0) * hsu;
+    const double hpg = -GRAV * hsu * W_(e2u, 0, 0) * (W_(sshn, 1, 0) - W_(sshn, 0, 0));
     // ssha_u is never written by the serial model (:302, :315): it is 0.
-    return (un_c * hsu + p.rdt * (adv + vis + cor + hpg) / st.e12u(o)) / (st.hu(o) + 0.0) / p.fric;
+    return (un_c * hsu + p.rdt * (adv + vis + cor + hpg) / W_(e12u, 0, 0)) / (W_(hu, 0, 0) + 0.0) /
+           p.fric;
 }
 
-// momentum_v at V point o (flags f), original/nemolite2d.f90:527-606.
-template <class S>
-__device__ __forceinline__ double momentum_v_at(const S& st, const NState& n, long o, long P,
-                                                unsigned f, const Params& p) {
-    const double vn_c = n.vn[o], vn_n = n.vn[o + P], vn_s = n.vn[o - P];
-    const double v_n = 0.5 * (vn_c + vn_n) * st.e1t(o + P);
-    const double depn = st.ht(o + P) + n.sshn[o + P];
-    const double v_s = 0.5 * (vn_c + vn_s) * st.e1t(o);
-    const double deps = st.ht(o) + n.sshn[o];
-    const double u_wc = 0.5 * (n.un[o - 1] + n.un[o - 1 + P]);
-    const double u_w = 0.5 * u_wc * (st.e2u(o - 1) + st.e2u(o - 1 + P));
-    const double depw = 0.50 * (st.hu(o - 1) + n.sshn_u[o - 1] + st.hu(o - 1 + P) + n.sshn_u[o - 1 + P]);
-    const double u_ec = 0.5 * (n.un[o] + n.un[o + P]);
-    const double u_e = 0.5 * u_ec * (st.e2u(o) + st.e2u(o + P));
-    const double depe = 0.50 * (st.hu(o) + n.sshn_u[o] + st.hu(o + P) + n.sshn_u[o + P]);
+// momentum_v at V point (i, j) with flags f, original/nemolite2d.f90:527-606.
+template <class W>
+__device__ __forceinline__ double momentum_v_w(const W& w, unsigned f, const Params& p) {
+    const double vn_c = W_(vn, 0, 0), vn_n = W_(vn, 0, 1), vn_s = W_(vn, 0, -1);
+    const double v_n = 0.5 * (vn_c + vn_n) * W_(e1t, 0, 1);
+    const double depn = W_(ht, 0, 1) + W_(sshn, 0, 1);
+    const double v_s = 0.5 * (vn_c + vn_s) * W_(e1t, 0, 0);
+    const double deps = W_(ht, 0, 0) + W_(sshn, 0, 0);
+    const double u_wc = 0.5 * (W_(un, -1, 0) + W_(un, -1, 1));
+    const double u_w = 0.5 * u_wc * (W_(e2u, -1, 0) + W_(e2u, -1, 1));
+    const double depw = 0.50 * (W_(hu, -1, 0) + W_(sshn_u, -1, 0) + W_(hu, -1, 1) + W_(sshn_u, -1, 1));
+    const double u_ec = 0.5 * (W_(un, 0, 0) + W_(un, 0, 1));
+    const double u_e = 0.5 * u_ec * (W_(e2u, 0, 0) + W_(e2u, 0, 1));
+    const double depe = 0.50 * (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
 
     const double vv_s = (0.5 - sgn_half(v_s)) * vn_c + (0.5 + sgn_half(v_s)) * vn_s;
     const double vv_n = (0.5 + sgn_half(v_n)) * vn_c + (0.5 - sgn_half(v_n)) * vn_n;
     const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
-    const double vn_w = n.vn[o - 1], vn_e = n.vn[o + 1];
+    const double vn_w = W_(vn, -1, 0), vn_e = W_(vn, 1, 0);
     double vv_w = (0.5 - sgn_half(u_w)) * vn_c;
     if (!ww) vv_w = vv_w + (0.5 + sgn_half(u_w)) * vn_w;
     double vv_e = (0.5 + sgn_half(u_e)) * vn_c;
     if (!we) vv_e = vv_e + (0.5 - sgn_half(u_e)) * vn_e;
     const double adv = vv_w * u_w * depw - vv_e * u_e * depe + vv_s * v_s * deps - vv_n * v_n * depn;
 
-    const double dvdy_n = (vn_n - vn_c) / st.e2t(o + P) * (st.ht(o + P) + n.sshn[o + P]);
-    const double dvdy_s = (vn_c - vn_s) / st.e2t(o) * (st.ht(o) + n.sshn[o]);
-    const double hsv = st.hv(o) + n.sshn_v[o];
+    const double dvdy_n = (vn_n - vn_c) / W_(e2t, 0, 1) * (W_(ht, 0, 1) + W_(sshn, 0, 1));
+    const double dvdy_s = (vn_c - vn_s) / W_(e2t, 0, 0) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const double hsv = W_(hv, 0, 0) + W_(sshn_v, 0, 0);
     double dvdx_w = 0.0, dvdx_e = 0.0;
     if (!ww)
-        dvdx_w = (vn_c - vn_w) / (st.e1v(o) + st.e1v(o - 1)) *
-                 (st.hv(o) + n.sshn_v[o] + st.hv(o - 1) + n.sshn_v[o - 1]);
+        dvdx_w = (vn_c - vn_w) / (W_(e1v, 0, 0) + W_(e1v, -1, 0)) *
+                 (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, -1, 0) + W_(sshn_v, -1, 0));
     if (!we)
-        dvdx_e = (vn_e - vn_c) / (st.e1v(o) + st.e1v(o + 1)) *
-                 (st.hv(o) + n.sshn_v[o] + st.hv(o + 1) + n.sshn_v[o + 1]);
-    double vis = (dvdy_n - dvdy_s) * st.e1v(o) + (dvdx_e - dvdx_w) * st.e2v(o) * 0.5;
+        dvdx_e = (vn_e - vn_c) / (W_(e1v, 0, 0) + W_(e1v, 1, 0)) *
+                 (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
+    double vis = (dvdy_n - dvdy_s) * W_(e1v, 0, 0) + (dvdx_e - dvdx_w) * W_(e2v, 0, 0) * 0.5;
     vis = p.visc * vis;
 
-    const double cor = -0.5 * (st.fcv(o) * (u_ec + u_wc)) * st.e12v(o) * hsv;
-    const double hpg = -GRAV * hsv * st.e1v(o) * (n.sshn[o + P] - n.sshn[o]);
-    return (vn_c * hsv + p.rdt * (adv + vis + cor + hpg) / st.e12v(o)) / (st.hv(o) + 0.0) / p.fric;
+    const double cor = -0.5 * (W_(fcv, 0, 0) * (u_ec + u_wc)) * W_(e12v, 0, 0) * hsv;
+    const double hpg = -GRAV * hsv * W_(e1v, 0, 0) * (W_(sshn, 0, 1) - W_(sshn, 0, 0));
+    return (vn_c * hsv + p.rdt * (adv + vis + cor + hpg) / W_(e12v, 0, 0)) / (W_(hv, 0, 0) + 0.0) /
+           p.fric;
 }
 
-// next: sshn_u from the new ssh at T points o (i) and o+1 (i+1),
+// Memory-window wrappers at offset o.
+template <class S>
+__device__ __forceinline__ double continuity_at(const S& st, const NState& n, long o, long P,
+                                                double rdt) {
+    return continuity_w(MemWin<S>{st, n, o, P}, rdt);
+}
+template <class S>
+__device__ __forceinline__ double momentum_u_at(const S& st, const NState& n, long o, long P,
+                                                unsigned f, const Params& p) {
+    return momentum_u_w(MemWin<S>{st, n, o, P}, f, p);
+}
+template <class S>
+__device__ __forceinline__ double momentum_v_at(const S& st, const NState& n, long o, long P,
+                                                unsigned f, const Params& p) {
+    return momentum_v_w(MemWin<S>{st, n, o, P}, f, p);
+}
+
+// next: sshn_u from the new ssh at T points (i, j) and (i+1, j),
 // original/nemolite2d.f90:738-746.  `keep` is the old value (skip class).
 template <class S>
 __device__ __forceinline__ double next_su(const S& st, unsigned cls, long o, double s_lo,
@@ -229,7 +269,7 @@ __device__ __forceinline__ double next_su(const S& st, unsigned cls, long o, dou
     }
 }
 
-// next: sshn_v from the new ssh at T points o (j) and o+P (j+1), :754-762.
+// next: sshn_v from the new ssh at T points (i, j) and (i, j+1), :754-762.
 template <class S>
 __device__ __forceinline__ double next_sv(const S& st, unsigned cls, long o, long P, double s_lo,
                                           double s_hi, double keep) {
