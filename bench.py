@@ -65,8 +65,15 @@ def main():
     ap.add_argument("--n", type=int, default=4096, help="interior grid size (jpi = jpj)")
     ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 1 GiB device copy (torch) whose HBM counters "
-                         "calibrate the FETCH_SIZE/WRITE_SIZE readings of the step kernel")
+                    help="also reset the state 3 more times: each reset issues device-to-"
+                         "device copies of the five padded state arrays (plain copy "
+                         "kernels of known size) whose HBM counters calibrate the "
+                         "FETCH_SIZE/WRITE_SIZE readings of the step kernel")
+    ap.add_argument("--variant", default="auto",
+                    help="fused-step kernel (nemo2d.VARIANTS); results do not depend on it")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every fused-step variant (200 steps each) and report them "
+                         "under 'variants'")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the "
@@ -78,6 +85,7 @@ def main():
     n, K, W = args.n, args.steps, args.warmup
     m = nemo2d.Model(n, n, dx=1000.0, dy=1000.0, dep_const=100.0, nit000=1, nitend=K,
                      irecord=K + 1, rdt=20.0, cbfr=0.0015, visc=0.1)
+    m.set_variant(args.variant)
     rest = nemo2d.initial_state(nemo2d.default_grid(n, n))
 
     # warm-up, then reset to rest
@@ -104,6 +112,24 @@ def main():
     m.timer_start()
     m.run(1, K)
     ms_kernel = m.timer_stop() / K
+
+    variants = None
+    if args.sweep:
+        variants = {}
+        for name in nemo2d.VARIANTS:
+            if name == "auto":
+                continue
+            m.set_variant(name)
+            m.set_state(rest)
+            m.run(1, 5)
+            m.set_state(rest)
+            m.timer_start()
+            m.run(1, 200)
+            variants[name] = m.timer_stop() / 200
+        m.set_variant(args.variant)
+    if args.copy_calibration:
+        for _ in range(3):
+            m.set_state(rest)
     m.close()
 
     cells = float(n) * n
@@ -135,18 +161,13 @@ def main():
                      "kernel": "k_step_march", "kernel_ms": ms_kernel,
                      "bytes_per_launch": BYTES_PER_CELL * cells},
     }
+    if variants:
+        out["variants_ms_per_step"] = variants
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-    if args.copy_calibration:
-        import torch
-        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

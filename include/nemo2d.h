@@ -104,6 +104,15 @@ int nemo2d_next(nemo2d_model *m);                    /* :704-769; replaces cuda_
  * stage calls above. */
 int nemo2d_step(nemo2d_model *m, int istp);
 
+/* Choose the kernel of nemo2d_step (results are identical for every
+ * choice): 0 default, 1 one thread per cell, 2/3/4 column-marching with
+ * 16/32/64-row segments, 5 marching with a 4-waves-per-SIMD register
+ * budget, 6 marching and 7 per-cell with Flather points recomputed inline
+ * instead of in the epilogue.  Marching needs uniform metrics and depths;
+ * other grids use 1 (or 7).  No counterpart in the reference: a tuning
+ * hook. */
+int nemo2d_set_variant(nemo2d_model *m, int variant);
+
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the
  * per-step checksums on the device when checksum recording is on. */
 int nemo2d_run(nemo2d_model *m, int nit000, int nitend);
@@ -138,6 +147,25 @@ int nemo2d_retrieve_aux(nemo2d_model *m, double *sshn_u, double *sshn_v, double 
  * stream).  timer_stop synchronises. */
 int nemo2d_timer_start(nemo2d_model *m);
 int nemo2d_timer_stop(nemo2d_model *m, double *ms);
+
+/* --- single-precision form (no counterpart in the reference) ------------
+ * Same model, FP32 state (sshn_u, sshn_v, un, vn) and FP32 stencil
+ * arithmetic; the new ssh is accumulated as sshn + increment in FP64 with
+ * sshn kept in double when compensated != 0, in FP32 otherwise.  Needs
+ * uniform metrics and latitude (depths may vary per point).  Only the fused
+ * step (nemo2d_step / nemo2d_run) is available; the stage calls return
+ * NEMO2D_ERR_STATE.  Results are not bitwise those of the serial model:
+ * their error against the FP64 path is measured by the tests and bench. */
+int nemo2d_create_mixed(nemo2d_model **model, int jpi, int jpj, double dx, double dy,
+                        double dep_const, int nit000, int nitend, int irecord,
+                        double rdt, double cbfr, double visc, int compensated);
+
+/* FP64 diagnostics of the current now-state over the T points (no
+ * counterpart in the reference): out[0] = volume sum(e12t*ssh),
+ * out[1] = energy sum(e12t*(0.5*(ht+ssh)*(ubar^2+vbar^2) + 0.5*g*ssh^2))
+ * with ubar, vbar the means of each cell's two faces, out[2] = wet area
+ * sum(e12t).  Compensated, order-fixed reduction.  Synchronises. */
+int nemo2d_diagnostics(nemo2d_model *m, double out[3]);
 
 /* Free the model's device memory and stream.  Replaces cuda_finalise_
  * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */

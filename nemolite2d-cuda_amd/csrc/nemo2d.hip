@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nemo2d.h"
@@ -100,6 +101,14 @@ struct nemo2d_model {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool grid_set = false, state_set = false;
+
+    // single-precision form: float state (sshn in double when compensated),
+    // float depths, scalar float metrics
+    bool mixed = false, compensated = true;
+    int variant = 0;              // fused-step kernel variant (0 = default)
+    void* X[2][NF] = {};          // float (or double sshn) now-state ping-pong
+    float *hf_t = nullptr, *hf_u = nullptr, *hf_v = nullptr;
+    double* diag = nullptr;       // 2 doubles: volume, energy
 };
 
 // ---------------------------------------------------------------------------
@@ -144,7 +153,9 @@ __device__ __forceinline__ double ssha_at(const S& st, const NState& n, const ui
 // T point (i, j), the U point (i, j) (east face), the V point (i, j) (north
 // face) and sshn_u/sshn_v there.  The new ssh east and north of the cell,
 // needed by the interpolation of next, is recomputed from the now-state.
-template <class S>
+// Flather points are left to k_epilogue unless FLIN (then the inner
+// neighbour's momentum is recomputed here).
+template <class S, bool FLIN>
 __global__ void __launch_bounds__(TBX * TBY)
 k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L,
@@ -164,18 +175,18 @@ k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
         }
         if (j >= 1) {
             // U point (i, j)
-            double u;
             const unsigned c = ucls(f);
-            if (c == CLS_COMP) {
-                u = momentum_u_at(st, n, o, P, f, p);
-            } else if (c == CLS_FLATHER) {
-                const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
-                u = u_pre_flather(st, n, fl, oi, P, p) + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
-            } else {
-                u = 0.0;
+            if (c != CLS_FLATHER || FLIN) {
+                double u = 0.0;
+                if (c == CLS_COMP) {
+                    u = momentum_u_at(st, n, o, P, f, p);
+                } else if (c == CLS_FLATHER) {
+                    const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
+                    u = u_pre_flather(st, n, fl, oi, P, p) + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
+                }
+                a.un[o] = u;
+                if (i >= 1) dd_add(cu, fabs(u));
             }
-            a.un[o] = u;
-            if (i >= 1) dd_add(cu, fabs(u));
             const unsigned su = sucls(f);
             double s_hi = 0.0;
             if (su == NX_AVG || su == NX_HI) s_hi = ssha_at(st, n, fl, o + 1, P, p.rdt, tide);
@@ -183,18 +194,18 @@ k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
         }
         if (i >= 1) {
             // V point (i, j)
-            double v;
             const unsigned c = vcls(f);
-            if (c == CLS_COMP) {
-                v = momentum_v_at(st, n, o, P, f, p);
-            } else if (c == CLS_FLATHER) {
-                const long oi = (f & V_FL_NORTH) ? o + P : o - P;
-                v = v_pre_flather(st, n, fl, oi, P, p) + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
-            } else {
-                v = 0.0;
+            if (c != CLS_FLATHER || FLIN) {
+                double v = 0.0;
+                if (c == CLS_COMP) {
+                    v = momentum_v_at(st, n, o, P, f, p);
+                } else if (c == CLS_FLATHER) {
+                    const long oi = (f & V_FL_NORTH) ? o + P : o - P;
+                    v = v_pre_flather(st, n, fl, oi, P, p) + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
+                }
+                a.vn[o] = v;
+                if (j >= 1) dd_add(cv, fabs(v));
             }
-            a.vn[o] = v;
-            if (j >= 1) dd_add(cv, fabs(v));
             const unsigned sv = svcls(f);
             double s_hi = 0.0;
             if (sv == NX_AVG || sv == NX_HI) s_hi = ssha_at(st, n, fl, o + P, P, p.rdt, tide);
@@ -215,6 +226,7 @@ constexpr int MW = 4;     // waves per block (side by side in i)
 constexpr int MOUT = 63;  // output columns per wave
 
 struct RegWin {
+    using real = double;
     Scal s;
     double un_[3][3], vn_[3][3], sshn_[3][3], su_[3][3], sv_[3][3];  // [DJ+1][DI+1]
 #define NEMO2D_REGF(F, A) \
@@ -261,8 +273,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int NJ>
-__global__ void __launch_bounds__(64 * MW)
+template <int NJ, bool FLIN, int WPE>
+__global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
              double tide, double* __restrict__ partials, int nbx, int ntiles) {
@@ -300,37 +312,39 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                         a.sshn[o] = s_c;
                         dd_add(ct, fabs(s_c));
                     }
-                    double u;
                     const unsigned c = ucls(f);
-                    if (c == CLS_COMP) {
-                        u = momentum_u_w(w, f, p);
-                    } else if (c == CLS_FLATHER) {
-                        const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
-                        const unsigned fi = fl[oi];
-                        const double ui = ucls(fi) == CLS_COMP ? momentum_u_at(st, n, oi, P, fi, p) : 0.0;
-                        u = ui + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
-                    } else {
-                        u = 0.0;
+                    if (c != CLS_FLATHER || FLIN) {
+                        double u = 0.0;
+                        if (c == CLS_COMP) {
+                            u = momentum_u_w(w, f, p);
+                        } else if (c == CLS_FLATHER) {
+                            const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
+                            const unsigned fi = fl[oi];
+                            const double ui =
+                                ucls(fi) == CLS_COMP ? momentum_u_at(st, n, oi, P, fi, p) : 0.0;
+                            u = ui + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
+                        }
+                        a.un[o] = u;
+                        if (i >= 1) dd_add(cu, fabs(u));
                     }
-                    a.un[o] = u;
-                    if (i >= 1) dd_add(cu, fabs(u));
                     a.sshn_u[o] = next_su(st, sucls(f), o, s_c, s_e, w.su_[1][1]);
                 }
                 if (i >= 1) {
-                    double v;
                     const unsigned c = vcls(f);
-                    if (c == CLS_COMP) {
-                        v = momentum_v_w(w, f, p);
-                    } else if (c == CLS_FLATHER) {
-                        const long oi = (f & V_FL_NORTH) ? o + P : o - P;
-                        const unsigned fi = fl[oi];
-                        const double vi = vcls(fi) == CLS_COMP ? momentum_v_at(st, n, oi, P, fi, p) : 0.0;
-                        v = vi + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
-                    } else {
-                        v = 0.0;
+                    if (c != CLS_FLATHER || FLIN) {
+                        double v = 0.0;
+                        if (c == CLS_COMP) {
+                            v = momentum_v_w(w, f, p);
+                        } else if (c == CLS_FLATHER) {
+                            const long oi = (f & V_FL_NORTH) ? o + P : o - P;
+                            const unsigned fi = fl[oi];
+                            const double vi =
+                                vcls(fi) == CLS_COMP ? momentum_v_at(st, n, oi, P, fi, p) : 0.0;
+                            v = vi + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
+                        }
+                        a.vn[o] = v;
+                        if (j >= 1) dd_add(cv, fabs(v));
                     }
-                    a.vn[o] = v;
-                    if (j >= 1) dd_add(cv, fabs(v));
                     if (j > jb) a.sshn_v[o - P] = next_sv(st, svcls(f_prev), o - P, P, s_prev, s_c, sv_prev);
                 }
             }
@@ -351,6 +365,246 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         }
     }
     block_reduce3(ct, cu, cv, partials + 6 * (long)blockIdx.x);
+}
+
+// ---- single-precision form ------------------------------------------------------
+// FP32 state and stencil arithmetic; the new ssh is accumulated as
+// sshn + incr in FP64 when `COMP` (sshn kept in double), in FP32 otherwise
+// (the ablation).  Uniform metrics and Coriolis parameter, per-point depths.
+struct ScalF {
+    float e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, fcu, fcv;
+};
+
+template <bool COMP>
+struct MixedState {
+    using ssh_t = typename std::conditional<COMP, double, float>::type;
+    const ssh_t* __restrict__ sshn;
+    const float *__restrict__ sshn_u, *__restrict__ sshn_v, *__restrict__ un, *__restrict__ vn;
+};
+template <bool COMP>
+struct MixedOut {
+    typename MixedState<COMP>::ssh_t* sshn;
+    float *sshn_u, *sshn_v, *un, *vn;
+};
+
+template <bool COMP>
+struct MemWinF {
+    using real = float;
+    const MixedState<COMP>& n;
+    const ScalF& s;
+    const float *ht_, *hu_, *hv_;
+    long o, P;
+    template <int DI, int DJ> __device__ long at() const { return o + DI + (long)DJ * P; }
+    template <int DI, int DJ> __device__ float un() const { return n.un[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float vn() const { return n.vn[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float sshn() const { return (float)n.sshn[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float sshn_u() const { return n.sshn_u[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float sshn_v() const { return n.sshn_v[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float ht() const { return ht_[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float hu() const { return hu_[at<DI, DJ>()]; }
+    template <int DI, int DJ> __device__ float hv() const { return hv_[at<DI, DJ>()]; }
+#define NEMO2D_SF(F) \
+    template <int DI, int DJ> __device__ float F() const { return s.F; }
+    NEMO2D_SF(e1t) NEMO2D_SF(e2t) NEMO2D_SF(e1u) NEMO2D_SF(e2u) NEMO2D_SF(e1v) NEMO2D_SF(e2v)
+    NEMO2D_SF(e12t) NEMO2D_SF(e12u) NEMO2D_SF(e12v) NEMO2D_SF(fcu) NEMO2D_SF(fcv)
+#undef NEMO2D_SF
+};
+
+template <bool COMP>
+__device__ __forceinline__ typename MixedState<COMP>::ssh_t mixed_ssha(
+    const MemWinF<COMP>& w, unsigned f, float rdt, double tide) {
+    using ssh_t = typename MixedState<COMP>::ssh_t;
+    if (f & T_CLAMP) return (ssh_t)tide;
+    return w.n.sshn[w.o] + (ssh_t)continuity_incr_w(w, rdt);
+}
+
+template <bool COMP>
+__global__ void __launch_bounds__(TBX * TBY)
+k_step_mixed(ScalF s, MixedState<COMP> n, MixedOut<COMP> a, const float* __restrict__ ht,
+             const float* __restrict__ hu, const float* __restrict__ hv,
+             const uint16_t* __restrict__ fl, const double* __restrict__ flu,
+             const double* __restrict__ flv, Layout L, ParamsT<float> p, double tide,
+             double* __restrict__ partials) {
+    const int i = blockIdx.x * TBX + threadIdx.x;
+    const int j = blockIdx.y * TBY + threadIdx.y;
+    const long P = L.pitch;
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (i <= L.jpi && j <= L.jpj) {
+        const long o = L.at(i, j);
+        const unsigned f = fl[o];
+        MemWinF<COMP> w{n, s, ht, hu, hv, o, P};
+        double s_c = 0.0;
+        if (i >= 1 && j >= 1) {
+            const auto sa = mixed_ssha(w, f, p.rdt, tide);
+            a.sshn[o] = sa;
+            s_c = (double)sa;
+            dd_add(ct, fabs(s_c));
+        }
+        if (j >= 1) {
+            float u;
+            const unsigned c = ucls(f);
+            if (c == CLS_COMP) {
+                u = momentum_u_w(w, f, p);
+            } else if (c == CLS_FLATHER) {
+                const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
+                const unsigned fi = fl[oi];
+                MemWinF<COMP> wi{n, s, ht, hu, hv, oi, P};
+                const float ui = ucls(fi) == CLS_COMP ? momentum_u_w(wi, fi, p) : 0.0f;
+                u = ui + (float)flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
+            } else {
+                u = 0.0f;
+            }
+            a.un[o] = u;
+            if (i >= 1) dd_add(cu, fabs((double)u));
+            const unsigned su = sucls(f);
+            float r = n.sshn_u[o];
+            if (su != NX_SKIP) {
+                float s_hi = 0.0f;
+                if (su == NX_AVG || su == NX_HI) {
+                    const long oe = o + 1;
+                    MemWinF<COMP> we_{n, s, ht, hu, hv, oe, P};
+                    s_hi = (float)mixed_ssha(we_, fl[oe], p.rdt, tide);
+                }
+                const float s_lo = (float)s_c;
+                r = su == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12u
+                                 : (su == NX_HI ? s_hi : s_lo);
+            }
+            a.sshn_u[o] = r;
+        }
+        if (i >= 1) {
+            float v;
+            const unsigned c = vcls(f);
+            if (c == CLS_COMP) {
+                v = momentum_v_w(w, f, p);
+            } else if (c == CLS_FLATHER) {
+                const long oi = (f & V_FL_NORTH) ? o + P : o - P;
+                const unsigned fi = fl[oi];
+                MemWinF<COMP> wi{n, s, ht, hu, hv, oi, P};
+                const float vi = vcls(fi) == CLS_COMP ? momentum_v_w(wi, fi, p) : 0.0f;
+                v = vi + (float)flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
+            } else {
+                v = 0.0f;
+            }
+            a.vn[o] = v;
+            if (j >= 1) dd_add(cv, fabs((double)v));
+            const unsigned sv = svcls(f);
+            float r = n.sshn_v[o];
+            if (sv != NX_SKIP) {
+                float s_hi = 0.0f;
+                if (sv == NX_AVG || sv == NX_HI) {
+                    const long on = o + P;
+                    MemWinF<COMP> wn_{n, s, ht, hu, hv, on, P};
+                    s_hi = (float)mixed_ssha(wn_, fl[on], p.rdt, tide);
+                }
+                const float s_lo = (float)s_c;
+                r = sv == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12v
+                                 : (sv == NX_HI ? s_hi : s_lo);
+            }
+            a.sshn_v[o] = r;
+        }
+    }
+    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of a state of element
+// types TS (ssh) and TV (the rest).
+template <class TS, class TV>
+__global__ void __launch_bounds__(TBX * TBY)
+k_checksum_partials_t(const TS* __restrict__ sshn, const TV* __restrict__ un,
+                      const TV* __restrict__ vn, Layout L, double* __restrict__ partials) {
+    const int i = blockIdx.x * TBX + threadIdx.x;
+    const int j = blockIdx.y * TBY + threadIdx.y;
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (in_rect(i, j, 1, L.jpi, 1, L.jpj)) {
+        const long o = L.at(i, j);
+        dd_add(ct, fabs((double)sshn[o]));
+        dd_add(cu, fabs((double)un[o]));
+        dd_add(cv, fabs((double)vn[o]));
+    }
+    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// Volume and energy diagnostics in FP64 over the T points:
+//   volume = sum e12t*ssh
+//   energy = sum e12t*(0.5*(ht+ssh)*(ubar^2+vbar^2) + 0.5*g*ssh^2)
+// with ubar, vbar the averages of the two faces of the cell.  The third
+// accumulator carries sum e12t (the wet area).
+template <class TS, class TV, class TH>
+__global__ void __launch_bounds__(TBX * TBY)
+k_diag_partials(const TS* __restrict__ sshn, const TV* __restrict__ un, const TV* __restrict__ vn,
+                const TH* __restrict__ ht, double ht0, const double* __restrict__ e12t, double e12t0,
+                Layout L, double* __restrict__ partials) {
+    const int i = blockIdx.x * TBX + threadIdx.x;
+    const int j = blockIdx.y * TBY + threadIdx.y;
+    DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
+    if (in_rect(i, j, 1, L.jpi, 1, L.jpj)) {
+        const long o = L.at(i, j);
+        const double ar = e12t ? e12t[o] : e12t0;
+        const double h = ht ? (double)ht[o] : ht0;
+        const double z = (double)sshn[o];
+        const double ub = 0.5 * ((double)un[o - 1] + (double)un[o]);
+        const double vb = 0.5 * ((double)vn[o - L.pitch] + (double)vn[o]);
+        dd_add(a0, ar * z);
+        dd_add(a1, ar * (0.5 * (h + z) * (ub * ub + vb * vb) + 0.5 * GRAV * z * z));
+        dd_add(a2, ar);
+    }
+    block_reduce3(a0, a1, a2, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// ---- step epilogue -------------------------------------------------------------
+// One block.  Phase 1: the Flather points of the fused step (bc_flather_u/v,
+// original/nemolite2d.f90:669-695) from the new after-velocities of their
+// inner neighbours and the old sshn_u/sshn_v, one thread per line in loop
+// order; their |values| inside (1:jpi,1:jpj) go to partial slot nparts.
+// Phase 2 (when out is given): the fixed-order reduction of the nparts+1
+// partials into the step's checksums.
+__device__ __forceinline__ void finish_partials(const double* __restrict__ partials, int nparts,
+                                                double* __restrict__ out) {
+    DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        const double* q = partials + 6 * (long)k;
+        a0 = dd_merge(a0, DD{q[0], q[1]});
+        a1 = dd_merge(a1, DD{q[2], q[3]});
+        a2 = dd_merge(a2, DD{q[4], q[5]});
+    }
+    __shared__ double r[6];
+    block_reduce3(a0, a1, a2, r);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = r[0] + r[1];
+        out[1] = r[2] + r[3];
+        out[2] = r[4] + r[5];
+    }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
+           const int* __restrict__ vptr, const long* __restrict__ vpts, int nvl,
+           const uint16_t* __restrict__ fl, const double* __restrict__ flu,
+           const double* __restrict__ flv, const double* __restrict__ sshn_u,
+           const double* __restrict__ sshn_v, double* __restrict__ un, double* __restrict__ vn,
+           Layout L, double* __restrict__ partials, int nparts, double* __restrict__ out) {
+    const long P = L.pitch;
+    DD cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    for (int line = threadIdx.x; line < nul; line += RED_THREADS)
+        for (int k = uptr[line]; k < uptr[line + 1]; ++k) {
+            const long o = upts[k];
+            const long oi = (fl[o] & U_FL_EAST) ? o + 1 : o - 1;
+            const double u = un[oi] + flu[o] * (sshn_u[o] - sshn_u[oi]);
+            un[o] = u;
+            if ((o - L.base) - (long)(line + 1) * P >= 1) dd_add(cu, fabs(u));  // i >= 1
+        }
+    for (int line = threadIdx.x; line < nvl; line += RED_THREADS)
+        for (int k = vptr[line]; k < vptr[line + 1]; ++k) {
+            const long o = vpts[k];
+            const long oi = (fl[o] & V_FL_NORTH) ? o + P : o - P;
+            const double v = vn[oi] + flv[o] * (sshn_v[o] - sshn_v[oi]);
+            vn[o] = v;
+            if ((o - L.base) / P >= 1) dd_add(cv, fabs(v));  // j >= 1
+        }
+    block_reduce3(DD{0.0, 0.0}, cu, cv, partials + 6 * (long)nparts);
+    __syncthreads();
+    if (out) finish_partials(partials, nparts + 1, out);
 }
 
 // ---- checksum ----------------------------------------------------------------
@@ -499,16 +753,7 @@ __global__ void k_next(S st, const uint16_t* __restrict__ fl, const double* __re
 // ---------------------------------------------------------------------------
 dim3 grid2d(int ni, int nj) { return dim3((ni + TBX - 1) / TBX, (nj + TBY - 1) / TBY); }
 
-constexpr int MJ = 32;  // rows per marching segment
 
-// Tiles of the marching kernel: MW*MOUT columns (i = 0..jpi) by MJ rows
-// (j = 0..jpj); the launch is padded to a multiple of the 8 XCDs.
-void march_grid(const Layout& L, int& nbx, int& ntiles, int& nblocks) {
-    nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
-    const int nby = (L.jpj + 1 + MJ - 1) / MJ;
-    ntiles = nbx * nby;
-    nblocks = (ntiles + 7) / 8 * 8;
-}
 
 NState nstate(const nemo2d_model* m, int k) {
     NState n;
@@ -584,11 +829,48 @@ bool all_equal(const double* a, size_t n) {
     return true;
 }
 
+// Host Fortran array (lb1:ub1, lb2:ub2) of doubles -> device padded array
+// of element type E (converted on the host).
+template <class E>
+int upload_as(const nemo2d_model* m, E* dst, const double* src, int lb1, int ub1, int lb2, int ub2) {
+    const long n1 = ub1 - lb1 + 1;
+    std::vector<E> rows((size_t)n1 * (ub2 - lb2 + 1));
+    for (size_t k = 0; k < rows.size(); ++k) rows[k] = (E)src[k];
+    HIPCHK(hipMemcpy2DAsync(dst + m->L.at(lb1, lb2), m->L.pitch * sizeof(E), rows.data(),
+                            n1 * sizeof(E), n1 * sizeof(E), ub2 - lb2 + 1, hipMemcpyHostToDevice,
+                            m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));  // rows is a local buffer
+    return NEMO2D_OK;
+}
+
+template <class E>
+int download_as(const nemo2d_model* m, double* dst, const E* src, int lb1, int ub1, int lb2, int ub2) {
+    const long n1 = ub1 - lb1 + 1;
+    std::vector<E> rows((size_t)n1 * (ub2 - lb2 + 1));
+    HIPCHK(hipMemcpy2DAsync(rows.data(), n1 * sizeof(E), src + m->L.at(lb1, lb2),
+                            m->L.pitch * sizeof(E), n1 * sizeof(E), ub2 - lb2 + 1,
+                            hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (size_t k = 0; k < rows.size(); ++k) dst[k] = (double)rows[k];
+    return NEMO2D_OK;
+}
+
 int checksum_into(nemo2d_model* m, int k, double* out) {
     const Layout& L = m->L;
     dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
-    k_checksum_partials<<<g, dim3(TBX, TBY), 0, m->stream>>>(
-        m->S[k][F_SSHN], m->S[k][F_UN], m->S[k][F_VN], L, m->partials);
+    if (m->mixed) {
+        const float* un = (const float*)m->X[k][F_UN];
+        const float* vn = (const float*)m->X[k][F_VN];
+        if (m->compensated)
+            k_checksum_partials_t<double, float><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                (const double*)m->X[k][F_SSHN], un, vn, L, m->partials);
+        else
+            k_checksum_partials_t<float, float><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                (const float*)m->X[k][F_SSHN], un, vn, L, m->partials);
+    } else {
+        k_checksum_partials<<<g, dim3(TBX, TBY), 0, m->stream>>>(
+            m->S[k][F_SSHN], m->S[k][F_UN], m->S[k][F_VN], L, m->partials);
+    }
     LAUNCHCHK("checksum_partials");
     k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y), out);
     LAUNCHCHK("checksum_finish");
@@ -653,10 +935,9 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         hipMemset(m->flags, 0, n * sizeof(uint16_t)) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
     dim3 g = grid2d(jpi + 1, jpj + 1);
-    int nbx_, ntiles_, nblocks_;
-    march_grid(L, nbx_, ntiles_, nblocks_);
-    m->npart_max = (int)(g.x * g.y) > nblocks_ ? (int)(g.x * g.y) : nblocks_;
-    if ((rc = dalloc(&m->partials, 6 * (size_t)m->npart_max)) || (rc = dalloc(&m->cs_now, 3)))
+    // the thread-per-cell launch has the most blocks of all step kernels
+    m->npart_max = (int)(g.x * g.y);
+    if ((rc = dalloc(&m->partials, 6 * ((size_t)m->npart_max + 1))) || (rc = dalloc(&m->cs_now, 3)))
         return bail(rc);
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "pinned allocation failed"));
@@ -849,6 +1130,25 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     if (!vpts.empty())
         HIPCHK(hipMemcpyAsync(m->flv_pts, vpts.data(), vpts.size() * sizeof(long), hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
+    if (m->mixed) {
+        // the single-precision form takes per-point depths and uniform
+        // metrics and Coriolis parameter
+        for (int s2 = 0; s2 < NS; ++s2)
+            if (s2 != S_HT && s2 != S_HU && s2 != S_HV && !all_equal(sd[s2].a, (size_t)sd[s2].n))
+                return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: the single-precision form needs "
+                                            "uniform metrics and latitude (depth may vary)");
+        float** hf[3] = {&m->hf_t, &m->hf_u, &m->hf_v};
+        const int sidx[3] = {S_HT, S_HU, S_HV};
+        for (int q = 0; q < 3; ++q) {
+            if (!*hf[q]) {
+                HIPCHK(hipMalloc((void**)hf[q], n * sizeof(float)));
+                HIPCHK(hipMemset(*hf[q], 0, n * sizeof(float)));
+            }
+            const auto& d = sd[sidx[q]];
+            int rc = upload_as<float>(m, *hf[q], d.a, d.lb1, d.ub1, d.lb2, d.ub2);
+            if (rc) return rc;
+        }
+    }
     m->grid_set = true;
     return NEMO2D_OK;
 }
@@ -861,6 +1161,30 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     const int jpi = m->L.jpi, jpj = m->L.jpj;
     const size_t bytes = (size_t)m->L.size() * sizeof(double);
     m->cur = 0;
+    if (m->mixed) {
+        const size_t nb[NF] = {m->compensated ? bytes : bytes / 2, bytes / 2, bytes / 2, bytes / 2,
+                               bytes / 2};
+        for (int k = 0; k < 2; ++k)
+            for (int f = 0; f < NF; ++f) HIPCHK(hipMemsetAsync(m->X[k][f], 0, nb[f], m->stream));
+        int rc;
+        if (m->compensated)
+            rc = upload_as<double>(m, (double*)m->X[0][F_SSHN], sshn, 1, jpi, 1, jpj);
+        else
+            rc = upload_as<float>(m, (float*)m->X[0][F_SSHN], sshn, 1, jpi, 1, jpj);
+        if (rc || (rc = upload_as<float>(m, (float*)m->X[0][F_SSHN_U], sshn_u, 0, jpi, 1, jpj)) ||
+            (rc = upload_as<float>(m, (float*)m->X[0][F_SSHN_V], sshn_v, 1, jpi, 0, jpj)) ||
+            (rc = upload_as<float>(m, (float*)m->X[0][F_UN], un, 0, jpi, 1, jpj)) ||
+            (rc = upload_as<float>(m, (float*)m->X[0][F_VN], vn, 1, jpi, 0, jpj)))
+            return rc;
+        for (int f = 0; f < NF; ++f)
+            HIPCHK(hipMemcpyAsync(m->X[1][f], m->X[0][f], nb[f], hipMemcpyDeviceToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        m->a_current = false;
+        m->state_set = true;
+        m->cs_next = 0;
+        m->cs_n = 0;
+        return NEMO2D_OK;
+    }
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f) HIPCHK(hipMemsetAsync(m->S[k][f], 0, bytes, m->stream));
     HIPCHK(hipMemsetAsync(m->ssha, 0, bytes, m->stream));
@@ -892,8 +1216,16 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
             return fail(NEMO2D_ERR_STATE, std::string(who) + ": grid and state must be set first"); \
     } while (0)
 
+#define READY_STAGE(m, who)                                                                \
+    do {                                                                                   \
+        READY(m, who);                                                                     \
+        if ((m)->mixed)                                                                    \
+            return fail(NEMO2D_ERR_STATE, std::string(who) +                               \
+                                              ": the single-precision form has the fused step only"); \
+    } while (0)
+
 int nemo2d_continuity(nemo2d_model* m) {
-    READY(m, "nemo2d_continuity");
+    READY_STAGE(m, "nemo2d_continuity");
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_continuity<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -903,7 +1235,7 @@ int nemo2d_continuity(nemo2d_model* m) {
 }
 
 int nemo2d_momentum_u(nemo2d_model* m) {
-    READY(m, "nemo2d_momentum_u");
+    READY_STAGE(m, "nemo2d_momentum_u");
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_momentum_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -914,7 +1246,7 @@ int nemo2d_momentum_u(nemo2d_model* m) {
 }
 
 int nemo2d_momentum_v(nemo2d_model* m) {
-    READY(m, "nemo2d_momentum_v");
+    READY_STAGE(m, "nemo2d_momentum_v");
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_momentum_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -925,7 +1257,7 @@ int nemo2d_momentum_v(nemo2d_model* m) {
 }
 
 int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
-    READY(m, "nemo2d_bc_ssh");
+    READY_STAGE(m, "nemo2d_bc_ssh");
     const Layout& L = m->L;
     k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ssha, L,
                                                                             tide_value(rtime));
@@ -934,7 +1266,7 @@ int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
 }
 
 int nemo2d_bc_solid_u(nemo2d_model* m) {
-    READY(m, "nemo2d_bc_solid_u");
+    READY_STAGE(m, "nemo2d_bc_solid_u");
     const Layout& L = m->L;
     k_bc_solid_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ua, L);
     LAUNCHCHK("bc_solid_u");
@@ -942,7 +1274,7 @@ int nemo2d_bc_solid_u(nemo2d_model* m) {
 }
 
 int nemo2d_bc_solid_v(nemo2d_model* m) {
-    READY(m, "nemo2d_bc_solid_v");
+    READY_STAGE(m, "nemo2d_bc_solid_v");
     const Layout& L = m->L;
     k_bc_solid_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->va, L);
     LAUNCHCHK("bc_solid_v");
@@ -950,7 +1282,7 @@ int nemo2d_bc_solid_v(nemo2d_model* m) {
 }
 
 int nemo2d_bc_flather_u(nemo2d_model* m) {
-    READY(m, "nemo2d_bc_flather_u");
+    READY_STAGE(m, "nemo2d_bc_flather_u");
     if (m->nflu == 0) return NEMO2D_OK;
     const int lines = m->L.jpj;
     k_bc_flather_u<<<(lines + 255) / 256, 256, 0, m->stream>>>(
@@ -960,7 +1292,7 @@ int nemo2d_bc_flather_u(nemo2d_model* m) {
 }
 
 int nemo2d_bc_flather_v(nemo2d_model* m) {
-    READY(m, "nemo2d_bc_flather_v");
+    READY_STAGE(m, "nemo2d_bc_flather_v");
     if (m->nflv == 0) return NEMO2D_OK;
     const int lines = m->L.jpi;
     k_bc_flather_v<<<(lines + 255) / 256, 256, 0, m->stream>>>(
@@ -970,7 +1302,7 @@ int nemo2d_bc_flather_v(nemo2d_model* m) {
 }
 
 int nemo2d_next(nemo2d_model* m) {
-    READY(m, "nemo2d_next");
+    READY_STAGE(m, "nemo2d_next");
     const Layout& L = m->L;
     double** s = m->S[m->cur];
     WITH_STATICS(m, st, (k_next<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -987,6 +1319,50 @@ int nemo2d_next(nemo2d_model* m) {
     return NEMO2D_OK;
 }
 
+static int step_mixed(nemo2d_model* m, int istp) {
+    if (!m->fused_ok)
+        return fail(NEMO2D_ERR_MASK, "nemo2d_step: chained Flather points are not supported by "
+                                     "the single-precision form");
+    const Layout& L = m->L;
+    const int k = m->cur;
+    const Scal& d = m->scal;
+    ScalF sf{(float)d.e1t, (float)d.e2t, (float)d.e1u, (float)d.e2u, (float)d.e1v, (float)d.e2v,
+             (float)d.e12t, (float)d.e12u, (float)d.e12v, (float)d.fcu, (float)d.fcv};
+    ParamsT<float> pf{(float)m->prm.rdt, (float)m->prm.cbfr, (float)m->prm.visc, (float)m->prm.fric};
+    const double tide = tide_value((double)istp * m->prm.rdt);
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    void** x = m->X[k];
+    void** y = m->X[1 - k];
+    if (m->compensated) {
+        MixedState<true> n{(const double*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
+                           (const float*)x[F_UN], (const float*)x[F_VN]};
+        MixedOut<true> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
+                         (float*)y[F_VN]};
+        k_step_mixed<true><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
+                                                                m->flags, m->flu, m->flv, L, pf, tide,
+                                                                m->partials);
+    } else {
+        MixedState<false> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
+                            (const float*)x[F_UN], (const float*)x[F_VN]};
+        MixedOut<false> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
+                          (float*)y[F_VN]};
+        k_step_mixed<false><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
+                                                                 m->flags, m->flu, m->flv, L, pf, tide,
+                                                                 m->partials);
+    }
+    LAUNCHCHK("step_mixed");
+    m->cur = 1 - k;
+    const int slot = istp - m->nit000;
+    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y),
+                                                            m->cs_log + 3 * (long)slot);
+        LAUNCHCHK("checksum_finish");
+        if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
+        m->cs_next = slot + 1;
+    }
+    return NEMO2D_OK;
+}
+
 static int run_stages(nemo2d_model* m, int istp) {
     int rc;
     if ((rc = nemo2d_continuity(m)) || (rc = nemo2d_momentum_u(m)) || (rc = nemo2d_momentum_v(m)) ||
@@ -999,8 +1375,52 @@ static int run_stages(nemo2d_model* m, int istp) {
     return nemo2d_next(m);
 }
 
+}  // extern "C"
+
+// Fused-step kernel variants (nemo2d_set_variant); 0 picks the default.
+enum {
+    VAR_AUTO = 0,
+    VAR_CELL = 1,       // one thread per cell, Flather in the epilogue
+    VAR_MARCH16 = 2,    // column-marching, 16-row segments
+    VAR_MARCH32 = 3,    //                  32-row segments
+    VAR_MARCH64 = 4,    //                  64-row segments
+    VAR_MARCH32_W4 = 5, // 32-row segments, register budget for 4 waves/SIMD
+    VAR_MARCH32_FLIN = 6,  // 32-row segments, Flather points recomputed inline
+    VAR_CELL_FLIN = 7,  // one thread per cell, Flather inline
+    VAR_LAST = 7
+};
+
+template <int NJ, bool FLIN, int WPE>
+static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    const Layout& L = m->L;
+    const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    k_step_march<NJ, FLIN, WPE><<<nblocks, 64 * MW, 0, m->stream>>>(
+        m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
+    LAUNCHCHK("step_march");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+template <bool FLIN>
+static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    const Layout& L = m->L;
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    WITH_STATICS(m, st, (k_step_fused<decltype(st), FLIN><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                            st, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials)));
+    LAUNCHCHK("step_fused");
+    nparts = (int)(g.x * g.y);
+    return NEMO2D_OK;
+}
+
+extern "C" {
+
 int nemo2d_step(nemo2d_model* m, int istp) {
     READY(m, "nemo2d_step");
+    if (m->mixed) return step_mixed(m, istp);
     if (!m->fused_ok) return run_stages(m, istp);
     const Layout& L = m->L;
     const int k = m->cur;
@@ -1008,32 +1428,48 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     AOut a{m->S[1 - k][F_SSHN], m->S[1 - k][F_SSHN_U], m->S[1 - k][F_SSHN_V], m->S[1 - k][F_UN],
            m->S[1 - k][F_VN]};
     const double tide = tide_value((double)istp * m->prm.rdt);
-    int nparts;
-    if (m->uniform) {
-        int nbx, ntiles, nblocks;
-        march_grid(L, nbx, ntiles, nblocks);
-        k_step_march<MJ><<<nblocks, 64 * MW, 0, m->stream>>>(m->scal, n, a, m->flags, m->flu, m->flv,
-                                                             L, m->prm, tide, m->partials, nbx, ntiles);
-        LAUNCHCHK("step_march");
-        nparts = nblocks;
-    } else {
-        dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
-        StArrays st = starrays(m);
-        k_step_fused<<<g, dim3(TBX, TBY), 0, m->stream>>>(st, n, a, m->flags, m->flu, m->flv, L,
-                                                          m->prm, tide, m->partials);
-        LAUNCHCHK("step_fused");
-        nparts = (int)(g.x * g.y);
+    int var = m->variant;
+    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH32 : VAR_CELL;
+    if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
+    int nparts = 0, rc = NEMO2D_OK;
+    bool flin = false;
+    switch (var) {
+        case VAR_CELL: rc = launch_cell<false>(m, n, a, tide, nparts); break;
+        case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
+        case VAR_MARCH16: rc = launch_march<16, false, 0>(m, n, a, tide, nparts); break;
+        case VAR_MARCH64: rc = launch_march<64, false, 0>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_W4: rc = launch_march<32, false, 4>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_FLIN: rc = launch_march<32, true, 0>(m, n, a, tide, nparts); flin = true; break;
+        default: rc = launch_march<32, false, 0>(m, n, a, tide, nparts); break;
     }
+    if (rc) return rc;
     m->cur = 1 - k;
     m->a_current = false;
     const int slot = istp - m->nit000;
-    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
-        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts,
-                                                            m->cs_log + 3 * (long)slot);
+    double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
+    const bool has_fl = (m->nflu + m->nflv) > 0;
+    if (!flin && (has_fl || out)) {
+        // Flather points of the new state, then (if recording) the checksums
+        k_epilogue<<<1, RED_THREADS, 0, m->stream>>>(
+            m->flu_ptr, m->flu_pts, m->nflu ? L.jpj : 0, m->flv_ptr, m->flv_pts, m->nflv ? L.jpi : 0,
+            m->flags, m->flu, m->flv, n.sshn_u, n.sshn_v, a.un, a.vn, L, m->partials, nparts, out);
+        LAUNCHCHK("epilogue");
+    } else if (out) {
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts, out);
         LAUNCHCHK("checksum_finish");
+    }
+    if (out) {
         if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
         m->cs_next = slot + 1;
     }
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_variant(nemo2d_model* m, int variant) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_variant: null model");
+    if (variant < 0 || variant > VAR_LAST)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_set_variant: unknown variant " + std::to_string(variant));
+    m->variant = variant;
     return NEMO2D_OK;
 }
 
@@ -1089,6 +1525,20 @@ int nemo2d_retrieve_results(nemo2d_model* m, double* sshn, double* un, double* v
                             double* va) {
     READY(m, "nemo2d_retrieve_results");
     const int jpi = m->L.jpi, jpj = m->L.jpj;
+    if (m->mixed) {
+        void** x = m->X[m->cur];
+        int rc = NEMO2D_OK;
+        if (sshn) {
+            rc = m->compensated ? download_as<double>(m, sshn, (const double*)x[F_SSHN], 1, jpi, 1, jpj)
+                                : download_as<float>(m, sshn, (const float*)x[F_SSHN], 1, jpi, 1, jpj);
+            if (rc) return rc;
+        }
+        if (un && (rc = download_as<float>(m, un, (const float*)x[F_UN], 0, jpi, 1, jpj))) return rc;
+        if (vn && (rc = download_as<float>(m, vn, (const float*)x[F_VN], 1, jpi, 0, jpj))) return rc;
+        if (ua && (rc = download_as<float>(m, ua, (const float*)x[F_UN], 0, jpi, 1, jpj))) return rc;
+        if (va && (rc = download_as<float>(m, va, (const float*)x[F_VN], 1, jpi, 0, jpj))) return rc;
+        return NEMO2D_OK;
+    }
     double** s = m->S[m->cur];
     // after a fused step, ua/va equal un/vn (next copied them)
     const double* dua = m->a_current ? m->ua : s[F_UN];
@@ -1106,6 +1556,18 @@ int nemo2d_retrieve_results(nemo2d_model* m, double* sshn, double* un, double* v
 int nemo2d_retrieve_aux(nemo2d_model* m, double* sshn_u, double* sshn_v, double* ssha) {
     READY(m, "nemo2d_retrieve_aux");
     const int jpi = m->L.jpi, jpj = m->L.jpj;
+    if (m->mixed) {
+        void** x = m->X[m->cur];
+        int rc = NEMO2D_OK;
+        if (sshn_u && (rc = download_as<float>(m, sshn_u, (const float*)x[F_SSHN_U], 0, jpi, 1, jpj))) return rc;
+        if (sshn_v && (rc = download_as<float>(m, sshn_v, (const float*)x[F_SSHN_V], 1, jpi, 0, jpj))) return rc;
+        if (ssha) {
+            rc = m->compensated ? download_as<double>(m, ssha, (const double*)x[F_SSHN], 1, jpi, 1, jpj)
+                                : download_as<float>(m, ssha, (const float*)x[F_SSHN], 1, jpi, 1, jpj);
+            if (rc) return rc;
+        }
+        return NEMO2D_OK;
+    }
     double** s = m->S[m->cur];
     int rc = NEMO2D_OK;
     if (sshn_u && (rc = download(m, sshn_u, s[F_SSHN_U], 0, jpi, 1, jpj))) return rc;
@@ -1132,6 +1594,62 @@ int nemo2d_timer_stop(nemo2d_model* m, double* ms) {
     return NEMO2D_OK;
 }
 
+int nemo2d_create_mixed(nemo2d_model** out, int jpi, int jpj, double dx, double dy,
+                        double dep_const, int nit000, int nitend, int irecord, double rdt,
+                        double cbfr, double visc, int compensated) {
+    int rc = nemo2d_create(out, jpi, jpj, dx, dy, dep_const, nit000, nitend, irecord, rdt, cbfr, visc);
+    if (rc) return rc;
+    nemo2d_model* m = *out;
+    m->mixed = true;
+    m->compensated = compensated != 0;
+    const size_t n = (size_t)m->L.size();
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < NF; ++f) {
+            const size_t es = (f == F_SSHN && m->compensated) ? sizeof(double) : sizeof(float);
+            if (hipMalloc(&m->X[k][f], n * es) != hipSuccess || hipMemset(m->X[k][f], 0, n * es) != hipSuccess) {
+                nemo2d_destroy(m);
+                *out = nullptr;
+                return fail(NEMO2D_ERR_HIP, "nemo2d_create_mixed: allocation failed");
+            }
+        }
+    return NEMO2D_OK;
+}
+
+int nemo2d_diagnostics(nemo2d_model* m, double out[3]) {
+    READY(m, "nemo2d_diagnostics");
+    if (!out) return fail(NEMO2D_ERR_ARG, "nemo2d_diagnostics: null output");
+    const Layout& L = m->L;
+    if (!m->diag) {
+        int rc = dalloc(&m->diag, 3);
+        if (rc) return rc;
+    }
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    const double* e12t = m->uniform ? nullptr : m->st[S_E12T];
+    if (m->mixed) {
+        void** x = m->X[m->cur];
+        if (m->compensated)
+            k_diag_partials<double, float, float><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                (const double*)x[F_SSHN], (const float*)x[F_UN], (const float*)x[F_VN], m->hf_t, 0.0,
+                nullptr, m->scal.e12t, L, m->partials);
+        else
+            k_diag_partials<float, float, float><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                (const float*)x[F_SSHN], (const float*)x[F_UN], (const float*)x[F_VN], m->hf_t, 0.0,
+                nullptr, m->scal.e12t, L, m->partials);
+    } else {
+        double** x = m->S[m->cur];
+        k_diag_partials<double, double, double><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+            x[F_SSHN], x[F_UN], x[F_VN], m->uniform ? nullptr : m->st[S_HT], m->scal.ht, e12t,
+            m->scal.e12t, L, m->partials);
+    }
+    LAUNCHCHK("diag_partials");
+    k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y), m->diag);
+    LAUNCHCHK("diag_finish");
+    HIPCHK(hipMemcpyAsync(m->h_pinned, m->diag, 3 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (int k = 0; k < 3; ++k) out[k] = m->h_pinned[k];
+    return NEMO2D_OK;
+}
+
 int nemo2d_destroy(nemo2d_model* m) {
     if (!m) return NEMO2D_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
@@ -1142,6 +1660,9 @@ int nemo2d_destroy(nemo2d_model* m) {
     dfree(m->flags); dfree(m->flu); dfree(m->flv);
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
     dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log);
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < NF; ++f) dfree(m->X[k][f]);
+    dfree(m->hf_t); dfree(m->hf_u); dfree(m->hf_v); dfree(m->diag);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);

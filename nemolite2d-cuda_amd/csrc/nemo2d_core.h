@@ -55,10 +55,12 @@ __host__ __device__ inline unsigned vcls(unsigned f) { return (f >> V_CLS_SHIFT)
 __host__ __device__ inline unsigned sucls(unsigned f) { return (f >> SU_SHIFT) & 3u; }
 __host__ __device__ inline unsigned svcls(unsigned f) { return (f >> SV_SHIFT) & 3u; }
 
-struct Params {
-    double rdt, cbfr, visc;
-    double fric;  // 1 + cbfr*rdt, as the serial model forms it (:514)
+template <class T>
+struct ParamsT {
+    T rdt, cbfr, visc;
+    T fric;  // 1 + cbfr*rdt, as the serial model forms it (:514)
 };
+using Params = ParamsT<double>;
 
 // Scalar statics, used when every element of a metric/depth/Coriolis array
 // holds the same value (the serial model's jphgr_msh=1 grid).
@@ -116,6 +118,7 @@ struct NState {
 // kernel (a register window with the same interface).
 template <class S>
 struct MemWin {
+    using real = double;
     const S& st;
     const NState& n;
     long o, P;
@@ -134,106 +137,121 @@ struct MemWin {
 
 #define W_(F, DI, DJ) w.template F<DI, DJ>()
 
-// continuity at T point (i, j), original/nemolite2d.f90:399-403.
-template <class W>
-__device__ __forceinline__ double continuity_w(const W& w, double rdt) {
-    const double r1 = (W_(sshn_u, 0, 0) + W_(hu, 0, 0)) * W_(un, 0, 0);
-    const double r2 = (W_(sshn_u, -1, 0) + W_(hu, -1, 0)) * W_(un, -1, 0);
-    const double r3 = (W_(sshn_v, 0, 0) + W_(hv, 0, 0)) * W_(vn, 0, 0);
-    const double r4 = (W_(sshn_v, 0, -1) + W_(hv, 0, -1)) * W_(vn, 0, -1);
-    return W_(sshn, 0, 0) + (r2 - r1 + r4 - r3) * rdt / W_(e12t, 0, 0);
-}
+// Division used by the point functions: IEEE for double (bitwise parity);
+// the single-precision form uses the fast reciprocal-based quotient.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b); }
 
 // Upwind weight SIGN(0.5, x) with Fortran's signed-zero rule.
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
+__device__ __forceinline__ float sgn_half(float x) { return copysignf(0.5f, x); }
+
+// continuity increment at T point (i, j): ssha - sshn, original/nemolite2d.f90:399-403
+// (ssha = sshn + incr is the same expression tree as :403).
+template <class W, class T = typename W::real>
+__device__ __forceinline__ T continuity_incr_w(const W& w, T rdt) {
+    const T r1 = (W_(sshn_u, 0, 0) + W_(hu, 0, 0)) * W_(un, 0, 0);
+    const T r2 = (W_(sshn_u, -1, 0) + W_(hu, -1, 0)) * W_(un, -1, 0);
+    const T r3 = (W_(sshn_v, 0, 0) + W_(hv, 0, 0)) * W_(vn, 0, 0);
+    const T r4 = (W_(sshn_v, 0, -1) + W_(hv, 0, -1)) * W_(vn, 0, -1);
+    return dv((r2 - r1 + r4 - r3) * rdt, W_(e12t, 0, 0));
+}
+template <class W>
+__device__ __forceinline__ double continuity_w(const W& w, double rdt) {
+    return W_(sshn, 0, 0) + continuity_incr_w(w, rdt);
+}
 
 // momentum_u at U point (i, j) with flags f, original/nemolite2d.f90:439-514.
-template <class W>
-__device__ __forceinline__ double momentum_u_w(const W& w, unsigned f, const Params& p) {
-    const double un_c = W_(un, 0, 0), un_e = W_(un, 1, 0), un_w = W_(un, -1, 0);
-    const double u_e = 0.5 * (un_c + un_e) * W_(e2t, 1, 0);
-    const double depe = W_(ht, 1, 0) + W_(sshn, 1, 0);
-    const double u_w = 0.5 * (un_c + un_w) * W_(e2t, 0, 0);
-    const double depw = W_(ht, 0, 0) + W_(sshn, 0, 0);
-    const double v_sc = 0.5 * (W_(vn, 0, -1) + W_(vn, 1, -1));
-    const double v_s = 0.5 * v_sc * (W_(e1v, 0, -1) + W_(e1v, 1, -1));
-    const double deps = 0.5 * (W_(hv, 0, -1) + W_(sshn_v, 0, -1) + W_(hv, 1, -1) + W_(sshn_v, 1, -1));
-    const double v_nc = 0.5 * (W_(vn, 0, 0) + W_(vn, 1, 0));
-    const double v_n = 0.5 * v_nc * (W_(e1v, 0, 0) + W_(e1v, 1, 0));
-    const double depn = 0.5 * (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
+template <class W, class T = typename W::real>
+__device__ __forceinline__ T momentum_u_w(const W& w, unsigned f, const ParamsT<T>& p) {
+    const T h = T(0.5);
+    const T un_c = W_(un, 0, 0), un_e = W_(un, 1, 0), un_w = W_(un, -1, 0);
+    const T u_e = h * (un_c + un_e) * W_(e2t, 1, 0);
+    const T depe = W_(ht, 1, 0) + W_(sshn, 1, 0);
+    const T u_w = h * (un_c + un_w) * W_(e2t, 0, 0);
+    const T depw = W_(ht, 0, 0) + W_(sshn, 0, 0);
+    const T v_sc = h * (W_(vn, 0, -1) + W_(vn, 1, -1));
+    const T v_s = h * v_sc * (W_(e1v, 0, -1) + W_(e1v, 1, -1));
+    const T deps = h * (W_(hv, 0, -1) + W_(sshn_v, 0, -1) + W_(hv, 1, -1) + W_(sshn_v, 1, -1));
+    const T v_nc = h * (W_(vn, 0, 0) + W_(vn, 1, 0));
+    const T v_n = h * v_nc * (W_(e1v, 0, 0) + W_(e1v, 1, 0));
+    const T depn = h * (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
 
-    const double uu_w = (0.5 - sgn_half(u_w)) * un_c + (0.5 + sgn_half(u_w)) * un_w;
-    const double uu_e = (0.5 + sgn_half(u_e)) * un_c + (0.5 - sgn_half(u_e)) * un_e;
+    const T uu_w = (h - sgn_half(u_w)) * un_c + (h + sgn_half(u_w)) * un_w;
+    const T uu_e = (h + sgn_half(u_e)) * un_c + (h - sgn_half(u_e)) * un_e;
     const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
-    const double un_s = W_(un, 0, -1), un_n = W_(un, 0, 1);
-    double uu_s = (0.5 - sgn_half(v_s)) * un_c;
-    if (!ws) uu_s = uu_s + (0.5 + sgn_half(v_s)) * un_s;
-    double uu_n = (0.5 + sgn_half(v_n)) * un_c;
-    if (!wn) uu_n = uu_n + (0.5 - sgn_half(v_n)) * un_n;
-    const double adv = uu_w * u_w * depw - uu_e * u_e * depe + uu_s * v_s * deps - uu_n * v_n * depn;
+    const T un_s = W_(un, 0, -1), un_n = W_(un, 0, 1);
+    T uu_s = (h - sgn_half(v_s)) * un_c;
+    if (!ws) uu_s = uu_s + (h + sgn_half(v_s)) * un_s;
+    T uu_n = (h + sgn_half(v_n)) * un_c;
+    if (!wn) uu_n = uu_n + (h - sgn_half(v_n)) * un_n;
+    const T adv = uu_w * u_w * depw - uu_e * u_e * depe + uu_s * v_s * deps - uu_n * v_n * depn;
 
-    const double dudx_e = (un_e - un_c) / W_(e1t, 1, 0) * (W_(ht, 1, 0) + W_(sshn, 1, 0));
-    const double dudx_w = (un_c - un_w) / W_(e1t, 0, 0) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
-    const double hsu = W_(hu, 0, 0) + W_(sshn_u, 0, 0);
-    double dudy_s = 0.0, dudy_n = 0.0;
+    const T dudx_e = dv(un_e - un_c, W_(e1t, 1, 0)) * (W_(ht, 1, 0) + W_(sshn, 1, 0));
+    const T dudx_w = dv(un_c - un_w, W_(e1t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const T hsu = W_(hu, 0, 0) + W_(sshn_u, 0, 0);
+    T dudy_s = T(0.0), dudy_n = T(0.0);
     if (!ws)
-        dudy_s = (un_c - un_s) / (W_(e2u, 0, 0) + W_(e2u, 0, -1)) *
+        dudy_s = dv(un_c - un_s, W_(e2u, 0, 0) + W_(e2u, 0, -1)) *
                  (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, -1) + W_(sshn_u, 0, -1));
     if (!wn)
-        dudy_n = (un_n - un_c) / (W_(e2u, 0, 0) + W_(e2u, 0, 1)) *
+        dudy_n = dv(un_n - un_c, W_(e2u, 0, 0) + W_(e2u, 0, 1)) *
                  (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
-    double vis = (dudx_e - dudx_w) * W_(e2u, 0, 0) + (dudy_n - dudy_s) * W_(e1u, 0, 0) * 0.5;
+    T vis = (dudx_e - dudx_w) * W_(e2u, 0, 0) + (dudy_n - dudy_s) * W_(e1u, 0, 0) * h;
     vis = p.visc * vis;
 
-    const double cor = 0.5 * (W_(fcu, 0, 0) * (v_sc + v_nc)) * W_(e12u, 0, 0) * hsu;
-    const double hpg = -GRAV * hsu * W_(e2u, 0, 0) * (W_(sshn, 1, 0) - W_(sshn, 0, 0));
+    const T cor = h * (W_(fcu, 0, 0) * (v_sc + v_nc)) * W_(e12u, 0, 0) * hsu;
+    const T hpg = -T(GRAV) * hsu * W_(e2u, 0, 0) * (W_(sshn, 1, 0) - W_(sshn, 0, 0));
     // ssha_u is never written by the serial model (:302, :315): it is 0.
-    return (un_c * hsu + p.rdt * (adv + vis + cor + hpg) / W_(e12u, 0, 0)) / (W_(hu, 0, 0) + 0.0) /
-           p.fric;
+    return dv(dv(un_c * hsu + dv(p.rdt * (adv + vis + cor + hpg), W_(e12u, 0, 0)),
+                 W_(hu, 0, 0) + T(0.0)),
+              p.fric);
 }
 
 // momentum_v at V point (i, j) with flags f, original/nemolite2d.f90:527-606.
-template <class W>
-__device__ __forceinline__ double momentum_v_w(const W& w, unsigned f, const Params& p) {
-    const double vn_c = W_(vn, 0, 0), vn_n = W_(vn, 0, 1), vn_s = W_(vn, 0, -1);
-    const double v_n = 0.5 * (vn_c + vn_n) * W_(e1t, 0, 1);
-    const double depn = W_(ht, 0, 1) + W_(sshn, 0, 1);
-    const double v_s = 0.5 * (vn_c + vn_s) * W_(e1t, 0, 0);
-    const double deps = W_(ht, 0, 0) + W_(sshn, 0, 0);
-    const double u_wc = 0.5 * (W_(un, -1, 0) + W_(un, -1, 1));
-    const double u_w = 0.5 * u_wc * (W_(e2u, -1, 0) + W_(e2u, -1, 1));
-    const double depw = 0.50 * (W_(hu, -1, 0) + W_(sshn_u, -1, 0) + W_(hu, -1, 1) + W_(sshn_u, -1, 1));
-    const double u_ec = 0.5 * (W_(un, 0, 0) + W_(un, 0, 1));
-    const double u_e = 0.5 * u_ec * (W_(e2u, 0, 0) + W_(e2u, 0, 1));
-    const double depe = 0.50 * (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
+template <class W, class T = typename W::real>
+__device__ __forceinline__ T momentum_v_w(const W& w, unsigned f, const ParamsT<T>& p) {
+    const T h = T(0.5);
+    const T vn_c = W_(vn, 0, 0), vn_n = W_(vn, 0, 1), vn_s = W_(vn, 0, -1);
+    const T v_n = h * (vn_c + vn_n) * W_(e1t, 0, 1);
+    const T depn = W_(ht, 0, 1) + W_(sshn, 0, 1);
+    const T v_s = h * (vn_c + vn_s) * W_(e1t, 0, 0);
+    const T deps = W_(ht, 0, 0) + W_(sshn, 0, 0);
+    const T u_wc = h * (W_(un, -1, 0) + W_(un, -1, 1));
+    const T u_w = h * u_wc * (W_(e2u, -1, 0) + W_(e2u, -1, 1));
+    const T depw = h * (W_(hu, -1, 0) + W_(sshn_u, -1, 0) + W_(hu, -1, 1) + W_(sshn_u, -1, 1));
+    const T u_ec = h * (W_(un, 0, 0) + W_(un, 0, 1));
+    const T u_e = h * u_ec * (W_(e2u, 0, 0) + W_(e2u, 0, 1));
+    const T depe = h * (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
 
-    const double vv_s = (0.5 - sgn_half(v_s)) * vn_c + (0.5 + sgn_half(v_s)) * vn_s;
-    const double vv_n = (0.5 + sgn_half(v_n)) * vn_c + (0.5 - sgn_half(v_n)) * vn_n;
+    const T vv_s = (h - sgn_half(v_s)) * vn_c + (h + sgn_half(v_s)) * vn_s;
+    const T vv_n = (h + sgn_half(v_n)) * vn_c + (h - sgn_half(v_n)) * vn_n;
     const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
-    const double vn_w = W_(vn, -1, 0), vn_e = W_(vn, 1, 0);
-    double vv_w = (0.5 - sgn_half(u_w)) * vn_c;
-    if (!ww) vv_w = vv_w + (0.5 + sgn_half(u_w)) * vn_w;
-    double vv_e = (0.5 + sgn_half(u_e)) * vn_c;
-    if (!we) vv_e = vv_e + (0.5 - sgn_half(u_e)) * vn_e;
-    const double adv = vv_w * u_w * depw - vv_e * u_e * depe + vv_s * v_s * deps - vv_n * v_n * depn;
+    const T vn_w = W_(vn, -1, 0), vn_e = W_(vn, 1, 0);
+    T vv_w = (h - sgn_half(u_w)) * vn_c;
+    if (!ww) vv_w = vv_w + (h + sgn_half(u_w)) * vn_w;
+    T vv_e = (h + sgn_half(u_e)) * vn_c;
+    if (!we) vv_e = vv_e + (h - sgn_half(u_e)) * vn_e;
+    const T adv = vv_w * u_w * depw - vv_e * u_e * depe + vv_s * v_s * deps - vv_n * v_n * depn;
 
-    const double dvdy_n = (vn_n - vn_c) / W_(e2t, 0, 1) * (W_(ht, 0, 1) + W_(sshn, 0, 1));
-    const double dvdy_s = (vn_c - vn_s) / W_(e2t, 0, 0) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
-    const double hsv = W_(hv, 0, 0) + W_(sshn_v, 0, 0);
-    double dvdx_w = 0.0, dvdx_e = 0.0;
+    const T dvdy_n = dv(vn_n - vn_c, W_(e2t, 0, 1)) * (W_(ht, 0, 1) + W_(sshn, 0, 1));
+    const T dvdy_s = dv(vn_c - vn_s, W_(e2t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const T hsv = W_(hv, 0, 0) + W_(sshn_v, 0, 0);
+    T dvdx_w = T(0.0), dvdx_e = T(0.0);
     if (!ww)
-        dvdx_w = (vn_c - vn_w) / (W_(e1v, 0, 0) + W_(e1v, -1, 0)) *
+        dvdx_w = dv(vn_c - vn_w, W_(e1v, 0, 0) + W_(e1v, -1, 0)) *
                  (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, -1, 0) + W_(sshn_v, -1, 0));
     if (!we)
-        dvdx_e = (vn_e - vn_c) / (W_(e1v, 0, 0) + W_(e1v, 1, 0)) *
+        dvdx_e = dv(vn_e - vn_c, W_(e1v, 0, 0) + W_(e1v, 1, 0)) *
                  (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
-    double vis = (dvdy_n - dvdy_s) * W_(e1v, 0, 0) + (dvdx_e - dvdx_w) * W_(e2v, 0, 0) * 0.5;
+    T vis = (dvdy_n - dvdy_s) * W_(e1v, 0, 0) + (dvdx_e - dvdx_w) * W_(e2v, 0, 0) * h;
     vis = p.visc * vis;
 
-    const double cor = -0.5 * (W_(fcv, 0, 0) * (u_ec + u_wc)) * W_(e12v, 0, 0) * hsv;
-    const double hpg = -GRAV * hsv * W_(e1v, 0, 0) * (W_(sshn, 0, 1) - W_(sshn, 0, 0));
-    return (vn_c * hsv + p.rdt * (adv + vis + cor + hpg) / W_(e12v, 0, 0)) / (W_(hv, 0, 0) + 0.0) /
-           p.fric;
+    const T cor = -h * (W_(fcv, 0, 0) * (u_ec + u_wc)) * W_(e12v, 0, 0) * hsv;
+    const T hpg = -T(GRAV) * hsv * W_(e1v, 0, 0) * (W_(sshn, 0, 1) - W_(sshn, 0, 0));
+    return dv(dv(vn_c * hsv + dv(p.rdt * (adv + vis + cor + hpg), W_(e12v, 0, 0)),
+                 W_(hv, 0, 0) + T(0.0)),
+              p.fric);
 }
 
 // Memory-window wrappers at offset o.

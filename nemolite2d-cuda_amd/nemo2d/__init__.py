@@ -28,8 +28,13 @@ SYMBOLS = (
     "nemo2d_bc_flather_v", "nemo2d_next", "nemo2d_step", "nemo2d_run",
     "nemo2d_record_checksums", "nemo2d_checksum_log", "nemo2d_checksums",
     "nemo2d_retrieve_results", "nemo2d_retrieve_aux", "nemo2d_timer_start",
-    "nemo2d_timer_stop", "nemo2d_destroy",
+    "nemo2d_timer_stop", "nemo2d_destroy", "nemo2d_create_mixed", "nemo2d_diagnostics",
+    "nemo2d_set_variant",
 )
+
+# nemo2d_set_variant choices
+VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_w4": 5,
+            "march32_flin": 6, "cell_flin": 7}
 
 
 class Nemo2dError(RuntimeError):
@@ -54,6 +59,8 @@ def lib():
         L.nemo2d_last_error.restype = ctypes.c_char_p
         L.nemo2d_version.restype = ctypes.c_char_p
         L.nemo2d_create.argtypes = [ctypes.POINTER(vp), i, i, d, d, d, i, i, i, d, d, d]
+        L.nemo2d_create_mixed.argtypes = [ctypes.POINTER(vp), i, i, d, d, d, i, i, i, d, d, d, i]
+        L.nemo2d_diagnostics.argtypes = [vp, dp]
         L.nemo2d_set_grid.argtypes = [vp, ip] + [dp] * 14
         L.nemo2d_set_state.argtypes = [vp] + [dp] * 5
         for n in ("continuity", "momentum_u", "momentum_v", "bc_solid_u", "bc_solid_v",
@@ -61,6 +68,7 @@ def lib():
             getattr(L, "nemo2d_" + n).argtypes = [vp]
         L.nemo2d_bc_ssh.argtypes = [vp, d]
         L.nemo2d_step.argtypes = [vp, i]
+        L.nemo2d_set_variant.argtypes = [vp, i]
         L.nemo2d_run.argtypes = [vp, i, i]
         L.nemo2d_record_checksums.argtypes = [vp, i]
         L.nemo2d_checksum_log.argtypes = [vp, dp, i, ip]
@@ -148,13 +156,25 @@ class Model:
     STATE_KEYS = ("sshn", "sshn_u", "sshn_v", "un", "vn")
 
     def __init__(self, jpi, jpj, dx=1000.0, dy=1000.0, dep_const=100.0, nit000=1, nitend=1,
-                 irecord=1, rdt=20.0, cbfr=0.0015, visc=0.1, grid=None, state=None):
+                 irecord=1, rdt=20.0, cbfr=0.0015, visc=0.1, grid=None, state=None,
+                 precision="fp64", compensated=True):
+        """precision: "fp64" (the drop-in, bitwise the serial model) or "fp32"
+        (single-precision state and stencils; compensated=True keeps the ssh
+        accumulation in FP64)."""
         self.jpi, self.jpj = int(jpi), int(jpj)
         self.rdt = float(rdt)
         self.nit000, self.nitend = int(nit000), int(nitend)
+        self.precision = precision
         h = ctypes.c_void_p()
-        _check(lib().nemo2d_create(ctypes.byref(h), self.jpi, self.jpj, dx, dy, dep_const,
-                                   int(nit000), int(nitend), int(irecord), rdt, cbfr, visc))
+        if precision == "fp64":
+            _check(lib().nemo2d_create(ctypes.byref(h), self.jpi, self.jpj, dx, dy, dep_const,
+                                       int(nit000), int(nitend), int(irecord), rdt, cbfr, visc))
+        elif precision == "fp32":
+            _check(lib().nemo2d_create_mixed(ctypes.byref(h), self.jpi, self.jpj, dx, dy, dep_const,
+                                             int(nit000), int(nitend), int(irecord), rdt, cbfr,
+                                             visc, 1 if compensated else 0))
+        else:
+            raise ValueError("precision must be 'fp64' or 'fp32'")
         self._h = h
         g = default_grid(self.jpi, self.jpj, dx, dy, dep_const) if grid is None else grid
         self.set_grid(g)
@@ -228,6 +248,10 @@ class Model:
         """One fused step."""
         _check(lib().nemo2d_step(self._h, int(istp)))
 
+    def set_variant(self, name):
+        """Kernel of the fused step (see VARIANTS); results do not depend on it."""
+        _check(lib().nemo2d_set_variant(self._h, VARIANTS[name] if isinstance(name, str) else int(name)))
+
     def run(self, nit000, nitend):
         _check(lib().nemo2d_run(self._h, int(nit000), int(nitend)))
 
@@ -257,6 +281,12 @@ class Model:
         _check(lib().nemo2d_retrieve_aux(self._h, _dp(r["sshn_u"]), _dp(r["sshn_v"]), _dp(r["ssha"])))
         return r
 
+    def diagnostics(self):
+        """(volume, energy, wet area) of the current state, FP64 reductions."""
+        out = np.zeros(3)
+        _check(lib().nemo2d_diagnostics(self._h, _dp(out)))
+        return out
+
     def timer_start(self):
         _check(lib().nemo2d_timer_start(self._h))
 
@@ -264,3 +294,52 @@ class Model:
         ms = ctypes.c_double(0.0)
         _check(lib().nemo2d_timer_stop(self._h, ctypes.byref(ms)))
         return ms.value
+
+
+def seamount_grid(jpi, jpj, seed, n_mounts=32, dx=1000.0, dy=1000.0, dep_const=100.0,
+                  height=(10.0, 60.0), radius=(20e3, 200e3), min_depth=10.0):
+    """The reference grid (land ring, open south edge, f-plane) with seeded
+    Gaussian seamounts: ht = max(dep_const - sum_k h_k exp(-r^2 / R_k^2),
+    min_depth), h_k ~ U(height), R_k ~ U(radius), centres uniform over the
+    basin; hu, hv are the means of the two adjacent ht (the outer faces take
+    the adjacent cell's depth).  Returns (grid, parameters of the mounts)."""
+    rng = np.random.default_rng(seed)
+    g = default_grid(jpi, jpj, dx, dy, dep_const)
+    hk = rng.uniform(*height, size=n_mounts)
+    rk = rng.uniform(*radius, size=n_mounts)
+    xk = rng.uniform(0.0, jpi * dx, size=n_mounts)
+    yk = rng.uniform(0.0, jpj * dy, size=n_mounts)
+    bump = np.zeros((jpj, jpi))
+    for h, r, x0, y0 in zip(hk, rk, xk, yk):
+        # only within 6 radii (exp(-36) ~ 2e-16 of the height beyond)
+        i0 = max(int((x0 - 6 * r) / dx), 0)
+        i1 = min(int((x0 + 6 * r) / dx) + 1, jpi)
+        j0 = max(int((y0 - 6 * r) / dy), 0)
+        j1 = min(int((y0 + 6 * r) / dy) + 1, jpj)
+        if i0 >= i1 or j0 >= j1:
+            continue
+        x = (np.arange(i0, i1) + 0.5) * dx - x0
+        y = (np.arange(j0, j1) + 0.5) * dy - y0
+        bump[j0:j1, i0:i1] += h * np.exp(-(y[:, None] ** 2 + x[None, :] ** 2) / (r * r))
+    ht = np.maximum(dep_const - bump, min_depth)
+    hu = np.empty((jpj, jpi + 1))
+    hu[:, 1:jpi] = 0.5 * (ht[:, :-1] + ht[:, 1:])
+    hu[:, 0] = ht[:, 0]
+    hu[:, jpi] = ht[:, -1]
+    hv = np.empty((jpj + 1, jpi))
+    hv[1:jpj, :] = 0.5 * (ht[:-1, :] + ht[1:, :])
+    hv[0, :] = ht[0, :]
+    hv[jpj, :] = ht[-1, :]
+    g["ht"], g["hu"], g["hv"] = ht, hu, hv
+    return g, {"height": hk, "radius": rk, "x": xk, "y": yk}
+
+
+def field_errors(a, b):
+    """RMS and max |a-b| of ssh (T), u (U interior faces), v (V interior faces)."""
+    out = {}
+    for k, sl in (("ssh", np.s_[:, :]), ("u", np.s_[:, 1:-1]), ("v", np.s_[1:-1, :])):
+        key = {"ssh": "sshn", "u": "un", "v": "vn"}[k]
+        d = (a[key] - b[key])[sl]
+        out[k] = {"rms": float(np.sqrt(np.mean(d * d))), "max": float(np.max(np.abs(d))),
+                  "ref_rms": float(np.sqrt(np.mean(b[key][sl] ** 2)))}
+    return out

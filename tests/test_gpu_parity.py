@@ -310,3 +310,19 @@ def test_fortran_driver(tmp_path, fused):
         assert d.shape == ((g - 2) ** 2, 6)
     d = np.loadtxt(tmp_path / "go2d_00300.dat")
     assert np.allclose(d[:, 3], o.arr("sshn").ravel(), rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
+def test_every_kernel_variant_is_bitwise_equal(variant):
+    """Default grid (uniform statics: all variants apply) and a general mask
+    with uniform metrics (Flather points on three edges)."""
+    for o in (Oracle(300, 70), random_grid_oracle(137, 45, 11, vary_metrics=False)):
+        with model_like(o) as m:
+            m.set_variant(variant)
+            m.record_checksums(30)
+            for istp in range(1, 31):
+                m.step(istp)
+                o.step(istp)
+            assert_fields_equal(m, o, variant)
+            log = m.checksum_log(30)
+            assert np.all(np.isfinite(log))
