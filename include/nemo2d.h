@@ -104,6 +104,12 @@ int nemo2d_next(nemo2d_model *m);                    /* :704-769; replaces cuda_
  * stage calls above. */
 int nemo2d_step(nemo2d_model *m, int istp);
 
+/* Tidal forcing of the clamped-ssh boundary: ssh = amp*sin(2*3.14159/
+ * (period_hours*3600) * rtime).  The default is the serial model's
+ * (amp 0.2 m, period 12.42 h, original/nemolite2d.f90:629-630); other values
+ * have no counterpart in the reference (ensemble members). */
+int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
+
 /* Choose the kernel of nemo2d_step (results are identical for every
  * choice): 0 default, 1 one thread per cell, 2/3/4 column-marching with
  * 16/32/64-row segments, 5 marching with a 4-waves-per-SIMD register

@@ -24,6 +24,8 @@
 
 #include "nemo2d.h"
 #include "nemo2d_core.h"
+#include "nemo2d_internal.h"
+#include "nemo2d_step.h"
 
 using namespace nemo2d;
 
@@ -56,8 +58,6 @@ enum {
     S_HT, S_HU, S_HV, S_FCU, S_FCV, NS
 };
 
-constexpr int TBX = 64, TBY = 4;   // 2-D tiles: one wave per row segment
-constexpr int RED_THREADS = 1024;  // checksum finishing kernel
 
 }  // namespace
 
@@ -106,6 +106,8 @@ struct nemo2d_model {
     // float depths, scalar float metrics
     bool mixed = false, compensated = true;
     int variant = 0;              // fused-step kernel variant (0 = default)
+    double tide_amp = 0.2;                      // original/nemolite2d.f90:629
+    double tide_omega = tide_omega_of(12.42);   // :630
     void* X[2][NF] = {};          // float (or double sshn) now-state ping-pong
     float *hf_t = nullptr, *hf_u = nullptr, *hf_v = nullptr;
     double* diag = nullptr;       // 2 doubles: volume, energy
@@ -116,102 +118,30 @@ struct nemo2d_model {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct AOut {
-    double *sshn, *sshn_u, *sshn_v, *un, *vn;
-};
-
-__device__ __forceinline__ bool in_rect(int i, int j, int i0, int i1, int j0, int j1) {
-    return i >= i0 && i <= i1 && j >= j0 && j <= j1;
-}
-
-// Pre-Flather value of the after-velocity at U point o, for a point that is
-// the inner neighbour of a Flather point (never itself a Flather point when
-// the fused path is used): momentum if computed, else 0 (solid points are
-// zeroed, untouched points keep the zero of the allocation).
-template <class S>
-__device__ __forceinline__ double u_pre_flather(const S& st, const NState& n, const uint16_t* fl,
-                                                long o, long P, const Params& p) {
-    const unsigned f = fl[o];
-    return ucls(f) == CLS_COMP ? momentum_u_at(st, n, o, P, f, p) : 0.0;
-}
-template <class S>
-__device__ __forceinline__ double v_pre_flather(const S& st, const NState& n, const uint16_t* fl,
-                                                long o, long P, const Params& p) {
-    const unsigned f = fl[o];
-    return vcls(f) == CLS_COMP ? momentum_v_at(st, n, o, P, f, p) : 0.0;
-}
-
-// ssha at T point o after bc_ssh: the tide value where clamped.
-template <class S>
-__device__ __forceinline__ double ssha_at(const S& st, const NState& n, const uint16_t* fl, long o,
-                                          long P, double rdt, double tide) {
-    return (fl[o] & T_CLAMP) ? tide : continuity_at(st, n, o, P, rdt);
-}
-
-// ---- fused step ------------------------------------------------------------
-// One thread per cell (i, j), i in [0, jpi], j in [0, jpj]: it produces the
-// T point (i, j), the U point (i, j) (east face), the V point (i, j) (north
-// face) and sshn_u/sshn_v there.  The new ssh east and north of the cell,
-// needed by the interpolation of next, is recomputed from the now-state.
-// Flather points are left to k_epilogue unless FLIN (then the inner
-// neighbour's momentum is recomputed here).
+// ---- fused step, one thread per cell (kernel wrapper) ---------------------------
 template <class S, bool FLIN>
 __global__ void __launch_bounds__(TBX * TBY)
 k_step_fused(S st, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L,
              Params p, double tide, double* __restrict__ partials) {
-    const int i = blockIdx.x * TBX + threadIdx.x;
-    const int j = blockIdx.y * TBY + threadIdx.y;
-    const long P = L.pitch;
     DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
-    if (i <= L.jpi && j <= L.jpj) {
-        const long o = L.at(i, j);
-        const unsigned f = fl[o];
-        double s_c = 0.0;
-        if (i >= 1 && j >= 1) {
-            s_c = (f & T_CLAMP) ? tide : continuity_at(st, n, o, P, p.rdt);
-            a.sshn[o] = s_c;
-            dd_add(ct, fabs(s_c));
-        }
-        if (j >= 1) {
-            // U point (i, j)
-            const unsigned c = ucls(f);
-            if (c != CLS_FLATHER || FLIN) {
-                double u = 0.0;
-                if (c == CLS_COMP) {
-                    u = momentum_u_at(st, n, o, P, f, p);
-                } else if (c == CLS_FLATHER) {
-                    const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
-                    u = u_pre_flather(st, n, fl, oi, P, p) + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
-                }
-                a.un[o] = u;
-                if (i >= 1) dd_add(cu, fabs(u));
-            }
-            const unsigned su = sucls(f);
-            double s_hi = 0.0;
-            if (su == NX_AVG || su == NX_HI) s_hi = ssha_at(st, n, fl, o + 1, P, p.rdt, tide);
-            a.sshn_u[o] = next_su(st, su, o, s_c, s_hi, n.sshn_u[o]);
-        }
-        if (i >= 1) {
-            // V point (i, j)
-            const unsigned c = vcls(f);
-            if (c != CLS_FLATHER || FLIN) {
-                double v = 0.0;
-                if (c == CLS_COMP) {
-                    v = momentum_v_at(st, n, o, P, f, p);
-                } else if (c == CLS_FLATHER) {
-                    const long oi = (f & V_FL_NORTH) ? o + P : o - P;
-                    v = v_pre_flather(st, n, fl, oi, P, p) + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
-                }
-                a.vn[o] = v;
-                if (j >= 1) dd_add(cv, fabs(v));
-            }
-            const unsigned sv = svcls(f);
-            double s_hi = 0.0;
-            if (sv == NX_AVG || sv == NX_HI) s_hi = ssha_at(st, n, fl, o + P, P, p.rdt, tide);
-            a.sshn_v[o] = next_sv(st, sv, o, P, s_c, s_hi, n.sshn_v[o]);
-        }
-    }
+    cell_step<S, FLIN>(st, n, a, fl, flu, flv, L, p, tide, blockIdx.x * TBX + threadIdx.x,
+                       blockIdx.y * TBY + threadIdx.y, ct, cu, cv);
+    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
+}
+
+// single-precision form, per-point depths
+template <bool COMP>
+__global__ void __launch_bounds__(TBX * TBY)
+k_step_mixed(ScalF s, MixedState<COMP> n, MixedOut<COMP> a, const float* __restrict__ ht,
+             const float* __restrict__ hu, const float* __restrict__ hv,
+             const uint16_t* __restrict__ fl, const double* __restrict__ flu,
+             const double* __restrict__ flv, Layout L, ParamsT<float> p, double tide,
+             double* __restrict__ partials) {
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    cell_step_mixed<COMP, false>(s, n, a, ht, hu, hv, fl, flu, flv, L, p, tide,
+                                 blockIdx.x * TBX + threadIdx.x, blockIdx.y * TBY + threadIdx.y,
+                                 ct, cu, cv);
     block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
 }
 
@@ -367,145 +297,6 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
     block_reduce3(ct, cu, cv, partials + 6 * (long)blockIdx.x);
 }
 
-// ---- single-precision form ------------------------------------------------------
-// FP32 state and stencil arithmetic; the new ssh is accumulated as
-// sshn + incr in FP64 when `COMP` (sshn kept in double), in FP32 otherwise
-// (the ablation).  Uniform metrics and Coriolis parameter, per-point depths.
-struct ScalF {
-    float e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, fcu, fcv;
-};
-
-template <bool COMP>
-struct MixedState {
-    using ssh_t = typename std::conditional<COMP, double, float>::type;
-    const ssh_t* __restrict__ sshn;
-    const float *__restrict__ sshn_u, *__restrict__ sshn_v, *__restrict__ un, *__restrict__ vn;
-};
-template <bool COMP>
-struct MixedOut {
-    typename MixedState<COMP>::ssh_t* sshn;
-    float *sshn_u, *sshn_v, *un, *vn;
-};
-
-template <bool COMP>
-struct MemWinF {
-    using real = float;
-    const MixedState<COMP>& n;
-    const ScalF& s;
-    const float *ht_, *hu_, *hv_;
-    long o, P;
-    template <int DI, int DJ> __device__ long at() const { return o + DI + (long)DJ * P; }
-    template <int DI, int DJ> __device__ float un() const { return n.un[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float vn() const { return n.vn[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float sshn() const { return (float)n.sshn[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float sshn_u() const { return n.sshn_u[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float sshn_v() const { return n.sshn_v[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float ht() const { return ht_[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float hu() const { return hu_[at<DI, DJ>()]; }
-    template <int DI, int DJ> __device__ float hv() const { return hv_[at<DI, DJ>()]; }
-#define NEMO2D_SF(F) \
-    template <int DI, int DJ> __device__ float F() const { return s.F; }
-    NEMO2D_SF(e1t) NEMO2D_SF(e2t) NEMO2D_SF(e1u) NEMO2D_SF(e2u) NEMO2D_SF(e1v) NEMO2D_SF(e2v)
-    NEMO2D_SF(e12t) NEMO2D_SF(e12u) NEMO2D_SF(e12v) NEMO2D_SF(fcu) NEMO2D_SF(fcv)
-#undef NEMO2D_SF
-};
-
-template <bool COMP>
-__device__ __forceinline__ typename MixedState<COMP>::ssh_t mixed_ssha(
-    const MemWinF<COMP>& w, unsigned f, float rdt, double tide) {
-    using ssh_t = typename MixedState<COMP>::ssh_t;
-    if (f & T_CLAMP) return (ssh_t)tide;
-    return w.n.sshn[w.o] + (ssh_t)continuity_incr_w(w, rdt);
-}
-
-template <bool COMP>
-__global__ void __launch_bounds__(TBX * TBY)
-k_step_mixed(ScalF s, MixedState<COMP> n, MixedOut<COMP> a, const float* __restrict__ ht,
-             const float* __restrict__ hu, const float* __restrict__ hv,
-             const uint16_t* __restrict__ fl, const double* __restrict__ flu,
-             const double* __restrict__ flv, Layout L, ParamsT<float> p, double tide,
-             double* __restrict__ partials) {
-    const int i = blockIdx.x * TBX + threadIdx.x;
-    const int j = blockIdx.y * TBY + threadIdx.y;
-    const long P = L.pitch;
-    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
-    if (i <= L.jpi && j <= L.jpj) {
-        const long o = L.at(i, j);
-        const unsigned f = fl[o];
-        MemWinF<COMP> w{n, s, ht, hu, hv, o, P};
-        double s_c = 0.0;
-        if (i >= 1 && j >= 1) {
-            const auto sa = mixed_ssha(w, f, p.rdt, tide);
-            a.sshn[o] = sa;
-            s_c = (double)sa;
-            dd_add(ct, fabs(s_c));
-        }
-        if (j >= 1) {
-            float u;
-            const unsigned c = ucls(f);
-            if (c == CLS_COMP) {
-                u = momentum_u_w(w, f, p);
-            } else if (c == CLS_FLATHER) {
-                const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
-                const unsigned fi = fl[oi];
-                MemWinF<COMP> wi{n, s, ht, hu, hv, oi, P};
-                const float ui = ucls(fi) == CLS_COMP ? momentum_u_w(wi, fi, p) : 0.0f;
-                u = ui + (float)flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
-            } else {
-                u = 0.0f;
-            }
-            a.un[o] = u;
-            if (i >= 1) dd_add(cu, fabs((double)u));
-            const unsigned su = sucls(f);
-            float r = n.sshn_u[o];
-            if (su != NX_SKIP) {
-                float s_hi = 0.0f;
-                if (su == NX_AVG || su == NX_HI) {
-                    const long oe = o + 1;
-                    MemWinF<COMP> we_{n, s, ht, hu, hv, oe, P};
-                    s_hi = (float)mixed_ssha(we_, fl[oe], p.rdt, tide);
-                }
-                const float s_lo = (float)s_c;
-                r = su == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12u
-                                 : (su == NX_HI ? s_hi : s_lo);
-            }
-            a.sshn_u[o] = r;
-        }
-        if (i >= 1) {
-            float v;
-            const unsigned c = vcls(f);
-            if (c == CLS_COMP) {
-                v = momentum_v_w(w, f, p);
-            } else if (c == CLS_FLATHER) {
-                const long oi = (f & V_FL_NORTH) ? o + P : o - P;
-                const unsigned fi = fl[oi];
-                MemWinF<COMP> wi{n, s, ht, hu, hv, oi, P};
-                const float vi = vcls(fi) == CLS_COMP ? momentum_v_w(wi, fi, p) : 0.0f;
-                v = vi + (float)flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
-            } else {
-                v = 0.0f;
-            }
-            a.vn[o] = v;
-            if (j >= 1) dd_add(cv, fabs((double)v));
-            const unsigned sv = svcls(f);
-            float r = n.sshn_v[o];
-            if (sv != NX_SKIP) {
-                float s_hi = 0.0f;
-                if (sv == NX_AVG || sv == NX_HI) {
-                    const long on = o + P;
-                    MemWinF<COMP> wn_{n, s, ht, hu, hv, on, P};
-                    s_hi = (float)mixed_ssha(wn_, fl[on], p.rdt, tide);
-                }
-                const float s_lo = (float)s_c;
-                r = sv == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12v
-                                 : (sv == NX_HI ? s_hi : s_lo);
-            }
-            a.sshn_v[o] = r;
-        }
-    }
-    block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
-}
-
 // sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of a state of element
 // types TS (ssh) and TV (the rest).
 template <class TS, class TV>
@@ -558,25 +349,6 @@ k_diag_partials(const TS* __restrict__ sshn, const TV* __restrict__ un, const TV
 // order; their |values| inside (1:jpi,1:jpj) go to partial slot nparts.
 // Phase 2 (when out is given): the fixed-order reduction of the nparts+1
 // partials into the step's checksums.
-__device__ __forceinline__ void finish_partials(const double* __restrict__ partials, int nparts,
-                                                double* __restrict__ out) {
-    DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-        const double* q = partials + 6 * (long)k;
-        a0 = dd_merge(a0, DD{q[0], q[1]});
-        a1 = dd_merge(a1, DD{q[2], q[3]});
-        a2 = dd_merge(a2, DD{q[4], q[5]});
-    }
-    __shared__ double r[6];
-    block_reduce3(a0, a1, a2, r);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        out[0] = r[0] + r[1];
-        out[1] = r[2] + r[3];
-        out[2] = r[4] + r[5];
-    }
-}
-
 __global__ void __launch_bounds__(RED_THREADS)
 k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
            const int* __restrict__ vptr, const long* __restrict__ vpts, int nvl,
@@ -788,13 +560,6 @@ StArrays starrays(const nemo2d_model* m) {
         }                                   \
     } while (0)
 
-// Tide value of the clamped-ssh boundary, original/nemolite2d.f90:629-635,
-// evaluated with the host C library.
-double tide_value(double rtime) {
-    const double amp_tide = 0.2;
-    const double omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0);
-    return amp_tide * sin(omega_tide * rtime);
-}
 
 void dfree(void* p) {
     if (p) (void)hipFree(p);
@@ -879,6 +644,152 @@ int checksum_into(nemo2d_model* m, int k, double* out) {
 
 }  // namespace
 
+namespace nemo2d {
+
+// Per-point classes of a mask (see nemo2d_core.h), the Flather coefficients
+// sqrt(g/h) and the Flather point lists in the serial model's loop order.
+// Rejects masks the serial model would index outside its arrays with.
+int classify(const Layout& L, const int* pt, const double* hu, const double* hv, Classes& c,
+             std::string& err) {
+    const int jpi = L.jpi, jpj = L.jpj;
+    const int pw = jpi + 2;
+    auto PT = [&](int i, int j) { return pt[i + (long)j * pw]; };
+    for (long k = 0; k < (long)pw * (jpj + 2); ++k)
+        if (pt[k] < -1 || pt[k] > 1) {
+            err = "mask values must be -1, 0 or 1";
+            return NEMO2D_ERR_MASK;
+        }
+    auto U = [&](const double* a, int i, int j) { return a[i + (long)(j - 1) * (jpi + 1)]; };
+    auto V = [&](const double* a, int i, int j) { return a[(i - 1) + (long)j * jpi]; };
+    // ---- per-point classes --------------------------------------------------
+    std::vector<uint16_t>& fl = c.fl;
+    std::vector<double>& hflu = c.flu;
+    std::vector<double>& hflv = c.flv;
+    fl.assign((size_t)L.size(), 0);
+    hflu.assign((size_t)L.size(), 0.0);
+    hflv.assign((size_t)L.size(), 0.0);
+    auto err_oob = [&](const char* what, int i, int j) {
+        err = std::string("mask makes the serial model read outside its arrays (") + what +
+              " at i=" + std::to_string(i) + ", j=" + std::to_string(j) + ")";
+        return NEMO2D_ERR_MASK;
+    };
+    for (int j = 1; j <= jpj; ++j)
+        for (int i = 0; i <= jpi; ++i) {  // U points
+            unsigned f = 0;
+            const int a = PT(i, j), b = PT(i + 1, j);
+            unsigned cls = CLS_NONE;
+            if (i >= 1 && i <= jpi - 1 && !(a + b <= 0) && !(a <= 0 || b <= 0)) {
+                cls = CLS_COMP;
+                if (PT(i, j - 1) <= 0 || PT(i + 1, j - 1) <= 0) f |= U_WALL_S;
+                else if (j == 1) return err_oob("momentum_u south", i, j);
+                if (PT(i, j + 1) <= 0 || PT(i + 1, j + 1) <= 0) f |= U_WALL_N;
+                else if (j == jpj) return err_oob("momentum_u north", i, j);
+            }
+            if (a * b == 0) cls = CLS_SOLID;
+            if (!(a + b <= -1) && (a < 0 || b < 0)) {
+                cls = CLS_FLATHER;
+                const bool east = a < 0;
+                if (east) f |= U_FL_EAST;
+                const int ii = east ? i + 1 : i - 1;
+                if (ii < 0 || ii > jpi) return err_oob("bc_flather_u", i, j);
+                hflu[L.at(i, j)] = sqrt(GRAV / U(hu, i, j));
+            }
+            f |= cls << U_CLS_SHIFT;
+            unsigned nx = NX_SKIP;
+            if (!(a + b <= 0)) {
+                if (a * b > 0) nx = NX_AVG;
+                else if (a <= 0) nx = NX_HI;
+                else if (b <= 0) nx = NX_LO;
+            }
+            if ((nx == NX_AVG && (i == 0 || i == jpi)) || (nx == NX_HI && i == jpi) ||
+                (nx == NX_LO && i == 0))
+                return err_oob("next sshn_u", i, j);
+            f |= nx << SU_SHIFT;
+            fl[L.at(i, j)] |= (uint16_t)f;
+        }
+    for (int j = 0; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i) {  // V points
+            unsigned f = 0;
+            const int a = PT(i, j), b = PT(i, j + 1);
+            unsigned cls = CLS_NONE;
+            // the first skip test looks east (original/nemolite2d.f90:523)
+            if (j >= 1 && j <= jpj - 1 && !(a + PT(i + 1, j) <= 0) && !(a <= 0 || b <= 0)) {
+                cls = CLS_COMP;
+                if (PT(i - 1, j) <= 0 || PT(i - 1, j + 1) <= 0) f |= V_WALL_W;
+                else if (i == 1) return err_oob("momentum_v west", i, j);
+                if (PT(i + 1, j) <= 0 || PT(i + 1, j + 1) <= 0) f |= V_WALL_E;
+                else if (i == jpi) return err_oob("momentum_v east", i, j);
+            }
+            if (a * b == 0) cls = CLS_SOLID;
+            if (!(a + b <= -1) && (a < 0 || b < 0)) {
+                cls = CLS_FLATHER;
+                const bool north = a < 0;
+                if (north) f |= V_FL_NORTH;
+                const int jj = north ? j + 1 : j - 1;
+                if (jj < 0 || jj > jpj) return err_oob("bc_flather_v", i, j);
+                hflv[L.at(i, j)] = sqrt(GRAV / V(hv, i, j));
+            }
+            f |= cls << V_CLS_SHIFT;
+            unsigned nx = NX_SKIP;
+            if (!(a + b <= 0)) {
+                if (a * b > 0) nx = NX_AVG;
+                else if (a <= 0) nx = NX_HI;
+                else if (b <= 0) nx = NX_LO;
+            }
+            if ((nx == NX_AVG && (j == 0 || j == jpj)) || (nx == NX_HI && j == jpj) ||
+                (nx == NX_LO && j == 0))
+                return err_oob("next sshn_v", i, j);
+            f |= nx << SV_SHIFT;
+            fl[L.at(i, j)] |= (uint16_t)f;
+        }
+    for (int j = 1; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i) {  // T points: bc_ssh
+            if (PT(i, j) <= 0) continue;
+            if (PT(i, j - 1) < 0 || PT(i, j + 1) < 0 || PT(i + 1, j) < 0 || PT(i - 1, j) < 0)
+                fl[L.at(i, j)] |= T_CLAMP;
+        }
+
+    // Flather lines, in loop order; the fused path needs every Flather
+    // point's inner neighbour to be a non-Flather point.
+    std::vector<int>& uptr = c.uptr;
+    std::vector<int>& vptr = c.vptr;
+    std::vector<long>& upts = c.upts;
+    std::vector<long>& vpts = c.vpts;
+    uptr.assign(jpj + 1, 0);
+    vptr.assign(jpi + 1, 0);
+    upts.clear();
+    vpts.clear();
+    bool& chained = c.chained;
+    chained = false;
+    for (int j = 1; j <= jpj; ++j) {
+        uptr[j - 1] = (int)upts.size();
+        for (int i = 0; i <= jpi; ++i) {
+            const long o = L.at(i, j);
+            if (ucls(fl[o]) == CLS_FLATHER) {
+                upts.push_back(o);
+                const long oi = (fl[o] & U_FL_EAST) ? o + 1 : o - 1;
+                if (ucls(fl[oi]) == CLS_FLATHER) chained = true;
+            }
+        }
+    }
+    uptr[jpj] = (int)upts.size();
+    for (int i = 1; i <= jpi; ++i) {
+        vptr[i - 1] = (int)vpts.size();
+        for (int j = 0; j <= jpj; ++j) {
+            const long o = L.at(i, j);
+            if (vcls(fl[o]) == CLS_FLATHER) {
+                vpts.push_back(o);
+                const long oi = (fl[o] & V_FL_NORTH) ? o + L.pitch : o - L.pitch;
+                if (vcls(fl[oi]) == CLS_FLATHER) chained = true;
+            }
+        }
+    }
+    vptr[jpi] = (int)vpts.size();
+    return NEMO2D_OK;
+}
+
+}  // namespace nemo2d
+
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
@@ -959,129 +870,21 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     if (!pt) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null mask");
     const Layout& L = m->L;
     const int jpi = L.jpi, jpj = L.jpj;
-    const int pw = jpi + 2;
-    auto PT = [&](int i, int j) { return pt[i + (long)j * pw]; };
-    for (long k = 0; k < (long)pw * (jpj + 2); ++k)
-        if (pt[k] < -1 || pt[k] > 1)
-            return fail(NEMO2D_ERR_MASK, "nemo2d_set_grid: mask values must be -1, 0 or 1");
     const long nu = (long)(jpi + 1) * jpj, nv = (long)jpi * (jpj + 1), nt = (long)jpi * jpj;
-    auto U = [&](const double* a, int i, int j) { return a[i + (long)(j - 1) * (jpi + 1)]; };
-    auto V = [&](const double* a, int i, int j) { return a[(i - 1) + (long)j * jpi]; };
-
-    // ---- per-point classes --------------------------------------------------
-    std::vector<uint16_t> fl((size_t)L.size(), 0);
-    std::vector<double> hflu((size_t)L.size(), 0.0), hflv((size_t)L.size(), 0.0);
-    auto err_oob = [&](const char* what, int i, int j) {
-        return fail(NEMO2D_ERR_MASK, std::string("nemo2d_set_grid: mask makes the serial model read "
-                                                 "outside its arrays (") + what + " at i=" +
-                                         std::to_string(i) + ", j=" + std::to_string(j) + ")");
-    };
-    for (int j = 1; j <= jpj; ++j)
-        for (int i = 0; i <= jpi; ++i) {  // U points
-            unsigned f = 0;
-            const int a = PT(i, j), b = PT(i + 1, j);
-            unsigned cls = CLS_NONE;
-            if (i >= 1 && i <= jpi - 1 && !(a + b <= 0) && !(a <= 0 || b <= 0)) {
-                cls = CLS_COMP;
-                if (PT(i, j - 1) <= 0 || PT(i + 1, j - 1) <= 0) f |= U_WALL_S;
-                else if (j == 1) return err_oob("momentum_u south", i, j);
-                if (PT(i, j + 1) <= 0 || PT(i + 1, j + 1) <= 0) f |= U_WALL_N;
-                else if (j == jpj) return err_oob("momentum_u north", i, j);
-            }
-            if (a * b == 0) cls = CLS_SOLID;
-            if (!(a + b <= -1) && (a < 0 || b < 0)) {
-                cls = CLS_FLATHER;
-                const bool east = a < 0;
-                if (east) f |= U_FL_EAST;
-                const int ii = east ? i + 1 : i - 1;
-                if (ii < 0 || ii > jpi) return err_oob("bc_flather_u", i, j);
-                hflu[L.at(i, j)] = sqrt(GRAV / U(hu, i, j));
-            }
-            f |= cls << U_CLS_SHIFT;
-            unsigned nx = NX_SKIP;
-            if (!(a + b <= 0)) {
-                if (a * b > 0) nx = NX_AVG;
-                else if (a <= 0) nx = NX_HI;
-                else if (b <= 0) nx = NX_LO;
-            }
-            if ((nx == NX_AVG && (i == 0 || i == jpi)) || (nx == NX_HI && i == jpi) ||
-                (nx == NX_LO && i == 0))
-                return err_oob("next sshn_u", i, j);
-            f |= nx << SU_SHIFT;
-            fl[L.at(i, j)] |= (uint16_t)f;
-        }
-    for (int j = 0; j <= jpj; ++j)
-        for (int i = 1; i <= jpi; ++i) {  // V points
-            unsigned f = 0;
-            const int a = PT(i, j), b = PT(i, j + 1);
-            unsigned cls = CLS_NONE;
-            // the first skip test looks east (original/nemolite2d.f90:523)
-            if (j >= 1 && j <= jpj - 1 && !(a + PT(i + 1, j) <= 0) && !(a <= 0 || b <= 0)) {
-                cls = CLS_COMP;
-                if (PT(i - 1, j) <= 0 || PT(i - 1, j + 1) <= 0) f |= V_WALL_W;
-                else if (i == 1) return err_oob("momentum_v west", i, j);
-                if (PT(i + 1, j) <= 0 || PT(i + 1, j + 1) <= 0) f |= V_WALL_E;
-                else if (i == jpi) return err_oob("momentum_v east", i, j);
-            }
-            if (a * b == 0) cls = CLS_SOLID;
-            if (!(a + b <= -1) && (a < 0 || b < 0)) {
-                cls = CLS_FLATHER;
-                const bool north = a < 0;
-                if (north) f |= V_FL_NORTH;
-                const int jj = north ? j + 1 : j - 1;
-                if (jj < 0 || jj > jpj) return err_oob("bc_flather_v", i, j);
-                hflv[L.at(i, j)] = sqrt(GRAV / V(hv, i, j));
-            }
-            f |= cls << V_CLS_SHIFT;
-            unsigned nx = NX_SKIP;
-            if (!(a + b <= 0)) {
-                if (a * b > 0) nx = NX_AVG;
-                else if (a <= 0) nx = NX_HI;
-                else if (b <= 0) nx = NX_LO;
-            }
-            if ((nx == NX_AVG && (j == 0 || j == jpj)) || (nx == NX_HI && j == jpj) ||
-                (nx == NX_LO && j == 0))
-                return err_oob("next sshn_v", i, j);
-            f |= nx << SV_SHIFT;
-            fl[L.at(i, j)] |= (uint16_t)f;
-        }
-    for (int j = 1; j <= jpj; ++j)
-        for (int i = 1; i <= jpi; ++i) {  // T points: bc_ssh
-            if (PT(i, j) <= 0) continue;
-            if (PT(i, j - 1) < 0 || PT(i, j + 1) < 0 || PT(i + 1, j) < 0 || PT(i - 1, j) < 0)
-                fl[L.at(i, j)] |= T_CLAMP;
-        }
-
-    // Flather lines, in loop order; the fused path needs every Flather
-    // point's inner neighbour to be a non-Flather point.
-    std::vector<int> uptr(jpj + 1, 0), vptr(jpi + 1, 0);
-    std::vector<long> upts, vpts;
-    bool chained = false;
-    for (int j = 1; j <= jpj; ++j) {
-        uptr[j - 1] = (int)upts.size();
-        for (int i = 0; i <= jpi; ++i) {
-            const long o = L.at(i, j);
-            if (ucls(fl[o]) == CLS_FLATHER) {
-                upts.push_back(o);
-                const long oi = (fl[o] & U_FL_EAST) ? o + 1 : o - 1;
-                if (ucls(fl[oi]) == CLS_FLATHER) chained = true;
-            }
-        }
+    Classes cls;
+    {
+        std::string err;
+        const int rc = classify(L, pt, hu, hv, cls, err);
+        if (rc) return fail(rc, "nemo2d_set_grid: " + err);
     }
-    uptr[jpj] = (int)upts.size();
-    for (int i = 1; i <= jpi; ++i) {
-        vptr[i - 1] = (int)vpts.size();
-        for (int j = 0; j <= jpj; ++j) {
-            const long o = L.at(i, j);
-            if (vcls(fl[o]) == CLS_FLATHER) {
-                vpts.push_back(o);
-                const long oi = (fl[o] & V_FL_NORTH) ? o + L.pitch : o - L.pitch;
-                if (vcls(fl[oi]) == CLS_FLATHER) chained = true;
-            }
-        }
-    }
-    vptr[jpi] = (int)vpts.size();
-    m->fused_ok = !chained;
+    std::vector<uint16_t>& fl = cls.fl;
+    std::vector<double>& hflu = cls.flu;
+    std::vector<double>& hflv = cls.flv;
+    std::vector<int>& uptr = cls.uptr;
+    std::vector<int>& vptr = cls.vptr;
+    std::vector<long>& upts = cls.upts;
+    std::vector<long>& vpts = cls.vpts;
+    m->fused_ok = !cls.chained;
 
     // ---- statics ------------------------------------------------------------
     std::vector<double> fcu((size_t)nu), fcv((size_t)nv);
@@ -1260,7 +1063,7 @@ int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
     READY_STAGE(m, "nemo2d_bc_ssh");
     const Layout& L = m->L;
     k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ssha, L,
-                                                                            tide_value(rtime));
+                                                                            tide_eval(m->tide_amp, m->tide_omega, rtime));
     LAUNCHCHK("bc_ssh");
     return NEMO2D_OK;
 }
@@ -1327,9 +1130,9 @@ static int step_mixed(nemo2d_model* m, int istp) {
     const int k = m->cur;
     const Scal& d = m->scal;
     ScalF sf{(float)d.e1t, (float)d.e2t, (float)d.e1u, (float)d.e2u, (float)d.e1v, (float)d.e2v,
-             (float)d.e12t, (float)d.e12u, (float)d.e12v, (float)d.fcu, (float)d.fcv};
+             (float)d.e12t, (float)d.e12u, (float)d.e12v, (float)d.fcu, (float)d.fcv, (float)d.ht, (float)d.hu, (float)d.hv};
     ParamsT<float> pf{(float)m->prm.rdt, (float)m->prm.cbfr, (float)m->prm.visc, (float)m->prm.fric};
-    const double tide = tide_value((double)istp * m->prm.rdt);
+    const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
     void** x = m->X[k];
     void** y = m->X[1 - k];
@@ -1427,7 +1230,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     NState n = nstate(m, k);
     AOut a{m->S[1 - k][F_SSHN], m->S[1 - k][F_SSHN_U], m->S[1 - k][F_SSHN_V], m->S[1 - k][F_UN],
            m->S[1 - k][F_VN]};
-    const double tide = tide_value((double)istp * m->prm.rdt);
+    const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
     if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH32 : VAR_CELL;
     if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
@@ -1462,6 +1265,15 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
         m->cs_next = slot + 1;
     }
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_tide(nemo2d_model* m, double amp, double period_hours) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_tide: null model");
+    if (!isfinite(amp) || !(period_hours > 0.0))
+        return fail(NEMO2D_ERR_ARG, "nemo2d_set_tide: amplitude must be finite, period > 0");
+    m->tide_amp = amp;
+    m->tide_omega = tide_omega_of(period_hours);
     return NEMO2D_OK;
 }
 

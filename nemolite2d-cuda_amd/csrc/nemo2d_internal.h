@@ -1,0 +1,30 @@
+// Host-side helpers shared by the library's translation units.
+#pragma once
+
+#include <math.h>
+
+#include <string>
+#include <vector>
+
+#include "nemo2d_core.h"
+
+namespace nemo2d {
+
+struct Classes {
+    std::vector<uint16_t> fl;        // per-point class words
+    std::vector<double> flu, flv;    // Flather sqrt(g/h) at Flather points
+    std::vector<int> uptr, vptr;     // Flather lines: rows (u) / columns (v)
+    std::vector<long> upts, vpts;    // Flather point offsets in loop order
+    bool chained = false;            // a Flather point's inner point is one too
+};
+
+// Tide of the clamped-ssh open boundary, original/nemolite2d.f90:629-635:
+// amp * sin(omega * rtime) with omega = 2*3.14159/(period*3600) (the serial
+// model's literal 3.14159, period 12.42 h, amp 0.2 m), on the host C library.
+inline double tide_omega_of(double period_hours) { return 2.0 * 3.14159 / (period_hours * 3600.0); }
+inline double tide_eval(double amp, double omega, double rtime) { return amp * sin(omega * rtime); }
+
+int classify(const Layout& L, const int* pt, const double* hu, const double* hv, Classes& c,
+             std::string& err);
+
+}  // namespace nemo2d

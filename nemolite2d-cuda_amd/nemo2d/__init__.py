@@ -29,7 +29,7 @@ SYMBOLS = (
     "nemo2d_record_checksums", "nemo2d_checksum_log", "nemo2d_checksums",
     "nemo2d_retrieve_results", "nemo2d_retrieve_aux", "nemo2d_timer_start",
     "nemo2d_timer_stop", "nemo2d_destroy", "nemo2d_create_mixed", "nemo2d_diagnostics",
-    "nemo2d_set_variant",
+    "nemo2d_set_variant", "nemo2d_set_tide",
 )
 
 # nemo2d_set_variant choices
@@ -69,6 +69,7 @@ def lib():
         L.nemo2d_bc_ssh.argtypes = [vp, d]
         L.nemo2d_step.argtypes = [vp, i]
         L.nemo2d_set_variant.argtypes = [vp, i]
+        L.nemo2d_set_tide.argtypes = [vp, d, d]
         L.nemo2d_run.argtypes = [vp, i, i]
         L.nemo2d_record_checksums.argtypes = [vp, i]
         L.nemo2d_checksum_log.argtypes = [vp, dp, i, ip]
@@ -142,10 +143,10 @@ def initial_state(grid, sshn=None):
             "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
 
 
-def tide(rtime):
+def tide(rtime, amp=0.2, period_hours=12.42):
     """Clamped-ssh tide of original/nemolite2d.f90:629-635 (host libm)."""
-    omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0)
-    return 0.2 * math.sin(omega_tide * rtime)
+    omega_tide = 2.0 * 3.14159 / (period_hours * 3600.0)
+    return amp * math.sin(omega_tide * rtime)
 
 
 class Model:
@@ -247,6 +248,10 @@ class Model:
     def step(self, istp):
         """One fused step."""
         _check(lib().nemo2d_step(self._h, int(istp)))
+
+    def set_tide(self, amp, period_hours):
+        """Tide amplitude (m) and period (hours) of the clamped-ssh boundary."""
+        _check(lib().nemo2d_set_tide(self._h, float(amp), float(period_hours)))
 
     def set_variant(self, name):
         """Kernel of the fused step (see VARIANTS); results do not depend on it."""

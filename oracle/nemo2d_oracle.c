@@ -40,6 +40,7 @@ typedef struct {
 struct o2d {
     int jpi, jpj;
     o2d_params prm;
+    double tide_amp, tide_omega;  /* original/nemolite2d.f90:629-630 */
     int *pt;             /* (0:jpi+1, 0:jpj+1) */
     farr e1t, e2t, e1u, e2u, e1f, e2f, e1v, e2v, e12t, e12u, e12v;
     farr gphiu, gphiv, gphif, xt, yt, ht, hu, hv, hf;
@@ -64,6 +65,8 @@ o2d *o2d_create(int jpi, int jpj) {
     o2d *m = (o2d *)calloc(1, sizeof(o2d));
     if (!m) return NULL;
     m->jpi = jpi; m->jpj = jpj;
+    m->tide_amp = 0.2;
+    m->tide_omega = 2.0 * 3.14159 / (12.42 * 3600.0);
     int ok = 1;
     ok &= falloc(&m->e1t, 1, jpi, 1, jpj);   ok &= falloc(&m->e2t, 1, jpi, 1, jpj);
     ok &= falloc(&m->e1u, 0, jpi, 1, jpj);   ok &= falloc(&m->e2u, 0, jpi, 1, jpj);
@@ -368,15 +371,24 @@ double o2d_tide(double rtime) {
     return amp_tide * sin(omega_tide * rtime);
 }
 
+/* Tide amplitude (m) and period (hours); the defaults are the serial
+ * model's.  Other values are for ensemble members (no reference
+ * counterpart). */
+void o2d_set_tide(o2d *m, double amp, double period_hours) {
+    m->tide_amp = amp;
+    m->tide_omega = 2.0 * 3.14159 / (period_hours * 3600.0);
+}
+
 /* bc_ssh, original/nemolite2d.f90:628-645. */
 void o2d_bc_ssh(o2d *m, double rtime) {
     const int jpi = m->jpi, jpj = m->jpj;
+    const double tide = m->tide_amp * sin(m->tide_omega * rtime);
     for (int jj = 1; jj <= jpj; ++jj)
         for (int ji = 1; ji <= jpi; ++ji) {
             if (PT(m, ji, jj) <= 0) continue;
             if (PT(m, ji, jj - 1) < 0 || PT(m, ji, jj + 1) < 0 || PT(m, ji + 1, jj) < 0 ||
                 PT(m, ji - 1, jj) < 0)
-                F(m->ssha, ji, jj) = o2d_tide(rtime);
+                F(m->ssha, ji, jj) = tide;
         }
 }
 

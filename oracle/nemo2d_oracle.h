@@ -28,6 +28,7 @@ void o2d_continuity(o2d *m);
 void o2d_momentum_u(o2d *m);
 void o2d_momentum_v(o2d *m);
 double o2d_tide(double rtime);
+void o2d_set_tide(o2d *m, double amp, double period_hours);
 void o2d_bc_ssh(o2d *m, double rtime);
 void o2d_bc_solid_u(o2d *m);
 void o2d_bc_solid_v(o2d *m);

@@ -49,6 +49,7 @@ def lib():
         L.o2d_bc.argtypes = [vp, d]
         L.o2d_tide.restype = d
         L.o2d_tide.argtypes = [d]
+        L.o2d_set_tide.argtypes = [vp, d, d]
         L.o2d_step.argtypes = [vp, i]
         L.o2d_checksums.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.o2d_run.restype = d
@@ -128,6 +129,7 @@ class Oracle:
     def bc_flather_u(self): lib().o2d_bc_flather_u(self._m)
     def bc_flather_v(self): lib().o2d_bc_flather_v(self._m)
     def bc(self, rtime): lib().o2d_bc(self._m, float(rtime))
+    def set_tide(self, amp, period_hours): lib().o2d_set_tide(self._m, float(amp), float(period_hours))
     def next(self): lib().o2d_next(self._m)
     def step(self, istp): lib().o2d_step(self._m, int(istp))
 
