@@ -173,6 +173,38 @@ int nemo2d_create_mixed(nemo2d_model **model, int jpi, int jpj, double dx, doubl
  * sum(e12t).  Compensated, order-fixed reduction.  Synchronises. */
 int nemo2d_diagnostics(nemo2d_model *m, double out[3]);
 
+/* --- ensembles of basins (no counterpart in the reference) ---------------
+ * nmem members, each the serial model's basin (land ring, open south edge,
+ * f-plane at 50 N, jphgr_msh=1 grid of original/nemolite2d.f90:180-279)
+ * with its own interior size jpi[k] x jpj[k], constant depth dep[k] (m) and
+ * tide amp[k] (m) / period_h[k] (hours); common dx, dy, rdt, cbfr, visc.
+ * precision 64: each member is bitwise the single-model FP64 path
+ * (nemo2d_create + nemo2d_set_tide on the same basin); precision 32: the
+ * single-precision form with FP64 ssh accumulation.  All members start at
+ * rest.  One persistent kernel advances every member by one step. */
+typedef struct nemo2d_ensemble nemo2d_ensemble;
+int nemo2d_ensemble_create(nemo2d_ensemble **e, int nmem, const int *jpi, const int *jpj,
+                           const double *dep, const double *amp, const double *period_h,
+                           double dx, double dy, double rdt, double cbfr, double visc,
+                           int precision);
+/* Back to rest. */
+int nemo2d_ensemble_reset(nemo2d_ensemble *e);
+/* Autotuner: time every candidate tiling over nsteps steps, keep the fastest
+ * (index in *choice, ms per step in *best_ms) and reset to rest. */
+int nemo2d_ensemble_tune(nemo2d_ensemble *e, int nsteps, int *choice, double *best_ms);
+int nemo2d_ensemble_set_tiling(nemo2d_ensemble *e, int choice);
+/* Work-item shape (columns x rows of cells) of tiling `choice`. */
+int nemo2d_ensemble_tiling(int choice, int *tx, int *ty);
+/* Steps nit000..nitend; use_graph != 0 captures the whole loop into a HIP
+ * graph first (capture not timed) and replays it.  *ms = device time of the
+ * steps.  Per-member, per-step checksums are logged. */
+int nemo2d_ensemble_run(nemo2d_ensemble *e, int nit000, int nitend, int use_graph, double *ms);
+int nemo2d_ensemble_checksum_log(nemo2d_ensemble *e, int member, double *out, int max_steps,
+                                 int *nsteps);
+/* sshn (T), un (U), vn (V) of one member, host layout. */
+int nemo2d_ensemble_retrieve(nemo2d_ensemble *e, int member, double *sshn, double *un, double *vn);
+int nemo2d_ensemble_destroy(nemo2d_ensemble *e);
+
 /* Free the model's device memory and stream.  Replaces cuda_finalise_
  * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */
 int nemo2d_destroy(nemo2d_model *m);

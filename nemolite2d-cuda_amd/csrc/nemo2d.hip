@@ -646,6 +646,8 @@ int checksum_into(nemo2d_model* m, int k, double* out) {
 
 namespace nemo2d {
 
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
 // Per-point classes of a mask (see nemo2d_core.h), the Flather coefficients
 // sqrt(g/h) and the Flather point lists in the serial model's loop order.
 // Rejects masks the serial model would index outside its arrays with.
@@ -846,8 +848,13 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         hipMemset(m->flags, 0, n * sizeof(uint16_t)) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
     dim3 g = grid2d(jpi + 1, jpj + 1);
-    // the thread-per-cell launch has the most blocks of all step kernels
-    m->npart_max = (int)(g.x * g.y);
+    // partial slots: the most blocks any step kernel launches (the
+    // thread-per-cell grid, or the 16-row marching grid padded to 8 XCDs)
+    {
+        const long mb = (long)((jpi + 1 + MW * MOUT - 1) / (MW * MOUT)) * ((jpj + 1 + 15) / 16);
+        const long mbp = (mb + 7) / 8 * 8;
+        m->npart_max = (int)((long)(g.x * g.y) > mbp ? (long)(g.x * g.y) : mbp);
+    }
     if ((rc = dalloc(&m->partials, 6 * ((size_t)m->npart_max + 1))) || (rc = dalloc(&m->cs_now, 3)))
         return bail(rc);
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)

@@ -1,0 +1,502 @@
+// Batched ensembles of NEMOLite2D basins of uneven size (no counterpart in
+// the reference).
+//
+// Every member is the serial model's basin (land ring, open south edge,
+// f-plane at 50 N, original/nemolite2d.f90:180-279) with its own size,
+// constant depth and tidal amplitude/period.  All members live in pooled
+// device arrays, each in its own padded layout.  One step of the whole
+// ensemble is one persistent kernel: a fixed grid of blocks walks a list of
+// (member, tile) work items; each cell uses the same per-cell body as the
+// single-model path, so an FP64 member is bitwise the same as that member
+// run alone.  Per-member checksums are reduced in a fixed order by a second
+// kernel.  A run of many steps is captured once into a HIP graph and
+// replayed.  The tile shape is chosen by timing the candidates
+// (nemo2d_ensemble_tune).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "nemo2d.h"
+#include "nemo2d_core.h"
+#include "nemo2d_internal.h"
+#include "nemo2d_step.h"
+
+using namespace nemo2d;
+
+namespace {
+
+#define EHIP(expr)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_error(NEMO2D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int EB = 256;  // threads per block of the ensemble step
+
+struct EnsMember {
+    Layout L;      // member layout; offsets relative to the member's base
+    long off;      // element offset of the member in the pooled arrays
+    Scal s;        // FP64 uniform statics
+    ScalF sf;      // FP32 uniform statics (with depths)
+    int tile0, ntiles;
+};
+
+// Candidate tilings: TX columns x (EB/TX)*ROWS rows per work item.
+struct Tiling {
+    int tx, rows;
+};
+constexpr Tiling TILINGS[] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 1}, {32, 2}};
+constexpr int NTILINGS = sizeof(TILINGS) / sizeof(TILINGS[0]);
+
+template <int TX, int ROWS>
+__global__ void __launch_bounds__(EB)
+k_ens_step64(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, int ntiles,
+             NState n0, AOut a0, const uint16_t* __restrict__ flp, const double* __restrict__ flup,
+             const double* __restrict__ flvp, Params p, const double* __restrict__ tide_tab,
+             int tide_stride, int step_idx, double* __restrict__ partials) {
+    constexpr int TY = EB / TX;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const EnsMember& M = mem[tl.x];
+        const long off = M.off;
+        const NState n{n0.sshn + off, n0.sshn_u + off, n0.sshn_v + off, n0.un + off, n0.vn + off};
+        const AOut a{a0.sshn + off, a0.sshn_u + off, a0.sshn_v + off, a0.un + off, a0.vn + off};
+        const StUniform st{M.s};
+        const Layout L = M.L;
+        const double tide = tide_tab[(long)tl.x * tide_stride + step_idx];
+        DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            cell_step<StUniform, true>(st, n, a, flp + off, flup + off, flvp + off, L, p, tide,
+                                       tl.y * TX + tx, (tl.z * ROWS + r) * TY + ty, ct, cu, cv);
+        block_reduce3(ct, cu, cv, partials + 6 * (long)t);
+        __syncthreads();  // the reduction's shared memory is reused by the next tile
+    }
+}
+
+template <int TX, int ROWS>
+__global__ void __launch_bounds__(EB)
+k_ens_step32(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, int ntiles,
+             MixedState<true> n0, MixedOut<true> a0, const uint16_t* __restrict__ flp,
+             const double* __restrict__ flup, const double* __restrict__ flvp, ParamsT<float> p,
+             const double* __restrict__ tide_tab, int tide_stride, int step_idx,
+             double* __restrict__ partials) {
+    constexpr int TY = EB / TX;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const EnsMember& M = mem[tl.x];
+        const long off = M.off;
+        const MixedState<true> n{n0.sshn + off, n0.sshn_u + off, n0.sshn_v + off, n0.un + off,
+                                 n0.vn + off};
+        const MixedOut<true> a{a0.sshn + off, a0.sshn_u + off, a0.sshn_v + off, a0.un + off,
+                               a0.vn + off};
+        const ScalF sf = M.sf;
+        const Layout L = M.L;
+        const double tide = tide_tab[(long)tl.x * tide_stride + step_idx];
+        DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            cell_step_mixed<true, true>(sf, n, a, nullptr, nullptr, nullptr, flp + off, flup + off,
+                                        flvp + off, L, p, tide, tl.y * TX + tx,
+                                        (tl.z * ROWS + r) * TY + ty, ct, cu, cv);
+        block_reduce3(ct, cu, cv, partials + 6 * (long)t);
+        __syncthreads();
+    }
+}
+
+// Per-member checksums: block m reduces its members' tile partials in order.
+__global__ void __launch_bounds__(256)
+k_ens_finish(const EnsMember* __restrict__ mem, const double* __restrict__ partials,
+             double* __restrict__ log, int cap, int slot) {
+    const EnsMember& M = mem[blockIdx.x];
+    finish_partials(partials + 6 * (long)M.tile0, M.ntiles, log + 3 * ((long)blockIdx.x * cap + slot));
+}
+
+}  // namespace
+
+struct nemo2d_ensemble {
+    int nmem = 0;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+    Params prm{};
+    std::vector<int> jpi, jpj;
+    std::vector<double> amp, omega;
+    std::vector<EnsMember> hmem;
+    EnsMember* dmem = nullptr;
+    long total = 0;  // pooled elements
+    void* X[2][5] = {};
+    int cur = 0;
+    uint16_t* fl = nullptr;
+    double *flu = nullptr, *flv = nullptr;
+    int tiling = 0;
+    int4* tiles = nullptr;
+    int ntiles = 0, ntiles_cap = 0;
+    double* partials = nullptr;
+    int grid = 0;
+    double* tide_tab = nullptr;
+    int tide_stride = 0;
+    double* log = nullptr;
+    int log_cap = 0, log_n = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int build_tiles(nemo2d_ensemble* e, int tiling) {
+    const Tiling T = TILINGS[tiling];
+    const int th = (EB / T.tx) * T.rows;
+    std::vector<int4> tl;
+    for (int k = 0; k < e->nmem; ++k) {
+        EnsMember& M = e->hmem[k];
+        const int nbx = (M.L.jpi + 1 + T.tx - 1) / T.tx, nby = (M.L.jpj + 1 + th - 1) / th;
+        M.tile0 = (int)tl.size();
+        M.ntiles = nbx * nby;
+        for (int by = 0; by < nby; ++by)
+            for (int bx = 0; bx < nbx; ++bx) tl.push_back(make_int4(k, bx, by, 0));
+    }
+    e->ntiles = (int)tl.size();
+    if (e->ntiles > e->ntiles_cap) {
+        if (e->tiles) (void)hipFree(e->tiles);
+        if (e->partials) (void)hipFree(e->partials);
+        e->tiles = nullptr;
+        e->partials = nullptr;
+        EHIP(hipMalloc((void**)&e->tiles, e->ntiles * sizeof(int4)));
+        EHIP(hipMalloc((void**)&e->partials, 6 * (size_t)e->ntiles * sizeof(double)));
+        e->ntiles_cap = e->ntiles;
+    }
+    EHIP(hipMemcpy(e->tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice));
+    EHIP(hipMemcpy(e->dmem, e->hmem.data(), e->hmem.size() * sizeof(EnsMember), hipMemcpyHostToDevice));
+    e->tiling = tiling;
+    return NEMO2D_OK;
+}
+
+template <int TX, int ROWS>
+void launch_step_t(nemo2d_ensemble* e, int step_idx) {
+    const int k = e->cur;
+    if (!e->fp32) {
+        double** x = (double**)e->X[k];
+        double** y = (double**)e->X[1 - k];
+        NState n{x[0], x[1], x[2], x[3], x[4]};
+        AOut a{y[0], y[1], y[2], y[3], y[4]};
+        k_ens_step64<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, n, a, e->fl,
+                                                              e->flu, e->flv, e->prm, e->tide_tab,
+                                                              e->tide_stride, step_idx, e->partials);
+    } else {
+        void** x = e->X[k];
+        void** y = e->X[1 - k];
+        MixedState<true> n{(const double*)x[0], (const float*)x[1], (const float*)x[2],
+                           (const float*)x[3], (const float*)x[4]};
+        MixedOut<true> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
+        ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
+                          (float)e->prm.fric};
+        k_ens_step32<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, n, a, e->fl,
+                                                              e->flu, e->flv, pf, e->tide_tab,
+                                                              e->tide_stride, step_idx, e->partials);
+    }
+}
+
+int launch_step(nemo2d_ensemble* e, int step_idx, int slot) {
+    switch (e->tiling) {
+        case 0: launch_step_t<64, 1>(e, step_idx); break;
+        case 1: launch_step_t<64, 2>(e, step_idx); break;
+        case 2: launch_step_t<64, 4>(e, step_idx); break;
+        case 3: launch_step_t<128, 1>(e, step_idx); break;
+        case 4: launch_step_t<128, 2>(e, step_idx); break;
+        case 5: launch_step_t<256, 1>(e, step_idx); break;
+        case 6: launch_step_t<32, 1>(e, step_idx); break;
+        default: launch_step_t<32, 2>(e, step_idx); break;
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble step: ") + hipGetErrorString(err));
+    if (slot >= 0) {
+        k_ens_finish<<<e->nmem, 256, 0, e->stream>>>(e->dmem, e->partials, e->log, e->log_cap, slot);
+        err = hipGetLastError();
+        if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble finish: ") + hipGetErrorString(err));
+    }
+    e->cur = 1 - e->cur;
+    return NEMO2D_OK;
+}
+
+// Tide values of every member for steps nit000..nitend (host C library).
+int upload_tides(nemo2d_ensemble* e, int nit000, int nsteps) {
+    std::vector<double> tab((size_t)e->nmem * nsteps);
+    for (int k = 0; k < e->nmem; ++k)
+        for (int s = 0; s < nsteps; ++s)
+            tab[(size_t)k * nsteps + s] = tide_eval(e->amp[k], e->omega[k], (double)(nit000 + s) * e->prm.rdt);
+    if (e->tide_tab) (void)hipFree(e->tide_tab);
+    e->tide_tab = nullptr;
+    EHIP(hipMalloc((void**)&e->tide_tab, tab.size() * sizeof(double)));
+    EHIP(hipMemcpy(e->tide_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    e->tide_stride = nsteps;
+    return NEMO2D_OK;
+}
+
+int ensure_log(nemo2d_ensemble* e, int nsteps) {
+    if (nsteps > e->log_cap) {
+        if (e->log) (void)hipFree(e->log);
+        e->log = nullptr;
+        EHIP(hipMalloc((void**)&e->log, 3 * (size_t)e->nmem * nsteps * sizeof(double)));
+        e->log_cap = nsteps;
+    }
+    EHIP(hipMemset(e->log, 0, 3 * (size_t)e->nmem * e->log_cap * sizeof(double)));
+    e->log_n = 0;
+    return NEMO2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nemo2d_ensemble_reset(nemo2d_ensemble* e) {
+    if (!e) return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_reset: null ensemble");
+    // rest state: every field zero (sshn = 0 makes sshn_u = sshn_v = 0)
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 5; ++f) {
+            const size_t es = (e->fp32 && f > 0) ? sizeof(float) : sizeof(double);
+            EHIP(hipMemsetAsync(e->X[k][f], 0, (size_t)e->total * es, e->stream));
+        }
+    EHIP(hipStreamSynchronize(e->stream));
+    e->cur = 0;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, const int* jpj,
+                           const double* dep, const double* amp, const double* period_h, double dx,
+                           double dy, double rdt, double cbfr, double visc, int precision) {
+    if (!out || nmem < 1 || !jpi || !jpj || !dep || !amp || !period_h)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_create: bad argument");
+    if (precision != 64 && precision != 32)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_create: precision must be 64 or 32");
+    if (!(rdt > 0.0) || !(dx > 0.0) || !(dy > 0.0))
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_create: dx, dy, rdt must be > 0");
+    *out = nullptr;
+    nemo2d_ensemble* e = new nemo2d_ensemble();
+    e->nmem = nmem;
+    e->fp32 = precision == 32;
+    e->prm = Params{rdt, cbfr, visc, 1.0 + cbfr * rdt};
+    auto bail = [&](int rc) {
+        nemo2d_ensemble_destroy(e);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: stream/event creation failed"));
+    // member layouts in the pools
+    long off = 0;
+    const double d2r = PI / 180.0;
+    const double fc = 2.0 * OMEGA * sin(50.0 * d2r);
+    for (int k = 0; k < nmem; ++k) {
+        if (jpi[k] < 1 || jpj[k] < 1 || !(dep[k] > 0.0) || !(period_h[k] > 0.0))
+            return bail(set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_create: bad member " + std::to_string(k)));
+        EnsMember M{};
+        M.L.jpi = jpi[k];
+        M.L.jpj = jpj[k];
+        M.L.pitch = ((long)COL0 + jpi[k] + 3 + 15) / 16 * 16;
+        M.L.rows = (long)jpj[k] + 4;
+        M.L.base = M.L.pitch + COL0;
+        M.off = off;
+        off += (M.L.size() + 63) / 64 * 64;
+        const double e12 = dx * dy;
+        M.s = Scal{dx, dy, dx, dy, dx, dy, e12, e12, e12, dep[k], dep[k], dep[k], fc, fc};
+        M.sf = ScalF{(float)dx, (float)dy, (float)dx, (float)dy, (float)dx, (float)dy, (float)e12,
+                     (float)e12, (float)e12, (float)fc, (float)fc, (float)dep[k], (float)dep[k],
+                     (float)dep[k]};
+        e->hmem.push_back(M);
+        e->jpi.push_back(jpi[k]);
+        e->jpj.push_back(jpj[k]);
+        e->amp.push_back(amp[k]);
+        e->omega.push_back(tide_omega_of(period_h[k]));
+    }
+    e->total = off;
+    if (hipMalloc((void**)&e->dmem, nmem * sizeof(EnsMember)) != hipSuccess)
+        return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: allocation failed"));
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 5; ++f) {
+            const size_t es = (e->fp32 && f > 0) ? sizeof(float) : sizeof(double);
+            if (hipMalloc(&e->X[k][f], (size_t)off * es) != hipSuccess)
+                return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: state allocation failed"));
+        }
+    if (hipMalloc((void**)&e->fl, (size_t)off * sizeof(uint16_t)) != hipSuccess ||
+        hipMalloc((void**)&e->flu, (size_t)off * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&e->flv, (size_t)off * sizeof(double)) != hipSuccess)
+        return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: allocation failed"));
+    // classes of each member's mask (the reference grid)
+    std::vector<uint16_t> hfl((size_t)off, 0);
+    std::vector<double> hflu((size_t)off, 0.0), hflv((size_t)off, 0.0);
+    for (int k = 0; k < nmem; ++k) {
+        const EnsMember& M = e->hmem[k];
+        const int ni = jpi[k], nj = jpj[k];
+        std::vector<int> pt((size_t)(ni + 2) * (nj + 2), 1);
+        for (int j = 0; j <= nj + 1; ++j) {
+            pt[(size_t)j * (ni + 2)] = 0;
+            pt[(size_t)j * (ni + 2) + ni + 1] = 0;
+        }
+        for (int i = 0; i <= ni + 1; ++i) pt[(size_t)(nj + 1) * (ni + 2) + i] = 0;
+        for (int i = 1; i <= ni; ++i) pt[i] = -1;
+        std::vector<double> hu((size_t)(ni + 1) * nj, dep[k]), hv((size_t)ni * (nj + 1), dep[k]);
+        Classes c;
+        std::string err;
+        int rc = classify(M.L, pt.data(), hu.data(), hv.data(), c, err);
+        if (rc) return bail(set_error(rc, "nemo2d_ensemble_create: member " + std::to_string(k) + ": " + err));
+        std::copy(c.fl.begin(), c.fl.end(), hfl.begin() + M.off);
+        std::copy(c.flu.begin(), c.flu.end(), hflu.begin() + M.off);
+        std::copy(c.flv.begin(), c.flv.end(), hflv.begin() + M.off);
+    }
+    if (hipMemcpy(e->fl, hfl.data(), hfl.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->flu, hflu.data(), hflu.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->flv, hflv.data(), hflv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: upload failed"));
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    e->grid = 4 * cus;
+    int rc;
+    if ((rc = build_tiles(e, 0)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
+    *out = e;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_set_tiling(nemo2d_ensemble* e, int choice) {
+    if (!e) return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_set_tiling: null ensemble");
+    if (choice < 0 || choice >= NTILINGS)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_set_tiling: choice out of range");
+    return build_tiles(e, choice);
+}
+
+int nemo2d_ensemble_tiling(int choice, int* tx, int* ty) {
+    if (choice < 0 || choice >= NTILINGS || !tx || !ty)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_tiling: bad argument");
+    *tx = TILINGS[choice].tx;
+    *ty = (EB / TILINGS[choice].tx) * TILINGS[choice].rows;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_run(nemo2d_ensemble* e, int nit000, int nitend, int use_graph, double* ms) {
+    if (!e || nitend < nit000) return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_run: bad argument");
+    const int nsteps = nitend - nit000 + 1;
+    int rc;
+    if ((rc = upload_tides(e, nit000, nsteps)) || (rc = ensure_log(e, nsteps))) return rc;
+    EHIP(hipDeviceSynchronize());
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph) {
+        const int cur0 = e->cur;
+        EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < nsteps && !rc; ++s) rc = launch_step(e, s, s);
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        e->cur = cur0;  // capture did not run anything
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ce != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        EHIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    EHIP(hipEventRecord(e->ev0, e->stream));
+    if (use_graph) {
+        hipError_t le = hipGraphLaunch(exec, e->stream);
+        if (le == hipSuccess && (nsteps & 1)) e->cur = 1 - e->cur;
+        if (le != hipSuccess) rc = set_error(NEMO2D_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(le));
+    } else {
+        for (int s = 0; s < nsteps && !rc; ++s) rc = launch_step(e, s, s);
+    }
+    EHIP(hipEventRecord(e->ev1, e->stream));
+    EHIP(hipEventSynchronize(e->ev1));
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    float t = 0.f;
+    EHIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    if (ms) *ms = (double)t;
+    e->log_n = nsteps;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_tune(nemo2d_ensemble* e, int nsteps, int* choice, double* best_ms) {
+    if (!e || nsteps < 1) return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_tune: bad argument");
+    int best = 0;
+    double bt = 1e300;
+    for (int c = 0; c < NTILINGS; ++c) {
+        int rc;
+        double t = 0.0;
+        if ((rc = build_tiles(e, c)) || (rc = nemo2d_ensemble_reset(e)) ||
+            (rc = nemo2d_ensemble_run(e, 1, 2, 0, nullptr)) ||   // warm-up
+            (rc = nemo2d_ensemble_run(e, 3, 2 + nsteps, 0, &t)))
+            return rc;
+        if (t < bt) {
+            bt = t;
+            best = c;
+        }
+    }
+    int rc;
+    if ((rc = build_tiles(e, best)) || (rc = nemo2d_ensemble_reset(e))) return rc;
+    if (choice) *choice = best;
+    if (best_ms) *best_ms = bt / nsteps;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_checksum_log(nemo2d_ensemble* e, int member, double* out, int max_steps,
+                                 int* nsteps) {
+    if (!e || member < 0 || member >= e->nmem || (!out && max_steps > 0))
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_checksum_log: bad argument");
+    EHIP(hipStreamSynchronize(e->stream));
+    const int n = std::min(e->log_n, max_steps);
+    if (n > 0)
+        EHIP(hipMemcpy(out, e->log + 3 * (size_t)member * e->log_cap, 3 * (size_t)n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    if (nsteps) *nsteps = e->log_n;
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_retrieve(nemo2d_ensemble* e, int member, double* sshn, double* un, double* vn) {
+    if (!e || member < 0 || member >= e->nmem)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_retrieve: bad argument");
+    EHIP(hipStreamSynchronize(e->stream));
+    const EnsMember& M = e->hmem[member];
+    const Layout& L = M.L;
+    const int ni = L.jpi, nj = L.jpj;
+    struct {
+        double* dst;
+        int f, lb1, ub1, lb2, ub2;
+    } w[3] = {{sshn, 0, 1, ni, 1, nj}, {un, 3, 0, ni, 1, nj}, {vn, 4, 1, ni, 0, nj}};
+    for (auto& q : w) {
+        if (!q.dst) continue;
+        const long n1 = q.ub1 - q.lb1 + 1, n2 = q.ub2 - q.lb2 + 1;
+        const bool dbl = !e->fp32 || q.f == 0;
+        const size_t es = dbl ? sizeof(double) : sizeof(float);
+        std::vector<char> buf((size_t)n1 * n2 * es);
+        const char* src = (const char*)e->X[e->cur][q.f] + (M.off + L.at(q.lb1, q.lb2)) * es;
+        EHIP(hipMemcpy2D(buf.data(), n1 * es, src, L.pitch * es, n1 * es, n2, hipMemcpyDeviceToHost));
+        for (long k = 0; k < n1 * n2; ++k)
+            q.dst[k] = dbl ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k];
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_ensemble_destroy(nemo2d_ensemble* e) {
+    if (!e) return NEMO2D_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 5; ++f)
+            if (e->X[k][f]) (void)hipFree(e->X[k][f]);
+    void* ptrs[] = {e->dmem, e->fl, e->flu, e->flv, e->tiles, e->partials, e->tide_tab, e->log};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return NEMO2D_OK;
+}
+
+}  // extern "C"

@@ -24,6 +24,9 @@ struct Classes {
 inline double tide_omega_of(double period_hours) { return 2.0 * 3.14159 / (period_hours * 3600.0); }
 inline double tide_eval(double amp, double omega, double rtime) { return amp * sin(omega * rtime); }
 
+// Record the message returned by nemo2d_last_error(); returns code.
+int set_error(int code, const std::string& msg);
+
 int classify(const Layout& L, const int* pt, const double* hu, const double* hv, Classes& c,
              std::string& err);
 

@@ -29,7 +29,10 @@ SYMBOLS = (
     "nemo2d_record_checksums", "nemo2d_checksum_log", "nemo2d_checksums",
     "nemo2d_retrieve_results", "nemo2d_retrieve_aux", "nemo2d_timer_start",
     "nemo2d_timer_stop", "nemo2d_destroy", "nemo2d_create_mixed", "nemo2d_diagnostics",
-    "nemo2d_set_variant", "nemo2d_set_tide",
+    "nemo2d_set_variant", "nemo2d_set_tide", "nemo2d_ensemble_create", "nemo2d_ensemble_reset",
+    "nemo2d_ensemble_tune", "nemo2d_ensemble_set_tiling", "nemo2d_ensemble_tiling",
+    "nemo2d_ensemble_run", "nemo2d_ensemble_checksum_log", "nemo2d_ensemble_retrieve",
+    "nemo2d_ensemble_destroy",
 )
 
 # nemo2d_set_variant choices
@@ -70,6 +73,15 @@ def lib():
         L.nemo2d_step.argtypes = [vp, i]
         L.nemo2d_set_variant.argtypes = [vp, i]
         L.nemo2d_set_tide.argtypes = [vp, d, d]
+        L.nemo2d_ensemble_create.argtypes = [ctypes.POINTER(vp), i, ip, ip, dp, dp, dp, d, d, d, d, d, i]
+        L.nemo2d_ensemble_reset.argtypes = [vp]
+        L.nemo2d_ensemble_tune.argtypes = [vp, i, ip, dp]
+        L.nemo2d_ensemble_set_tiling.argtypes = [vp, i]
+        L.nemo2d_ensemble_tiling.argtypes = [i, ip, ip]
+        L.nemo2d_ensemble_run.argtypes = [vp, i, i, i, dp]
+        L.nemo2d_ensemble_checksum_log.argtypes = [vp, i, dp, i, ip]
+        L.nemo2d_ensemble_retrieve.argtypes = [vp, i, dp, dp, dp]
+        L.nemo2d_ensemble_destroy.argtypes = [vp]
         L.nemo2d_run.argtypes = [vp, i, i]
         L.nemo2d_record_checksums.argtypes = [vp, i]
         L.nemo2d_checksum_log.argtypes = [vp, dp, i, ip]
@@ -348,3 +360,90 @@ def field_errors(a, b):
         out[k] = {"rms": float(np.sqrt(np.mean(d * d))), "max": float(np.max(np.abs(d))),
                   "ref_rms": float(np.sqrt(np.mean(b[key][sl] ** 2)))}
     return out
+
+
+def ensemble_members(n, seed, size=(64, 1024), depth=(50.0, 200.0), amp=(0.1, 0.3),
+                     period=(11.0, 13.0)):
+    """Seeded ensemble: square basins of side round(exp(U(ln a, ln b))),
+    constant depth U(depth) m, tide amplitude U(amp) m and period U(period) h."""
+    rng = np.random.default_rng(seed)
+    side = np.rint(np.exp(rng.uniform(np.log(size[0]), np.log(size[1]), n))).astype(np.int32)
+    return {"jpi": side, "jpj": side.copy(), "dep": rng.uniform(*depth, n),
+            "amp": rng.uniform(*amp, n), "period": rng.uniform(*period, n)}
+
+
+class Ensemble:
+    """Batched basins advanced by one persistent kernel per step."""
+
+    def __init__(self, members, dx=1000.0, dy=1000.0, rdt=12.0, cbfr=0.0015, visc=0.1,
+                 precision=64):
+        self.members = {k: np.ascontiguousarray(v) for k, v in members.items()}
+        self.n = len(self.members["jpi"])
+        self.rdt = rdt
+        m = self.members
+        jpi = np.ascontiguousarray(m["jpi"], dtype=np.int32)
+        jpj = np.ascontiguousarray(m["jpj"], dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        h = ctypes.c_void_p()
+        _check(lib().nemo2d_ensemble_create(
+            ctypes.byref(h), self.n, jpi.ctypes.data_as(ip), jpj.ctypes.data_as(ip),
+            _dp(np.ascontiguousarray(m["dep"], dtype=np.float64)),
+            _dp(np.ascontiguousarray(m["amp"], dtype=np.float64)),
+            _dp(np.ascontiguousarray(m["period"], dtype=np.float64)),
+            dx, dy, rdt, cbfr, visc, int(precision)))
+        self._h = h
+        self.cells = int(np.sum(jpi.astype(np.int64) * jpj))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nemo2d_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def reset(self):
+        _check(lib().nemo2d_ensemble_reset(self._h))
+
+    def tune(self, nsteps=20):
+        c = ctypes.c_int(0)
+        t = ctypes.c_double(0.0)
+        _check(lib().nemo2d_ensemble_tune(self._h, int(nsteps), ctypes.byref(c), ctypes.byref(t)))
+        return c.value, t.value
+
+    def set_tiling(self, choice):
+        _check(lib().nemo2d_ensemble_set_tiling(self._h, int(choice)))
+
+    @staticmethod
+    def tiling_shape(choice):
+        tx, ty = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().nemo2d_ensemble_tiling(int(choice), ctypes.byref(tx), ctypes.byref(ty)))
+        return tx.value, ty.value
+
+    def run(self, nit000, nitend, graph=True):
+        """Returns the device milliseconds of the steps."""
+        ms = ctypes.c_double(0.0)
+        _check(lib().nemo2d_ensemble_run(self._h, int(nit000), int(nitend), 1 if graph else 0,
+                                         ctypes.byref(ms)))
+        return ms.value
+
+    def checksum_log(self, member, max_steps):
+        out = np.zeros((max(int(max_steps), 1), 3))
+        n = ctypes.c_int(0)
+        _check(lib().nemo2d_ensemble_checksum_log(self._h, int(member), _dp(out), int(max_steps),
+                                                  ctypes.byref(n)))
+        return out[:min(n.value, int(max_steps))]
+
+    def retrieve(self, member):
+        ni, nj = int(self.members["jpi"][member]), int(self.members["jpj"][member])
+        s = shapes(ni, nj)
+        r = {"sshn": np.zeros(s["T"]), "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
+        _check(lib().nemo2d_ensemble_retrieve(self._h, int(member), _dp(r["sshn"]), _dp(r["un"]),
+                                              _dp(r["vn"])))
+        return r
