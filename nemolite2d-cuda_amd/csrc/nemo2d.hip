@@ -155,6 +155,7 @@ k_step_mixed(ScalF s, MixedState<COMP> n, MixedOut<COMP> a, const float* __restr
 constexpr int MW = 4;     // waves per block (side by side in i)
 constexpr int MOUT = 63;  // output columns per wave
 
+template <bool FAST>
 struct RegWin {
     using real = double;
     Scal s;
@@ -170,6 +171,10 @@ struct RegWin {
     NEMO2D_REGS(e2v) NEMO2D_REGS(e12t) NEMO2D_REGS(e12u) NEMO2D_REGS(e12v) NEMO2D_REGS(ht)
     NEMO2D_REGS(hu) NEMO2D_REGS(hv) NEMO2D_REGS(fcu) NEMO2D_REGS(fcv)
 #undef NEMO2D_REGS
+    template <int TAG> __device__ double qd(double x, double d) const {
+        if constexpr (FAST) return div_rcp(x, d, s.rcp[TAG]);
+        else return x / d;
+    }
 
     // rows move down one; row 2 is then reloaded
     __device__ __forceinline__ void shift() {
@@ -203,7 +208,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int NJ, bool FLIN, int WPE>
+template <int NJ, bool FLIN, int WPE, bool FAST>
 __global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
@@ -211,7 +216,9 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const long P = L.pitch;
-    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    // per-lane sums of at most NJ terms in plain double (relative error
+    // below NJ*2^-53), merged across lanes and blocks in double-double
+    double ct_ = 0.0, cu_ = 0.0, cv_ = 0.0;
     if (t < ntiles) {
         const int bx = t % nbx, by = t / nbx;
         const int i = (bx * MW + wave) * MOUT + lane;
@@ -221,7 +228,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         const int jb = by * NJ;
         const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
         StUniform st{sc_};
-        RegWin w;
+        RegWin<FAST> w;
         w.s = sc_;
         w.load(1, n, L.at(il - 1, jb - 1));
         w.load(2, n, L.at(il - 1, jb));
@@ -240,7 +247,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 if (j >= 1) {
                     if (i >= 1) {
                         a.sshn[o] = s_c;
-                        dd_add(ct, fabs(s_c));
+                        ct_ += fabs(s_c);
                     }
                     const unsigned c = ucls(f);
                     if (c != CLS_FLATHER || FLIN) {
@@ -255,7 +262,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                             u = ui + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
                         }
                         a.un[o] = u;
-                        if (i >= 1) dd_add(cu, fabs(u));
+                        if (i >= 1) cu_ += fabs(u);
                     }
                     a.sshn_u[o] = next_su(st, sucls(f), o, s_c, s_e, w.su_[1][1]);
                 }
@@ -273,7 +280,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                             v = vi + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
                         }
                         a.vn[o] = v;
-                        if (j >= 1) dd_add(cv, fabs(v));
+                        if (j >= 1) cv_ += fabs(v);
                     }
                     if (j > jb) a.sshn_v[o - P] = next_sv(st, svcls(f_prev), o - P, P, s_prev, s_c, sv_prev);
                 }
@@ -294,7 +301,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
             a.sshn_v[o] = next_sv(st, c, o, P, s_prev, s_n, sv_prev);
         }
     }
-    block_reduce3(ct, cu, cv, partials + 6 * (long)blockIdx.x);
+    block_reduce3(DD{ct_, 0.0}, DD{cu_, 0.0}, DD{cv_, 0.0}, partials + 6 * (long)blockIdx.x);
 }
 
 // sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of a state of element
@@ -648,6 +655,21 @@ namespace nemo2d {
 
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
+// Reciprocals of the uniform divisors (formed as the point functions form
+// them) for the division-by-reciprocal path; off unless every divisor is in
+// the range where that path is exact.
+void set_reciprocals(Scal& s, double fric) {
+    const double d[QD_N] = {s.e12t, s.e1t, s.e2t, s.e2u + s.e2u, s.e1v + s.e1v, s.e12u, s.e12v,
+                            s.hu + 0.0, s.hv + 0.0, fric};
+    bool ok = true;
+    for (int k = 0; k < QD_N; ++k) {
+        ok = ok && d[k] >= 0x1p-60 && d[k] <= 0x1p60;
+        s.rcp[k] = 1.0 / d[k];
+    }
+    s.fast_div = ok ? 1 : 0;
+}
+
+
 // Per-point classes of a mask (see nemo2d_core.h), the Flather coefficients
 // sqrt(g/h) and the Flather point lists in the serial model's loop order.
 // Rejects masks the serial model would index outside its arrays with.
@@ -912,7 +934,8 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     for (int s = 0; s < NS; ++s) uni = uni && all_equal(sd[s].a, (size_t)sd[s].n);
     m->uniform = uni;
     m->scal = Scal{e1t[0], e2t[0], e1u[0], e2u[0], e1v[0], e2v[0], e12t[0], e12u[0], e12v[0],
-                   ht[0], hu[0], hv[0], fcu[0], fcv[0]};
+                   ht[0], hu[0], hv[0], fcu[0], fcv[0], {}, 0};
+    set_reciprocals(m->scal, m->prm.fric);
     const size_t n = (size_t)L.size();
     for (int s = 0; s < NS; ++s) {
         if (!m->st[s]) {
@@ -1197,10 +1220,12 @@ enum {
     VAR_MARCH32_W4 = 5, // 32-row segments, register budget for 4 waves/SIMD
     VAR_MARCH32_FLIN = 6,  // 32-row segments, Flather points recomputed inline
     VAR_CELL_FLIN = 7,  // one thread per cell, Flather inline
-    VAR_LAST = 7
+    VAR_MARCH16_FD = 8,  // marching, 16-row segments, division by reciprocals
+    VAR_MARCH32_FD = 9,  //                32-row segments
+    VAR_LAST = 9
 };
 
-template <int NJ, bool FLIN, int WPE>
+template <int NJ, bool FLIN, int WPE, bool FAST = false>
 static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
@@ -1208,7 +1233,7 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march<NJ, FLIN, WPE><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march<NJ, FLIN, WPE, FAST><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
     LAUNCHCHK("step_march");
     nparts = nblocks;
@@ -1250,6 +1275,14 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         case VAR_MARCH64: rc = launch_march<64, false, 0>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_W4: rc = launch_march<32, false, 4>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_FLIN: rc = launch_march<32, true, 0>(m, n, a, tide, nparts); flin = true; break;
+        case VAR_MARCH16_FD:
+            rc = m->scal.fast_div ? launch_march<16, false, 0, true>(m, n, a, tide, nparts)
+                                  : launch_march<16, false, 0>(m, n, a, tide, nparts);
+            break;
+        case VAR_MARCH32_FD:
+            rc = m->scal.fast_div ? launch_march<32, false, 0, true>(m, n, a, tide, nparts)
+                                  : launch_march<32, false, 0>(m, n, a, tide, nparts);
+            break;
         default: rc = launch_march<32, false, 0>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;

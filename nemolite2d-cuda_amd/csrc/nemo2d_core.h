@@ -62,10 +62,18 @@ struct ParamsT {
 };
 using Params = ParamsT<double>;
 
+// Divisor tags of the point functions: a window with uniform statics divides
+// by precomputed reciprocals, any other window divides.
+enum {
+    QD_E12T = 0, QD_E1T, QD_E2T, QD_E2U2, QD_E1V2, QD_E12U, QD_E12V, QD_HU, QD_HV, QD_FRIC, QD_N
+};
+
 // Scalar statics, used when every element of a metric/depth/Coriolis array
 // holds the same value (the serial model's jphgr_msh=1 grid).
 struct Scal {
     double e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, ht, hu, hv, fcu, fcv;
+    double rcp[QD_N];  // RN(1/divisor) for each QD_ tag (uniform statics)
+    int fast_div;      // divisors in range: divide by reciprocals
 };
 
 struct StUniform {
@@ -133,14 +141,37 @@ struct MemWin {
     NEMO2D_MEMS(e2v) NEMO2D_MEMS(e12t) NEMO2D_MEMS(e12u) NEMO2D_MEMS(e12v) NEMO2D_MEMS(ht)
     NEMO2D_MEMS(hu) NEMO2D_MEMS(hv) NEMO2D_MEMS(fcu) NEMO2D_MEMS(fcv)
 #undef NEMO2D_MEMS
+    template <int TAG> __device__ double qd(double x, double d) const { return x / d; }
 };
 
 #define W_(F, DI, DJ) w.template F<DI, DJ>()
+#define QD_(TAG, X, D) w.template qd<TAG>((X), (D))
 
 // Division used by the point functions: IEEE for double (bitwise parity);
 // the single-precision form uses the fast reciprocal-based quotient.
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
 __device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b); }
+
+// Correctly rounded x / d for a divisor d > 0 in [2^-60, 2^60] with y =
+// RN(1/d) precomputed: q0 = RN(x*y) and two FMA remainder corrections
+// (Markstein: y within half an ulp of 1/d and a faithful quotient make
+// RN(q + (x - d*q)*y) the correctly rounded quotient; the first correction
+// makes the quotient faithful, the remainder x - d*q is exact by FMA while
+// nothing underflows).  Zeros keep their sign through x*y; |x| outside
+// [2^-960, 2^960] (where the remainder could underflow) and non-finite x take
+// the IEEE division.  Same bits as x / d for every x.
+__device__ __forceinline__ double div_rcp(double x, double d, double y) {
+    double q = x * y;
+    double r = fma(-d, q, x);
+    q = fma(r, y, q);
+    r = fma(-d, q, x);
+    const double q2 = fma(r, y, q);
+    const double ax = fabs(x);
+    double res = ax == 0.0 ? x * y : q2;
+    if (__builtin_expect(!(ax >= 0x1p-960 && ax <= 0x1p960) && ax != 0.0, 0)) res = x / d;
+    return res;
+}
+
 
 // Upwind weight SIGN(0.5, x) with Fortran's signed-zero rule.
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
@@ -154,7 +185,7 @@ __device__ __forceinline__ T continuity_incr_w(const W& w, T rdt) {
     const T r2 = (W_(sshn_u, -1, 0) + W_(hu, -1, 0)) * W_(un, -1, 0);
     const T r3 = (W_(sshn_v, 0, 0) + W_(hv, 0, 0)) * W_(vn, 0, 0);
     const T r4 = (W_(sshn_v, 0, -1) + W_(hv, 0, -1)) * W_(vn, 0, -1);
-    return dv((r2 - r1 + r4 - r3) * rdt, W_(e12t, 0, 0));
+    return QD_(QD_E12T, (r2 - r1 + r4 - r3) * rdt, W_(e12t, 0, 0));
 }
 template <class W>
 __device__ __forceinline__ double continuity_w(const W& w, double rdt) {
@@ -187,15 +218,15 @@ __device__ __forceinline__ T momentum_u_w(const W& w, unsigned f, const ParamsT<
     if (!wn) uu_n = uu_n + (h - sgn_half(v_n)) * un_n;
     const T adv = uu_w * u_w * depw - uu_e * u_e * depe + uu_s * v_s * deps - uu_n * v_n * depn;
 
-    const T dudx_e = dv(un_e - un_c, W_(e1t, 1, 0)) * (W_(ht, 1, 0) + W_(sshn, 1, 0));
-    const T dudx_w = dv(un_c - un_w, W_(e1t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const T dudx_e = QD_(QD_E1T, un_e - un_c, W_(e1t, 1, 0)) * (W_(ht, 1, 0) + W_(sshn, 1, 0));
+    const T dudx_w = QD_(QD_E1T, un_c - un_w, W_(e1t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
     const T hsu = W_(hu, 0, 0) + W_(sshn_u, 0, 0);
     T dudy_s = T(0.0), dudy_n = T(0.0);
     if (!ws)
-        dudy_s = dv(un_c - un_s, W_(e2u, 0, 0) + W_(e2u, 0, -1)) *
+        dudy_s = QD_(QD_E2U2, un_c - un_s, W_(e2u, 0, 0) + W_(e2u, 0, -1)) *
                  (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, -1) + W_(sshn_u, 0, -1));
     if (!wn)
-        dudy_n = dv(un_n - un_c, W_(e2u, 0, 0) + W_(e2u, 0, 1)) *
+        dudy_n = QD_(QD_E2U2, un_n - un_c, W_(e2u, 0, 0) + W_(e2u, 0, 1)) *
                  (W_(hu, 0, 0) + W_(sshn_u, 0, 0) + W_(hu, 0, 1) + W_(sshn_u, 0, 1));
     T vis = (dudx_e - dudx_w) * W_(e2u, 0, 0) + (dudy_n - dudy_s) * W_(e1u, 0, 0) * h;
     vis = p.visc * vis;
@@ -203,9 +234,10 @@ __device__ __forceinline__ T momentum_u_w(const W& w, unsigned f, const ParamsT<
     const T cor = h * (W_(fcu, 0, 0) * (v_sc + v_nc)) * W_(e12u, 0, 0) * hsu;
     const T hpg = -T(GRAV) * hsu * W_(e2u, 0, 0) * (W_(sshn, 1, 0) - W_(sshn, 0, 0));
     // ssha_u is never written by the serial model (:302, :315): it is 0.
-    return dv(dv(un_c * hsu + dv(p.rdt * (adv + vis + cor + hpg), W_(e12u, 0, 0)),
-                 W_(hu, 0, 0) + T(0.0)),
-              p.fric);
+    return QD_(QD_FRIC,
+               QD_(QD_HU, un_c * hsu + QD_(QD_E12U, p.rdt * (adv + vis + cor + hpg), W_(e12u, 0, 0)),
+                   W_(hu, 0, 0) + T(0.0)),
+               p.fric);
 }
 
 // momentum_v at V point (i, j) with flags f, original/nemolite2d.f90:527-606.
@@ -234,24 +266,25 @@ __device__ __forceinline__ T momentum_v_w(const W& w, unsigned f, const ParamsT<
     if (!we) vv_e = vv_e + (h - sgn_half(u_e)) * vn_e;
     const T adv = vv_w * u_w * depw - vv_e * u_e * depe + vv_s * v_s * deps - vv_n * v_n * depn;
 
-    const T dvdy_n = dv(vn_n - vn_c, W_(e2t, 0, 1)) * (W_(ht, 0, 1) + W_(sshn, 0, 1));
-    const T dvdy_s = dv(vn_c - vn_s, W_(e2t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
+    const T dvdy_n = QD_(QD_E2T, vn_n - vn_c, W_(e2t, 0, 1)) * (W_(ht, 0, 1) + W_(sshn, 0, 1));
+    const T dvdy_s = QD_(QD_E2T, vn_c - vn_s, W_(e2t, 0, 0)) * (W_(ht, 0, 0) + W_(sshn, 0, 0));
     const T hsv = W_(hv, 0, 0) + W_(sshn_v, 0, 0);
     T dvdx_w = T(0.0), dvdx_e = T(0.0);
     if (!ww)
-        dvdx_w = dv(vn_c - vn_w, W_(e1v, 0, 0) + W_(e1v, -1, 0)) *
+        dvdx_w = QD_(QD_E1V2, vn_c - vn_w, W_(e1v, 0, 0) + W_(e1v, -1, 0)) *
                  (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, -1, 0) + W_(sshn_v, -1, 0));
     if (!we)
-        dvdx_e = dv(vn_e - vn_c, W_(e1v, 0, 0) + W_(e1v, 1, 0)) *
+        dvdx_e = QD_(QD_E1V2, vn_e - vn_c, W_(e1v, 0, 0) + W_(e1v, 1, 0)) *
                  (W_(hv, 0, 0) + W_(sshn_v, 0, 0) + W_(hv, 1, 0) + W_(sshn_v, 1, 0));
     T vis = (dvdy_n - dvdy_s) * W_(e1v, 0, 0) + (dvdx_e - dvdx_w) * W_(e2v, 0, 0) * h;
     vis = p.visc * vis;
 
     const T cor = -h * (W_(fcv, 0, 0) * (u_ec + u_wc)) * W_(e12v, 0, 0) * hsv;
     const T hpg = -T(GRAV) * hsv * W_(e1v, 0, 0) * (W_(sshn, 0, 1) - W_(sshn, 0, 0));
-    return dv(dv(vn_c * hsv + dv(p.rdt * (adv + vis + cor + hpg), W_(e12v, 0, 0)),
-                 W_(hv, 0, 0) + T(0.0)),
-              p.fric);
+    return QD_(QD_FRIC,
+               QD_(QD_HV, vn_c * hsv + QD_(QD_E12V, p.rdt * (adv + vis + cor + hpg), W_(e12v, 0, 0)),
+                   W_(hv, 0, 0) + T(0.0)),
+               p.fric);
 }
 
 // Memory-window wrappers at offset o.

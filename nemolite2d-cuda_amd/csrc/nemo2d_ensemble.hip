@@ -305,7 +305,8 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         M.off = off;
         off += (M.L.size() + 63) / 64 * 64;
         const double e12 = dx * dy;
-        M.s = Scal{dx, dy, dx, dy, dx, dy, e12, e12, e12, dep[k], dep[k], dep[k], fc, fc};
+        M.s = Scal{dx, dy, dx, dy, dx, dy, e12, e12, e12, dep[k], dep[k], dep[k], fc, fc, {}, 0};
+        set_reciprocals(M.s, e->prm.fric);
         M.sf = ScalF{(float)dx, (float)dy, (float)dx, (float)dy, (float)dx, (float)dy, (float)e12,
                      (float)e12, (float)e12, (float)fc, (float)fc, (float)dep[k], (float)dep[k],
                      (float)dep[k]};

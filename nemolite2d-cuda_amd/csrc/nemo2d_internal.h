@@ -24,6 +24,9 @@ struct Classes {
 inline double tide_omega_of(double period_hours) { return 2.0 * 3.14159 / (period_hours * 3600.0); }
 inline double tide_eval(double amp, double omega, double rtime) { return amp * sin(omega * rtime); }
 
+// Reciprocals of a uniform grid's divisors (see nemo2d_core.h div_rcp).
+void set_reciprocals(Scal& s, double fric);
+
 // Record the message returned by nemo2d_last_error(); returns code.
 int set_error(int code, const std::string& msg);
 

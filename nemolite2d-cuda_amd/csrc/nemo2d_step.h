@@ -152,6 +152,7 @@ struct MemWinF {
     NEMO2D_SF(e1t) NEMO2D_SF(e2t) NEMO2D_SF(e1u) NEMO2D_SF(e2u) NEMO2D_SF(e1v) NEMO2D_SF(e2v)
     NEMO2D_SF(e12t) NEMO2D_SF(e12u) NEMO2D_SF(e12v) NEMO2D_SF(fcu) NEMO2D_SF(fcv)
 #undef NEMO2D_SF
+    template <int TAG> __device__ float qd(float x, float d) const { return dv(x, d); }
 };
 
 template <bool COMP, bool UNI>

@@ -41,7 +41,7 @@ def test_fp64_members_bitwise_equal_single_model_and_oracle():
             for f in ("sshn", "un", "vn"):
                 assert np.array_equal(r[f], ref[f]), (k, f)
             log = e.checksum_log(k, nsteps)
-            assert np.allclose(log, log_ref, rtol=1e-15, atol=0), k
+            assert np.allclose(log, log_ref, rtol=1e-13, atol=0), k
         # two members against the oracle directly
         for k in (1, 4):
             o = Oracle(int(m["jpi"][k]), int(m["jpj"][k]), dep_const=float(m["dep"][k]), rdt=RDT)
