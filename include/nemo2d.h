@@ -206,6 +206,34 @@ int nemo2d_ensemble_checksum_log(nemo2d_ensemble *e, int member, double *out, in
 int nemo2d_ensemble_retrieve(nemo2d_ensemble *e, int member, double *sshn, double *un, double *vn);
 int nemo2d_ensemble_destroy(nemo2d_ensemble *e);
 
+/* --- learned-closure network kernels (no counterpart in the reference) ---
+ * Device pointers, NHWC activations with the channel stride rounded up to
+ * 8 (BF16 stored as 16-bit integers), 1 <= channels <= 32, stream a
+ * hipStream_t (NULL = default stream).  Convolutions are 'same'-padded 3x3,
+ * implicit-GEMM on BF16 matrix cores with FP32 accumulation.
+ *   wf: forward weights [9][32][pad8(cin)], wb: data-gradient weights
+ *   [9][32][pad8(cout)] (both from nemo2d_nn_pack_weights of FP32
+ *   W[9][cin][cout]). */
+int nemo2d_nn_conv_fwd(const void *x, int n, int h, int w, int cin, const void *wf,
+                       const float *bias, int cout, float *z, void *act, int use_gelu,
+                       void *stream);
+int nemo2d_nn_conv_bwd_data(const void *dz, int n, int h, int w, int cout, const void *wb,
+                            int cin, float *dx, void *stream);
+/* Work buffer size (floats) for nemo2d_nn_conv_bwd_weight over `pixels`. */
+long nemo2d_nn_wgrad_work_floats(long pixels);
+int nemo2d_nn_conv_bwd_weight(const void *x, const void *dz, int n, int h, int w, int cin,
+                              int cout, float *dw, float *db, float *work, void *stream);
+/* dz = dact * GELU'(z), BF16. */
+int nemo2d_nn_gelu_bwd(const float *dact, const float *z, long count, void *dz, void *stream);
+/* Weighted MSE over channels < ch: gradient dy (BF16) and per-block loss
+ * parts (nblocks floats, to be summed). */
+int nemo2d_nn_mse(const float *y, const float *t, long pixels, int ch, const float *wch,
+                  void *dy, float *part, int nblocks, void *stream);
+int nemo2d_nn_adam(float *p, const float *g, float *m, float *v, long n, float lr, float b1,
+                   float b2, float eps, int step, void *stream);
+int nemo2d_nn_pack_weights(const float *w, int cin, int cout, void *wf, void *wb, void *stream);
+int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
+
 /* Free the model's device memory and stream.  Replaces cuda_finalise_
  * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */
 int nemo2d_destroy(nemo2d_model *m);

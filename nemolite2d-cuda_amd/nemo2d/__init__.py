@@ -32,7 +32,9 @@ SYMBOLS = (
     "nemo2d_set_variant", "nemo2d_set_tide", "nemo2d_ensemble_create", "nemo2d_ensemble_reset",
     "nemo2d_ensemble_tune", "nemo2d_ensemble_set_tiling", "nemo2d_ensemble_tiling",
     "nemo2d_ensemble_run", "nemo2d_ensemble_checksum_log", "nemo2d_ensemble_retrieve",
-    "nemo2d_ensemble_destroy",
+    "nemo2d_ensemble_destroy", "nemo2d_nn_conv_fwd", "nemo2d_nn_conv_bwd_data",
+    "nemo2d_nn_wgrad_work_floats", "nemo2d_nn_conv_bwd_weight", "nemo2d_nn_gelu_bwd",
+    "nemo2d_nn_mse", "nemo2d_nn_adam", "nemo2d_nn_pack_weights", "nemo2d_nn_to_bf16",
 )
 
 # nemo2d_set_variant choices
@@ -90,7 +92,7 @@ def lib():
         L.nemo2d_retrieve_aux.argtypes = [vp] + [dp] * 3
         L.nemo2d_timer_stop.argtypes = [vp, dp]
         for s in SYMBOLS:
-            if s not in ("nemo2d_last_error", "nemo2d_version"):
+            if s not in ("nemo2d_last_error", "nemo2d_version") and not s.startswith("nemo2d_nn_"):
                 getattr(L, s).restype = i
         _lib = L
     return _lib

@@ -1,0 +1,96 @@
+"""The closure network's GPU kernels (BF16 MFMA implicit-GEMM convolutions,
+GELU, MSE, Adam) against the FP64 numpy reference oracle/closure_ref.py."""
+import numpy as np
+import pytest
+
+import closure_ref as cr
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round FP64 values to BF16 (nearest even), back to FP64."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def setup(seed, n=2, h=11, w=13):
+    import torch
+    from nemo2d.closure import ConvNet, LAYERS
+    rng = np.random.default_rng(seed)
+    net = ConvNet(seed=seed)
+    ws = [0.5 * rng.standard_normal((9, ci, co)) / np.sqrt(9 * ci) for ci, co, _ in LAYERS]
+    bs = [0.1 * rng.standard_normal(co) for _, co, _ in LAYERS]
+    net.set_weights(ws, bs)
+    x = rng.standard_normal((n, h, w, 4))
+    t = rng.standard_normal((n, h, w, 3)) * 0.1
+    xp = np.zeros((n * h * w, 8), dtype=np.float32)
+    xp[:, :4] = x.reshape(-1, 4)
+    tp = np.zeros((n * h * w, 8), dtype=np.float32)
+    tp[:, :3] = t.reshape(-1, 3)
+    xg = net.to_bf16(torch.tensor(xp, device="cuda"))
+    tg = torch.tensor(tp, device="cuda")
+    # the reference sees the BF16-rounded operands the GPU sees
+    params = [(bf16(w_), b_.astype(np.float32).astype(np.float64)) for w_, b_ in zip(ws, bs)]
+    return net, params, bf16(x), t, xg, tg, (n, h, w)
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_matches_reference():
+    net, params, x, t, xg, tg, (n, h, w) = setup(0)
+    y = net.forward(xg, n, h, w).cpu().numpy()[:, :3].reshape(n, h, w, 3)
+    ref, _ = cr.forward(params, x)
+    assert relerr(y, ref) < 2e-2, relerr(y, ref)
+
+
+def test_first_layer_exact_accumulation():
+    """One convolution with BF16 operands: FP32 accumulation of exactly
+    representable products; agrees with FP64 to FP32 rounding."""
+    import torch
+    from nemo2d import closure as C
+    rng = np.random.default_rng(3)
+    n, h, w, cin, cout = 1, 7, 9, 4, 32
+    net = C.ConvNet(seed=3)
+    wgt = bf16(rng.standard_normal((9, cin, cout)))
+    bias = rng.standard_normal(cout).astype(np.float32).astype(np.float64)
+    net.set_weights([wgt] + [np.zeros((9, a, b)) for a, b, _ in C.LAYERS[1:]],
+                    [bias] + [np.zeros(b) for _, b, _ in C.LAYERS[1:]])
+    x = bf16(rng.standard_normal((n, h, w, cin)))
+    xp = np.zeros((n * h * w, 8), dtype=np.float32)
+    xp[:, :4] = x.reshape(-1, 4)
+    net.forward(net.to_bf16(torch.tensor(xp, device="cuda")), n, h, w)
+    z0 = net.z[0].cpu().numpy()[:, :cout].reshape(n, h, w, cout)
+    ref = cr.conv_fwd(x, wgt, bias)
+    assert np.allclose(z0, ref, rtol=2e-6, atol=2e-5)
+
+
+def test_gradients_match_reference():
+    import torch
+    net, params, x, t, xg, tg, (n, h, w) = setup(1)
+    wch = torch.tensor([1.0, 0.5, 2.0], dtype=torch.float32, device="cuda")
+    net.forward(xg, n, h, w)
+    loss = float(net.backward(tg, wch))
+    gw, gb = net.grads()
+    ref_loss, ref_grads = cr.loss_and_grads(params, x, t, np.array([1.0, 0.5, 2.0]))
+    assert abs(loss - ref_loss) <= 3e-2 * ref_loss
+    for k in range(4):
+        assert relerr(gw[k], ref_grads[k][0]) < 5e-2, (k, relerr(gw[k], ref_grads[k][0]))
+        assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
+
+
+def test_training_reduces_loss():
+    """Adam on a fixed batch: the loss falls by at least 5x in 300 steps."""
+    import torch
+    net, params, x, t, xg, tg, (n, h, w) = setup(2, n=4, h=16, w=16)
+    wch = torch.ones(3, dtype=torch.float32, device="cuda")
+    losses = []
+    for it in range(300):
+        net.forward(xg, n, h, w)
+        losses.append(float(net.backward(tg, wch)))
+        net.step(lr=3e-3)
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
