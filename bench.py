@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "nemolite2d-cuda_amd"))
 
@@ -49,6 +51,117 @@ def cpu_baseline(n, steps):
                       "per-step checksums off" % (steps, n, n, t)}
 
 
+def form_fp32(n, steps, seed=1):
+    """Single-precision form on a seamount basin against the FP64 path on
+    the same inputs."""
+    import nemo2d
+    grid, _ = nemo2d.seamount_grid(n, n, seed)
+    rest = nemo2d.initial_state(grid)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        m = nemo2d.Model(n, n, grid=grid, state=rest, precision=prec, rdt=20.0, cbfr=0.0015,
+                         visc=0.1)
+        m.run(1, 2)
+        m.set_state(rest)
+        m.timer_start()
+        m.run(1, steps)
+        ms = m.timer_stop()
+        out[prec] = (ms, m.retrieve(), m.diagnostics())
+        m.close()
+    err = nemo2d.field_errors(out["fp32"][1], out["fp64"][1])
+    cells = float(n) * n * steps
+    return {"workload": "%dx%d seamount basin (32 seeded Gaussian mounts, depth 100 m minus "
+                        "heights U(10,60) m, radii U(20,200) km, floor 10 m), %d steps, "
+                        "rdt=20 s, cbfr=0.0015" % (n, n, steps),
+            "fp32_cell_updates_per_s": cells / (out["fp32"][0] * 1e-3),
+            "fp64_cell_updates_per_s": cells / (out["fp64"][0] * 1e-3),
+            "errors_vs_fp64": err,
+            "volume": {"fp64": out["fp64"][2][0], "fp32": out["fp32"][2][0]},
+            "energy": {"fp64": out["fp64"][2][1], "fp32": out["fp32"][2][1]}}
+
+
+def form_ensemble(nmem, steps, seed=7):
+    """Batched basins: one persistent kernel per step, graph-captured loop,
+    autotuned tiling; FP64 and FP32 on the same members."""
+    import nemo2d
+    mem = nemo2d.ensemble_members(nmem, seed)
+    res = {}
+    states = {}
+    for prec in (64, 32):
+        e = nemo2d.Ensemble(mem, rdt=12.0, precision=prec)
+        choice, _ = e.tune(nsteps=10)
+        ms = e.run(1, steps, graph=True)
+        res[prec] = {"cell_updates_per_s": e.cells * steps / (ms * 1e-3), "ms": ms,
+                     "tiling": nemo2d.Ensemble.tiling_shape(choice)}
+        states[prec] = [e.retrieve(k) for k in range(0, nmem, max(1, nmem // 16))]
+        cells = e.cells
+        e.close()
+    errs = [nemo2d.field_errors(a, b) for a, b in zip(states[32], states[64])]
+    rel = {f: float(np.sqrt(np.mean([(x[f]["rms"] / max(x[f]["ref_rms"], 1e-300)) ** 2 for x in errs])))
+           for f in ("ssh", "u", "v")}
+    return {"workload": "%d square basins, side round(exp U(ln64, ln1024)) (%d cells in all), depth "
+                        "U(50,200) m, tide amplitude U(0.1,0.3) m, period U(11,13) h, seed %d, %d "
+                        "steps, rdt=12 s" % (nmem, cells, seed, steps),
+            "fp64": res[64], "fp32": res[32],
+            "fp32_rms_error_relative_to_fp64_rms (16 members)": rel}
+
+
+def form_closure(nc, runs, snaps, iters, windows):
+    """Coarse (nc^2, dx=2 km) + learned closure against the fine solver
+    ((2nc)^2, dx=1 km)."""
+    import nemo2d
+    from nemo2d import closure_pipeline as cp
+    t0 = time.perf_counter()
+    Xs, Ys = [], []
+    for seed in range(runs):
+        X, Y = cp.generate_pairs(cp.Setup(nc, seed), n_snap=snaps)
+        Xs.append(X)
+        Ys.append(Y)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    t_data = time.perf_counter() - t0
+    clos = cp.Closure(X, Y, seed=0)
+    t0 = time.perf_counter()
+    losses = clos.train(X, Y, iters=iters, batch=4, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    held = cp.Setup(nc, 1000)
+    t0 = time.perf_counter()
+    ep, ec, fell = cp.evaluate(held, clos, n_windows=windows)
+    t_eval = time.perf_counter() - t0
+    # throughput: plain coarse, coarse with the correction every 10 steps
+    # (device round trip included), fine
+    csteps = 10 * windows
+    rates = {}
+    for name in ("coarse", "coarse+closure", "fine"):
+        m = held.fine_model() if name == "fine" else held.coarse_model()
+        nst = 2 * csteps if name == "fine" else csteps
+        m.run(1, 2)
+        t1 = time.perf_counter()
+        if name == "coarse+closure":
+            for w in range(windows):
+                m.run(10 * w + 1, 10 * w + 10)
+                s, _ = clos.correct(m.retrieve(), held.coarse_grid)
+                m.set_state(s)
+            m.checksums()
+        else:
+            m.run(1, nst)
+            m.checksums()
+        dt = time.perf_counter() - t1
+        n = held.nf if name == "fine" else held.nc
+        rates[name] = n * n * nst / dt
+        m.close()
+    return {"workload": "coarse %dx%d (dx=2 km, rdt=40 s) + 4-layer 3x3-conv closure (32 channels, "
+                        "GELU, BF16 MFMA, FP32 accumulation) every 10 steps; trained on %d pairs "
+                        "from %d seeded seamount fine runs %dx%d (dx=1 km, rdt=20 s); judged on a "
+                        "held-out basin for %d coarse steps" % (nc, nc, len(X), runs, 2 * nc, 2 * nc,
+                                                               csteps),
+            "ssh_rms_vs_fine": {"coarse": float(ep.mean()), "coarse+closure": float(ec.mean())},
+            "fallback_fired": bool(fell),
+            "train_loss_first_last": [losses[0], losses[-1]],
+            "cell_updates_per_s": rates,
+            "fine_equivalent_rate_of_coarse+closure": rates["coarse+closure"] * 8.0,
+            "seconds": {"data": t_data, "train": t_train, "eval": t_eval}}
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic_step_fused_4096.json")
     if not os.path.exists(p):
@@ -71,6 +184,11 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE readings of the step kernel")
     ap.add_argument("--variant", default="auto",
                     help="fused-step kernel (nemo2d.VARIANTS); results do not depend on it")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the single-precision, ensemble and closure forms")
+    ap.add_argument("--forms-scale", type=float, default=1.0,
+                    help="scale of the forms' workloads (1.0: 8192^2 x 5000 FP32, 256-basin "
+                         "ensemble x 3000, closure at 256^2)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every fused-step variant (200 steps each) and report them "
                          "under 'variants'")
@@ -163,6 +281,13 @@ def main():
     }
     if variants:
         out["variants_ms_per_step"] = variants
+    if not args.no_forms:
+        sc = args.forms_scale
+        out["forms"] = {
+            "fp32_mixed": form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
+            "ensemble": form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc))),
+            "closure": form_closure(max(32, int(256 * sc)), 4, 16, 300, 20),
+        }
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     else:

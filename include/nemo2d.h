@@ -115,7 +115,8 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
  * 16/32/64-row segments, 5 marching with a 4-waves-per-SIMD register
  * budget, 6 marching and 7 per-cell with Flather points recomputed inline
  * instead of in the epilogue, 8/9 marching (16/32 rows) dividing by
- * precomputed reciprocals with two exact FMA corrections.  Marching needs uniform metrics and depths;
+ * precomputed reciprocals with two exact FMA corrections, 10/11 marching
+ * (16/32 rows) with the next row's loads issued one iteration ahead.  Marching needs uniform metrics and depths;
  * other grids use 1 (or 7).  No counterpart in the reference: a tuning
  * hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);

@@ -187,6 +187,28 @@ struct RegWin {
             sv_[0][c] = sv_[1][c]; sv_[1][c] = sv_[2][c];
         }
     }
+    // prefetched next row (software pipelining: loaded one iteration ahead)
+    double nun_[3], nvn_[3], nsshn_[3], nsu_[3], nsv_[3];
+    __device__ __forceinline__ void shift_in_next() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            un_[0][c] = un_[1][c]; un_[1][c] = un_[2][c]; un_[2][c] = nun_[c];
+            vn_[0][c] = vn_[1][c]; vn_[1][c] = vn_[2][c]; vn_[2][c] = nvn_[c];
+            sshn_[0][c] = sshn_[1][c]; sshn_[1][c] = sshn_[2][c]; sshn_[2][c] = nsshn_[c];
+            su_[0][c] = su_[1][c]; su_[1][c] = su_[2][c]; su_[2][c] = nsu_[c];
+            sv_[0][c] = sv_[1][c]; sv_[1][c] = sv_[2][c]; sv_[2][c] = nsv_[c];
+        }
+    }
+    __device__ __forceinline__ void load_next(const NState& n, long rowoff) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nun_[c] = n.un[rowoff + c];
+            nvn_[c] = n.vn[rowoff + c];
+            nsshn_[c] = n.sshn[rowoff + c];
+            nsu_[c] = n.sshn_u[rowoff + c];
+            nsv_[c] = n.sshn_v[rowoff + c];
+        }
+    }
     // load window row r from storage row j, columns i-1..i+1 (rowoff = offset
     // of (i-1, j))
     __device__ __forceinline__ void load(int r, const NState& n, long rowoff) {
@@ -208,7 +230,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int NJ, bool FLIN, int WPE, bool FAST>
+template <int NJ, bool FLIN, int WPE, bool FAST, bool PF = false>
 __global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
@@ -232,14 +254,31 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         w.s = sc_;
         w.load(1, n, L.at(il - 1, jb - 1));
         w.load(2, n, L.at(il - 1, jb));
+        unsigned f_nxt = 0;
+        if constexpr (PF) {
+            w.load_next(n, L.at(il - 1, jb + 1));
+            f_nxt = fl[L.at(il, jb)];
+        }
         double s_prev = 0.0;   // new ssh at (i, j-1)
         double sv_prev = 0.0;  // old sshn_v at (i, j-1)
         unsigned f_prev = 0;
         for (int j = jb; j <= je; ++j) {
-            w.shift();
-            w.load(2, n, L.at(il - 1, j + 1));
             const long o = L.at(il, j);
-            const unsigned f = fl[o];
+            unsigned f;
+            if constexpr (PF) {
+                // rows j-1..j+1 from the window and the prefetched row; the
+                // loads of row j+2 (and the flags of row j+1) have a whole
+                // iteration to arrive
+                w.shift_in_next();
+                f = f_nxt;
+                const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
+                w.load_next(n, L.at(il - 1, jn));
+                f_nxt = fl[L.at(il, j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1)];
+            } else {
+                w.shift();
+                w.load(2, n, L.at(il - 1, j + 1));
+                f = fl[o];
+            }
             double s_c = 0.0;
             if (j >= 1 && tcol) s_c = (f & T_CLAMP) ? tide : continuity_w(w, p.rdt);
             const double s_e = __shfl_down(s_c, 1, 64);
@@ -1222,10 +1261,12 @@ enum {
     VAR_CELL_FLIN = 7,  // one thread per cell, Flather inline
     VAR_MARCH16_FD = 8,  // marching, 16-row segments, division by reciprocals
     VAR_MARCH32_FD = 9,  //                32-row segments
-    VAR_LAST = 9
+    VAR_MARCH16_PF = 10,  // marching, 16-row segments, next row prefetched
+    VAR_MARCH32_PF = 11,  //                32-row segments
+    VAR_LAST = 11
 };
 
-template <int NJ, bool FLIN, int WPE, bool FAST = false>
+template <int NJ, bool FLIN, int WPE, bool FAST = false, bool PF = false>
 static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
@@ -1233,7 +1274,7 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march<NJ, FLIN, WPE, FAST><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march<NJ, FLIN, WPE, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
     LAUNCHCHK("step_march");
     nparts = nblocks;
@@ -1264,7 +1305,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
            m->S[1 - k][F_VN]};
     const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
-    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH32 : VAR_CELL;
+    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH16 : VAR_CELL;
     if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
@@ -1283,6 +1324,8 @@ int nemo2d_step(nemo2d_model* m, int istp) {
             rc = m->scal.fast_div ? launch_march<32, false, 0, true>(m, n, a, tide, nparts)
                                   : launch_march<32, false, 0>(m, n, a, tide, nparts);
             break;
+        case VAR_MARCH16_PF: rc = launch_march<16, false, 0, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_PF: rc = launch_march<32, false, 0, false, true>(m, n, a, tide, nparts); break;
         default: rc = launch_march<32, false, 0>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;

@@ -39,7 +39,8 @@ SYMBOLS = (
 
 # nemo2d_set_variant choices
 VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_w4": 5,
-            "march32_flin": 6, "cell_flin": 7, "march16_fd": 8, "march32_fd": 9}
+            "march32_flin": 6, "cell_flin": 7, "march16_fd": 8, "march32_fd": 9, "march16_pf": 10,
+            "march32_pf": 11}
 
 
 class Nemo2dError(RuntimeError):

@@ -1,0 +1,246 @@
+"""Closure pipeline: fine runs -> restricted training pairs -> training ->
+coarse+closure runs judged against the fine solver (no counterpart in the
+reference).
+
+Grids: the fine basin has N_f = 2*N_c cells per side at dx_f and time step
+rdt_f; the coarse basin N_c at dx_c = 2*dx_f, rdt_c = 2*rdt_f, so one coarse
+step covers two fine steps and both see the same tide at the same time.
+Both carry seeded Gaussian seamounts (nemo2d.seamount_grid on the fine grid,
+restricted to the coarse one).
+
+Restriction R (fine -> coarse) on the C-grid:
+  ssh_c  = mean of the 4 fine T cells (volume-conserving: sum e12t*ssh is
+           unchanged);
+  u_c    = depth-weighted mean of the 2 fine U faces lying on the coarse U
+           face (the coarse face carries the fine faces' volume flux when the
+           coarse depth is their mean); v_c likewise;
+  sshn_u, sshn_v are re-derived from ssh_c with the serial model's next
+  rules (original/nemolite2d.f90:735-765), row 0 of sshn_v staying 0.
+Training pairs: from C_t = R(F_t) the coarse model runs K = 10 steps to C';
+the target is R(F_{t+K}) - C' at T points (ssh, u and v averaged to the
+cell); the input is C' (ssh, u, v at T points, depth).  Applying the closure
+adds the predicted increment every K coarse steps.
+"""
+import numpy as np
+
+from . import Model, default_grid, initial_state, seamount_grid, shapes
+
+K_STEPS = 10
+
+
+# ---- grids and restriction ----------------------------------------------------
+def face_depths(ht):
+    jpj, jpi = ht.shape
+    hu = np.empty((jpj, jpi + 1))
+    hu[:, 1:jpi] = 0.5 * (ht[:, :-1] + ht[:, 1:])
+    hu[:, 0], hu[:, jpi] = ht[:, 0], ht[:, -1]
+    hv = np.empty((jpj + 1, jpi))
+    hv[1:jpj, :] = 0.5 * (ht[:-1, :] + ht[1:, :])
+    hv[0, :], hv[jpj, :] = ht[0, :], ht[-1, :]
+    return hu, hv
+
+
+def coarse_grid(fine, dx_c, dy_c):
+    nf = fine["ht"].shape[0]
+    nc = nf // 2
+    g = default_grid(nc, nc, dx_c, dy_c, 100.0)
+    htf = fine["ht"]
+    g["ht"] = 0.25 * (htf[0::2, 0::2] + htf[0::2, 1::2] + htf[1::2, 0::2] + htf[1::2, 1::2])
+    g["hu"], g["hv"] = face_depths(g["ht"])
+    return g
+
+
+def next_interp(ssh, e12t=None, e12u=None, e12v=None):
+    """sshn_u, sshn_v from ssh by the serial model's next rules on the
+    reference basin (uniform areas): interior faces average, outer faces copy
+    the adjacent cell, V row 0 (open boundary) stays 0."""
+    jpj, jpi = ssh.shape
+    su = np.empty((jpj, jpi + 1))
+    sv = np.zeros((jpj + 1, jpi))
+    e = 1.0 if e12t is None else e12t
+    su[:, 1:jpi] = 0.5 * (e * ssh[:, :-1] + e * ssh[:, 1:]) / (1.0 if e12u is None else e12u)
+    su[:, 0], su[:, jpi] = ssh[:, 0], ssh[:, -1]
+    sv[1:jpj, :] = 0.5 * (e * ssh[:-1, :] + e * ssh[1:, :]) / (1.0 if e12v is None else e12v)
+    sv[jpj, :] = ssh[-1, :]
+    return su, sv
+
+
+def restrict_state(fine_state, fine_grid, coarse_g):
+    f = fine_state
+    ssh = 0.25 * (f["sshn"][0::2, 0::2] + f["sshn"][0::2, 1::2] + f["sshn"][1::2, 0::2] +
+                  f["sshn"][1::2, 1::2])
+    huf, hvf = fine_grid["hu"], fine_grid["hv"]
+    u = f["un"][:, 0::2]
+    hu = huf[:, 0::2]
+    un = (hu[0::2] * u[0::2] + hu[1::2] * u[1::2]) / (hu[0::2] + hu[1::2])
+    v = f["vn"][0::2, :]
+    hv = hvf[0::2, :]
+    vn = (hv[:, 0::2] * v[:, 0::2] + hv[:, 1::2] * v[:, 1::2]) / (hv[:, 0::2] + hv[:, 1::2])
+    e = coarse_g["e12t"][0, 0]
+    su, sv = next_interp(ssh, e, coarse_g["e12u"][0, 0], coarse_g["e12v"][0, 0])
+    return {"sshn": ssh, "sshn_u": su, "sshn_v": sv, "un": np.ascontiguousarray(un),
+            "vn": np.ascontiguousarray(vn)}
+
+
+def t_fields(state):
+    """ssh, u, v at T points."""
+    u = 0.5 * (state["un"][:, :-1] + state["un"][:, 1:])
+    v = 0.5 * (state["vn"][:-1, :] + state["vn"][1:, :])
+    return np.stack([state["sshn"], u, v], -1)
+
+
+def features(state, ht):
+    return np.concatenate([t_fields(state), ht[..., None]], -1)
+
+
+def apply_increment(state, d, grid):
+    """Add T-point increments d[..., 0:3] (ssh, u, v): ssh directly, face
+    velocities by the mean of the two adjacent cells on interior faces (outer
+    faces keep their boundary values)."""
+    s = {k: v.copy() for k, v in state.items()}
+    s["sshn"] += d[..., 0]
+    s["un"][:, 1:-1] += 0.5 * (d[:, :-1, 1] + d[:, 1:, 1])
+    s["vn"][1:-1, :] += 0.5 * (d[:-1, :, 2] + d[1:, :, 2])
+    su, sv = next_interp(s["sshn"], grid["e12t"][0, 0], grid["e12u"][0, 0], grid["e12v"][0, 0])
+    s["sshn_u"], s["sshn_v"] = su, sv
+    return s
+
+
+def rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+# ---- data ------------------------------------------------------------------------
+class Setup:
+    """One fine/coarse basin pair."""
+
+    def __init__(self, nc, seed, dx_f=1000.0, rdt_f=20.0, cbfr=0.0015, visc=0.1,
+                 radius=(20e3, 200e3)):
+        self.nc, self.nf = nc, 2 * nc
+        self.dx_f, self.rdt_f = dx_f, rdt_f
+        self.fine_grid, _ = seamount_grid(self.nf, self.nf, seed, dx=dx_f, dy=dx_f, radius=radius)
+        self.coarse_grid = coarse_grid(self.fine_grid, 2 * dx_f, 2 * dx_f)
+        self.kw = dict(cbfr=cbfr, visc=visc)
+
+    def fine_model(self):
+        return Model(self.nf, self.nf, dx=self.dx_f, dy=self.dx_f, rdt=self.rdt_f,
+                     grid=self.fine_grid, **self.kw)
+
+    def coarse_model(self):
+        return Model(self.nc, self.nc, dx=2 * self.dx_f, dy=2 * self.dx_f, rdt=2 * self.rdt_f,
+                     grid=self.coarse_grid, **self.kw)
+
+
+def generate_pairs(setup, n_snap, spinup_snaps=2):
+    """Returns (inputs [S, nc, nc, 4], targets [S, nc, nc, 3]) from one fine
+    run of (spinup + n_snap + 1) * K coarse steps."""
+    fm, cm = setup.fine_model(), setup.coarse_model()
+    snaps = []
+    total = spinup_snaps + n_snap + 1
+    for s in range(total):
+        fm.run(2 * K_STEPS * s + 1, 2 * K_STEPS * (s + 1))
+        if s >= spinup_snaps - 1:
+            snaps.append((K_STEPS * (s + 1), fm.retrieve()))
+    X, Y = [], []
+    for (tc, f0), (_, f1) in zip(snaps[:-1], snaps[1:]):
+        c0 = restrict_state(f0, setup.fine_grid, setup.coarse_grid)
+        cm.set_state(c0)
+        cm.run(tc + 1, tc + K_STEPS)
+        cp = cm.retrieve()
+        truth = t_fields(restrict_state(f1, setup.fine_grid, setup.coarse_grid))
+        X.append(features(cp, setup.coarse_grid["ht"]))
+        Y.append(truth - t_fields(cp))
+    fm.close()
+    cm.close()
+    return np.array(X), np.array(Y)
+
+
+# ---- the learned closure ---------------------------------------------------------
+class Closure:
+    """Normalisation + ConvNet; corrects a coarse state every K steps."""
+
+    def __init__(self, X, Y, seed=0):
+        from .closure import ConvNet
+        self.net = ConvNet(seed=seed)
+        self.xm = X.reshape(-1, 4).mean(0)
+        self.xs = X.reshape(-1, 4).std(0) + 1e-12
+        self.ys = Y.reshape(-1, 3).std(0) + 1e-30
+        self.torch = self.net.torch
+
+    def _x(self, X):
+        t = self.torch
+        n, h, w, _ = X.shape
+        xp = np.zeros((n * h * w, 8), dtype=np.float32)
+        xp[:, :4] = ((X - self.xm) / self.xs).reshape(-1, 4)
+        return self.net.to_bf16(t.tensor(xp, device="cuda"))
+
+    def _y(self, Y):
+        n, h, w, _ = Y.shape
+        yp = np.zeros((n * h * w, 8), dtype=np.float32)
+        yp[:, :3] = (Y / self.ys).reshape(-1, 3)
+        return self.torch.tensor(yp, device="cuda")
+
+    def train(self, X, Y, iters=400, batch=8, lr=2e-3, seed=0, log=None):
+        rng = np.random.default_rng(seed)
+        t = self.torch
+        wch = t.ones(3, dtype=t.float32, device="cuda")
+        n, h, w, _ = X.shape
+        losses = []
+        for it in range(iters):
+            idx = rng.choice(n, size=min(batch, n), replace=False)
+            xb, yb = self._x(X[idx]), self._y(Y[idx])
+            self.net.forward(xb, len(idx), h, w)
+            loss = self.net.backward(yb, wch)
+            self.net.step(lr=lr * (0.5 * (1 + np.cos(np.pi * it / iters))))
+            if it % 50 == 0 or it == iters - 1:
+                losses.append(float(loss))
+                if log:
+                    log("closure train it %d loss %.4g" % (it, losses[-1]))
+        return losses
+
+    def predict(self, X):
+        n, h, w, _ = X.shape
+        y = self.net.forward(self._x(X), n, h, w)
+        return y.cpu().numpy()[:, :3].reshape(n, h, w, 3) * self.ys
+
+    def correct(self, state, grid):
+        d = self.predict(features(state, grid["ht"])[None])[0]
+        return apply_increment(state, d, grid), d
+
+
+def evaluate(setup, closure, n_windows, tol=2.0, judge_every=5):
+    """Fine run (the judge) beside a plain coarse run and a coarse+closure run
+    from the same restricted start.  Every judge_every windows of K coarse
+    steps the closure run is compared with the restricted fine state; when
+    its ssh RMS error exceeds tol times the plain run's, the closure is
+    switched off (fallback to the plain run's state).  Returns per-window
+    ssh RMS errors and whether the fallback fired."""
+    fm, cp, cc = setup.fine_model(), setup.coarse_model(), setup.coarse_model()
+    fm.run(1, 2 * K_STEPS)
+    f = fm.retrieve()
+    c0 = restrict_state(f, setup.fine_grid, setup.coarse_grid)
+    cp.set_state(c0)
+    cc.set_state(c0)
+    err_plain, err_clos = [], []
+    use = True
+    fell_back = False
+    for wdx in range(1, n_windows + 1):
+        t0 = K_STEPS * wdx
+        fm.run(2 * t0 + 1, 2 * (t0 + K_STEPS))
+        cp.run(t0 + 1, t0 + K_STEPS)
+        cc.run(t0 + 1, t0 + K_STEPS)
+        sc = cc.retrieve()
+        if use:
+            sc, _ = closure.correct(sc, setup.coarse_grid)
+            cc.set_state(sc)
+        truth = restrict_state(fm.retrieve(), setup.fine_grid, setup.coarse_grid)
+        sp = cp.retrieve()
+        err_plain.append(rms(sp["sshn"], truth["sshn"]))
+        err_clos.append(rms(sc["sshn"], truth["sshn"]))
+        if use and wdx % judge_every == 0 and err_clos[-1] > tol * err_plain[-1]:
+            use = False
+            fell_back = True
+            cc.set_state(sp)
+    for m in (fm, cp, cc):
+        m.close()
+    return np.array(err_plain), np.array(err_clos), fell_back

@@ -1,0 +1,37 @@
+"""End-to-end closure at small size: training pairs from the solver's own
+fine runs, training with the library's kernels, and a held-out run where
+coarse+closure beats the plain coarse run against the fine solver; the
+fallback fires when the tolerance is made impossible to meet."""
+import numpy as np
+import pytest
+
+from nemo2d import closure_pipeline as cp
+
+pytestmark = pytest.mark.gpu
+
+NC = 48
+
+
+def train_small():
+    Xs, Ys = [], []
+    for seed in range(3):
+        X, Y = cp.generate_pairs(cp.Setup(NC, seed, radius=(8e3, 40e3)), n_snap=24)
+        Xs.append(X)
+        Ys.append(Y)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    clos = cp.Closure(X, Y, seed=0)
+    losses = clos.train(X, Y, iters=600, batch=8, lr=2e-3)
+    return clos, losses
+
+
+def test_closure_trains_and_helps_on_held_out_basin():
+    clos, losses = train_small()
+    assert losses[-1] < 0.6 * losses[0], losses
+    held = cp.Setup(NC, 100, radius=(8e3, 40e3))
+    ep, ec, fell = cp.evaluate(held, clos, n_windows=24, tol=2.0)
+    print("plain ssh rms", ep.mean(), "closure ssh rms", ec.mean())
+    assert not fell
+    assert ec.mean() < ep.mean(), (ec.mean(), ep.mean())
+    # impossible tolerance: the judge switches the closure off
+    _, _, fell2 = cp.evaluate(held, clos, n_windows=10, tol=1e-9, judge_every=5)
+    assert fell2
