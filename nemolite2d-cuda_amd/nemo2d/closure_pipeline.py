@@ -115,10 +115,11 @@ class Setup:
     """One fine/coarse basin pair."""
 
     def __init__(self, nc, seed, dx_f=1000.0, rdt_f=20.0, cbfr=0.0015, visc=0.1,
-                 radius=(20e3, 200e3)):
+                 radius=(20e3, 200e3), n_mounts=32, height=(10.0, 60.0)):
         self.nc, self.nf = nc, 2 * nc
         self.dx_f, self.rdt_f = dx_f, rdt_f
-        self.fine_grid, _ = seamount_grid(self.nf, self.nf, seed, dx=dx_f, dy=dx_f, radius=radius)
+        self.fine_grid, _ = seamount_grid(self.nf, self.nf, seed, n_mounts=n_mounts, dx=dx_f,
+                                          dy=dx_f, radius=radius, height=height)
         self.coarse_grid = coarse_grid(self.fine_grid, 2 * dx_f, 2 * dx_f)
         self.kw = dict(cbfr=cbfr, visc=visc)
 

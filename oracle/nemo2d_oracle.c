@@ -46,6 +46,9 @@ struct o2d {
     farr gphiu, gphiv, gphif, xt, yt, ht, hu, hv, hf;
     farr sshb, sshb_u, sshb_v, sshn, sshn_u, sshn_v, ssha, ssha_u, ssha_v;
     farr un, vn, ua, va;
+    /* sin(gphi*d2r) per U/V point, filled for the duration of o2d_run (the
+     * latitudes do not change within a run); NULL: evaluate per point */
+    double *sin_u, *sin_v;
 };
 
 /* Fortran-style element access with explicit lower bounds. */
@@ -282,7 +285,9 @@ void o2d_momentum_u(o2d *m) {
             double vis = (dudx_e - dudx_w) * F(m->e2u, ji, jj) + (dudy_n - dudy_s) * F(m->e1u, ji, jj) * 0.5;
             vis = visc * vis;
 
-            double cor = 0.5 * (2.0 * OMEGA * sin(F(m->gphiu, ji, jj) * D2R) * (v_sc + v_nc)) *
+            const double sphi = m->sin_u ? m->sin_u[(ji - m->gphiu.lb1) + (long)(jj - m->gphiu.lb2) * (m->jpi + 1)]
+                                         : sin(F(m->gphiu, ji, jj) * D2R);
+            double cor = 0.5 * (2.0 * OMEGA * sphi * (v_sc + v_nc)) *
                          F(m->e12u, ji, jj) * (F(m->hu, ji, jj) + F(m->sshn_u, ji, jj));
             double hpg = -GRAV * (F(m->hu, ji, jj) + F(m->sshn_u, ji, jj)) * F(m->e2u, ji, jj) *
                          (F(m->sshn, ji + 1, jj) - F(m->sshn, ji, jj));
@@ -354,7 +359,9 @@ void o2d_momentum_v(o2d *m) {
             double vis = (dvdy_n - dvdy_s) * F(m->e1v, ji, jj) + (dvdx_e - dvdx_w) * F(m->e2v, ji, jj) * 0.5;
             vis = visc * vis;
 
-            double cor = -0.5 * (2.0 * OMEGA * sin(F(m->gphiv, ji, jj) * D2R) * (u_ec + u_wc)) *
+            const double sphi = m->sin_v ? m->sin_v[(ji - m->gphiv.lb1) + (long)(jj - m->gphiv.lb2) * m->jpi]
+                                         : sin(F(m->gphiv, ji, jj) * D2R);
+            double cor = -0.5 * (2.0 * OMEGA * sphi * (u_ec + u_wc)) *
                          F(m->e12v, ji, jj) * (F(m->hv, ji, jj) + F(m->sshn_v, ji, jj));
             double hpg = -GRAV * (F(m->hv, ji, jj) + F(m->sshn_v, ji, jj)) * F(m->e1v, ji, jj) *
                          (F(m->sshn, ji, jj + 1) - F(m->sshn, ji, jj));
@@ -518,11 +525,24 @@ void o2d_checksums(o2d *m, double out[3]) {
 #include <time.h>
 double o2d_run(o2d *m, int nit000, int nitend, double *sums) {
     struct timespec t0, t1;
+    /* the same sin values the per-point evaluation gives, computed once */
+    const long nu = (long)(m->jpi + 1) * m->jpj, nv = (long)m->jpi * (m->jpj + 1);
+    m->sin_u = (double *)malloc(nu * sizeof(double));
+    m->sin_v = (double *)malloc(nv * sizeof(double));
+    if (m->sin_u && m->sin_v) {
+        for (long k = 0; k < nu; ++k) m->sin_u[k] = sin(m->gphiu.p[k] * D2R);
+        for (long k = 0; k < nv; ++k) m->sin_v[k] = sin(m->gphiv.p[k] * D2R);
+    } else {
+        free(m->sin_u); free(m->sin_v);
+        m->sin_u = m->sin_v = NULL;
+    }
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (int istp = nit000; istp <= nitend; ++istp) {
         o2d_step(m, istp);
         if (sums) o2d_checksums(m, sums + 3 * (long)(istp - nit000));
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(m->sin_u); free(m->sin_v);
+    m->sin_u = m->sin_v = NULL;
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
