@@ -111,14 +111,15 @@ int nemo2d_step(nemo2d_model *m, int istp);
 int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default, 1 one thread per cell, 2/3/4 column-marching with
- * 16/32/64-row segments, 5 marching with a 4-waves-per-SIMD register
- * budget, 6 marching and 7 per-cell with Flather points recomputed inline
- * instead of in the epilogue, 8/9 marching (16/32 rows) dividing by
- * precomputed reciprocals with two exact FMA corrections, 10/11 marching
- * (16/32 rows) with the next row's loads issued one iteration ahead.  Marching needs uniform metrics and depths;
- * other grids use 1 (or 7).  No counterpart in the reference: a tuning
- * hook. */
+ * choice): 0 default (10 on grids with uniform metrics and depths, else 1),
+ * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
+ * 5 marching with a 4-waves-per-SIMD register budget, 6 marching and 7
+ * per-cell with Flather points recomputed inline instead of in the
+ * epilogue, 8/9 marching (16/32 rows) dividing by precomputed reciprocals
+ * with two exact FMA corrections, 10/11 marching (16/32 rows) with the next
+ * row's loads issued one iteration ahead.  Marching needs uniform metrics
+ * and depths; other grids use 1 (or 7).  No counterpart in the reference:
+ * a tuning hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
 
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the
@@ -234,6 +235,15 @@ int nemo2d_nn_adam(float *p, const float *g, float *m, float *v, long n, float l
                    float b2, float eps, int step, void *stream);
 int nemo2d_nn_pack_weights(const float *w, int cin, int cout, void *wf, void *wb, void *stream);
 int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
+
+/* Device memory for the closure network's tensors, on the current HIP
+ * device (no counterpart in the reference).  Copies and zeroing are
+ * synchronous with the host and ordered after work on the default stream. */
+int nemo2d_dev_alloc(long bytes, void **out);
+int nemo2d_dev_free(void *p);
+int nemo2d_dev_upload(void *dst, const void *src, long bytes);
+int nemo2d_dev_download(void *dst, const void *src, long bytes);
+int nemo2d_dev_zero(void *p, long bytes);
 
 /* Free the model's device memory and stream.  Replaces cuda_finalise_
  * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */

@@ -1305,7 +1305,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
            m->S[1 - k][F_VN]};
     const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
-    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH16 : VAR_CELL;
+    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH16_PF : VAR_CELL;
     if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;

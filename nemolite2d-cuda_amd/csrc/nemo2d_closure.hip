@@ -375,4 +375,48 @@ int nemo2d_nn_to_bf16(const float* a, long n, void* b, void* stream) {
     return launched("to_bf16");
 }
 
+int nemo2d_dev_alloc(long bytes, void** out) {
+    if (!out || bytes < 0) return set_error(NEMO2D_ERR_ARG, "nemo2d_dev_alloc: bad argument");
+    *out = nullptr;
+    if (bytes == 0) return NEMO2D_OK;
+    const hipError_t e = hipMalloc(out, (size_t)bytes);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return set_error(NEMO2D_ERR_HIP, std::string("nemo2d_dev_alloc: ") + hipGetErrorString(e));
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_dev_free(void* p) {
+    if (!p) return NEMO2D_OK;
+    const hipError_t e = hipFree(p);
+    if (e != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("nemo2d_dev_free: ") + hipGetErrorString(e));
+    return NEMO2D_OK;
+}
+
+static int dev_copy(void* dst, const void* src, long bytes, hipMemcpyKind k, const char* what) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) return set_error(NEMO2D_ERR_ARG, std::string(what) + ": bad argument");
+    if (bytes == 0) return NEMO2D_OK;
+    const hipError_t e = hipMemcpy(dst, src, (size_t)bytes, k);
+    if (e != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return NEMO2D_OK;
+}
+
+int nemo2d_dev_upload(void* dst, const void* src, long bytes) {
+    return dev_copy(dst, src, bytes, hipMemcpyHostToDevice, "nemo2d_dev_upload");
+}
+
+int nemo2d_dev_download(void* dst, const void* src, long bytes) {
+    return dev_copy(dst, src, bytes, hipMemcpyDeviceToHost, "nemo2d_dev_download");
+}
+
+int nemo2d_dev_zero(void* p, long bytes) {
+    if (bytes < 0 || (bytes > 0 && !p)) return set_error(NEMO2D_ERR_ARG, "nemo2d_dev_zero: bad argument");
+    if (bytes == 0) return NEMO2D_OK;
+    hipError_t e = hipMemset(p, 0, (size_t)bytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("nemo2d_dev_zero: ") + hipGetErrorString(e));
+    return NEMO2D_OK;
+}
+
 }  // extern "C"

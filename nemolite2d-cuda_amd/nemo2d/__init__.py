@@ -35,6 +35,8 @@ SYMBOLS = (
     "nemo2d_ensemble_destroy", "nemo2d_nn_conv_fwd", "nemo2d_nn_conv_bwd_data",
     "nemo2d_nn_wgrad_work_floats", "nemo2d_nn_conv_bwd_weight", "nemo2d_nn_gelu_bwd",
     "nemo2d_nn_mse", "nemo2d_nn_adam", "nemo2d_nn_pack_weights", "nemo2d_nn_to_bf16",
+    "nemo2d_dev_alloc", "nemo2d_dev_free", "nemo2d_dev_upload", "nemo2d_dev_download",
+    "nemo2d_dev_zero",
 )
 
 # nemo2d_set_variant choices
@@ -85,6 +87,11 @@ def lib():
         L.nemo2d_ensemble_checksum_log.argtypes = [vp, i, dp, i, ip]
         L.nemo2d_ensemble_retrieve.argtypes = [vp, i, dp, dp, dp]
         L.nemo2d_ensemble_destroy.argtypes = [vp]
+        L.nemo2d_dev_alloc.argtypes = [ctypes.c_long, ctypes.POINTER(vp)]
+        L.nemo2d_dev_free.argtypes = [vp]
+        L.nemo2d_dev_upload.argtypes = [vp, vp, ctypes.c_long]
+        L.nemo2d_dev_download.argtypes = [vp, vp, ctypes.c_long]
+        L.nemo2d_dev_zero.argtypes = [vp, ctypes.c_long]
         L.nemo2d_run.argtypes = [vp, i, i]
         L.nemo2d_record_checksums.argtypes = [vp, i]
         L.nemo2d_checksum_log.argtypes = [vp, dp, i, ip]

@@ -1,15 +1,15 @@
 """Learned closure for a 2x-coarsened NEMOLite2D run (no counterpart in the
 reference).
 
+* DevArray -- a typed device buffer owned by the library
+  (nemo2d_dev_alloc/free/upload/download/zero); the network needs no other
+  device runtime.
 * ConvNet -- 4 'same' 3x3 convolutions (4 -> 32 -> 32 -> 32 -> 3 channels),
   exact GELU between them, BF16 weights/activations with FP32 accumulation,
   trained with Adam on FP32 master weights.  Every operation runs in the
   library's own kernels (libnemo2d.so, nemo2d_nn_*: BF16-MFMA implicit-GEMM
-  convolutions forward and backward, GELU backward, MSE, Adam); torch only
-  allocates device memory and supplies the stream.
-* restrict_state / features / apply_correction -- the fine->coarse
-  restriction on the C-grid (volume-conserving ssh average, depth-weighted
-  face velocities), the network's input channels and the correction.
+  convolutions forward and backward, GELU backward, MSE, Adam) on the
+  default stream.
 * Closure data generation, training and evaluation live in
   closure_pipeline (same package).
 """
@@ -26,12 +26,58 @@ def pad8(c):
     return (c + 7) // 8 * 8
 
 
-def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr())
+class DevArray:
+    """Device buffer of `shape` and numpy `dtype` (float32 or int16 = BF16
+    bits) on the current HIP device, zero-initialised."""
 
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(x) for x in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._L = lib()
+        p = ctypes.c_void_p()
+        _check(self._L.nemo2d_dev_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+        self.zero()
 
-def _fptr(t):
-    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), ctypes.POINTER(ctypes.c_float))
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("DevArray.upload: %d bytes into %d" % (a.nbytes, self.nbytes))
+        _check(self._L.nemo2d_dev_upload(self.ptr, a.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self._L.nemo2d_dev_download(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        _check(self._L.nemo2d_dev_zero(self.ptr, self.nbytes))
+
+    def fptr(self):
+        return ctypes.cast(self.ptr, ctypes.POINTER(ctypes.c_float))
+
+    def close(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self._L.nemo2d_dev_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _setup_nn(L):
@@ -58,116 +104,125 @@ def _setup_nn(L):
 class ConvNet:
     """The closure network on the current HIP device."""
 
-    def __init__(self, seed=0, layers=LAYERS, init_scale=1.0):
-        import torch
-        self.torch = torch
+    # weights start N(0, init_scale^2 * 2 / (9 cin)); the default scale
+    # sqrt(1/6) gives variance 1 / (27 cin), that of the uniform fan-in rule
+    # U(-1/sqrt(9 cin), 1/sqrt(9 cin)).  The He scale (1.0) trains to a lower
+    # loss but the coarse+closure run can then drift away from the fine one.
+    INIT_SCALE = float(np.sqrt(1.0 / 6.0))
+    STREAM = None  # the default stream
+
+    def __init__(self, seed=0, layers=LAYERS, init_scale=None):
         self.L = lib()
         _setup_nn(self.L)
         self.layers = layers
         rng = np.random.default_rng(seed)
-        dev = "cuda"
+        if init_scale is None:
+            init_scale = self.INIT_SCALE
         self.w, self.b, self.wf, self.wb = [], [], [], []
         self.mw, self.vw, self.mb, self.vb = [], [], [], []
         for cin, cout, _ in layers:
             std = init_scale * np.sqrt(2.0 / (9 * cin))
-            w = torch.tensor(rng.standard_normal((9, cin, cout)) * std, dtype=torch.float32, device=dev)
-            self.w.append(w.contiguous())
-            self.b.append(torch.zeros(cout, dtype=torch.float32, device=dev))
-            self.wf.append(torch.zeros(9 * 32 * pad8(cin), dtype=torch.int16, device=dev))
-            self.wb.append(torch.zeros(9 * 32 * pad8(cout), dtype=torch.int16, device=dev))
+            self.w.append(DevArray.from_numpy(rng.standard_normal((9, cin, cout)) * std, np.float32))
+            self.b.append(DevArray((cout,), np.float32))
+            self.wf.append(DevArray((9 * 32 * pad8(cin),), np.int16))
+            self.wb.append(DevArray((9 * 32 * pad8(cout),), np.int16))
             for lst, shape in ((self.mw, (9, cin, cout)), (self.vw, (9, cin, cout)),
                                (self.mb, (cout,)), (self.vb, (cout,))):
-                lst.append(torch.zeros(shape, dtype=torch.float32, device=dev))
+                lst.append(DevArray(shape, np.float32))
         self.step_count = 0
         self.pack()
         self._shape = None
 
-    @property
-    def stream(self):
-        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
-
     def pack(self):
         for k, (cin, cout, _) in enumerate(self.layers):
-            _check(self.L.nemo2d_nn_pack_weights(_fptr(self.w[k]), cin, cout, _ptr(self.wf[k]),
-                                                 _ptr(self.wb[k]), self.stream))
+            _check(self.L.nemo2d_nn_pack_weights(self.w[k].fptr(), cin, cout, self.wf[k].ptr,
+                                                 self.wb[k].ptr, self.STREAM))
 
     def set_weights(self, ws, bs):
         """FP32 numpy weights [9, cin, cout] and biases [cout]."""
         for k in range(len(self.layers)):
-            self.w[k].copy_(self.torch.tensor(ws[k], dtype=self.torch.float32))
-            self.b[k].copy_(self.torch.tensor(bs[k], dtype=self.torch.float32))
+            self.w[k].upload(np.asarray(ws[k], dtype=np.float32))
+            self.b[k].upload(np.asarray(bs[k], dtype=np.float32))
         self.pack()
 
     def weights(self):
-        return ([w.cpu().numpy().astype(np.float64) for w in self.w],
-                [b.cpu().numpy().astype(np.float64) for b in self.b])
+        return ([w.download().astype(np.float64) for w in self.w],
+                [b.download().astype(np.float64) for b in self.b])
+
+    def upload(self, a):
+        """FP32 device copy of a host array (targets, channel weights)."""
+        return DevArray.from_numpy(a, np.float32)
 
     def to_bf16(self, x):
-        """FP32 [P, C] tensor (C a multiple of 8) -> BF16 bits (int16)."""
-        out = self.torch.empty(x.shape, dtype=self.torch.int16, device=x.device)
-        _check(self.L.nemo2d_nn_to_bf16(_fptr(x), x.numel(), _ptr(out), self.stream))
+        """FP32 host array [P, C] (C a multiple of 8) -> device BF16 bits."""
+        xf = DevArray.from_numpy(x, np.float32)
+        out = DevArray(xf.shape, np.int16)
+        _check(self.L.nemo2d_nn_to_bf16(xf.fptr(), xf.size, out.ptr, self.STREAM))
         return out
 
     def _buffers(self, n, h, w):
         if self._shape == (n, h, w):
             return
-        t = self.torch
         P = n * h * w
-        self.z = [t.zeros((P, pad8(co)), dtype=t.float32, device="cuda") for _, co, _ in self.layers]
-        self.act = [t.zeros((P, pad8(co)), dtype=t.int16, device="cuda") for _, co, _ in self.layers]
-        self.dz = [t.zeros((P, pad8(co)), dtype=t.int16, device="cuda") for _, co, _ in self.layers]
-        self.dact = t.zeros((P, 32), dtype=t.float32, device="cuda")
-        self.gw = [t.zeros((9, ci, co), dtype=t.float32, device="cuda") for ci, co, _ in self.layers]
-        self.gb = [t.zeros(co, dtype=t.float32, device="cuda") for _, co, _ in self.layers]
-        self.work = t.empty(int(self.L.nemo2d_nn_wgrad_work_floats(P)), dtype=t.float32, device="cuda")
+        self.z = [DevArray((P, pad8(co)), np.float32) for _, co, _ in self.layers]
+        self.act = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
+        self.dz = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
+        self.dact = DevArray((P, 32), np.float32)
+        self.gw = [DevArray((9, ci, co), np.float32) for ci, co, _ in self.layers]
+        self.gb = [DevArray((co,), np.float32) for _, co, _ in self.layers]
+        self.work = DevArray((int(self.L.nemo2d_nn_wgrad_work_floats(P)),), np.float32)
         self.nparts = 512
-        self.part = t.zeros(self.nparts, dtype=t.float32, device="cuda")
+        self.part = DevArray((self.nparts,), np.float32)
         self._shape = (n, h, w)
 
     def forward(self, x_bf16, n, h, w):
-        """x_bf16: [n*h*w, 8] BF16 bits (channels 0..3 used).  Returns the
-        FP32 output [P, 8] (channels 0..2)."""
+        """x_bf16: device [n*h*w, 8] BF16 bits (channels 0..3 used).  Returns
+        the FP32 output, a device [P, 8] array (channels 0..2)."""
         self._buffers(n, h, w)
         self.x = x_bf16
         a = x_bf16
         for k, (cin, cout, g) in enumerate(self.layers):
-            _check(self.L.nemo2d_nn_conv_fwd(_ptr(a), n, h, w, cin, _ptr(self.wf[k]), _fptr(self.b[k]),
-                                             cout, _fptr(self.z[k]), _ptr(self.act[k]), 1 if g else 0,
-                                             self.stream))
+            _check(self.L.nemo2d_nn_conv_fwd(a.ptr, n, h, w, cin, self.wf[k].ptr, self.b[k].fptr(),
+                                             cout, self.z[k].fptr(), self.act[k].ptr, 1 if g else 0,
+                                             self.STREAM))
             a = self.act[k]
         return self.z[-1]
 
-    def backward(self, target, wch):
-        """target: FP32 [P, 8]; wch: FP32 tensor [3] of channel weights.
-        Gradients into self.gw/self.gb; returns the loss (a device tensor)."""
+    def backward(self, target, wch, want_loss=True):
+        """target: device FP32 [P, 8]; wch: device FP32 [3] channel weights.
+        Gradients into self.gw/self.gb; returns the loss (None when
+        want_loss is false: no host synchronisation)."""
         n, h, w = self._shape
         P = n * h * w
         nl = len(self.layers)
-        _check(self.L.nemo2d_nn_mse(_fptr(self.z[-1]), _fptr(target), P, self.layers[-1][1], _fptr(wch),
-                                    _ptr(self.dz[-1]), _fptr(self.part), self.nparts, self.stream))
+        _check(self.L.nemo2d_nn_mse(self.z[-1].fptr(), target.fptr(), P, self.layers[-1][1],
+                                    wch.fptr(), self.dz[-1].ptr, self.part.fptr(), self.nparts,
+                                    self.STREAM))
         for k in range(nl - 1, -1, -1):
             cin, cout, _ = self.layers[k]
             xin = self.act[k - 1] if k > 0 else self.x
-            _check(self.L.nemo2d_nn_conv_bwd_weight(_ptr(xin), _ptr(self.dz[k]), n, h, w, cin, cout,
-                                                    _fptr(self.gw[k]), _fptr(self.gb[k]),
-                                                    _fptr(self.work), self.stream))
+            _check(self.L.nemo2d_nn_conv_bwd_weight(xin.ptr, self.dz[k].ptr, n, h, w, cin, cout,
+                                                    self.gw[k].fptr(), self.gb[k].fptr(),
+                                                    self.work.fptr(), self.STREAM))
             if k > 0:
-                dact = self.torch.as_strided(self.dact, (P, pad8(cin)), (pad8(cin), 1))
-                _check(self.L.nemo2d_nn_conv_bwd_data(_ptr(self.dz[k]), n, h, w, cout, _ptr(self.wb[k]),
-                                                      cin, _fptr(dact), self.stream))
-                _check(self.L.nemo2d_nn_gelu_bwd(_fptr(dact), _fptr(self.z[k - 1]), P * pad8(cin),
-                                                 _ptr(self.dz[k - 1]), self.stream))
-        return self.part.sum()
+                # dact as a contiguous [P, pad8(cin)] array at the start of its buffer
+                _check(self.L.nemo2d_nn_conv_bwd_data(self.dz[k].ptr, n, h, w, cout, self.wb[k].ptr,
+                                                      cin, self.dact.fptr(), self.STREAM))
+                _check(self.L.nemo2d_nn_gelu_bwd(self.dact.fptr(), self.z[k - 1].fptr(),
+                                                 P * pad8(cin), self.dz[k - 1].ptr, self.STREAM))
+        if not want_loss:
+            return None
+        return float(self.part.download().astype(np.float64).sum())
 
     def step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
         for k in range(len(self.layers)):
             for p, g, m, v in ((self.w[k], self.gw[k], self.mw[k], self.vw[k]),
                                (self.b[k], self.gb[k], self.mb[k], self.vb[k])):
-                _check(self.L.nemo2d_nn_adam(_fptr(p), _fptr(g), _fptr(m), _fptr(v), p.numel(), lr, b1,
-                                             b2, eps, self.step_count, self.stream))
+                _check(self.L.nemo2d_nn_adam(p.fptr(), g.fptr(), m.fptr(), v.fptr(), p.size, lr, b1,
+                                             b2, eps, self.step_count, self.STREAM))
         self.pack()
 
     def grads(self):
-        return ([g.cpu().numpy().astype(np.float64) for g in self.gw],
-                [g.cpu().numpy().astype(np.float64) for g in self.gb])
+        return ([g.download().astype(np.float64) for g in self.gw],
+                [g.download().astype(np.float64) for g in self.gb])

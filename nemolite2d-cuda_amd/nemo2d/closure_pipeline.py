@@ -17,9 +17,13 @@ Restriction R (fine -> coarse) on the C-grid:
   sshn_u, sshn_v are re-derived from ssh_c with the serial model's next
   rules (original/nemolite2d.f90:735-765), row 0 of sshn_v staying 0.
 Training pairs: from C_t = R(F_t) the coarse model runs K = 10 steps to C';
-the target is R(F_{t+K}) - C' at T points (ssh, u and v averaged to the
-cell); the input is C' (ssh, u, v at T points, depth).  Applying the closure
-adds the predicted increment every K coarse steps.
+the target is R(F_{t+K}) - C' and the input is C' plus the depth, one
+pixel per T cell carrying the cell's own C-grid points: ssh at the centre,
+u on its east face, v on its north face (no averaging, so an increment
+lands exactly where it was measured; the west and south outer faces, one a
+wall and one the open boundary's Flather face, are never corrected, and the
+east and north wall faces stay 0).  Applying the closure adds the predicted
+increment every K coarse steps.
 """
 import numpy as np
 
@@ -83,10 +87,8 @@ def restrict_state(fine_state, fine_grid, coarse_g):
 
 
 def t_fields(state):
-    """ssh, u, v at T points."""
-    u = 0.5 * (state["un"][:, :-1] + state["un"][:, 1:])
-    v = 0.5 * (state["vn"][:-1, :] + state["vn"][1:, :])
-    return np.stack([state["sshn"], u, v], -1)
+    """Per T cell: ssh, u on the east face, v on the north face."""
+    return np.stack([state["sshn"], state["un"][:, 1:], state["vn"][1:, :]], -1)
 
 
 def features(state, ht):
@@ -94,13 +96,13 @@ def features(state, ht):
 
 
 def apply_increment(state, d, grid):
-    """Add T-point increments d[..., 0:3] (ssh, u, v): ssh directly, face
-    velocities by the mean of the two adjacent cells on interior faces (outer
-    faces keep their boundary values)."""
+    """Add per-cell increments d[..., 0:3] (ssh at the centre, u on the east
+    face, v on the north face, as in t_fields); the outer faces keep their
+    boundary values."""
     s = {k: v.copy() for k, v in state.items()}
     s["sshn"] += d[..., 0]
-    s["un"][:, 1:-1] += 0.5 * (d[:, :-1, 1] + d[:, 1:, 1])
-    s["vn"][1:-1, :] += 0.5 * (d[:-1, :, 2] + d[1:, :, 2])
+    s["un"][:, 1:-1] += d[:, :-1, 1]
+    s["vn"][1:-1, :] += d[:-1, :, 2]
     su, sv = next_interp(s["sshn"], grid["e12t"][0, 0], grid["e12u"][0, 0], grid["e12v"][0, 0])
     s["sshn_u"], s["sshn_v"] = su, sv
     return s
@@ -166,35 +168,33 @@ class Closure:
         self.xm = X.reshape(-1, 4).mean(0)
         self.xs = X.reshape(-1, 4).std(0) + 1e-12
         self.ys = Y.reshape(-1, 3).std(0) + 1e-30
-        self.torch = self.net.torch
 
     def _x(self, X):
-        t = self.torch
         n, h, w, _ = X.shape
         xp = np.zeros((n * h * w, 8), dtype=np.float32)
         xp[:, :4] = ((X - self.xm) / self.xs).reshape(-1, 4)
-        return self.net.to_bf16(t.tensor(xp, device="cuda"))
+        return self.net.to_bf16(xp)
 
     def _y(self, Y):
         n, h, w, _ = Y.shape
         yp = np.zeros((n * h * w, 8), dtype=np.float32)
         yp[:, :3] = (Y / self.ys).reshape(-1, 3)
-        return self.torch.tensor(yp, device="cuda")
+        return self.net.upload(yp)
 
     def train(self, X, Y, iters=400, batch=8, lr=2e-3, seed=0, log=None):
         rng = np.random.default_rng(seed)
-        t = self.torch
-        wch = t.ones(3, dtype=t.float32, device="cuda")
+        wch = self.net.upload(np.ones(3, dtype=np.float32))
         n, h, w, _ = X.shape
         losses = []
         for it in range(iters):
             idx = rng.choice(n, size=min(batch, n), replace=False)
             xb, yb = self._x(X[idx]), self._y(Y[idx])
             self.net.forward(xb, len(idx), h, w)
-            loss = self.net.backward(yb, wch)
+            record = it % 50 == 0 or it == iters - 1
+            loss = self.net.backward(yb, wch, want_loss=record)
             self.net.step(lr=lr * (0.5 * (1 + np.cos(np.pi * it / iters))))
-            if it % 50 == 0 or it == iters - 1:
-                losses.append(float(loss))
+            if record:
+                losses.append(loss)
                 if log:
                     log("closure train it %d loss %.4g" % (it, losses[-1]))
         return losses
@@ -202,7 +202,7 @@ class Closure:
     def predict(self, X):
         n, h, w, _ = X.shape
         y = self.net.forward(self._x(X), n, h, w)
-        return y.cpu().numpy()[:, :3].reshape(n, h, w, 3) * self.ys
+        return y.download()[:, :3].reshape(n, h, w, 3) * self.ys
 
     def correct(self, state, grid):
         d = self.predict(features(state, grid["ht"])[None])[0]

@@ -62,3 +62,18 @@ def test_increment_goes_to_interior_faces_only():
     assert np.all(s["un"][:, 1:-1] == 1.0) and np.all(s["un"][:, [0, -1]] == 0.0)
     assert np.all(s["vn"][1:-1, :] == 2.0) and np.all(s["vn"][[0, -1], :] == 0.0)
     assert np.all(s["sshn"] == 0.5)
+
+
+def test_increment_of_field_difference_recovers_target():
+    """apply_increment(a, t_fields(b) - t_fields(a)) puts b's values on every
+    corrected point: the cell's ssh, its east face and its north face."""
+    rng = np.random.default_rng(3)
+    g = nemo2d.default_grid(7, 6)
+    a = {k: rng.standard_normal(v.shape) for k, v in nemo2d.initial_state(g).items()}
+    b = {k: rng.standard_normal(v.shape) for k, v in a.items()}
+    s = cp.apply_increment(a, cp.t_fields(b) - cp.t_fields(a), g)
+    assert np.allclose(s["sshn"], b["sshn"], rtol=0, atol=1e-14)
+    assert np.allclose(s["un"][:, 1:-1], b["un"][:, 1:-1], rtol=0, atol=1e-14)
+    assert np.allclose(s["vn"][1:-1, :], b["vn"][1:-1, :], rtol=0, atol=1e-14)
+    assert np.array_equal(s["un"][:, [0, -1]], a["un"][:, [0, -1]])
+    assert np.array_equal(s["vn"][[0, -1], :], a["vn"][[0, -1], :])

@@ -17,7 +17,6 @@ def bf16(a):
 
 
 def setup(seed, n=2, h=11, w=13):
-    import torch
     from nemo2d.closure import ConvNet, LAYERS
     rng = np.random.default_rng(seed)
     net = ConvNet(seed=seed)
@@ -30,8 +29,8 @@ def setup(seed, n=2, h=11, w=13):
     xp[:, :4] = x.reshape(-1, 4)
     tp = np.zeros((n * h * w, 8), dtype=np.float32)
     tp[:, :3] = t.reshape(-1, 3)
-    xg = net.to_bf16(torch.tensor(xp, device="cuda"))
-    tg = torch.tensor(tp, device="cuda")
+    xg = net.to_bf16(xp)
+    tg = net.upload(tp)
     # the reference sees the BF16-rounded operands the GPU sees
     params = [(bf16(w_), b_.astype(np.float32).astype(np.float64)) for w_, b_ in zip(ws, bs)]
     return net, params, bf16(x), t, xg, tg, (n, h, w)
@@ -43,7 +42,7 @@ def relerr(a, b):
 
 def test_forward_matches_reference():
     net, params, x, t, xg, tg, (n, h, w) = setup(0)
-    y = net.forward(xg, n, h, w).cpu().numpy()[:, :3].reshape(n, h, w, 3)
+    y = net.forward(xg, n, h, w).download()[:, :3].reshape(n, h, w, 3)
     ref, _ = cr.forward(params, x)
     assert relerr(y, ref) < 2e-2, relerr(y, ref)
 
@@ -51,7 +50,6 @@ def test_forward_matches_reference():
 def test_first_layer_exact_accumulation():
     """One convolution with BF16 operands: FP32 accumulation of exactly
     representable products; agrees with FP64 to FP32 rounding."""
-    import torch
     from nemo2d import closure as C
     rng = np.random.default_rng(3)
     n, h, w, cin, cout = 1, 7, 9, 4, 32
@@ -63,16 +61,15 @@ def test_first_layer_exact_accumulation():
     x = bf16(rng.standard_normal((n, h, w, cin)))
     xp = np.zeros((n * h * w, 8), dtype=np.float32)
     xp[:, :4] = x.reshape(-1, 4)
-    net.forward(net.to_bf16(torch.tensor(xp, device="cuda")), n, h, w)
-    z0 = net.z[0].cpu().numpy()[:, :cout].reshape(n, h, w, cout)
+    net.forward(net.to_bf16(xp), n, h, w)
+    z0 = net.z[0].download()[:, :cout].reshape(n, h, w, cout)
     ref = cr.conv_fwd(x, wgt, bias)
     assert np.allclose(z0, ref, rtol=2e-6, atol=2e-5)
 
 
 def test_gradients_match_reference():
-    import torch
     net, params, x, t, xg, tg, (n, h, w) = setup(1)
-    wch = torch.tensor([1.0, 0.5, 2.0], dtype=torch.float32, device="cuda")
+    wch = net.upload(np.array([1.0, 0.5, 2.0]))
     net.forward(xg, n, h, w)
     loss = float(net.backward(tg, wch))
     gw, gb = net.grads()
@@ -83,11 +80,38 @@ def test_gradients_match_reference():
         assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
 
 
+def test_gradients_match_reference_over_several_pixel_chunks():
+    """P = 3*41*43 = 5289 pixels: the weight gradient is summed over two
+    4096-pixel chunks and the loss over many blocks."""
+    net, params, x, t, xg, tg, (n, h, w) = setup(4, n=3, h=41, w=43)
+    wch = net.upload(np.array([1.0, 1.0, 1.0]))
+    net.forward(xg, n, h, w)
+    loss = float(net.backward(tg, wch))
+    gw, gb = net.grads()
+    ref_loss, ref_grads = cr.loss_and_grads(params, x, t, np.ones(3))
+    assert abs(loss - ref_loss) <= 3e-2 * ref_loss
+    for k in range(4):
+        assert relerr(gw[k], ref_grads[k][0]) < 5e-2, (k, relerr(gw[k], ref_grads[k][0]))
+        assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
+
+
+def test_device_arrays_round_trip():
+    from nemo2d.closure import DevArray
+    a = np.arange(1000, dtype=np.float32) * 0.5
+    d = DevArray.from_numpy(a)
+    assert np.array_equal(d.download(), a)
+    d.zero()
+    assert not d.download().any()
+    b = DevArray((7, 3), np.int16)
+    assert b.download().shape == (7, 3) and not b.download().any()
+    d.close()
+    b.close()
+
+
 def test_training_reduces_loss():
     """Adam on a fixed batch: the loss falls by at least 5x in 300 steps."""
-    import torch
     net, params, x, t, xg, tg, (n, h, w) = setup(2, n=4, h=16, w=16)
-    wch = torch.ones(3, dtype=torch.float32, device="cuda")
+    wch = net.upload(np.ones(3))
     losses = []
     for it in range(300):
         net.forward(xg, n, h, w)
