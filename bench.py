@@ -133,8 +133,8 @@ def form_closure(nc, runs, snaps, iters, windows):
     t0 = time.perf_counter()
     ep, ec, fell = cp.evaluate(held, clos, n_windows=windows)
     t_eval = time.perf_counter() - t0
-    # throughput: plain coarse, coarse with the correction every 10 steps
-    # (device round trip included), fine
+    # throughput: plain coarse, coarse with the on-device correction every
+    # 10 steps, fine
     csteps = 10 * windows
     rates = {}
     for name in ("coarse", "coarse+closure", "fine"):
@@ -145,8 +145,7 @@ def form_closure(nc, runs, snaps, iters, windows):
         if name == "coarse+closure":
             for w in range(windows):
                 m.run(10 * w + 1, 10 * w + 10)
-                s, _ = clos.correct(m.retrieve(), held.coarse_grid)
-                m.set_state(s)
+                clos.correct_model(m)
             m.checksums()
         else:
             m.run(1, nst)

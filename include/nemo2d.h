@@ -117,8 +117,13 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
  * per-cell with Flather points recomputed inline instead of in the
  * epilogue, 8/9 marching (16/32 rows) dividing by precomputed reciprocals
  * with two exact FMA corrections, 10/11 marching (16/32 rows) with the next
- * row's loads issued one iteration ahead.  Marching needs uniform metrics
- * and depths; other grids use 1 (or 7).  No counterpart in the reference:
+ * row's loads issued one iteration ahead, 12/13 marching (16/32 rows) with
+ * face terms shared between neighbouring points and loads one row ahead,
+ * 14/16 the same at 16/32 rows without the load-ahead, 15 march16_sh
+ * within a 3-waves-per-SIMD register budget, 17 march16_shn within a
+ * 4-waves-per-SIMD budget, 18/19/20 shared terms without load-ahead at
+ * 8/12/24 rows.  Marching needs uniform
+ * metrics and depths; other grids use 1 (or 7).  No counterpart in the reference:
  * a tuning hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
 
@@ -235,6 +240,20 @@ int nemo2d_nn_adam(float *p, const float *g, float *m, float *v, long n, float l
                    float b2, float eps, int step, void *stream);
 int nemo2d_nn_pack_weights(const float *w, int cin, int cout, void *wf, void *wb, void *stream);
 int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
+
+/* Learned closure on the model's device state (FP64 model; no counterpart
+ * in the reference).  Pixel p = (j-1)*jpi + (i-1) stands for T cell (i, j)
+ * with its east face U(i, j) and north face V(i, j).
+ * features: x (device, BF16 bits [jpj*jpi][8]) gets, per pixel, BF16 of
+ *   FP32((value - mean[c]) / scale[c]) for value = sshn, un, vn, ht of the
+ *   cell (c = 0..3); channels 4..7 are 0.  Synchronous.
+ * apply: adds (double)z[p][c] * scale[c] (z device FP32 [jpj*jpi][zstride])
+ *   to sshn (every cell, c = 0), un on the east faces with i < jpi (c = 1)
+ *   and vn on the north faces with j < jpj (c = 2), then re-derives sshn_u
+ *   and sshn_v from the new sshn by the next-stage rules.  Waits for the
+ *   default stream first (where z is produced); synchronous. */
+int nemo2d_closure_features(nemo2d_model *m, const double *mean, const double *scale, void *x);
+int nemo2d_closure_apply(nemo2d_model *m, const float *z, int zstride, const double *scale);
 
 /* Device memory for the closure network's tensors, on the current HIP
  * device (no counterpart in the reference).  Copies and zeroing are

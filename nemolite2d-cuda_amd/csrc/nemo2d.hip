@@ -88,6 +88,7 @@ struct nemo2d_model {
     int* flv_ptr = nullptr;   // jpi+1 column pointers (columns 1..jpi)
     long* flv_pts = nullptr;
     int nflu = 0, nflv = 0;
+    int nflv0 = 0;            // v points of row 0 whose inner point is north
 
     // checksum reduction
     double* partials = nullptr;  // 6 doubles per block
@@ -401,7 +402,8 @@ k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
            const uint16_t* __restrict__ fl, const double* __restrict__ flu,
            const double* __restrict__ flv, const double* __restrict__ sshn_u,
            const double* __restrict__ sshn_v, double* __restrict__ un, double* __restrict__ vn,
-           Layout L, double* __restrict__ partials, int nparts, double* __restrict__ out) {
+           Layout L, double* __restrict__ partials, int nparts, double* __restrict__ out,
+           int skip_v0) {
     const long P = L.pitch;
     DD cu = {0.0, 0.0}, cv = {0.0, 0.0};
     for (int line = threadIdx.x; line < nul; line += RED_THREADS)
@@ -415,6 +417,8 @@ k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
     for (int line = threadIdx.x; line < nvl; line += RED_THREADS)
         for (int k = vptr[line]; k < vptr[line + 1]; ++k) {
             const long o = vpts[k];
+            // row-0 points with the inner point north: done by the step kernel
+            if (skip_v0 && (fl[o] & V_FL_NORTH) && (o - L.base) / P == 0) continue;
             const long oi = (fl[o] & V_FL_NORTH) ? o + P : o - P;
             const double v = vn[oi] + flv[o] * (sshn_v[o] - sshn_v[oi]);
             vn[o] = v;
@@ -423,6 +427,306 @@ k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
     block_reduce3(DD{0.0, 0.0}, cu, cv, partials + 6 * (long)nparts);
     __syncthreads();
     if (out) finish_partials(partials, nparts + 1, out);
+}
+
+// ---- fused step, column-marching with shared face terms ----------------------------
+// Same work split and results as k_step_march (uniform metrics and depths),
+// with fewer operations per point:
+//  * terms on a face shared by two points of one column are computed once,
+//    at the lower point as its north term, and carried up a row as the
+//    upper point's south term (continuity's v flux, momentum_u's south
+//    advective flux and v average, momentum_v's south advective flux and
+//    dv/dy);
+//  * terms on a face shared by two points of one row are computed once, by
+//    the lane of the east point as its west term, and handed to the west
+//    lane by a lane shuffle (momentum_u's east advective flux and du/dx,
+//    momentum_v's east advective flux and u average);
+//  * the register window is a ring of four rows (three in use, one being
+//    loaded) whose roles rotate with the row, so no row is ever copied.
+// Every shared term is the same expression tree as the reference's own (an
+// IEEE sum a+b equals b+a), so the results are bit-identical.  At a wall the
+// shared advective term differs from the walled point's, which recomputes
+// its own.
+struct Row5 {
+    double un[3], vn[3], ss[3], su[3], sv[3];  // columns i-1, i, i+1
+};
+// Accesses by 32-bit byte offset from a kernel-argument base (the whole
+// padded array is below 4 GiB), so the address is base + offset in the
+// load/store itself.
+template <class T>
+__device__ __forceinline__ T ldb(const T* b, unsigned boff) {
+    return *(const T*)((const char*)b + boff);
+}
+template <class T>
+__device__ __forceinline__ void stb(T* b, unsigned boff, T v) {
+    *(T*)((char*)b + boff) = v;
+}
+// row at byte offset boff of column i-1
+__device__ __forceinline__ void load_row5(Row5& r, const NState& n, unsigned boff) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.un[c] = *(const double*)((const char*)n.un + boff + 8 * c);
+        r.vn[c] = *(const double*)((const char*)n.vn + boff + 8 * c);
+        r.ss[c] = *(const double*)((const char*)n.sshn + boff + 8 * c);
+        r.su[c] = *(const double*)((const char*)n.sshn_u + boff + 8 * c);
+        r.sv[c] = *(const double*)((const char*)n.sshn_v + boff + 8 * c);
+    }
+}
+
+// Terms on the north faces of the points of row `c` (row `nr` above it).
+struct NorthTerms {
+    double r3;     // continuity: (sshn_v + hv) * vn
+    double kn;     // momentum_u: uu_n * v_n * depn, no wall
+    double vnc;    // momentum_u: v_nc
+    double vnu;    // momentum_u: v_n
+    double depnu;  // momentum_u: depn
+    double hn;     // momentum_v: vv_n * v_n * depn
+    double dvdyn;  // momentum_v: dvdy_n
+};
+__device__ __forceinline__ NorthTerms north_terms(const Scal& s, const Row5& c, const Row5& nr) {
+    const double h = 0.5;
+    NorthTerms t;
+    t.r3 = (c.sv[1] + s.hv) * c.vn[1];
+    // momentum_u (original/nemolite2d.f90:439-514) at this row's U point
+    t.vnc = h * (c.vn[1] + c.vn[2]);
+    t.vnu = h * t.vnc * (s.e1v + s.e1v);
+    t.depnu = h * (s.hv + c.sv[1] + s.hv + c.sv[2]);
+    const double sn = sgn_half(t.vnu);
+    const double uun = (h + sn) * c.un[1] + (h - sn) * nr.un[1];
+    t.kn = uun * t.vnu * t.depnu;
+    // momentum_v (:527-606) at this row's V point
+    const double vn_n = h * (c.vn[1] + nr.vn[1]) * s.e1t;
+    const double depn = s.ht + nr.ss[1];
+    const double sv = sgn_half(vn_n);
+    const double vvn = (h + sv) * c.vn[1] + (h - sv) * nr.vn[1];
+    t.hn = vvn * vn_n * depn;
+    t.dvdyn = (nr.vn[1] - c.vn[1]) / s.e2t * (s.ht + nr.ss[1]);
+    return t;
+}
+
+struct ShCarry {
+    NorthTerms south;          // the row below's north terms
+    double s_prev, sv_prev;    // new ssh and old sshn_v at (i, j-1)
+    unsigned f_prev, f_nxt;
+    double ct, cu, cv;         // per-lane checksum sums
+};
+
+struct ShCtx {
+    Scal s;
+    Params p;
+    NState n;
+    AOut a;
+    const uint16_t* fl;
+    const double* flv;  // Flather sqrt(g/hv) (row-0 points)
+    Layout L;
+    double tide;
+    int i, il;
+    unsigned lb8;  // 8 * (il - 1): byte offset of the window's west column in a row
+    bool own, tcol;
+    __device__ unsigned rowb(int j) const { return (unsigned)(8 * L.at(0, j)); }
+};
+
+// One row j: A, B, C hold rows j-1, j, j+1; with PF, D receives row j+2
+// (C was loaded one row earlier), else C is loaded here.
+template <bool PF>
+__device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A, const Row5& B,
+                                       Row5& C, Row5& D, int j) {
+    const Layout& L = x.L;
+    const Scal& s = x.s;
+    const long P = L.pitch;
+    const double h = 0.5;
+    const long o = L.at(x.il, j);
+    const unsigned f = k.f_nxt;
+    if constexpr (PF) {
+        const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
+        load_row5(D, x.n, x.rowb(jn) + x.lb8);
+    } else {
+        load_row5(C, x.n, x.rowb(j + 1) + x.lb8);
+    }
+    {
+        const int jf = j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1;
+        k.f_nxt = ldb(x.fl, (x.rowb(jf) + x.lb8 + 8) >> 2);
+    }
+    const unsigned o8 = x.rowb(j) + x.lb8 + 8;  // byte offset of (i, j)
+    const NorthTerms nt = north_terms(s, B, C);
+    const NorthTerms& st = k.south;
+
+    // continuity (original/nemolite2d.f90:399-403)
+    double s_c = 0.0;
+    if (j >= 1 && x.tcol) {
+        if (f & T_CLAMP) {
+            s_c = x.tide;
+        } else {
+            const double r1 = (B.su[1] + s.hu) * B.un[1];
+            const double r2 = (B.su[0] + s.hu) * B.un[0];
+            s_c = B.ss[1] + (r2 - r1 + st.r3 - nt.r3) * x.p.rdt / s.e12t;
+        }
+    }
+    const double s_e = __shfl_down(s_c, 1, 64);
+
+    // west-face terms of this lane's points; the east ones come from lane+1
+    const double un_c = B.un[1];
+    const double u_w = h * (un_c + B.un[0]) * s.e2t;
+    const double depw_u = s.ht + B.ss[1];
+    const double sw_u = sgn_half(u_w);
+    const double fw = ((h - sw_u) * un_c + (h + sw_u) * B.un[0]) * u_w * depw_u;
+    const double dudx_w = (un_c - B.un[0]) / s.e1t * (s.ht + B.ss[1]);
+    const double vn_c = B.vn[1];
+    const double u_wc = h * (B.un[0] + C.un[0]);
+    const double u_wv = h * u_wc * (s.e2u + s.e2u);
+    const double depw_v = h * (s.hu + B.su[0] + s.hu + C.su[0]);
+    const double sw_v = sgn_half(u_wv);
+    const double gw = ((h - sw_v) * vn_c + (h + sw_v) * B.vn[0]) * u_wv * depw_v;
+    const double fe = __shfl_down(fw, 1, 64);
+    const double dudx_e = __shfl_down(dudx_w, 1, 64);
+    const double ge = __shfl_down(gw, 1, 64);
+    const double u_ec = __shfl_down(u_wc, 1, 64);
+
+    const double su_old = B.su[1], sv_old = B.sv[1];
+    if (x.own) {
+        const int i = x.i;
+        double u = 0.0, v = 0.0;
+        const unsigned cu = ucls(f), cv = vcls(f);
+        if (j >= 1 && cu == CLS_COMP) {
+            const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
+            double ts = st.kn, tn = nt.kn;
+            if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
+            if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
+            const double adv = fw - fe + ts - tn;
+            double dudy_s = 0.0, dudy_n = 0.0;
+            if (!ws) dudy_s = (un_c - A.un[1]) / (s.e2u + s.e2u) * (s.hu + B.su[1] + s.hu + A.su[1]);
+            if (!wn) dudy_n = (C.un[1] - un_c) / (s.e2u + s.e2u) * (s.hu + B.su[1] + s.hu + C.su[1]);
+            double vis = (dudx_e - dudx_w) * s.e2u + (dudy_n - dudy_s) * s.e1u * h;
+            vis = x.p.visc * vis;
+            const double hsu = s.hu + B.su[1];
+            const double cor = h * (s.fcu * (st.vnc + nt.vnc)) * s.e12u * hsu;
+            const double hpg = -GRAV * hsu * s.e2u * (B.ss[2] - B.ss[1]);
+            u = (un_c * hsu + x.p.rdt * (adv + vis + cor + hpg) / s.e12u) / (s.hu + 0.0) / x.p.fric;
+        }
+        if (i >= 1 && cv == CLS_COMP) {
+            const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
+            double tw = gw, te = ge;
+            if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
+            if (we) {
+                const double u_e = h * u_ec * (s.e2u + s.e2u);
+                const double depe = h * (s.hu + B.su[1] + s.hu + C.su[1]);
+                te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
+            }
+            const double adv = tw - te + st.hn - nt.hn;
+            double dvdx_w = 0.0, dvdx_e = 0.0;
+            if (!ww) dvdx_w = (vn_c - B.vn[0]) / (s.e1v + s.e1v) * (s.hv + B.sv[1] + s.hv + B.sv[0]);
+            if (!we) dvdx_e = (B.vn[2] - vn_c) / (s.e1v + s.e1v) * (s.hv + B.sv[1] + s.hv + B.sv[2]);
+            double vis = (nt.dvdyn - st.dvdyn) * s.e1v + (dvdx_e - dvdx_w) * s.e2v * h;
+            vis = x.p.visc * vis;
+            const double hsv = s.hv + B.sv[1];
+            const double cor = -h * (s.fcv * (u_ec + u_wc)) * s.e12v * hsv;
+            const double hpg = -GRAV * hsv * s.e1v * (C.ss[1] - B.ss[1]);
+            v = (vn_c * hsv + x.p.rdt * (adv + vis + cor + hpg) / s.e12v) / (s.hv + 0.0) / x.p.fric;
+        }
+        const StUniform stu{s};
+        if (j >= 1) {
+            if (i >= 1) {
+                stb(x.a.sshn, o8, s_c);
+                k.ct += fabs(s_c);
+            }
+            if (cu != CLS_FLATHER) {
+                stb(x.a.un, o8, u);
+                if (i >= 1) k.cu += fabs(u);
+            }
+            stb(x.a.sshn_u, o8, next_su(stu, sucls(f), o, s_c, s_e, su_old));
+        }
+        if (i >= 1) {
+            if (cv != CLS_FLATHER) {
+                stb(x.a.vn, o8, v);
+                if (j >= 1) k.cv += fabs(v);
+            }
+            // bc_flather_v (original/nemolite2d.f90:676-690) at a row-0 point
+            // whose inner point is this one; row 0 is outside the checksums
+            if (j == 1 && vcls(k.f_prev) == CLS_FLATHER && (k.f_prev & V_FL_NORTH)) {
+                const unsigned o08 = o8 - (unsigned)(8 * P);
+                stb(x.a.vn, o08, v + ldb(x.flv, o08) * (A.sv[1] - B.sv[1]));
+            }
+            if (k.f_prev != 0xffffffffu)
+                stb(x.a.sshn_v, o8 - (unsigned)(8 * P),
+                    next_sv(stu, svcls(k.f_prev), o - P, P, k.s_prev, s_c, k.sv_prev));
+        }
+    }
+    k.south = nt;
+    k.s_prev = s_c;
+    k.sv_prev = sv_old;
+    k.f_prev = f;
+}
+
+template <int NJ, int WPE, bool PF>
+__global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
+                const double* __restrict__ flv, Layout L, Params p, double tide,
+                double* __restrict__ partials, int nbx, int ntiles) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    ShCarry k;
+    k.ct = k.cu = k.cv = 0.0;
+    if (t < ntiles) {
+        const int bx = t % nbx, by = t / nbx;
+        ShCtx x;
+        x.s = sc_;
+        x.p = p;
+        x.n = n;
+        x.a = a;
+        x.fl = fl;
+        x.flv = flv;
+        x.L = L;
+        x.tide = tide;
+        x.i = (bx * MW + wave) * MOUT + lane;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+        x.lb8 = (unsigned)(8 * (x.il - 1));
+        x.own = lane < MOUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = by * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        Row5 R0, R1, R2, R3;
+        load_row5(R0, n, x.rowb(jb - 1) + x.lb8);
+        load_row5(R1, n, x.rowb(jb) + x.lb8);
+        if constexpr (PF) load_row5(R2, n, x.rowb(jb + 1) + x.lb8);
+        k.f_nxt = fl[L.at(x.il, jb)];
+        k.f_prev = 0xffffffffu;  // no row below inside this segment
+        k.s_prev = 0.0;
+        k.sv_prev = 0.0;
+        k.south = north_terms(sc_, R0, R1);
+        if constexpr (PF) {
+            for (int j = jb; j <= je; j += 4) {
+                sh_row<true>(x, k, R0, R1, R2, R3, j);
+                if (j + 1 > je) break;
+                sh_row<true>(x, k, R1, R2, R3, R0, j + 1);
+                if (j + 2 > je) break;
+                sh_row<true>(x, k, R2, R3, R0, R1, j + 2);
+                if (j + 3 > je) break;
+                sh_row<true>(x, k, R3, R0, R1, R2, j + 3);
+            }
+        } else {
+            for (int j = jb; j <= je; j += 3) {
+                sh_row<false>(x, k, R0, R1, R2, R3, j);
+                if (j + 1 > je) break;
+                sh_row<false>(x, k, R1, R2, R0, R3, j + 1);
+                if (j + 2 > je) break;
+                sh_row<false>(x, k, R2, R0, R1, R3, j + 2);
+            }
+        }
+        // the last row's sshn_v needs the new ssh one row up
+        if (x.own && x.i >= 1) {
+            StUniform st{sc_};
+            const long P = L.pitch;
+            const long o = L.at(x.il, je);
+            const unsigned c = svcls(k.f_prev);
+            double s_n = 0.0;
+            if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI)) {
+                const long on = o + P;
+                s_n = (fl[on] & T_CLAMP) ? tide : continuity_at(st, n, on, P, p.rdt);
+            }
+            a.sshn_v[o] = next_sv(st, c, o, P, k.s_prev, s_n, k.sv_prev);
+        }
+    }
+    block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
 
 // ---- checksum ----------------------------------------------------------------
@@ -563,6 +867,65 @@ __global__ void k_next(S st, const uint16_t* __restrict__ fl, const double* __re
         vn[o] = va[o];
         const unsigned c = svcls(f);
         if (c != NX_SKIP) sshn_v[o] = next_sv(st, c, o, P, ssha[o], ssha[o + P], 0.0);
+    }
+}
+
+// ---- learned closure on the device state ------------------------------------------
+// Pixel p = (j-1)*jpi + (i-1) stands for T cell (i, j) with its east face U(i, j)
+// and north face V(i, j).
+struct Norm4 {
+    double m[4], s[4];
+};
+struct Scale3 {
+    double s[3];
+};
+
+template <class S>
+__global__ void __launch_bounds__(TBX * TBY)
+k_closure_features(S st, const double* __restrict__ sshn, const double* __restrict__ un,
+                   const double* __restrict__ vn, Layout L, Norm4 nm, uint16_t* __restrict__ x) {
+    const int i = blockIdx.x * TBX + threadIdx.x + 1, j = blockIdx.y * TBY + threadIdx.y + 1;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j);
+    const long p = (long)(j - 1) * L.jpi + (i - 1);
+    const double v[4] = {sshn[o], un[o], vn[o], st.ht(o)};
+    uint16_t* xp = x + 8 * p;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xp[c] = bf16_rne((float)((v[c] - nm.m[c]) / nm.s[c]));
+#pragma unroll
+    for (int c = 4; c < 8; ++c) xp[c] = 0;
+}
+
+// Increments: sshn at every T cell, un on the interior east faces
+// (i < jpi), vn on the interior north faces (j < jpj).
+__global__ void __launch_bounds__(TBX * TBY)
+k_closure_add(const float* __restrict__ z, int zs, Scale3 sc, Layout L, double* __restrict__ sshn,
+              double* __restrict__ un, double* __restrict__ vn) {
+    const int i = blockIdx.x * TBX + threadIdx.x + 1, j = blockIdx.y * TBY + threadIdx.y + 1;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j);
+    const float* zp = z + (long)zs * ((long)(j - 1) * L.jpi + (i - 1));
+    sshn[o] = sshn[o] + (double)zp[0] * sc.s[0];
+    if (i < L.jpi) un[o] = un[o] + (double)zp[1] * sc.s[1];
+    if (j < L.jpj) vn[o] = vn[o] + (double)zp[2] * sc.s[2];
+}
+
+// sshn_u, sshn_v from the (new) sshn by the next-stage rules.
+template <class S>
+__global__ void __launch_bounds__(TBX * TBY)
+k_closure_interp(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn, Layout L,
+                 double* __restrict__ sshn_u, double* __restrict__ sshn_v) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j), P = L.pitch;
+    const unsigned f = fl[o];
+    if (j >= 1) {
+        const unsigned c = sucls(f);
+        if (c != NX_SKIP) sshn_u[o] = next_su(st, c, o, sshn[o], sshn[o + 1], 0.0);
+    }
+    if (i >= 1) {
+        const unsigned c = svcls(f);
+        if (c != NX_SKIP) sshn_v[o] = next_sv(st, c, o, P, sshn[o], sshn[o + P], 0.0);
     }
 }
 
@@ -912,7 +1275,8 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
     // partial slots: the most blocks any step kernel launches (the
     // thread-per-cell grid, or the 16-row marching grid padded to 8 XCDs)
     {
-        const long mb = (long)((jpi + 1 + MW * MOUT - 1) / (MW * MOUT)) * ((jpj + 1 + 15) / 16);
+        // marching grids: segments of at least 8 rows
+        const long mb = (long)((jpi + 1 + MW * MOUT - 1) / (MW * MOUT)) * ((jpj + 1 + 7) / 8);
         const long mbp = (mb + 7) / 8 * 8;
         m->npart_max = (int)((long)(g.x * g.y) > mbp ? (long)(g.x * g.y) : mbp);
     }
@@ -991,6 +1355,9 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     m->flu_ptr = nullptr; m->flu_pts = nullptr; m->flv_ptr = nullptr; m->flv_pts = nullptr;
     m->nflu = (int)upts.size();
     m->nflv = (int)vpts.size();
+    m->nflv0 = 0;
+    for (long o : vpts)
+        if ((o - L.base) / L.pitch == 0 && (fl[o] & V_FL_NORTH)) ++m->nflv0;
     HIPCHK(hipMalloc((void**)&m->flu_ptr, uptr.size() * sizeof(int)));
     HIPCHK(hipMalloc((void**)&m->flv_ptr, vptr.size() * sizeof(int)));
     HIPCHK(hipMalloc((void**)&m->flu_pts, (upts.size() + 1) * sizeof(long)));
@@ -1263,7 +1630,16 @@ enum {
     VAR_MARCH32_FD = 9,  //                32-row segments
     VAR_MARCH16_PF = 10,  // marching, 16-row segments, next row prefetched
     VAR_MARCH32_PF = 11,  //                32-row segments
-    VAR_LAST = 11
+    VAR_MARCH16_SH = 12,  // marching with shared face terms, 16-row segments
+    VAR_MARCH32_SH = 13,  //                                  32-row segments
+    VAR_MARCH16_SHN = 14, // shared face terms, 16 rows, no load-ahead
+    VAR_MARCH16_SH3 = 15, // march16_sh within 168 VGPRs (3 waves per SIMD)
+    VAR_MARCH32_SHN = 16, // shared face terms, 32 rows, no load-ahead
+    VAR_MARCH16_SHN4 = 17, // march16_shn within 128 VGPRs (4 waves per SIMD)
+    VAR_MARCH8_SHN = 18,  // shared face terms, no load-ahead, 8/12/24-row segments
+    VAR_MARCH12_SHN = 19,
+    VAR_MARCH24_SHN = 20,
+    VAR_LAST = 20
 };
 
 template <int NJ, bool FLIN, int WPE, bool FAST = false, bool PF = false>
@@ -1277,6 +1653,25 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
     k_step_march<NJ, FLIN, WPE, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
     LAUNCHCHK("step_march");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+// the shared-term kernels address the arrays by 32-bit byte offsets
+static bool sh_fits(const Layout& L) { return (unsigned long)L.size() * sizeof(double) < (1ul << 32); }
+
+template <int NJ, int WPE, bool PF>
+static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    const Layout& L = m->L;
+    if (!sh_fits(L)) return launch_march<16, false, 0, false, true>(m, n, a, tide, nparts);
+    const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    k_step_march_sh<NJ, WPE, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
+    LAUNCHCHK("step_march_sh");
     nparts = nblocks;
     return NEMO2D_OK;
 }
@@ -1326,6 +1721,15 @@ int nemo2d_step(nemo2d_model* m, int istp) {
             break;
         case VAR_MARCH16_PF: rc = launch_march<16, false, 0, false, true>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_PF: rc = launch_march<32, false, 0, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SH: rc = launch_march_sh<16, 0, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_SH: rc = launch_march_sh<32, 0, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SHN: rc = launch_march_sh<16, 0, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SH3: rc = launch_march_sh<16, 3, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_SHN: rc = launch_march_sh<32, 0, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SHN4: rc = launch_march_sh<16, 4, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH8_SHN: rc = launch_march_sh<8, 0, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_SHN: rc = launch_march_sh<12, 0, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH24_SHN: rc = launch_march_sh<24, 0, false>(m, n, a, tide, nparts); break;
         default: rc = launch_march<32, false, 0>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
@@ -1333,12 +1737,15 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     m->a_current = false;
     const int slot = istp - m->nit000;
     double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
-    const bool has_fl = (m->nflu + m->nflv) > 0;
+    // the shared-term kernels set the row-0 Flather v points themselves
+    const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L);
+    const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && (has_fl || out)) {
         // Flather points of the new state, then (if recording) the checksums
         k_epilogue<<<1, RED_THREADS, 0, m->stream>>>(
             m->flu_ptr, m->flu_pts, m->nflu ? L.jpj : 0, m->flv_ptr, m->flv_pts, m->nflv ? L.jpi : 0,
-            m->flags, m->flu, m->flv, n.sshn_u, n.sshn_v, a.un, a.vn, L, m->partials, nparts, out);
+            m->flags, m->flu, m->flv, n.sshn_u, n.sshn_v, a.un, a.vn, L, m->partials, nparts, out,
+            sh ? 1 : 0);
         LAUNCHCHK("epilogue");
     } else if (out) {
         k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts, out);
@@ -1413,6 +1820,43 @@ int nemo2d_checksums(nemo2d_model* m, double out[3]) {
     for (int k = 0; k < 3; ++k) out[k] = m->h_pinned[k];
     for (int k = 0; k < 3; ++k)
         if (!isfinite(out[k])) return fail(NEMO2D_ERR_NONFINITE, "nemo2d_checksums: non-finite checksum");
+    return NEMO2D_OK;
+}
+
+int nemo2d_closure_features(nemo2d_model* m, const double* mean, const double* scale, void* x) {
+    READY_STAGE(m, "nemo2d_closure_features");
+    if (!mean || !scale || !x) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_features: null argument");
+    Norm4 nm;
+    for (int c = 0; c < 4; ++c) {
+        nm.m[c] = mean[c];
+        nm.s[c] = scale[c];
+        if (!(scale[c] != 0.0)) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_features: zero scale");
+    }
+    const Layout& L = m->L;
+    double** s = m->S[m->cur];
+    WITH_STATICS(m, st, (k_closure_features<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, s[F_SSHN], s[F_UN], s[F_VN], L, nm, (uint16_t*)x)));
+    LAUNCHCHK("closure_features");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const double* scale) {
+    READY_STAGE(m, "nemo2d_closure_apply");
+    if (!z || !scale || zstride < 3) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_apply: bad argument");
+    // the increments come from work on the default stream
+    HIPCHK(hipStreamSynchronize(nullptr));
+    const Layout& L = m->L;
+    double** s = m->S[m->cur];
+    const Scale3 sc{{scale[0], scale[1], scale[2]}};
+    k_closure_add<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(z, zstride, sc, L, s[F_SSHN],
+                                                                          s[F_UN], s[F_VN]);
+    LAUNCHCHK("closure_add");
+    WITH_STATICS(m, st, (k_closure_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
+    LAUNCHCHK("closure_interp");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->a_current = false;
     return NEMO2D_OK;
 }
 

@@ -20,6 +20,7 @@
 #include <string>
 
 #include "nemo2d.h"
+#include "nemo2d_core.h"
 #include "nemo2d_internal.h"
 
 using namespace nemo2d;
@@ -31,12 +32,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int NB = 256;  // threads per block (4 waves)
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+__device__ __forceinline__ uint16_t f2bf(float f) { return bf16_rne(f); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }

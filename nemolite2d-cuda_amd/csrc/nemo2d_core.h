@@ -397,4 +397,11 @@ __device__ __forceinline__ void block_reduce3(DD a0, DD a1, DD a2, double* out) 
     }
 }
 
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 }  // namespace nemo2d

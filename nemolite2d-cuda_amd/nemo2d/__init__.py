@@ -36,13 +36,15 @@ SYMBOLS = (
     "nemo2d_nn_wgrad_work_floats", "nemo2d_nn_conv_bwd_weight", "nemo2d_nn_gelu_bwd",
     "nemo2d_nn_mse", "nemo2d_nn_adam", "nemo2d_nn_pack_weights", "nemo2d_nn_to_bf16",
     "nemo2d_dev_alloc", "nemo2d_dev_free", "nemo2d_dev_upload", "nemo2d_dev_download",
-    "nemo2d_dev_zero",
+    "nemo2d_dev_zero", "nemo2d_closure_features", "nemo2d_closure_apply",
 )
 
 # nemo2d_set_variant choices
 VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_w4": 5,
             "march32_flin": 6, "cell_flin": 7, "march16_fd": 8, "march32_fd": 9, "march16_pf": 10,
-            "march32_pf": 11}
+            "march32_pf": 11, "march16_sh": 12, "march32_sh": 13,
+            "march16_shn": 14, "march16_sh3": 15, "march32_shn": 16,
+            "march16_shn4": 17, "march8_shn": 18, "march12_shn": 19, "march24_shn": 20}
 
 
 class Nemo2dError(RuntimeError):
@@ -87,6 +89,8 @@ def lib():
         L.nemo2d_ensemble_checksum_log.argtypes = [vp, i, dp, i, ip]
         L.nemo2d_ensemble_retrieve.argtypes = [vp, i, dp, dp, dp]
         L.nemo2d_ensemble_destroy.argtypes = [vp]
+        L.nemo2d_closure_features.argtypes = [vp, dp, dp, vp]
+        L.nemo2d_closure_apply.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, dp]
         L.nemo2d_dev_alloc.argtypes = [ctypes.c_long, ctypes.POINTER(vp)]
         L.nemo2d_dev_free.argtypes = [vp]
         L.nemo2d_dev_upload.argtypes = [vp, vp, ctypes.c_long]
@@ -313,6 +317,19 @@ class Model:
         out = np.zeros(3)
         _check(lib().nemo2d_diagnostics(self._h, _dp(out)))
         return out
+
+    def closure_features(self, mean, scale, x):
+        """Closure input features of the now-state into the device array x
+        (BF16 bits [jpj*jpi, 8]); mean, scale: 4 values each."""
+        m = np.ascontiguousarray(mean, dtype=np.float64)
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        _check(lib().nemo2d_closure_features(self._h, _dp(m), _dp(sc), x.ptr))
+
+    def closure_apply(self, z, scale):
+        """Adds the device FP32 increments z [jpj*jpi, C] times scale (3
+        values) to the now-state and re-derives sshn_u, sshn_v."""
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        _check(lib().nemo2d_closure_apply(self._h, z.fptr(), int(z.shape[-1]), _dp(sc)))
 
     def timer_start(self):
         _check(lib().nemo2d_timer_start(self._h))

@@ -204,6 +204,17 @@ class Closure:
         y = self.net.forward(self._x(X), n, h, w)
         return y.download()[:, :3].reshape(n, h, w, 3) * self.ys
 
+    def correct_model(self, model):
+        """The same correction applied to a Model's device state in place:
+        features, network and increments never leave the GPU."""
+        from .closure import DevArray
+        P = model.jpi * model.jpj
+        if getattr(self, "_feat", None) is None or self._feat.shape[0] != P:
+            self._feat = DevArray((P, 8), np.int16)
+        model.closure_features(self.xm, self.xs, self._feat)
+        z = self.net.forward(self._feat, 1, model.jpj, model.jpi)
+        model.closure_apply(z, self.ys)
+
     def correct(self, state, grid):
         d = self.predict(features(state, grid["ht"])[None])[0]
         return apply_increment(state, d, grid), d
@@ -230,10 +241,9 @@ def evaluate(setup, closure, n_windows, tol=2.0, judge_every=5):
         fm.run(2 * t0 + 1, 2 * (t0 + K_STEPS))
         cp.run(t0 + 1, t0 + K_STEPS)
         cc.run(t0 + 1, t0 + K_STEPS)
-        sc = cc.retrieve()
         if use:
-            sc, _ = closure.correct(sc, setup.coarse_grid)
-            cc.set_state(sc)
+            closure.correct_model(cc)
+        sc = cc.retrieve()
         truth = restrict_state(fm.retrieve(), setup.fine_grid, setup.coarse_grid)
         sp = cp.retrieve()
         err_plain.append(rms(sp["sshn"], truth["sshn"]))

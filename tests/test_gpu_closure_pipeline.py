@@ -38,3 +38,23 @@ def test_closure_trains_and_helps_on_held_out_basin():
     # impossible tolerance: the judge switches the closure off
     _, _, fell2 = cp.evaluate(held, clos, n_windows=10, tol=1e-9, judge_every=5)
     assert fell2
+
+
+def test_device_correction_matches_host_correction_bitwise():
+    """Closure.correct_model (features, network and increments on the
+    device) leaves the same state as the host path Closure.correct."""
+    setup = cp.Setup(24, 5, **KW)
+    X, Y = cp.generate_pairs(setup, n_snap=3)
+    clos = cp.Closure(X, Y, seed=1)
+    clos.train(X, Y, iters=5, batch=2)
+    m = setup.coarse_model()
+    m.run(1, 30)
+    host, _ = clos.correct(m.retrieve(), setup.coarse_grid)
+    clos.correct_model(m)
+    dev = m.retrieve()
+    for k in ("sshn", "un", "vn", "sshn_u", "sshn_v"):
+        assert np.array_equal(dev[k], host[k]), (k, np.abs(dev[k] - host[k]).max())
+    # the corrected model steps on from the corrected state
+    m.run(31, 40)
+    assert np.all(np.isfinite(m.retrieve()["sshn"]))
+    m.close()
