@@ -111,20 +111,16 @@ int nemo2d_step(nemo2d_model *m, int istp);
 int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default (10 on grids with uniform metrics and depths, else 1),
+ * choice): 0 default (13 on grids with uniform metrics and depths, else 1),
  * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
- * 5 marching with a 4-waves-per-SIMD register budget, 6 marching and 7
- * per-cell with Flather points recomputed inline instead of in the
- * epilogue, 8/9 marching (16/32 rows) dividing by precomputed reciprocals
- * with two exact FMA corrections, 10/11 marching (16/32 rows) with the next
- * row's loads issued one iteration ahead, 12/13 marching (16/32 rows) with
- * face terms shared between neighbouring points and loads one row ahead,
- * 14/16 the same at 16/32 rows without the load-ahead, 15 march16_sh
- * within a 3-waves-per-SIMD register budget, 17 march16_shn within a
- * 4-waves-per-SIMD budget, 18/19/20 shared terms without load-ahead at
- * 8/12/24 rows.  Marching needs uniform
- * metrics and depths; other grids use 1 (or 7).  No counterpart in the reference:
- * a tuning hook. */
+ * 5 marching and 6 per-cell with Flather points recomputed inline instead
+ * of in the epilogue, 7/8 marching (16/32 rows) dividing by precomputed
+ * reciprocals with two exact FMA corrections, 9/10 marching (16/32 rows)
+ * with the next row's loads issued one iteration ahead, 11/12 marching
+ * (16/32 rows) with face terms shared between neighbouring points and the
+ * next row's loads issued ahead, 13/14/15/16 shared face terms at
+ * 12/16/24/32 rows.  Marching needs uniform metrics and depths; other
+ * grids use 1 (or 6).  No counterpart in the reference: a tuning hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
 
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the

@@ -231,8 +231,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int NJ, bool FLIN, int WPE, bool FAST, bool PF = false>
-__global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+template <int NJ, bool FLIN, bool FAST, bool PF = false>
+__global__ void __launch_bounds__(64 * MW)
 k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
              const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
              double tide, double* __restrict__ partials, int nbx, int ntiles) {
@@ -435,12 +435,12 @@ k_epilogue(const int* __restrict__ uptr, const long* __restrict__ upts, int nul,
 //  * terms on a face shared by two points of one column are computed once,
 //    at the lower point as its north term, and carried up a row as the
 //    upper point's south term (continuity's v flux, momentum_u's south
-//    advective flux and v average, momentum_v's south advective flux and
-//    dv/dy);
+//    advective flux, v average and du/dy quotient, momentum_v's south
+//    advective flux and dv/dy);
 //  * terms on a face shared by two points of one row are computed once, by
 //    the lane of the east point as its west term, and handed to the west
 //    lane by a lane shuffle (momentum_u's east advective flux and du/dx,
-//    momentum_v's east advective flux and u average);
+//    momentum_v's east advective flux, u average and dv/dx quotient);
 //  * the register window is a ring of four rows (three in use, one being
 //    loaded) whose roles rotate with the row, so no row is ever copied.
 // Every shared term is the same expression tree as the reference's own (an
@@ -482,6 +482,7 @@ struct NorthTerms {
     double depnu;  // momentum_u: depn
     double hn;     // momentum_v: vv_n * v_n * depn
     double dvdyn;  // momentum_v: dvdy_n
+    double qdudy;  // momentum_u: (un_n - un_c) / (e2u + e2u), dudy_n's quotient
 };
 __device__ __forceinline__ NorthTerms north_terms(const Scal& s, const Row5& c, const Row5& nr) {
     const double h = 0.5;
@@ -501,6 +502,7 @@ __device__ __forceinline__ NorthTerms north_terms(const Scal& s, const Row5& c, 
     const double vvn = (h + sv) * c.vn[1] + (h - sv) * nr.vn[1];
     t.hn = vvn * vn_n * depn;
     t.dvdyn = (nr.vn[1] - c.vn[1]) / s.e2t * (s.ht + nr.ss[1]);
+    t.qdudy = (nr.un[1] - c.un[1]) / (s.e2u + s.e2u);
     return t;
 }
 
@@ -577,10 +579,13 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
     const double depw_v = h * (s.hu + B.su[0] + s.hu + C.su[0]);
     const double sw_v = sgn_half(u_wv);
     const double gw = ((h - sw_v) * vn_c + (h + sw_v) * B.vn[0]) * u_wv * depw_v;
+    // dvdx_w's quotient; lane+1's is dvdx_e's
+    const double qdvdx_w = (vn_c - B.vn[0]) / (s.e1v + s.e1v);
     const double fe = __shfl_down(fw, 1, 64);
     const double dudx_e = __shfl_down(dudx_w, 1, 64);
     const double ge = __shfl_down(gw, 1, 64);
     const double u_ec = __shfl_down(u_wc, 1, 64);
+    const double qdvdx_e = __shfl_down(qdvdx_w, 1, 64);
 
     const double su_old = B.su[1], sv_old = B.sv[1];
     if (x.own) {
@@ -594,8 +599,8 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
             if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
             const double adv = fw - fe + ts - tn;
             double dudy_s = 0.0, dudy_n = 0.0;
-            if (!ws) dudy_s = (un_c - A.un[1]) / (s.e2u + s.e2u) * (s.hu + B.su[1] + s.hu + A.su[1]);
-            if (!wn) dudy_n = (C.un[1] - un_c) / (s.e2u + s.e2u) * (s.hu + B.su[1] + s.hu + C.su[1]);
+            if (!ws) dudy_s = st.qdudy * (s.hu + B.su[1] + s.hu + A.su[1]);
+            if (!wn) dudy_n = nt.qdudy * (s.hu + B.su[1] + s.hu + C.su[1]);
             double vis = (dudx_e - dudx_w) * s.e2u + (dudy_n - dudy_s) * s.e1u * h;
             vis = x.p.visc * vis;
             const double hsu = s.hu + B.su[1];
@@ -614,8 +619,8 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
             }
             const double adv = tw - te + st.hn - nt.hn;
             double dvdx_w = 0.0, dvdx_e = 0.0;
-            if (!ww) dvdx_w = (vn_c - B.vn[0]) / (s.e1v + s.e1v) * (s.hv + B.sv[1] + s.hv + B.sv[0]);
-            if (!we) dvdx_e = (B.vn[2] - vn_c) / (s.e1v + s.e1v) * (s.hv + B.sv[1] + s.hv + B.sv[2]);
+            if (!ww) dvdx_w = qdvdx_w * (s.hv + B.sv[1] + s.hv + B.sv[0]);
+            if (!we) dvdx_e = qdvdx_e * (s.hv + B.sv[1] + s.hv + B.sv[2]);
             double vis = (nt.dvdyn - st.dvdyn) * s.e1v + (dvdx_e - dvdx_w) * s.e2v * h;
             vis = x.p.visc * vis;
             const double hsv = s.hv + B.sv[1];
@@ -657,8 +662,8 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
     k.f_prev = f;
 }
 
-template <int NJ, int WPE, bool PF>
-__global__ void __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+template <int NJ, bool PF>
+__global__ void __launch_bounds__(64 * MW)
 k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 const double* __restrict__ flv, Layout L, Params p, double tide,
                 double* __restrict__ partials, int nbx, int ntiles) {
@@ -749,21 +754,7 @@ k_checksum_partials(const double* __restrict__ sshn, const double* __restrict__ 
 // Finish: reduce nparts block partials in a fixed order into out[0..2].
 __global__ void __launch_bounds__(RED_THREADS)
 k_checksum_finish(const double* __restrict__ partials, int nparts, double* __restrict__ out) {
-    DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += RED_THREADS) {
-        const double* q = partials + 6 * (long)k;
-        a0 = dd_merge(a0, DD{q[0], q[1]});
-        a1 = dd_merge(a1, DD{q[2], q[3]});
-        a2 = dd_merge(a2, DD{q[4], q[5]});
-    }
-    __shared__ double r[6];
-    block_reduce3(a0, a1, a2, r);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        out[0] = r[0] + r[1];
-        out[1] = r[2] + r[3];
-        out[2] = r[4] + r[5];
-    }
+    finish_partials(partials, nparts, out);
 }
 
 // ---- stage kernels (one thread per point of the stage's Fortran range) ------
@@ -1619,30 +1610,26 @@ static int run_stages(nemo2d_model* m, int istp) {
 // Fused-step kernel variants (nemo2d_set_variant); 0 picks the default.
 enum {
     VAR_AUTO = 0,
-    VAR_CELL = 1,       // one thread per cell, Flather in the epilogue
-    VAR_MARCH16 = 2,    // column-marching, 16-row segments
-    VAR_MARCH32 = 3,    //                  32-row segments
-    VAR_MARCH64 = 4,    //                  64-row segments
-    VAR_MARCH32_W4 = 5, // 32-row segments, register budget for 4 waves/SIMD
-    VAR_MARCH32_FLIN = 6,  // 32-row segments, Flather points recomputed inline
-    VAR_CELL_FLIN = 7,  // one thread per cell, Flather inline
-    VAR_MARCH16_FD = 8,  // marching, 16-row segments, division by reciprocals
-    VAR_MARCH32_FD = 9,  //                32-row segments
-    VAR_MARCH16_PF = 10,  // marching, 16-row segments, next row prefetched
-    VAR_MARCH32_PF = 11,  //                32-row segments
-    VAR_MARCH16_SH = 12,  // marching with shared face terms, 16-row segments
-    VAR_MARCH32_SH = 13,  //                                  32-row segments
-    VAR_MARCH16_SHN = 14, // shared face terms, 16 rows, no load-ahead
-    VAR_MARCH16_SH3 = 15, // march16_sh within 168 VGPRs (3 waves per SIMD)
-    VAR_MARCH32_SHN = 16, // shared face terms, 32 rows, no load-ahead
-    VAR_MARCH16_SHN4 = 17, // march16_shn within 128 VGPRs (4 waves per SIMD)
-    VAR_MARCH8_SHN = 18,  // shared face terms, no load-ahead, 8/12/24-row segments
-    VAR_MARCH12_SHN = 19,
-    VAR_MARCH24_SHN = 20,
-    VAR_LAST = 20
+    VAR_CELL = 1,          // one thread per cell, Flather in the epilogue
+    VAR_MARCH16 = 2,       // column-marching, 16-row segments
+    VAR_MARCH32 = 3,       //                  32-row segments
+    VAR_MARCH64 = 4,       //                  64-row segments
+    VAR_MARCH32_FLIN = 5,  // 32-row segments, Flather points recomputed inline
+    VAR_CELL_FLIN = 6,     // one thread per cell, Flather inline
+    VAR_MARCH16_FD = 7,    // marching, 16-row segments, division by reciprocals
+    VAR_MARCH32_FD = 8,    //                32-row segments
+    VAR_MARCH16_PF = 9,    // marching, 16-row segments, next row prefetched
+    VAR_MARCH32_PF = 10,   //                32-row segments
+    VAR_MARCH16_SH = 11,   // shared face terms, next row prefetched, 16 rows
+    VAR_MARCH32_SH = 12,   //                                         32 rows
+    VAR_MARCH12_SHN = 13,  // shared face terms, 12/16/24/32-row segments
+    VAR_MARCH16_SHN = 14,
+    VAR_MARCH24_SHN = 15,
+    VAR_MARCH32_SHN = 16,
+    VAR_LAST = 16
 };
 
-template <int NJ, bool FLIN, int WPE, bool FAST = false, bool PF = false>
+template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
 static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
@@ -1650,7 +1637,7 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march<NJ, FLIN, WPE, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march<NJ, FLIN, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
     LAUNCHCHK("step_march");
     nparts = nblocks;
@@ -1660,16 +1647,16 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
 // the shared-term kernels address the arrays by 32-bit byte offsets
 static bool sh_fits(const Layout& L) { return (unsigned long)L.size() * sizeof(double) < (1ul << 32); }
 
-template <int NJ, int WPE, bool PF>
+template <int NJ, bool PF>
 static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (!sh_fits(L)) return launch_march<16, false, 0, false, true>(m, n, a, tide, nparts);
+    if (!sh_fits(L)) return launch_march<16, false, false, true>(m, n, a, tide, nparts);
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
     const int nby = (L.jpj + 1 + NJ - 1) / NJ;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_sh<NJ, WPE, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march_sh<NJ, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
     LAUNCHCHK("step_march_sh");
     nparts = nblocks;
@@ -1700,37 +1687,33 @@ int nemo2d_step(nemo2d_model* m, int istp) {
            m->S[1 - k][F_VN]};
     const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
-    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH16_PF : VAR_CELL;
+    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH12_SHN : VAR_CELL;
     if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
     switch (var) {
         case VAR_CELL: rc = launch_cell<false>(m, n, a, tide, nparts); break;
         case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
-        case VAR_MARCH16: rc = launch_march<16, false, 0>(m, n, a, tide, nparts); break;
-        case VAR_MARCH64: rc = launch_march<64, false, 0>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_W4: rc = launch_march<32, false, 4>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_FLIN: rc = launch_march<32, true, 0>(m, n, a, tide, nparts); flin = true; break;
+        case VAR_MARCH16: rc = launch_march<16, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH64: rc = launch_march<64, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_FLIN: rc = launch_march<32, true>(m, n, a, tide, nparts); flin = true; break;
         case VAR_MARCH16_FD:
-            rc = m->scal.fast_div ? launch_march<16, false, 0, true>(m, n, a, tide, nparts)
-                                  : launch_march<16, false, 0>(m, n, a, tide, nparts);
+            rc = m->scal.fast_div ? launch_march<16, false, true>(m, n, a, tide, nparts)
+                                  : launch_march<16, false>(m, n, a, tide, nparts);
             break;
         case VAR_MARCH32_FD:
-            rc = m->scal.fast_div ? launch_march<32, false, 0, true>(m, n, a, tide, nparts)
-                                  : launch_march<32, false, 0>(m, n, a, tide, nparts);
+            rc = m->scal.fast_div ? launch_march<32, false, true>(m, n, a, tide, nparts)
+                                  : launch_march<32, false>(m, n, a, tide, nparts);
             break;
-        case VAR_MARCH16_PF: rc = launch_march<16, false, 0, false, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_PF: rc = launch_march<32, false, 0, false, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SH: rc = launch_march_sh<16, 0, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_SH: rc = launch_march_sh<32, 0, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SHN: rc = launch_march_sh<16, 0, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SH3: rc = launch_march_sh<16, 3, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_SHN: rc = launch_march_sh<32, 0, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SHN4: rc = launch_march_sh<16, 4, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH8_SHN: rc = launch_march_sh<8, 0, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_SHN: rc = launch_march_sh<12, 0, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH24_SHN: rc = launch_march_sh<24, 0, false>(m, n, a, tide, nparts); break;
-        default: rc = launch_march<32, false, 0>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_PF: rc = launch_march<16, false, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_PF: rc = launch_march<32, false, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SH: rc = launch_march_sh<16, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_SH: rc = launch_march_sh<32, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_SHN: rc = launch_march_sh<12, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH16_SHN: rc = launch_march_sh<16, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH24_SHN: rc = launch_march_sh<24, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH32_SHN: rc = launch_march_sh<32, false>(m, n, a, tide, nparts); break;
+        default: rc = launch_march<32, false>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
     m->cur = 1 - k;
@@ -1740,7 +1723,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     // the shared-term kernels set the row-0 Flather v points themselves
     const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
-    if (!flin && (has_fl || out)) {
+    if (!flin && has_fl) {
         // Flather points of the new state, then (if recording) the checksums
         k_epilogue<<<1, RED_THREADS, 0, m->stream>>>(
             m->flu_ptr, m->flu_pts, m->nflu ? L.jpj : 0, m->flv_ptr, m->flv_pts, m->nflv ? L.jpi : 0,

@@ -254,11 +254,22 @@ __device__ __forceinline__ void cell_step_mixed(const ScalF& s, const MixedState
 __device__ __forceinline__ void finish_partials(const double* __restrict__ partials, int nparts,
                                                 double* __restrict__ out) {
     DD a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-        const double* q = partials + 6 * (long)k;
-        a0 = dd_merge(a0, DD{q[0], q[1]});
-        a1 = dd_merge(a1, DD{q[2], q[3]});
-        a2 = dd_merge(a2, DD{q[4], q[5]});
+    // four partials per thread per pass, all loads issued before the merges
+    const int nt = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < nparts; k0 += 4 * nt) {
+        double q[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * nt;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) q[u][c] = k < nparts ? partials[6 * (long)k + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a0 = dd_merge(a0, DD{q[u][0], q[u][1]});
+            a1 = dd_merge(a1, DD{q[u][2], q[u][3]});
+            a2 = dd_merge(a2, DD{q[u][4], q[u][5]});
+        }
     }
     __shared__ double r[6];
     block_reduce3(a0, a1, a2, r);

@@ -40,11 +40,10 @@ SYMBOLS = (
 )
 
 # nemo2d_set_variant choices
-VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_w4": 5,
-            "march32_flin": 6, "cell_flin": 7, "march16_fd": 8, "march32_fd": 9, "march16_pf": 10,
-            "march32_pf": 11, "march16_sh": 12, "march32_sh": 13,
-            "march16_shn": 14, "march16_sh3": 15, "march32_shn": 16,
-            "march16_shn4": 17, "march8_shn": 18, "march12_shn": 19, "march24_shn": 20}
+VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_flin": 5,
+            "cell_flin": 6, "march16_fd": 7, "march32_fd": 8, "march16_pf": 9, "march32_pf": 10,
+            "march16_sh": 11, "march32_sh": 12, "march12_shn": 13, "march16_shn": 14,
+            "march24_shn": 15, "march32_shn": 16}
 
 
 class Nemo2dError(RuntimeError):
