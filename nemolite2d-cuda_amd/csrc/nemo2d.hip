@@ -20,6 +20,7 @@
 
 #include <string>
 #include <type_traits>
+#include <algorithm>
 #include <vector>
 
 #include "nemo2d.h"
@@ -95,6 +96,15 @@ struct nemo2d_model {
     int npart_max = 0;
     double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
     double* cs_log = nullptr;    // 3 doubles per logged step
+    // Deferred checksum finish: the fused step leaves each logged step's
+    // block partials in its own slot of cs_part; reading the log reduces
+    // all pending slots in one launch.
+    double* cs_part = nullptr;   // cs_cap slots of cs_slot doubles (or null)
+    long cs_slot = 0;
+    std::vector<int> cs_pend;    // partials waiting in each slot (0: none)
+    int* d_pend = nullptr;       // device copy of (slot, nparts) pairs
+    bool pend_any = false;
+    double* part_dst = nullptr;  // where the step kernel writes its partials
     int cs_cap = 0;
     int cs_next = 0;             // next free slot for stage-path logging
     int cs_n = 0;                // steps logged (max slot + 1)
@@ -757,6 +767,15 @@ k_checksum_finish(const double* __restrict__ partials, int nparts, double* __res
     finish_partials(partials, nparts, out);
 }
 
+// Pending logged steps: block b finishes slot pend[2b] from its pend[2b+1]
+// partials.
+__global__ void __launch_bounds__(RED_THREADS)
+k_checksum_finish_batch(const double* __restrict__ part, long slot_doubles,
+                        const int* __restrict__ pend, double* __restrict__ log) {
+    const int slot = pend[2 * blockIdx.x], np = pend[2 * blockIdx.x + 1];
+    finish_partials(part + slot * slot_doubles, np, log + 3 * (long)slot);
+}
+
 // ---- stage kernels (one thread per point of the stage's Fortran range) ------
 template <class S>
 __global__ void k_continuity(S st, NState n, double* __restrict__ ssha, Layout L, double rdt) {
@@ -963,6 +982,12 @@ StArrays starrays(const nemo2d_model* m) {
 
 void dfree(void* p) {
     if (p) (void)hipFree(p);
+}
+
+// Forget logged steps whose checksums were not finished yet.
+void drop_pending(nemo2d_model* m) {
+    std::fill(m->cs_pend.begin(), m->cs_pend.end(), 0);
+    m->pend_any = false;
 }
 
 int dalloc(double** p, size_t n) {
@@ -1413,6 +1438,7 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
         m->state_set = true;
         m->cs_next = 0;
         m->cs_n = 0;
+        drop_pending(m);
         return NEMO2D_OK;
     }
     for (int k = 0; k < 2; ++k)
@@ -1436,6 +1462,7 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     m->state_set = true;
     m->cs_next = 0;
     m->cs_n = 0;
+    drop_pending(m);
     return NEMO2D_OK;
 }
 
@@ -1541,6 +1568,7 @@ int nemo2d_next(nemo2d_model* m) {
     LAUNCHCHK("next");
     m->a_current = true;
     if (m->cs_cap > 0 && m->cs_next < m->cs_cap) {
+        if (!m->cs_pend.empty()) m->cs_pend[m->cs_next] = 0;  // this slot is final now
         int rc = checksum_into(m, m->cur, m->cs_log + 3 * (long)m->cs_next);
         if (rc) return rc;
         ++m->cs_next;
@@ -1638,7 +1666,7 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
     k_step_march<NJ, FLIN, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
-        m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
+        m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march");
     nparts = nblocks;
     return NEMO2D_OK;
@@ -1657,7 +1685,7 @@ static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, doub
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
     k_step_march_sh<NJ, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
-        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->partials, nbx, ntiles);
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march_sh");
     nparts = nblocks;
     return NEMO2D_OK;
@@ -1668,7 +1696,7 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
     const Layout& L = m->L;
     dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
     WITH_STATICS(m, st, (k_step_fused<decltype(st), FLIN><<<g, dim3(TBX, TBY), 0, m->stream>>>(
-                            st, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->partials)));
+                            st, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->part_dst)));
     LAUNCHCHK("step_fused");
     nparts = (int)(g.x * g.y);
     return NEMO2D_OK;
@@ -1691,6 +1719,10 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
+    const int slot = istp - m->nit000;
+    double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
+    const bool defer = out && m->cs_part;
+    m->part_dst = defer ? m->cs_part + slot * m->cs_slot : m->partials;
     switch (var) {
         case VAR_CELL: rc = launch_cell<false>(m, n, a, tide, nparts); break;
         case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
@@ -1718,21 +1750,25 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     if (rc) return rc;
     m->cur = 1 - k;
     m->a_current = false;
-    const int slot = istp - m->nit000;
-    double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
     // the shared-term kernels set the row-0 Flather v points themselves
     const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && has_fl) {
         // Flather points of the new state, then (if recording) the checksums
+        // (partial nparts holds the Flather points' sums)
         k_epilogue<<<1, RED_THREADS, 0, m->stream>>>(
             m->flu_ptr, m->flu_pts, m->nflu ? L.jpj : 0, m->flv_ptr, m->flv_pts, m->nflv ? L.jpi : 0,
-            m->flags, m->flu, m->flv, n.sshn_u, n.sshn_v, a.un, a.vn, L, m->partials, nparts, out,
-            sh ? 1 : 0);
+            m->flags, m->flu, m->flv, n.sshn_u, n.sshn_v, a.un, a.vn, L, m->part_dst, nparts,
+            defer ? nullptr : out, sh ? 1 : 0);
         LAUNCHCHK("epilogue");
-    } else if (out) {
-        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts, out);
+        ++nparts;
+    } else if (out && !defer) {
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->part_dst, nparts, out);
         LAUNCHCHK("checksum_finish");
+    }
+    if (defer) {
+        m->cs_pend[slot] = nparts;
+        m->pend_any = true;
     }
     if (out) {
         if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
@@ -1767,12 +1803,43 @@ int nemo2d_run(nemo2d_model* m, int nit000, int nitend) {
     return NEMO2D_OK;
 }
 
+// Reduce the partials of every pending logged step into the log.
+static int flush_checksums(nemo2d_model* m) {
+    if (!m->pend_any) return NEMO2D_OK;
+    std::vector<int> pairs;
+    for (int k = 0; k < m->cs_cap; ++k)
+        if (m->cs_pend[k] > 0) {
+            pairs.push_back(k);
+            pairs.push_back(m->cs_pend[k]);
+            m->cs_pend[k] = 0;
+        }
+    m->pend_any = false;
+    if (pairs.empty()) return NEMO2D_OK;
+    HIPCHK(hipMemcpyAsync(m->d_pend, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice,
+                          m->stream));
+    k_checksum_finish_batch<<<(unsigned)(pairs.size() / 2), RED_THREADS, 0, m->stream>>>(
+        m->cs_part, m->cs_slot, m->d_pend, m->cs_log);
+    LAUNCHCHK("checksum_finish_batch");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+// Slot buffers for the deferred finish up to this many bytes; beyond it
+// each logged step finishes its checksums at once.
+constexpr size_t CS_DEFER_MAX_BYTES = size_t(8) << 30;
+
 int nemo2d_record_checksums(nemo2d_model* m, int capacity) {
     if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_record_checksums: null model");
     if (capacity < 0) return fail(NEMO2D_ERR_ARG, "nemo2d_record_checksums: negative capacity");
     HIPCHK(hipStreamSynchronize(m->stream));
     dfree(m->cs_log);
+    dfree(m->cs_part);
+    dfree(m->d_pend);
     m->cs_log = nullptr;
+    m->cs_part = nullptr;
+    m->d_pend = nullptr;
+    m->cs_pend.clear();
+    m->pend_any = false;
     m->cs_cap = 0;
     m->cs_n = 0;
     m->cs_next = 0;
@@ -1780,6 +1847,20 @@ int nemo2d_record_checksums(nemo2d_model* m, int capacity) {
         int rc = dalloc(&m->cs_log, 3 * (size_t)capacity);
         if (rc) return rc;
         m->cs_cap = capacity;
+        m->cs_slot = 6 * ((long)m->npart_max + 1);
+        const size_t bytes = (size_t)capacity * (size_t)m->cs_slot * sizeof(double);
+        if (bytes <= CS_DEFER_MAX_BYTES) {
+            if (hipMalloc((void**)&m->cs_part, bytes) != hipSuccess ||
+                hipMalloc((void**)&m->d_pend, 2 * (size_t)capacity * sizeof(int)) != hipSuccess) {
+                (void)hipGetLastError();  // no room: finish at once instead
+                dfree(m->cs_part);
+                dfree(m->d_pend);
+                m->cs_part = nullptr;
+                m->d_pend = nullptr;
+            } else {
+                m->cs_pend.assign((size_t)capacity, 0);
+            }
+        }
     }
     return NEMO2D_OK;
 }
@@ -1787,6 +1868,8 @@ int nemo2d_record_checksums(nemo2d_model* m, int capacity) {
 int nemo2d_checksum_log(nemo2d_model* m, double* out, int max_steps, int* nsteps) {
     if (!m || (!out && max_steps > 0)) return fail(NEMO2D_ERR_ARG, "nemo2d_checksum_log: bad argument");
     HIPCHK(hipStreamSynchronize(m->stream));
+    int rc = flush_checksums(m);
+    if (rc) return rc;
     const int n = m->cs_n < max_steps ? m->cs_n : max_steps;
     if (n > 0) HIPCHK(hipMemcpy(out, m->cs_log, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     if (nsteps) *nsteps = m->cs_n;
@@ -1981,7 +2064,7 @@ int nemo2d_destroy(nemo2d_model* m) {
     dfree(m->ssha); dfree(m->ua); dfree(m->va);
     dfree(m->flags); dfree(m->flu); dfree(m->flv);
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
-    dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log);
+    dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log); dfree(m->cs_part); dfree(m->d_pend);
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f) dfree(m->X[k][f]);
     dfree(m->hf_t); dfree(m->hf_u); dfree(m->hf_v); dfree(m->diag);

@@ -6,6 +6,7 @@ the serial model's operation order with FMA contraction off, and evaluates
 sin/sqrt on the host with the same C library as the oracle); per-step
 checksums within 1e-12 relative (both sides use compensated summation, so in
 practice they differ by at most a few ulps)."""
+import math
 import os
 
 import numpy as np
@@ -113,6 +114,35 @@ def test_fused_and_stages_mix():
         assert_fields_equal(m, o, "mixed")
         r = m.retrieve()
         assert np.array_equal(r["ua"], r["un"]) or np.array_equal(r["ua"], o.arr("ua"))
+
+
+@pytest.mark.parametrize("variant", ["auto", "cell", "march16_pf"])
+def test_checksum_log_with_mixed_paths_and_restarts(variant):
+    """The per-step log (fused steps leave block partials that the log read
+    reduces; stage steps log at once) matches exact sums of the oracle's
+    fields when fused and stage steps interleave and after set_state has
+    discarded an unread recording; reading it twice gives the same log."""
+    o = Oracle(150, 70)
+    start = state_from_oracle(o)
+    with model_like(o) as m:
+        m.set_variant(variant)
+        m.record_checksums(30)
+        for istp in range(1, 11):
+            m.step(istp)
+        m.set_state(start)  # drops the 10 unread steps
+        m.record_checksums(30)
+        ref = []
+        for istp in range(1, 31):
+            (m.step_stages if istp % 4 == 0 else m.step)(istp)
+            o.step(istp)
+            # exact sums over (1:jpi, 1:jpj) of the oracle's fields
+            ref.append([math.fsum(np.abs(o.arr("sshn")).ravel()),
+                        math.fsum(np.abs(o.arr("un")[:, 1:]).ravel()),
+                        math.fsum(np.abs(o.arr("vn")[1:, :]).ravel())])
+        log1 = m.checksum_log(30)
+        log2 = m.checksum_log(30)
+        assert np.array_equal(log1, log2)
+        assert np.allclose(log1, np.array(ref), rtol=1e-13, atol=0), np.abs(log1 / np.array(ref) - 1).max()
 
 
 @pytest.mark.parametrize("traj", trajectories(load_golden(), max_cells=130 * 130),
