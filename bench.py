@@ -168,12 +168,23 @@ def form_closure(nc, runs, snaps, iters, windows):
             "seconds": {"data": t_data, "train": t_train, "eval": t_eval}}
 
 
-def load_traffic():
+# the fused-step kernel each variant runs on the uniform benchmark grid
+KERNELS = {"auto": "k_step_march_sh<12, false>", "march12_shn": "k_step_march_sh<12, false>",
+           "march16_shn": "k_step_march_sh<16, false>", "march24_shn": "k_step_march_sh<24, false>",
+           "march32_shn": "k_step_march_sh<32, false>", "march16_pf": "k_step_march<16, PF>",
+           "cell": "k_step_fused"}
+
+
+def load_traffic(variant, n):
+    """Measured HBM bytes per launch (profiles/traffic_step_fused_4096.json)
+    when it was measured for this kernel and size, else None."""
     p = os.path.join(ROOT, "profiles", "traffic_step_fused_4096.json")
-    if not os.path.exists(p):
+    if n != 4096 or not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    same = variant == d.get("variant") or (variant == "auto" and d.get("variant") == "march12_shn")
+    return d.get("bytes_per_launch") if same else None
 
 
 def main():
@@ -261,7 +272,7 @@ def main():
     cells = float(n) * n
     value = cells * K / (ms * 1e-3)
     achieved = BYTES_PER_CELL * cells / (ms_kernel * 1e-3) / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(args.variant, n)
     out = {
         "metric": METRIC,
         "value": value,
@@ -284,9 +295,15 @@ def main():
         "checksum_last_step": [float(x) for x in log[-1]] if len(log) else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step_march", "kernel_ms": ms_kernel,
+                     "kernel": KERNELS.get(args.variant, "fused step, variant " + args.variant),
+                     "kernel_ms": ms_kernel,
                      "bytes_per_launch": BYTES_PER_CELL * cells},
     }
+    # practical ceiling: plain streaming of the same five-in/five-out bytes
+    # over arrays of the padded size ((jpj+4) rows of the 16-padded pitch)
+    pitch = (16 + n + 3 + 15) // 16 * 16
+    out["stream_5r5w_gbs"] = nemo2d.stream_bandwidth((n + 4) * pitch)
+    out["roofline"]["frac_of_stream"] = achieved / out["stream_5r5w_gbs"]
     if variants:
         out["variants_ms_per_step"] = variants
     if not args.no_forms:

@@ -260,6 +260,14 @@ int nemo2d_dev_upload(void *dst, const void *src, long bytes);
 int nemo2d_dev_download(void *dst, const void *src, long bytes);
 int nemo2d_dev_zero(void *p, long bytes);
 
+/* Practical bandwidth ceiling for the fused step's traffic: a kernel that
+ * reads five FP64 arrays of n elements and writes five (elementwise, no
+ * stencil) is timed over `repeats` launches on the current device; *gbps
+ * gets 80 * n bytes per launch over the mean launch time.  Allocates and
+ * frees 10 * n doubles.  No counterpart in the reference (a measurement
+ * aid). */
+int nemo2d_stream_bandwidth(long n, int repeats, double *gbps);
+
 /* Free the model's device memory and stream.  Replaces cuda_finalise_
  * (cuda/cuda_nemolite2d.cu:267), without resetting the device. */
 int nemo2d_destroy(nemo2d_model *m);

@@ -744,6 +744,21 @@ k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
 
+// ---- streaming reference ---------------------------------------------------------
+// Reads five arrays and writes five, elementwise: the fused step's algorithmic
+// traffic without its stencil, for the practical bandwidth ceiling.
+struct Stream5 {
+    const double* in[5];
+    double* out[5];
+};
+__global__ void __launch_bounds__(256) k_stream5(Stream5 s, long n) {
+    const long stride = (long)gridDim.x * 256;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s.out[k][i] = s.in[k][i] * 1.0000001;
+    }
+}
+
 // ---- checksum ----------------------------------------------------------------
 // sum|sshn|, sum|un|, sum|vn| over (1:jpi, 1:jpj) of the given state.
 __global__ void __launch_bounds__(TBX * TBY)
@@ -1775,6 +1790,38 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         m->cs_next = slot + 1;
     }
     return NEMO2D_OK;
+}
+
+int nemo2d_stream_bandwidth(long n, int repeats, double* gbps) {
+    if (n < 1 || repeats < 1 || !gbps) return fail(NEMO2D_ERR_ARG, "nemo2d_stream_bandwidth: bad argument");
+    double* a[10] = {};
+    int rc = NEMO2D_OK;
+    for (int k = 0; k < 10 && rc == NEMO2D_OK; ++k) rc = dalloc(&a[k], (size_t)n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == NEMO2D_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(NEMO2D_ERR_HIP, "nemo2d_stream_bandwidth: event");
+    if (rc == NEMO2D_OK) {
+        Stream5 s;
+        for (int k = 0; k < 5; ++k) {
+            s.in[k] = a[k];
+            s.out[k] = a[5 + k];
+        }
+        const unsigned grid = 65536;
+        for (int it = 0; it < 3; ++it) k_stream5<<<grid, 256>>>(s, n);
+        (void)hipEventRecord(e0, nullptr);
+        for (int it = 0; it < repeats; ++it) k_stream5<<<grid, 256>>>(s, n);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+            hipGetLastError() != hipSuccess)
+            rc = fail(NEMO2D_ERR_HIP, "nemo2d_stream_bandwidth: launch");
+        else
+            *gbps = 80.0 * (double)n * repeats / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (int k = 0; k < 10; ++k) dfree(a[k]);
+    return rc;
 }
 
 int nemo2d_set_tide(nemo2d_model* m, double amp, double period_hours) {

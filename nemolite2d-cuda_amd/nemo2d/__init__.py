@@ -37,6 +37,7 @@ SYMBOLS = (
     "nemo2d_nn_mse", "nemo2d_nn_adam", "nemo2d_nn_pack_weights", "nemo2d_nn_to_bf16",
     "nemo2d_dev_alloc", "nemo2d_dev_free", "nemo2d_dev_upload", "nemo2d_dev_download",
     "nemo2d_dev_zero", "nemo2d_closure_features", "nemo2d_closure_apply",
+    "nemo2d_stream_bandwidth",
 )
 
 # nemo2d_set_variant choices
@@ -88,6 +89,7 @@ def lib():
         L.nemo2d_ensemble_checksum_log.argtypes = [vp, i, dp, i, ip]
         L.nemo2d_ensemble_retrieve.argtypes = [vp, i, dp, dp, dp]
         L.nemo2d_ensemble_destroy.argtypes = [vp]
+        L.nemo2d_stream_bandwidth.argtypes = [ctypes.c_long, i, dp]
         L.nemo2d_closure_features.argtypes = [vp, dp, dp, vp]
         L.nemo2d_closure_apply.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, dp]
         L.nemo2d_dev_alloc.argtypes = [ctypes.c_long, ctypes.POINTER(vp)]
@@ -337,6 +339,14 @@ class Model:
         ms = ctypes.c_double(0.0)
         _check(lib().nemo2d_timer_stop(self._h, ctypes.byref(ms)))
         return ms.value
+
+
+def stream_bandwidth(n, repeats=20):
+    """GB/s of a 5-array-read, 5-array-write streaming kernel over n FP64
+    elements (the fused step's traffic without the stencil)."""
+    out = ctypes.c_double(0.0)
+    _check(lib().nemo2d_stream_bandwidth(int(n), int(repeats), ctypes.byref(out)))
+    return out.value
 
 
 def seamount_grid(jpi, jpj, seed, n_mounts=32, dx=1000.0, dy=1000.0, dep_const=100.0,
