@@ -119,8 +119,13 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
  * with the next row's loads issued one iteration ahead, 11/12 marching
  * (16/32 rows) with face terms shared between neighbouring points and the
  * next row's loads issued ahead, 13/14/15/16 shared face terms at
- * 12/16/24/32 rows.  Marching needs uniform metrics and depths; other
- * grids use 1 (or 6).  No counterpart in the reference: a tuning hook. */
+ * 12/16/24/32 rows, 17 shared face terms at 12 rows reading per-point
+ * depths.  Marching needs uniform metrics, 2-16 also uniform depths, and
+ * 17 Flather points only on row 0 with the inner point north; other grids
+ * use 1 (or 6).  The default for grids with uniform metrics but varying
+ * depths is 17 where it applies.  The single-precision form marches (as 17)
+ * where it applies unless 1 is chosen.  No counterpart in the reference: a
+ * tuning hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
 
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the

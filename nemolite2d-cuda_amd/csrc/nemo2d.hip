@@ -27,6 +27,7 @@
 #include "nemo2d_core.h"
 #include "nemo2d_internal.h"
 #include "nemo2d_step.h"
+#include "nemo2d_march.h"
 
 using namespace nemo2d;
 
@@ -79,6 +80,7 @@ struct nemo2d_model {
     double *flu = nullptr, *flv = nullptr;  // Flather sqrt(g/h) at Flather points
     double* st[NS] = {};
     bool uniform = false;
+    bool uni_metrics = false;  // metrics and Coriolis uniform (depths may vary)
     Scal scal{};
     bool fused_ok = false;
 
@@ -1364,9 +1366,14 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
         {e12t, 1, jpi, 1, jpj, nt},  {e12u, 0, jpi, 1, jpj, nu},  {e12v, 1, jpi, 0, jpj, nv},
         {ht, 1, jpi, 1, jpj, nt},    {hu, 0, jpi, 1, jpj, nu},    {hv, 1, jpi, 0, jpj, nv},
         {fcu.data(), 0, jpi, 1, jpj, nu}, {fcv.data(), 1, jpi, 0, jpj, nv}};
-    bool uni = true;
-    for (int s = 0; s < NS; ++s) uni = uni && all_equal(sd[s].a, (size_t)sd[s].n);
+    bool uni = true, unim = true;
+    for (int s = 0; s < NS; ++s) {
+        const bool e = all_equal(sd[s].a, (size_t)sd[s].n);
+        uni = uni && e;
+        if (s != S_HT && s != S_HU && s != S_HV) unim = unim && e;
+    }
     m->uniform = uni;
+    m->uni_metrics = unim;
     m->scal = Scal{e1t[0], e2t[0], e1u[0], e2u[0], e1v[0], e2v[0], e12t[0], e12u[0], e12v[0],
                    ht[0], hu[0], hv[0], fcu[0], fcv[0], {}, 0};
     set_reciprocals(m->scal, m->prm.fric);
@@ -1592,49 +1599,6 @@ int nemo2d_next(nemo2d_model* m) {
     return NEMO2D_OK;
 }
 
-static int step_mixed(nemo2d_model* m, int istp) {
-    if (!m->fused_ok)
-        return fail(NEMO2D_ERR_MASK, "nemo2d_step: chained Flather points are not supported by "
-                                     "the single-precision form");
-    const Layout& L = m->L;
-    const int k = m->cur;
-    const Scal& d = m->scal;
-    ScalF sf{(float)d.e1t, (float)d.e2t, (float)d.e1u, (float)d.e2u, (float)d.e1v, (float)d.e2v,
-             (float)d.e12t, (float)d.e12u, (float)d.e12v, (float)d.fcu, (float)d.fcv, (float)d.ht, (float)d.hu, (float)d.hv};
-    ParamsT<float> pf{(float)m->prm.rdt, (float)m->prm.cbfr, (float)m->prm.visc, (float)m->prm.fric};
-    const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
-    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
-    void** x = m->X[k];
-    void** y = m->X[1 - k];
-    if (m->compensated) {
-        MixedState<true> n{(const double*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
-                           (const float*)x[F_UN], (const float*)x[F_VN]};
-        MixedOut<true> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
-                         (float*)y[F_VN]};
-        k_step_mixed<true><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
-                                                                m->flags, m->flu, m->flv, L, pf, tide,
-                                                                m->partials);
-    } else {
-        MixedState<false> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
-                            (const float*)x[F_UN], (const float*)x[F_VN]};
-        MixedOut<false> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
-                          (float*)y[F_VN]};
-        k_step_mixed<false><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
-                                                                 m->flags, m->flu, m->flv, L, pf, tide,
-                                                                 m->partials);
-    }
-    LAUNCHCHK("step_mixed");
-    m->cur = 1 - k;
-    const int slot = istp - m->nit000;
-    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
-        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y),
-                                                            m->cs_log + 3 * (long)slot);
-        LAUNCHCHK("checksum_finish");
-        if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
-        m->cs_next = slot + 1;
-    }
-    return NEMO2D_OK;
-}
 
 static int run_stages(nemo2d_model* m, int istp) {
     int rc;
@@ -1669,7 +1633,8 @@ enum {
     VAR_MARCH16_SHN = 14,
     VAR_MARCH24_SHN = 15,
     VAR_MARCH32_SHN = 16,
-    VAR_LAST = 16
+    VAR_MARCH12_SHD = 17,  // shared face terms, 12 rows, per-point depths
+    VAR_LAST = 17
 };
 
 template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
@@ -1706,6 +1671,40 @@ static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, doub
     return NEMO2D_OK;
 }
 
+// Flather points only on row 0 with the inner point north (the reference
+// basin's open boundary): the marching kernels set those themselves.
+static bool flather_row0_only(const nemo2d_model* m) { return m->nflu == 0 && m->nflv == m->nflv0; }
+
+template <class R, class SS, bool DARR, int NJ>
+static int launch_march_g(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& p,
+                          const StateG<R, SS, DARR>& n, const OutG<R, SS>& a, double tide,
+                          int& nparts) {
+    const Layout& L = m->L;
+    const int nbx = (L.jpi + 1 + GW * GOUT - 1) / (GW * GOUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    k_step_march_g<R, SS, DARR, NJ><<<nblocks, 64 * GW, 0, m->stream>>>(
+        mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles);
+    LAUNCHCHK("step_march_g");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+static MetG<double> metg(const Scal& d) {
+    return MetG<double>{d.e1t, d.e2t, d.e1u, d.e2u, d.e1v, d.e2v, d.e12t, d.e12u, d.e12v,
+                        d.fcu, d.fcv, d.ht, d.hu, d.hv};
+}
+
+// FP64 fused step by the generic marching kernel with per-point depths
+static int launch_march_depths(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
+                                    m->st[S_HV]};
+    OutG<double, double> ag{a.sshn, a.sshn_u, a.sshn_v, a.un, a.vn};
+    return launch_march_g<double, double, true, 12>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
+}
+
 template <bool FLIN>
 static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
@@ -1719,6 +1718,81 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
 
 extern "C" {
 
+static int step_mixed(nemo2d_model* m, int istp) {
+    if (!m->fused_ok)
+        return fail(NEMO2D_ERR_MASK, "nemo2d_step: chained Flather points are not supported by "
+                                     "the single-precision form");
+    const Layout& L = m->L;
+    const int k = m->cur;
+    const Scal& d = m->scal;
+    ScalF sf{(float)d.e1t, (float)d.e2t, (float)d.e1u, (float)d.e2u, (float)d.e1v, (float)d.e2v,
+             (float)d.e12t, (float)d.e12u, (float)d.e12v, (float)d.fcu, (float)d.fcv, (float)d.ht, (float)d.hu, (float)d.hv};
+    ParamsT<float> pf{(float)m->prm.rdt, (float)m->prm.cbfr, (float)m->prm.visc, (float)m->prm.fric};
+    const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
+    dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    void** x = m->X[k];
+    void** y = m->X[1 - k];
+    if (m->variant != VAR_CELL && flather_row0_only(m) && sh_fits(L)) {
+        const MetG<float> mt{sf.e1t, sf.e2t, sf.e1u, sf.e2u, sf.e1v, sf.e2v, sf.e12t, sf.e12u, sf.e12v,
+                             sf.fcu, sf.fcv, sf.ht, sf.hu, sf.hv};
+        int nparts = 0, rc;
+        m->part_dst = m->partials;
+        if (m->compensated) {
+            StateG<float, double, true> n{(const double*)x[F_SSHN], (const float*)x[F_SSHN_U],
+                                          (const float*)x[F_SSHN_V], (const float*)x[F_UN],
+                                          (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
+            OutG<float, double> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
+                                  (float*)y[F_UN], (float*)y[F_VN]};
+            rc = launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts);
+        } else {
+            StateG<float, float, true> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U],
+                                         (const float*)x[F_SSHN_V], (const float*)x[F_UN],
+                                         (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
+            OutG<float, float> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
+                                 (float*)y[F_UN], (float*)y[F_VN]};
+            rc = launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts);
+        }
+        if (rc) return rc;
+        m->cur = 1 - k;
+        const int slot = istp - m->nit000;
+        if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
+            k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, nparts, m->cs_log + 3 * (long)slot);
+            LAUNCHCHK("checksum_finish");
+            if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
+            m->cs_next = slot + 1;
+        }
+        return NEMO2D_OK;
+    }
+    if (m->compensated) {
+        MixedState<true> n{(const double*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
+                           (const float*)x[F_UN], (const float*)x[F_VN]};
+        MixedOut<true> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
+                         (float*)y[F_VN]};
+        k_step_mixed<true><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
+                                                                m->flags, m->flu, m->flv, L, pf, tide,
+                                                                m->partials);
+    } else {
+        MixedState<false> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
+                            (const float*)x[F_UN], (const float*)x[F_VN]};
+        MixedOut<false> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V], (float*)y[F_UN],
+                          (float*)y[F_VN]};
+        k_step_mixed<false><<<g, dim3(TBX, TBY), 0, m->stream>>>(sf, n, a, m->hf_t, m->hf_u, m->hf_v,
+                                                                 m->flags, m->flu, m->flv, L, pf, tide,
+                                                                 m->partials);
+    }
+    LAUNCHCHK("step_mixed");
+    m->cur = 1 - k;
+    const int slot = istp - m->nit000;
+    if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
+        k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(m->partials, (int)(g.x * g.y),
+                                                            m->cs_log + 3 * (long)slot);
+        LAUNCHCHK("checksum_finish");
+        if (slot + 1 > m->cs_n) m->cs_n = slot + 1;
+        m->cs_next = slot + 1;
+    }
+    return NEMO2D_OK;
+}
+
 int nemo2d_step(nemo2d_model* m, int istp) {
     READY(m, "nemo2d_step");
     if (m->mixed) return step_mixed(m, istp);
@@ -1730,8 +1804,12 @@ int nemo2d_step(nemo2d_model* m, int istp) {
            m->S[1 - k][F_VN]};
     const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
-    if (var == VAR_AUTO) var = m->uniform ? VAR_MARCH12_SHN : VAR_CELL;
-    if (!m->uniform && var != VAR_CELL_FLIN) var = VAR_CELL;  // marching needs uniform statics
+    const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
+    if (var == VAR_AUTO)
+        var = m->uniform ? VAR_MARCH12_SHN : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
+    if (var == VAR_MARCH12_SHD && !depth_march_ok) var = VAR_CELL;
+    // marching with scalar depths needs uniform statics
+    if (!m->uniform && var != VAR_CELL_FLIN && var != VAR_MARCH12_SHD) var = VAR_CELL;
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
     const int slot = istp - m->nit000;
@@ -1760,13 +1838,15 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         case VAR_MARCH16_SHN: rc = launch_march_sh<16, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH24_SHN: rc = launch_march_sh<24, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_SHN: rc = launch_march_sh<32, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_SHD: rc = launch_march_depths(m, n, a, tide, nparts); break;
         default: rc = launch_march<32, false>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
     m->cur = 1 - k;
     m->a_current = false;
     // the shared-term kernels set the row-0 Flather v points themselves
-    const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L);
+    const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L) &&
+                    (var != VAR_MARCH12_SHD || depth_march_ok);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && has_fl) {
         // Flather points of the new state, then (if recording) the checksums

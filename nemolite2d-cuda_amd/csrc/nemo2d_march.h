@@ -1,0 +1,369 @@
+// Column-marching fused step with shared face terms, generic over the
+// arithmetic type R (double for the drop-in, float for the single-precision
+// form), the storage type SS of sshn (double, or float for the
+// uncompensated single-precision form) and the source of the depths
+// (uniform scalars or per-point arrays); metrics and Coriolis parameters
+// are uniform.  The arithmetic of every point is the reference's
+// (original/nemolite2d.f90:399-403 continuity, :439-514 momentum_u,
+// :527-606 momentum_v, :735-765 next, :676-690 bc_flather_v), evaluated as
+// the per-point kernels of this library evaluate it, so the results equal
+// theirs bit for bit:
+//  * a term on a face shared by two points of a column is computed once at
+//    the lower point and carried up a row (continuity's v flux, the
+//    momentum_u south advective flux, v average and du/dy quotient,
+//    momentum_v's south advective flux and dv/dy);
+//  * a term on a face shared by two points of a row is computed by the
+//    lane of the east point and shuffled to the west lane (momentum_u's
+//    east advective flux and du/dx, momentum_v's east advective flux, u
+//    average and dv/dx quotient);
+//  * every shared term is the same expression tree as the point's own (an
+//    IEEE sum a+b equals b+a); a walled point recomputes its own advective
+//    term;
+//  * rows are held in a ring of three whose roles rotate, so no row is
+//    copied, and the arrays are addressed by 32-bit offsets from the kernel
+//    arguments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nemo2d_core.h"
+#include "nemo2d_step.h"
+
+namespace nemo2d {
+
+constexpr int GW = 4;      // waves per block (column strips side by side)
+constexpr int GOUT = 63;   // output columns per wave (lane 63 is the east halo)
+
+// uniform metrics, Coriolis parameters and (when not per point) depths
+template <class R>
+struct MetG {
+    R e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, fcu, fcv, ht, hu, hv;
+};
+
+template <class R, class SS, bool DARR>
+struct StateG {
+    const SS* sshn;
+    const R *sshn_u, *sshn_v, *un, *vn;
+    const R *ht, *hu, *hv;  // per-point depths (DARR)
+};
+template <class R, class SS>
+struct OutG {
+    SS* sshn;
+    R *sshn_u, *sshn_v, *un, *vn;
+};
+
+// element k of an array addressed by a 32-bit element offset
+template <class T>
+__device__ __forceinline__ T ldg_(const T* b, unsigned k) {
+    return *(const T*)((const char*)b + k * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stg_(T* b, unsigned k, T v) {
+    *(T*)((char*)b + k * (unsigned)sizeof(T)) = v;
+}
+
+// One row of the now-state at columns i-1, i, i+1.
+template <class R, class SS, bool DARR>
+struct RowG {
+    R un[3], vn[3], su[3], sv[3];
+    SS ss[3];
+    R ht_[DARR ? 3 : 1], hu_[DARR ? 3 : 1], hv_[DARR ? 3 : 1];
+};
+
+template <class R, class SS, bool DARR>
+__device__ __forceinline__ void load_rowg(RowG<R, SS, DARR>& r, const StateG<R, SS, DARR>& n,
+                                          unsigned k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.un[c] = ldg_(n.un, k + c);
+        r.vn[c] = ldg_(n.vn, k + c);
+        r.su[c] = ldg_(n.sshn_u, k + c);
+        r.sv[c] = ldg_(n.sshn_v, k + c);
+        r.ss[c] = ldg_(n.sshn, k + c);
+        if constexpr (DARR) {
+            r.ht_[c] = ldg_(n.ht, k + c);
+            r.hu_[c] = ldg_(n.hu, k + c);
+            r.hv_[c] = ldg_(n.hv, k + c);
+        }
+    }
+}
+
+template <class R, class SS, bool DARR>
+struct DepG {
+    const MetG<R>& m;
+    __device__ R ht(const RowG<R, SS, DARR>& r, int c) const { return DARR ? r.ht_[DARR ? c : 0] : m.ht; }
+    __device__ R hu(const RowG<R, SS, DARR>& r, int c) const { return DARR ? r.hu_[DARR ? c : 0] : m.hu; }
+    __device__ R hv(const RowG<R, SS, DARR>& r, int c) const { return DARR ? r.hv_[DARR ? c : 0] : m.hv; }
+};
+
+// Terms on the north faces of the points of row c (row nr above it).
+template <class R>
+struct NorthG {
+    R r3;     // continuity: (sshn_v + hv) * vn
+    R kn;     // momentum_u: uu_n * v_n * depn, no wall
+    R vnc;    // momentum_u: v_nc
+    R vnu;    // momentum_u: v_n
+    R depnu;  // momentum_u: depn
+    R hn;     // momentum_v: vv_n * v_n * depn
+    R dvdyn;  // momentum_v: dvdy_n
+    R qdudy;  // momentum_u: dudy_n's quotient (un_n - un_c) / (e2u + e2u)
+};
+
+template <class R, class SS, bool DARR>
+__device__ __forceinline__ NorthG<R> north_g(const MetG<R>& m, const DepG<R, SS, DARR>& d,
+                                             const RowG<R, SS, DARR>& c,
+                                             const RowG<R, SS, DARR>& nr) {
+    const R h = R(0.5);
+    NorthG<R> t;
+    t.r3 = (c.sv[1] + d.hv(c, 1)) * c.vn[1];
+    t.vnc = h * (c.vn[1] + c.vn[2]);
+    t.vnu = h * t.vnc * (m.e1v + m.e1v);
+    t.depnu = h * (d.hv(c, 1) + c.sv[1] + d.hv(c, 2) + c.sv[2]);
+    const R sn = sgn_half(t.vnu);
+    t.kn = ((h + sn) * c.un[1] + (h - sn) * nr.un[1]) * t.vnu * t.depnu;
+    const R vn_n = h * (c.vn[1] + nr.vn[1]) * m.e1t;
+    const R depn = d.ht(nr, 1) + R(nr.ss[1]);
+    const R sv = sgn_half(vn_n);
+    t.hn = ((h + sv) * c.vn[1] + (h - sv) * nr.vn[1]) * vn_n * depn;
+    t.dvdyn = dv(nr.vn[1] - c.vn[1], m.e2t) * (d.ht(nr, 1) + R(nr.ss[1]));
+    t.qdudy = dv(nr.un[1] - c.un[1], m.e2u + m.e2u);
+    return t;
+}
+
+// next (original/nemolite2d.f90:735-765) for uniform areas
+template <class R>
+__device__ __forceinline__ R next_g(unsigned cls, R lo, R hi, R keep, R e12t, R e12x) {
+    switch (cls) {
+        case NX_AVG: return R(0.5) * (e12t * lo + e12t * hi) / e12x;
+        case NX_HI: return hi;
+        case NX_LO: return lo;
+        default: return keep;
+    }
+}
+
+template <class R, class SS, bool DARR>
+struct CtxG {
+    MetG<R> m;
+    ParamsT<R> p;
+    StateG<R, SS, DARR> n;
+    OutG<R, SS> a;
+    const uint16_t* fl;
+    const double* flv;  // Flather sqrt(g/hv) (row-0 points)
+    Layout L;
+    double tide;
+    int i, il;
+    unsigned lk;  // element offset of the window's west column within a row
+    bool own, tcol;
+    __device__ unsigned rowk(int j) const { return (unsigned)L.at(0, j); }
+};
+
+template <class R, class SS>
+struct CarryG {
+    NorthG<R> south;  // the row below's north terms
+    SS s_prev;        // new ssh at (i, j-1)
+    R sv_prev;        // old sshn_v at (i, j-1)
+    unsigned f_prev, f_nxt;
+    double ct, cu, cv;  // per-lane checksum sums
+};
+
+// One row j: A, B hold rows j-1, j; C is loaded with row j+1 here.
+template <class R, class SS, bool DARR>
+__device__ __forceinline__ void row_g(const CtxG<R, SS, DARR>& x, CarryG<R, SS>& k,
+                                      const RowG<R, SS, DARR>& A, const RowG<R, SS, DARR>& B,
+                                      RowG<R, SS, DARR>& C, int j) {
+    const Layout& L = x.L;
+    const MetG<R>& m = x.m;
+    const DepG<R, SS, DARR> d{m};
+    const R h = R(0.5);
+    const unsigned f = k.f_nxt;
+    load_rowg(C, x.n, x.rowk(j + 1) + x.lk);
+    {
+        const int jf = j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1;
+        k.f_nxt = ldg_(x.fl, x.rowk(jf) + x.lk + 1);
+    }
+    const unsigned ko = x.rowk(j) + x.lk + 1;  // element offset of (i, j)
+    const unsigned kP = (unsigned)L.pitch;
+    const NorthG<R> nt = north_g(m, d, B, C);
+    const NorthG<R>& st = k.south;
+
+    // continuity
+    SS s_c = SS(0);
+    if (j >= 1 && x.tcol) {
+        if (f & T_CLAMP) {
+            s_c = SS(x.tide);
+        } else {
+            const R r1 = (B.su[1] + d.hu(B, 1)) * B.un[1];
+            const R r2 = (B.su[0] + d.hu(B, 0)) * B.un[0];
+            s_c = B.ss[1] + SS(dv((r2 - r1 + st.r3 - nt.r3) * x.p.rdt, m.e12t));
+        }
+    }
+    const SS s_e = __shfl_down(s_c, 1, 64);
+
+    // west-face terms; the east ones come from lane+1
+    const R un_c = B.un[1];
+    const R u_w = h * (un_c + B.un[0]) * m.e2t;
+    const R depw_u = d.ht(B, 1) + R(B.ss[1]);
+    const R sw_u = sgn_half(u_w);
+    const R fw = ((h - sw_u) * un_c + (h + sw_u) * B.un[0]) * u_w * depw_u;
+    const R dudx_w = dv(un_c - B.un[0], m.e1t) * (d.ht(B, 1) + R(B.ss[1]));
+    const R vn_c = B.vn[1];
+    const R u_wc = h * (B.un[0] + C.un[0]);
+    const R u_wv = h * u_wc * (m.e2u + m.e2u);
+    const R depw_v = h * (d.hu(B, 0) + B.su[0] + d.hu(C, 0) + C.su[0]);
+    const R sw_v = sgn_half(u_wv);
+    const R gw = ((h - sw_v) * vn_c + (h + sw_v) * B.vn[0]) * u_wv * depw_v;
+    const R qdvdx_w = dv(vn_c - B.vn[0], m.e1v + m.e1v);
+    const R fe = __shfl_down(fw, 1, 64);
+    const R dudx_e = __shfl_down(dudx_w, 1, 64);
+    const R ge = __shfl_down(gw, 1, 64);
+    const R u_ec = __shfl_down(u_wc, 1, 64);
+    const R qdvdx_e = __shfl_down(qdvdx_w, 1, 64);
+
+    const R su_old = B.su[1], sv_old = B.sv[1];
+    if (x.own) {
+        const int i = x.i;
+        R u = R(0), v = R(0);
+        const unsigned cu = ucls(f), cv = vcls(f);
+        if (j >= 1 && cu == CLS_COMP) {
+            const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
+            R ts = st.kn, tn = nt.kn;
+            if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
+            if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
+            const R adv = fw - fe + ts - tn;
+            R dudy_s = R(0), dudy_n = R(0);
+            if (!ws) dudy_s = st.qdudy * (d.hu(B, 1) + B.su[1] + d.hu(A, 1) + A.su[1]);
+            if (!wn) dudy_n = nt.qdudy * (d.hu(B, 1) + B.su[1] + d.hu(C, 1) + C.su[1]);
+            R vis = (dudx_e - dudx_w) * m.e2u + (dudy_n - dudy_s) * m.e1u * h;
+            vis = x.p.visc * vis;
+            const R hsu = d.hu(B, 1) + B.su[1];
+            const R cor = h * (m.fcu * (st.vnc + nt.vnc)) * m.e12u * hsu;
+            const R hpg = -R(GRAV) * hsu * m.e2u * (R(B.ss[2]) - R(B.ss[1]));
+            u = dv(dv(un_c * hsu + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12u), d.hu(B, 1) + R(0)),
+                   x.p.fric);
+        }
+        if (i >= 1 && cv == CLS_COMP) {
+            const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
+            R tw = gw, te = ge;
+            if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
+            if (we) {
+                const R u_e = h * u_ec * (m.e2u + m.e2u);
+                const R depe = h * (d.hu(B, 1) + B.su[1] + d.hu(C, 1) + C.su[1]);
+                te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
+            }
+            const R adv = tw - te + st.hn - nt.hn;
+            R dvdx_w = R(0), dvdx_e = R(0);
+            if (!ww) dvdx_w = qdvdx_w * (d.hv(B, 1) + B.sv[1] + d.hv(B, 0) + B.sv[0]);
+            if (!we) dvdx_e = qdvdx_e * (d.hv(B, 1) + B.sv[1] + d.hv(B, 2) + B.sv[2]);
+            R vis = (nt.dvdyn - st.dvdyn) * m.e1v + (dvdx_e - dvdx_w) * m.e2v * h;
+            vis = x.p.visc * vis;
+            const R hsv = d.hv(B, 1) + B.sv[1];
+            const R cor = -h * (m.fcv * (u_ec + u_wc)) * m.e12v * hsv;
+            const R hpg = -R(GRAV) * hsv * m.e1v * (R(C.ss[1]) - R(B.ss[1]));
+            v = dv(dv(vn_c * hsv + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12v), d.hv(B, 1) + R(0)),
+                   x.p.fric);
+        }
+        if (j >= 1) {
+            if (i >= 1) {
+                stg_(x.a.sshn, ko, s_c);
+                k.ct += fabs((double)s_c);
+            }
+            if (cu != CLS_FLATHER) {
+                stg_(x.a.un, ko, u);
+                if (i >= 1) k.cu += fabs((double)u);
+            }
+            stg_(x.a.sshn_u, ko, next_g<R>(sucls(f), R(s_c), R(s_e), su_old, m.e12t, m.e12u));
+        }
+        if (i >= 1) {
+            if (cv != CLS_FLATHER) {
+                stg_(x.a.vn, ko, v);
+                if (j >= 1) k.cv += fabs((double)v);
+            }
+            // bc_flather_v at a row-0 point whose inner point is this one
+            // (row 0 is outside the checksums)
+            if (j == 1 && vcls(k.f_prev) == CLS_FLATHER && (k.f_prev & V_FL_NORTH))
+                stg_(x.a.vn, ko - kP, v + R(ldg_(x.flv, ko - kP)) * (A.sv[1] - B.sv[1]));
+            if (k.f_prev != 0xffffffffu)
+                stg_(x.a.sshn_v, ko - kP,
+                     next_g<R>(svcls(k.f_prev), R(k.s_prev), R(s_c), k.sv_prev, m.e12t, m.e12v));
+        }
+    }
+    k.south = nt;
+    k.s_prev = s_c;
+    k.sv_prev = sv_old;
+    k.f_prev = f;
+}
+
+// continuity at (i, j) from memory, for the segment's last sshn_v
+template <class R, class SS, bool DARR>
+__device__ __forceinline__ SS ssha_at_g(const CtxG<R, SS, DARR>& x, unsigned k) {
+    const StateG<R, SS, DARR>& n = x.n;
+    const unsigned kP = (unsigned)x.L.pitch;
+    const R hu_c = DARR ? ldg_(n.hu, k) : x.m.hu, hu_w = DARR ? ldg_(n.hu, k - 1) : x.m.hu;
+    const R hv_c = DARR ? ldg_(n.hv, k) : x.m.hv, hv_s = DARR ? ldg_(n.hv, k - kP) : x.m.hv;
+    const R r1 = (ldg_(n.sshn_u, k) + hu_c) * ldg_(n.un, k);
+    const R r2 = (ldg_(n.sshn_u, k - 1) + hu_w) * ldg_(n.un, k - 1);
+    const R r3 = (ldg_(n.sshn_v, k) + hv_c) * ldg_(n.vn, k);
+    const R r4 = (ldg_(n.sshn_v, k - kP) + hv_s) * ldg_(n.vn, k - kP);
+    return ldg_(n.sshn, k) + SS(dv((r2 - r1 + r4 - r3) * x.p.rdt, x.m.e12t));
+}
+
+// Block (GW waves) = GW column strips of GOUT columns, one NJ-row segment;
+// tiles are handed out XCD by XCD (block b runs on XCD b % 8).
+template <class R, class SS, bool DARR, int NJ>
+__global__ void __launch_bounds__(64 * GW)
+k_step_march_g(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
+               const uint16_t* __restrict__ fl, const double* __restrict__ flv, Layout L,
+               double tide, double* __restrict__ partials, int nbx, int ntiles) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    CarryG<R, SS> k;
+    k.ct = k.cu = k.cv = 0.0;
+    if (t < ntiles) {
+        const int bx = t % nbx, by = t / nbx;
+        CtxG<R, SS, DARR> x;
+        x.m = m;
+        x.p = p;
+        x.n = n;
+        x.a = a;
+        x.fl = fl;
+        x.flv = flv;
+        x.L = L;
+        x.tide = tide;
+        x.i = (bx * GW + wave) * GOUT + lane;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+        x.lk = (unsigned)(x.il - 1);
+        x.own = lane < GOUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = by * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        const DepG<R, SS, DARR> d{m};
+        RowG<R, SS, DARR> R0, R1, R2;
+        load_rowg(R0, n, x.rowk(jb - 1) + x.lk);
+        load_rowg(R1, n, x.rowk(jb) + x.lk);
+        k.f_nxt = ldg_(fl, x.rowk(jb) + x.lk + 1);
+        k.f_prev = 0xffffffffu;  // no row below inside this segment
+        k.s_prev = SS(0);
+        k.sv_prev = R(0);
+        k.south = north_g(m, d, R0, R1);
+        for (int j = jb; j <= je; j += 3) {
+            row_g(x, k, R0, R1, R2, j);
+            if (j + 1 > je) break;
+            row_g(x, k, R1, R2, R0, j + 1);
+            if (j + 2 > je) break;
+            row_g(x, k, R2, R0, R1, j + 2);
+        }
+        // the last row's sshn_v needs the new ssh one row up
+        if (x.own && x.i >= 1) {
+            const unsigned ko = x.rowk(je) + x.lk + 1, kP = (unsigned)L.pitch;
+            const unsigned c = svcls(k.f_prev);
+            SS s_n = SS(0);
+            if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI))
+                s_n = (ldg_(fl, ko + kP) & T_CLAMP) ? SS(tide) : ssha_at_g(x, ko + kP);
+            stg_(a.sshn_v, ko, next_g<R>(c, R(k.s_prev), R(s_n), k.sv_prev, m.e12t, m.e12v));
+        }
+    }
+    block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
+}
+
+}  // namespace nemo2d

@@ -164,7 +164,7 @@ def test_reference_golden_checksums(traj):
 
 # ---- general masks and metrics ------------------------------------------------
 def random_grid_oracle(jpi, jpj, seed, islands=True, open_edges="WEN", vary_metrics=True,
-                       chained=False):
+                       chained=False, vary_depths=False):
     rng = np.random.default_rng(seed)
     o = Oracle(jpi, jpj, dx=900.0, dy=1100.0, dep_const=80.0, rdt=15.0, cbfr=0.002, visc=5.0,
                default_grid=True)
@@ -193,6 +193,10 @@ def random_grid_oracle(jpi, jpj, seed, islands=True, open_edges="WEN", vary_metr
             a = o.arr(k)
             a[:] = 30.0 + 20.0 * rng.random(a.shape)
         o.grid_areas()
+    elif vary_depths:
+        for k in ("ht", "hu", "hv"):
+            a = o.arr(k)
+            a[:] = 40.0 + 60.0 * rng.random(a.shape)
     o.arr("sshn")[:] = 0.05 * rng.standard_normal(o.arr("sshn").shape)
     o.initialise()
     un, vn = o.arr("un"), o.arr("vn")
@@ -345,6 +349,25 @@ def test_fortran_driver(tmp_path, fused):
         assert d.shape == ((g - 2) ** 2, 6)
     d = np.loadtxt(tmp_path / "go2d_00300.dat")
     assert np.allclose(d[:, 3], o.arr("sshn").ravel(), rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("variant", ["auto", "march12_shd", "cell"])
+@pytest.mark.parametrize("islands", [False, True])
+def test_per_point_depths_bitwise_vs_oracle(variant, islands):
+    """Uniform metrics, varying depths, the reference basin's open south edge
+    (and inner walls): the marching kernel with per-point depths (the
+    default for such grids) against the oracle."""
+    o = random_grid_oracle(203, 61, 21 + islands, islands=islands, open_edges="", vary_metrics=False,
+                           vary_depths=True)
+    with model_like(o) as m:
+        m.set_variant(variant)
+        m.record_checksums(25)
+        for istp in range(1, 26):
+            m.step(istp)
+            o.step(istp)
+            if istp % 5 == 0:
+                assert_fields_equal(m, o, "%s step %d" % (variant, istp))
+        assert np.all(np.isfinite(m.checksum_log(25)))
 
 
 @pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
