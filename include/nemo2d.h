@@ -202,7 +202,9 @@ int nemo2d_ensemble_reset(nemo2d_ensemble *e);
  * (index in *choice, ms per step in *best_ms) and reset to rest. */
 int nemo2d_ensemble_tune(nemo2d_ensemble *e, int nsteps, int *choice, double *best_ms);
 int nemo2d_ensemble_set_tiling(nemo2d_ensemble *e, int choice);
-/* Work-item shape (columns x rows of cells) of tiling `choice`. */
+/* Work-item shape (columns x rows of cells) of tiling `choice`: choices
+ * 0..7 are blocks of 256 cells, one per thread; choice 8 is one wave's
+ * strip of 63 columns by 12 rows, column-marching with shared face terms. */
 int nemo2d_ensemble_tiling(int choice, int *tx, int *ty);
 /* Steps nit000..nitend; use_graph != 0 captures the whole loop into a HIP
  * graph first (capture not timed) and replays it.  *ms = device time of the

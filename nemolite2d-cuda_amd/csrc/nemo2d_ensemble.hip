@@ -6,9 +6,11 @@
 // constant depth and tidal amplitude/period.  All members live in pooled
 // device arrays, each in its own padded layout.  One step of the whole
 // ensemble is one persistent kernel: a fixed grid of blocks walks a list of
-// (member, tile) work items; each cell uses the same per-cell body as the
-// single-model path, so an FP64 member is bitwise the same as that member
-// run alone.  Per-member checksums are reduced in a fixed order by a second
+// (member, tile) work items.  The work items are blocks of cells computed
+// one per thread with the single-model path's per-cell body, or (the last
+// tiling) one wave's column strip marched with the single-model path's
+// shared-face-term body; either way an FP64 member is bitwise the same as
+// that member run alone.  Per-member checksums are reduced in a fixed order by a second
 // kernel.  A run of many steps is captured once into a HIP graph and
 // replayed.  The tile shape is chosen by timing the candidates
 // (nemo2d_ensemble_tune).
@@ -25,6 +27,7 @@
 #include "nemo2d_core.h"
 #include "nemo2d_internal.h"
 #include "nemo2d_step.h"
+#include "nemo2d_march.h"
 
 using namespace nemo2d;
 
@@ -47,12 +50,17 @@ struct EnsMember {
     int tile0, ntiles;
 };
 
-// Candidate tilings: TX columns x (EB/TX)*ROWS rows per work item.
+// Candidate tilings: TX columns x (EB/TX)*ROWS rows per work item of a
+// block, one point per thread; the last is column-marching: a work item is
+// one wave's strip of GOUT columns over EMJ rows (shared face terms).
 struct Tiling {
     int tx, rows;
 };
-constexpr Tiling TILINGS[] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 1}, {32, 2}};
+constexpr int EMJ = 12;  // rows per marching work item
+constexpr Tiling TILINGS[] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 1}, {32, 2},
+                              {GOUT, EMJ}};
 constexpr int NTILINGS = sizeof(TILINGS) / sizeof(TILINGS[0]);
+constexpr int MARCH_TILING = NTILINGS - 1;
 
 template <int TX, int ROWS>
 __global__ void __launch_bounds__(EB)
@@ -112,6 +120,40 @@ k_ens_step32(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, 
     }
 }
 
+// Column-marching step of the whole ensemble: each wave takes work items
+// (member, strip, segment) in turn and writes the item's checksum partial.
+template <class R, class SS>
+__global__ void __launch_bounds__(EB)
+k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
+            StateG<R, SS, false> n0, OutG<R, SS> a0, const uint16_t* __restrict__ flp,
+            const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
+            int tide_stride, int step_idx, double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    for (int w = blockIdx.x * (EB / 64) + (threadIdx.x >> 6); w < nitems; w += gridDim.x * (EB / 64)) {
+        const int4 it = items[w];
+        const EnsMember& M = mem[it.x];
+        const long off = M.off;
+        MetG<R> mt;
+        if constexpr (sizeof(R) == sizeof(double)) {
+            const Scal& d = M.s;
+            mt = MetG<R>{d.e1t, d.e2t, d.e1u, d.e2u, d.e1v, d.e2v, d.e12t, d.e12u, d.e12v,
+                         d.fcu, d.fcv, d.ht, d.hu, d.hv};
+        } else {
+            const ScalF& d = M.sf;
+            mt = MetG<R>{d.e1t, d.e2t, d.e1u, d.e2u, d.e1v, d.e2v, d.e12t, d.e12u, d.e12v,
+                         d.fcu, d.fcv, d.ht, d.hu, d.hv};
+        }
+        const StateG<R, SS, false> n{n0.sshn + off, n0.sshn_u + off, n0.sshn_v + off, n0.un + off,
+                                     n0.vn + off, nullptr, nullptr, nullptr};
+        const OutG<R, SS> a{a0.sshn + off, a0.sshn_u + off, a0.sshn_v + off, a0.un + off, a0.vn + off};
+        const double tide = tide_tab[(long)it.x * tide_stride + step_idx];
+        CarryG<R, SS> k;
+        k.ct = k.cu = k.cv = 0.0;
+        march_wave_g<R, SS, false, EMJ>(mt, p, n, a, flp + off, flvp + off, M.L, tide, it.y, it.z, lane, k);
+        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)w);
+    }
+}
+
 // Per-member checksums: block m reduces its members' tile partials in order.
 __global__ void __launch_bounds__(256)
 k_ens_finish(const EnsMember* __restrict__ mem, const double* __restrict__ partials,
@@ -152,7 +194,8 @@ namespace {
 
 int build_tiles(nemo2d_ensemble* e, int tiling) {
     const Tiling T = TILINGS[tiling];
-    const int th = (EB / T.tx) * T.rows;
+    // rows per work item (a marching item is one wave's strip)
+    const int th = tiling == MARCH_TILING ? T.rows : (EB / T.tx) * T.rows;
     std::vector<int4> tl;
     for (int k = 0; k < e->nmem; ++k) {
         EnsMember& M = e->hmem[k];
@@ -203,8 +246,33 @@ void launch_step_t(nemo2d_ensemble* e, int step_idx) {
     }
 }
 
+void launch_march(nemo2d_ensemble* e, int step_idx) {
+    const int k = e->cur;
+    if (!e->fp32) {
+        double** x = (double**)e->X[k];
+        double** y = (double**)e->X[1 - k];
+        StateG<double, double, false> n{x[0], x[1], x[2], x[3], x[4], nullptr, nullptr, nullptr};
+        OutG<double, double> a{y[0], y[1], y[2], y[3], y[4]};
+        k_ens_march<double, double><<<e->grid, EB, 0, e->stream>>>(
+            e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
+            step_idx, e->partials);
+    } else {
+        void** x = e->X[k];
+        void** y = e->X[1 - k];
+        StateG<float, double, false> n{(const double*)x[0], (const float*)x[1], (const float*)x[2],
+                                       (const float*)x[3], (const float*)x[4], nullptr, nullptr, nullptr};
+        OutG<float, double> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
+        ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
+                          (float)e->prm.fric};
+        k_ens_march<float, double><<<e->grid, EB, 0, e->stream>>>(
+            e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
+            step_idx, e->partials);
+    }
+}
+
 int launch_step(nemo2d_ensemble* e, int step_idx, int slot) {
     switch (e->tiling) {
+        case MARCH_TILING: launch_march(e, step_idx); break;
         case 0: launch_step_t<64, 1>(e, step_idx); break;
         case 1: launch_step_t<64, 2>(e, step_idx); break;
         case 2: launch_step_t<64, 4>(e, step_idx); break;
@@ -362,7 +430,7 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
     }
     e->grid = 4 * cus;
     int rc;
-    if ((rc = build_tiles(e, 0)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
+    if ((rc = build_tiles(e, MARCH_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
     *out = e;
     return NEMO2D_OK;
 }
@@ -378,7 +446,7 @@ int nemo2d_ensemble_tiling(int choice, int* tx, int* ty) {
     if (choice < 0 || choice >= NTILINGS || !tx || !ty)
         return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_tiling: bad argument");
     *tx = TILINGS[choice].tx;
-    *ty = (EB / TILINGS[choice].tx) * TILINGS[choice].rows;
+    *ty = choice == MARCH_TILING ? TILINGS[choice].rows : (EB / TILINGS[choice].tx) * TILINGS[choice].rows;
     return NEMO2D_OK;
 }
 

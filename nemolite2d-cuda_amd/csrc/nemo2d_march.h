@@ -307,6 +307,59 @@ __device__ __forceinline__ SS ssha_at_g(const CtxG<R, SS, DARR>& x, unsigned k) 
     return ldg_(n.sshn, k) + SS(dv((r2 - r1 + r4 - r3) * x.p.rdt, x.m.e12t));
 }
 
+// One wave: column strip `strip` (columns strip*GOUT .. +GOUT-1; lane 63
+// is the east halo) over the NJ-row segment `seg`, adding its checksum
+// terms to k.ct/cu/cv.
+template <class R, class SS, bool DARR, int NJ>
+__device__ __forceinline__ void march_wave_g(const MetG<R>& m, const ParamsT<R>& p,
+                                             const StateG<R, SS, DARR>& n, const OutG<R, SS>& a,
+                                             const uint16_t* __restrict__ fl,
+                                             const double* __restrict__ flv, const Layout& L,
+                                             double tide, int strip, int seg, int lane,
+                                             CarryG<R, SS>& k) {
+    CtxG<R, SS, DARR> x;
+    x.m = m;
+    x.p = p;
+    x.n = n;
+    x.a = a;
+    x.fl = fl;
+    x.flv = flv;
+    x.L = L;
+    x.tide = tide;
+    x.i = strip * GOUT + lane;
+    x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+    x.lk = (unsigned)(x.il - 1);
+    x.own = lane < GOUT && x.i <= L.jpi;
+    x.tcol = x.i >= 1 && x.i <= L.jpi;
+    const int jb = seg * NJ;
+    const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+    const DepG<R, SS, DARR> d{m};
+    RowG<R, SS, DARR> R0, R1, R2;
+    load_rowg(R0, n, x.rowk(jb - 1) + x.lk);
+    load_rowg(R1, n, x.rowk(jb) + x.lk);
+    k.f_nxt = ldg_(fl, x.rowk(jb) + x.lk + 1);
+    k.f_prev = 0xffffffffu;  // no row below inside this segment
+    k.s_prev = SS(0);
+    k.sv_prev = R(0);
+    k.south = north_g(m, d, R0, R1);
+    for (int j = jb; j <= je; j += 3) {
+        row_g(x, k, R0, R1, R2, j);
+        if (j + 1 > je) break;
+        row_g(x, k, R1, R2, R0, j + 1);
+        if (j + 2 > je) break;
+        row_g(x, k, R2, R0, R1, j + 2);
+    }
+    // the last row's sshn_v needs the new ssh one row up
+    if (x.own && x.i >= 1) {
+        const unsigned ko = x.rowk(je) + x.lk + 1, kP = (unsigned)L.pitch;
+        const unsigned c = svcls(k.f_prev);
+        SS s_n = SS(0);
+        if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI))
+            s_n = (ldg_(fl, ko + kP) & T_CLAMP) ? SS(tide) : ssha_at_g(x, ko + kP);
+        stg_(a.sshn_v, ko, next_g<R>(c, R(k.s_prev), R(s_n), k.sv_prev, m.e12t, m.e12v));
+    }
+}
+
 // Block (GW waves) = GW column strips of GOUT columns, one NJ-row segment;
 // tiles are handed out XCD by XCD (block b runs on XCD b % 8).
 template <class R, class SS, bool DARR, int NJ>
@@ -319,51 +372,24 @@ k_step_march_g(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
     const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     CarryG<R, SS> k;
     k.ct = k.cu = k.cv = 0.0;
-    if (t < ntiles) {
-        const int bx = t % nbx, by = t / nbx;
-        CtxG<R, SS, DARR> x;
-        x.m = m;
-        x.p = p;
-        x.n = n;
-        x.a = a;
-        x.fl = fl;
-        x.flv = flv;
-        x.L = L;
-        x.tide = tide;
-        x.i = (bx * GW + wave) * GOUT + lane;
-        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
-        x.lk = (unsigned)(x.il - 1);
-        x.own = lane < GOUT && x.i <= L.jpi;
-        x.tcol = x.i >= 1 && x.i <= L.jpi;
-        const int jb = by * NJ;
-        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
-        const DepG<R, SS, DARR> d{m};
-        RowG<R, SS, DARR> R0, R1, R2;
-        load_rowg(R0, n, x.rowk(jb - 1) + x.lk);
-        load_rowg(R1, n, x.rowk(jb) + x.lk);
-        k.f_nxt = ldg_(fl, x.rowk(jb) + x.lk + 1);
-        k.f_prev = 0xffffffffu;  // no row below inside this segment
-        k.s_prev = SS(0);
-        k.sv_prev = R(0);
-        k.south = north_g(m, d, R0, R1);
-        for (int j = jb; j <= je; j += 3) {
-            row_g(x, k, R0, R1, R2, j);
-            if (j + 1 > je) break;
-            row_g(x, k, R1, R2, R0, j + 1);
-            if (j + 2 > je) break;
-            row_g(x, k, R2, R0, R1, j + 2);
-        }
-        // the last row's sshn_v needs the new ssh one row up
-        if (x.own && x.i >= 1) {
-            const unsigned ko = x.rowk(je) + x.lk + 1, kP = (unsigned)L.pitch;
-            const unsigned c = svcls(k.f_prev);
-            SS s_n = SS(0);
-            if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI))
-                s_n = (ldg_(fl, ko + kP) & T_CLAMP) ? SS(tide) : ssha_at_g(x, ko + kP);
-            stg_(a.sshn_v, ko, next_g<R>(c, R(k.s_prev), R(s_n), k.sv_prev, m.e12t, m.e12v));
-        }
-    }
+    if (t < ntiles)
+        march_wave_g<R, SS, DARR, NJ>(m, p, n, a, fl, flv, L, tide, (t % nbx) * GW + wave, t / nbx,
+                                      lane, k);
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
+}
+
+// Sums of the wave's lanes (fixed order), written by lane 0 to out[0..5].
+__device__ __forceinline__ void wave_reduce3(DD a0, DD a1, DD a2, double* out) {
+    for (int mk = 32; mk >= 1; mk >>= 1) {
+        a0 = dd_merge(a0, dd_shfl_xor(a0, mk));
+        a1 = dd_merge(a1, dd_shfl_xor(a1, mk));
+        a2 = dd_merge(a2, dd_shfl_xor(a2, mk));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        out[0] = a0.s; out[1] = a0.c;
+        out[2] = a1.s; out[3] = a1.c;
+        out[4] = a2.s; out[5] = a2.c;
+    }
 }
 
 }  // namespace nemo2d

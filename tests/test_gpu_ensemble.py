@@ -30,10 +30,12 @@ def single(k, m, nsteps):
         return s.retrieve(), s.checksum_log(nsteps)
 
 
-def test_fp64_members_bitwise_equal_single_model_and_oracle():
+@pytest.mark.parametrize("tiling", [0, 8])
+def test_fp64_members_bitwise_equal_single_model_and_oracle(tiling):
     m = small_members()
     nsteps = 80
     with nemo2d.Ensemble(m, rdt=RDT) as e:
+        e.set_tiling(tiling)
         e.run(1, nsteps, graph=True)
         for k in range(len(m["jpi"])):
             r = e.retrieve(k)
@@ -57,13 +59,14 @@ def test_graph_plain_and_tilings_agree():
     nsteps = 40
     results = []
     with nemo2d.Ensemble(m, rdt=RDT) as e:
-        for tiling, graph in ((0, True), (0, False), (3, True), (5, True), (7, False), (2, True)):
+        for tiling, graph in ((0, True), (0, False), (3, True), (5, True), (7, False), (2, True),
+                              (8, True), (8, False)):
             e.set_tiling(tiling)
             e.reset()
             e.run(1, nsteps, graph=graph)
             results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
         choice, ms = e.tune(nsteps=5)
-        assert 0 <= choice < 8 and ms > 0
+        assert 0 <= choice < 9 and ms > 0
         e.run(1, nsteps)
         results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
     for other in results[1:]:
@@ -86,6 +89,20 @@ def test_fp32_ensemble_close_to_fp64():
             for f in ("ssh", "u", "v"):
                 assert np.isfinite(err[f]["rms"])
                 assert err[f]["rms"] <= 1e-2 * err[f]["ref_rms"], (k, f, err[f])
+
+
+def test_fp32_marching_tiling_equals_per_cell_tiling():
+    m = nemo2d.ensemble_members(7, seed=9, size=(20, 150))
+    out = []
+    with nemo2d.Ensemble(m, rdt=RDT, precision=32) as e:
+        for tiling in (0, 8):
+            e.set_tiling(tiling)
+            e.reset()
+            e.run(1, 120)
+            out.append([e.retrieve(k) for k in range(7)])
+    for a, b in zip(*out):
+        for f in ("sshn", "un", "vn"):
+            assert np.array_equal(a[f], b[f]), f
 
 
 def test_ensemble_rejects_bad_members():
