@@ -1675,7 +1675,7 @@ static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, doub
 // basin's open boundary): the marching kernels set those themselves.
 static bool flather_row0_only(const nemo2d_model* m) { return m->nflu == 0 && m->nflv == m->nflv0; }
 
-template <class R, class SS, bool DARR, int NJ>
+template <class R, class SS, bool DARR, int NJ, int WPE = 0>
 static int launch_march_g(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& p,
                           const StateG<R, SS, DARR>& n, const OutG<R, SS>& a, double tide,
                           int& nparts) {
@@ -1685,7 +1685,7 @@ static int launch_march_g(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& 
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_g<R, SS, DARR, NJ><<<nblocks, 64 * GW, 0, m->stream>>>(
+    k_step_march_g<R, SS, DARR, NJ, WPE><<<nblocks, 64 * GW, 0, m->stream>>>(
         mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march_g");
     nparts = nblocks;
@@ -1698,11 +1698,13 @@ static MetG<double> metg(const Scal& d) {
 }
 
 // FP64 fused step by the generic marching kernel with per-point depths
+// (within 168 VGPRs: 3 waves per SIMD; measured 3% faster than 2 waves at
+// 170 VGPRs despite a 12-byte spill)
 static int launch_march_depths(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
                                     m->st[S_HV]};
     OutG<double, double> ag{a.sshn, a.sshn_u, a.sshn_v, a.un, a.vn};
-    return launch_march_g<double, double, true, 12>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
+    return launch_march_g<double, double, true, 12, 3>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
 }
 
 template <bool FLIN>
@@ -1807,9 +1809,10 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO)
         var = m->uniform ? VAR_MARCH12_SHN : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
-    if (var == VAR_MARCH12_SHD && !depth_march_ok) var = VAR_CELL;
+    const bool depth_var = var == VAR_MARCH12_SHD;
+    if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics
-    if (!m->uniform && var != VAR_CELL_FLIN && var != VAR_MARCH12_SHD) var = VAR_CELL;
+    if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
     const int slot = istp - m->nit000;

@@ -362,8 +362,8 @@ __device__ __forceinline__ void march_wave_g(const MetG<R>& m, const ParamsT<R>&
 
 // Block (GW waves) = GW column strips of GOUT columns, one NJ-row segment;
 // tiles are handed out XCD by XCD (block b runs on XCD b % 8).
-template <class R, class SS, bool DARR, int NJ>
-__global__ void __launch_bounds__(64 * GW)
+template <class R, class SS, bool DARR, int NJ, int WPE = 0>
+__global__ void __launch_bounds__(64 * GW) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 k_step_march_g(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
                const uint16_t* __restrict__ fl, const double* __restrict__ flv, Layout L,
                double tide, double* __restrict__ partials, int nbx, int ntiles) {
