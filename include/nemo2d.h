@@ -267,6 +267,40 @@ int nemo2d_dev_upload(void *dst, const void *src, long bytes);
 int nemo2d_dev_download(void *dst, const void *src, long bytes);
 int nemo2d_dev_zero(void *p, long bytes);
 
+/* Device timers of the serial model's timed regions (dl_timer's
+ * 'Continuity', 'Momentum', 'BCs', 'Next', original/nemolite2d.f90:391-769,
+ * shared/dl_timer/src/dl_timer.f90:713-773) for the nine-call step, plus
+ * the fused step as one region: on != 0 starts recording HIP events around
+ * each region on the model's stream (and clears the sums); stage_times
+ * fills, for regions 0..4 = Continuity, Momentum (momentum_u..momentum_v),
+ * BCs (bc_ssh..bc_flather_v), Next, fused step, the call count, the total
+ * seconds and the standard error of the mean per-call time.  No
+ * counterpart in the reference's C-ABI (its driver times on the host). */
+int nemo2d_stage_timers(nemo2d_model *m, int on);
+int nemo2d_stage_times(nemo2d_model *m, int *counts, double *total_s, double *stderr_s);
+
+/* --- snapshot files (go2d_NNNNN.dat), original/nemolite2d.f90:773-799 ---
+ * The reference's `output` retrieves the state and writes the file before
+ * stepping on (cuda/nemolite2d.f90:496-500 retrieves synchronously).  Here
+ * the copy and the file run behind the steps:
+ * set_output_grid: T-point coordinates and depth (host T arrays), copied once.
+ * snapshot_begin: queues device-to-host copies of sshn, un, vn on a second
+ *   stream after the work queued so far, and a host thread that writes
+ *   dir/go2d_<istp %05d>.dat in the reference's format once they land.
+ *   Later steps are queued at once (a step that would overwrite the buffer
+ *   being copied waits for the copy on the device).  A previous snapshot is
+ *   finished first.  FP64 model only.
+ * snapshot_wait: waits for the file; returns its write status.
+ * format_go2d: the file's text for host arrays in the reference's layout
+ *   (xt, yt, ht, sshn T; un U; vn V) into buf (cap bytes; buf NULL: *len
+ *   = the size only).  Host code; needs no device. */
+int nemo2d_set_output_grid(nemo2d_model *m, const double *xt, const double *yt, const double *ht);
+int nemo2d_snapshot_begin(nemo2d_model *m, int istp, const char *dir);
+int nemo2d_snapshot_wait(nemo2d_model *m);
+int nemo2d_format_go2d(int jpi, int jpj, const double *xt, const double *yt, const double *ht,
+                       const double *sshn, const double *un, const double *vn, char *buf, long cap,
+                       long *len);
+
 /* Practical bandwidth ceiling for the fused step's traffic: a kernel that
  * reads five FP64 arrays of n elements and writes five (elementwise, no
  * stencil) is timed over `repeats` launches on the current device; *gbps

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <algorithm>
 #include <vector>
@@ -124,6 +125,29 @@ struct nemo2d_model {
     void* X[2][NF] = {};          // float (or double sshn) now-state ping-pong
     float *hf_t = nullptr, *hf_u = nullptr, *hf_v = nullptr;
     double* diag = nullptr;       // 2 doubles: volume, energy
+
+    // asynchronous snapshots (go2d files): pinned copies of sshn, un, vn
+    // (padded layout) filled on copy_stream, written by snap_thread
+    std::vector<double> out_xt, out_yt, out_ht;  // T layout, from set_output_grid
+    double* h_snap[3] = {};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_state = nullptr, ev_copy = nullptr;
+    int snap_src = -1;            // now-state buffer being copied (-1: none)
+    std::thread snap_thread;
+    int snap_rc = NEMO2D_OK;
+    std::string snap_err;
+
+    // device timers of the serial model's regions (nemo2d_stage_timers)
+    bool timing = false;
+    std::vector<hipEvent_t> tev;  // event pool; tev_used in flight
+    int tev_used = 0;
+    struct TCall {
+        int region, b, e;
+    };
+    std::vector<TCall> tcalls;
+    int topen[5] = {-1, -1, -1, -1, -1};
+    long tcount[5] = {};
+    double tsum[5] = {}, tsq[5] = {};  // ms, ms^2
 };
 
 // ---------------------------------------------------------------------------
@@ -1001,6 +1025,83 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
+// Before work on m->stream writes now-state buffer `target` (0 or 1; -1:
+// both): if a snapshot is still copying that buffer, wait for the copy.
+int snap_guard(nemo2d_model* m, int target) {
+    if (m->snap_src >= 0 && (target < 0 || target == m->snap_src)) {
+        HIPCHK(hipStreamWaitEvent(m->stream, m->ev_copy, 0));
+        m->snap_src = -1;
+    }
+    return NEMO2D_OK;
+}
+
+// Finish the snapshot thread (if any); returns its status.
+int snap_join(nemo2d_model* m) {
+    if (m->snap_thread.joinable()) m->snap_thread.join();
+    const int rc = m->snap_rc;
+    m->snap_rc = NEMO2D_OK;
+    if (rc) return fail(rc, m->snap_err);
+    return NEMO2D_OK;
+}
+
+// ---- region timers: Continuity, Momentum, BCs, Next (the serial model's
+// dl_timer regions, original/nemolite2d.f90:391-769) and the fused step
+enum { TR_CONT = 0, TR_MOM, TR_BCS, TR_NEXT, TR_FUSED, TR_N };
+
+// fold the finished calls into the running sums and recycle their events
+void timers_fold(nemo2d_model* m) {
+    if (m->tcalls.empty()) return;
+    (void)hipStreamSynchronize(m->stream);
+    for (const auto& c : m->tcalls) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, m->tev[c.b], m->tev[c.e]) != hipSuccess) continue;
+        m->tcount[c.region] += 1;
+        m->tsum[c.region] += ms;
+        m->tsq[c.region] += (double)ms * ms;
+    }
+    m->tcalls.clear();
+    m->tev_used = 0;
+}
+
+int timer_event(nemo2d_model* m) {
+    if (m->tev_used == (int)m->tev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        m->tev.push_back(e);
+    }
+    const int k = m->tev_used++;
+    (void)hipEventRecord(m->tev[k], m->stream);
+    return k;
+}
+
+void region_begin(nemo2d_model* m, int r) {
+    if (!m->timing) return;
+    bool open = false;
+    for (int k = 0; k < TR_N; ++k) open = open || m->topen[k] >= 0;
+    if (!open && m->tev_used > 8192) timers_fold(m);
+    m->topen[r] = timer_event(m);
+}
+
+void region_end(nemo2d_model* m, int r) {
+    if (!m->timing || m->topen[r] < 0) return;
+    const int e = timer_event(m);
+    if (e >= 0) m->tcalls.push_back({r, m->topen[r], e});
+    m->topen[r] = -1;
+}
+
+// begin and/or end region r around the enclosing call
+struct RegionScope {
+    nemo2d_model* m;
+    int r;
+    bool end;
+    RegionScope(nemo2d_model* m_, int r_, bool begin, bool end_) : m(m_), r(r_), end(end_) {
+        if (begin) region_begin(m, r);
+    }
+    ~RegionScope() {
+        if (end) region_end(m, r);
+    }
+};
+
 // Forget logged steps whose checksums were not finished yet.
 void drop_pending(nemo2d_model* m) {
     std::fill(m->cs_pend.begin(), m->cs_pend.end(), 0);
@@ -1437,6 +1538,8 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
         return fail(NEMO2D_ERR_ARG, "nemo2d_set_state: null array");
     const int jpi = m->L.jpi, jpj = m->L.jpj;
     const size_t bytes = (size_t)m->L.size() * sizeof(double);
+    int grc = snap_guard(m, -1);
+    if (grc) return grc;
     m->cur = 0;
     if (m->mixed) {
         const size_t nb[NF] = {m->compensated ? bytes : bytes / 2, bytes / 2, bytes / 2, bytes / 2,
@@ -1505,6 +1608,7 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
 
 int nemo2d_continuity(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_continuity");
+    RegionScope rs_(m, TR_CONT, true, true);
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_continuity<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -1515,6 +1619,7 @@ int nemo2d_continuity(nemo2d_model* m) {
 
 int nemo2d_momentum_u(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_momentum_u");
+    RegionScope rs_(m, TR_MOM, true, false);
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_momentum_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -1526,6 +1631,7 @@ int nemo2d_momentum_u(nemo2d_model* m) {
 
 int nemo2d_momentum_v(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_momentum_v");
+    RegionScope rs_(m, TR_MOM, false, true);
     const Layout& L = m->L;
     NState n = nstate(m, m->cur);
     WITH_STATICS(m, st, (k_momentum_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -1537,6 +1643,7 @@ int nemo2d_momentum_v(nemo2d_model* m) {
 
 int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
     READY_STAGE(m, "nemo2d_bc_ssh");
+    RegionScope rs_(m, TR_BCS, true, false);
     const Layout& L = m->L;
     k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ssha, L,
                                                                             tide_eval(m->tide_amp, m->tide_omega, rtime));
@@ -1572,6 +1679,7 @@ int nemo2d_bc_flather_u(nemo2d_model* m) {
 
 int nemo2d_bc_flather_v(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_bc_flather_v");
+    RegionScope rs_(m, TR_BCS, false, true);
     if (m->nflv == 0) return NEMO2D_OK;
     const int lines = m->L.jpi;
     k_bc_flather_v<<<(lines + 255) / 256, 256, 0, m->stream>>>(
@@ -1582,6 +1690,9 @@ int nemo2d_bc_flather_v(nemo2d_model* m) {
 
 int nemo2d_next(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_next");
+    RegionScope rs_(m, TR_NEXT, true, true);
+    int grc = snap_guard(m, m->cur);
+    if (grc) return grc;
     const Layout& L = m->L;
     double** s = m->S[m->cur];
     WITH_STATICS(m, st, (k_next<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
@@ -1797,6 +1908,7 @@ static int step_mixed(nemo2d_model* m, int istp) {
 
 int nemo2d_step(nemo2d_model* m, int istp) {
     READY(m, "nemo2d_step");
+    RegionScope rs_(m, TR_FUSED, m->fused_ok || m->mixed, m->fused_ok || m->mixed);
     if (m->mixed) return step_mixed(m, istp);
     if (!m->fused_ok) return run_stages(m, istp);
     const Layout& L = m->L;
@@ -1819,6 +1931,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
     const bool defer = out && m->cs_part;
     m->part_dst = defer ? m->cs_part + slot * m->cs_slot : m->partials;
+    if ((rc = snap_guard(m, 1 - k))) return rc;
     switch (var) {
         case VAR_CELL: rc = launch_cell<false>(m, n, a, tide, nparts); break;
         case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
@@ -2042,6 +2155,8 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
     if (!z || !scale || zstride < 3) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_apply: bad argument");
     // the increments come from work on the default stream
     HIPCHK(hipStreamSynchronize(nullptr));
+    int grc = snap_guard(m, m->cur);
+    if (grc) return grc;
     const Layout& L = m->L;
     double** s = m->S[m->cur];
     const Scale3 sc{{scale[0], scale[1], scale[2]}};
@@ -2185,9 +2300,112 @@ int nemo2d_diagnostics(nemo2d_model* m, double out[3]) {
     return NEMO2D_OK;
 }
 
+int nemo2d_stage_timers(nemo2d_model* m, int on) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_stage_timers: null model");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->tcalls.clear();
+    m->tev_used = 0;
+    for (int k = 0; k < TR_N; ++k) {
+        m->topen[k] = -1;
+        m->tcount[k] = 0;
+        m->tsum[k] = m->tsq[k] = 0.0;
+    }
+    m->timing = on != 0;
+    return NEMO2D_OK;
+}
+
+int nemo2d_stage_times(nemo2d_model* m, int* counts, double* total_s, double* stderr_s) {
+    if (!m || !counts || !total_s || !stderr_s) return fail(NEMO2D_ERR_ARG, "nemo2d_stage_times: bad argument");
+    timers_fold(m);
+    for (int k = 0; k < TR_N; ++k) {
+        const long n = m->tcount[k];
+        counts[k] = (int)n;
+        total_s[k] = m->tsum[k] * 1e-3;
+        double se = 0.0;
+        if (n > 1) {
+            const double mean = m->tsum[k] / n;
+            const double var = (m->tsq[k] - n * mean * mean) / (n - 1);
+            se = var > 0.0 ? sqrt(var / n) * 1e-3 : 0.0;
+        }
+        stderr_s[k] = se;
+    }
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_output_grid(nemo2d_model* m, const double* xt, const double* yt, const double* ht) {
+    if (!m || !xt || !yt || !ht) return fail(NEMO2D_ERR_ARG, "nemo2d_set_output_grid: bad argument");
+    const size_t n = (size_t)m->L.jpi * m->L.jpj;
+    m->out_xt.assign(xt, xt + n);
+    m->out_yt.assign(yt, yt + n);
+    m->out_ht.assign(ht, ht + n);
+    return NEMO2D_OK;
+}
+
+int nemo2d_snapshot_begin(nemo2d_model* m, int istp, const char* dir) {
+    READY(m, "nemo2d_snapshot_begin");
+    if (m->mixed) return fail(NEMO2D_ERR_STATE, "nemo2d_snapshot_begin: FP64 model only");
+    if (m->out_xt.empty()) return fail(NEMO2D_ERR_STATE, "nemo2d_snapshot_begin: set_output_grid first");
+    if (istp < 0 || istp > 99999) return fail(NEMO2D_ERR_ARG, "nemo2d_snapshot_begin: istp out of range");
+    int rc = snap_join(m);  // the previous file first (its buffers are reused)
+    if (rc) return rc;
+    const size_t n = (size_t)m->L.size();
+    if (!m->copy_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&m->copy_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&m->ev_state, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&m->ev_copy, hipEventDisableTiming));
+        for (int k = 0; k < 3; ++k) HIPCHK(hipHostMalloc((void**)&m->h_snap[k], n * sizeof(double), 0));
+    }
+    // copies of the now-state, after the work queued so far
+    const int src = m->cur;
+    double* const* s = m->S[src];
+    HIPCHK(hipEventRecord(m->ev_state, m->stream));
+    HIPCHK(hipStreamWaitEvent(m->copy_stream, m->ev_state, 0));
+    HIPCHK(hipMemcpyAsync(m->h_snap[0], s[F_SSHN], n * sizeof(double), hipMemcpyDeviceToHost, m->copy_stream));
+    HIPCHK(hipMemcpyAsync(m->h_snap[1], s[F_UN], n * sizeof(double), hipMemcpyDeviceToHost, m->copy_stream));
+    HIPCHK(hipMemcpyAsync(m->h_snap[2], s[F_VN], n * sizeof(double), hipMemcpyDeviceToHost, m->copy_stream));
+    HIPCHK(hipEventRecord(m->ev_copy, m->copy_stream));
+    m->snap_src = src;
+    char name[32];
+    snprintf(name, sizeof name, "go2d_%05d.dat", istp);
+    std::string path = std::string(dir && *dir ? dir : ".") + "/" + name;
+    m->snap_thread = std::thread([m, path] {
+        if (hipEventSynchronize(m->ev_copy) != hipSuccess) {
+            m->snap_rc = NEMO2D_ERR_HIP;
+            m->snap_err = "nemo2d_snapshot: copy failed";
+            return;
+        }
+        const Layout& L = m->L;
+        const int jpi = L.jpi, jpj = L.jpj;
+        const Arr2 xt{m->out_xt.data(), -1L - jpi, jpi}, yt{m->out_yt.data(), -1L - jpi, jpi};
+        const Arr2 ht{m->out_ht.data(), -1L - jpi, jpi};
+        const Arr2 ss{m->h_snap[0], L.base, L.pitch}, un{m->h_snap[1], L.base, L.pitch},
+            vn{m->h_snap[2], L.base, L.pitch};
+        std::string err;
+        const int rc = write_go2d(path.c_str(), xt, yt, ht, ss, un, vn, jpi, jpj, err);
+        if (rc) {
+            m->snap_err = "nemo2d_snapshot: " + err;
+            m->snap_rc = rc;
+        }
+    });
+    return NEMO2D_OK;
+}
+
+int nemo2d_snapshot_wait(nemo2d_model* m) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_snapshot_wait: null model");
+    return snap_join(m);
+}
+
 int nemo2d_destroy(nemo2d_model* m) {
     if (!m) return NEMO2D_OK;
+    if (m->snap_thread.joinable()) m->snap_thread.join();
+    if (m->copy_stream) (void)hipStreamSynchronize(m->copy_stream);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (int k = 0; k < 3; ++k)
+        if (m->h_snap[k]) (void)hipHostFree(m->h_snap[k]);
+    for (hipEvent_t e : m->tev) (void)hipEventDestroy(e);
+    if (m->ev_state) (void)hipEventDestroy(m->ev_state);
+    if (m->ev_copy) (void)hipEventDestroy(m->ev_copy);
+    if (m->copy_stream) (void)hipStreamDestroy(m->copy_stream);
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f) dfree(m->S[k][f]);
     for (int s = 0; s < NS; ++s) dfree(m->st[s]);

@@ -33,4 +33,19 @@ int set_error(int code, const std::string& msg);
 int classify(const Layout& L, const int* pt, const double* hu, const double* hv, Classes& c,
              std::string& err);
 
+// A 2D host array read as p[off + j*stride + i] for point (i, j).
+struct Arr2 {
+    const double* p;
+    long off, stride;
+    double at(int i, int j) const { return p[off + (long)j * stride + i]; }
+};
+
+// go2d snapshot files (nemo2d_output.hip): bytes of a jpi x jpj file, rows
+// j0..j1-1 formatted into buf, and a whole file written with threads.
+long go2d_bytes(int jpi, int jpj);
+void format_go2d_rows(const Arr2& xt, const Arr2& yt, const Arr2& ht, const Arr2& ss, const Arr2& un,
+                      const Arr2& vn, int jpi, int j0, int j1, char* buf);
+int write_go2d(const char* path, const Arr2& xt, const Arr2& yt, const Arr2& ht, const Arr2& ss,
+               const Arr2& un, const Arr2& vn, int jpi, int jpj, std::string& err);
+
 }  // namespace nemo2d

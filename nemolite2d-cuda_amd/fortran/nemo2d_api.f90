@@ -15,7 +15,8 @@ module nemo2d_api
               nemo2d_step, nemo2d_run, nemo2d_record_checksums, nemo2d_checksum_log, &
               nemo2d_checksums, nemo2d_retrieve_results, nemo2d_retrieve_aux, &
               nemo2d_timer_start, nemo2d_timer_stop, nemo2d_destroy, &
-              nemo2d_check
+              nemo2d_set_output_grid, nemo2d_snapshot_begin, nemo2d_snapshot_wait, &
+              nemo2d_stage_timers, nemo2d_stage_times, nemo2d_check
 
     interface
         integer(c_int) function nemo2d_create(m, jpi, jpj, dx, dy, dep_const, nit000, nitend, &
@@ -135,6 +136,39 @@ module nemo2d_api
             real(c_double) :: ms
         end function
         integer(c_int) function nemo2d_destroy(m) bind(C, name="nemo2d_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+        end function
+
+        ! device timers of the Continuity/Momentum/BCs/Next regions and the
+        ! fused step (regions 1..5 of the arrays)
+        integer(c_int) function nemo2d_stage_timers(m, on) bind(C, name="nemo2d_stage_timers")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), value :: on
+        end function
+        integer(c_int) function nemo2d_stage_times(m, counts, total_s, stderr_s) &
+            bind(C, name="nemo2d_stage_times")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            integer(c_int) :: counts(*)
+            real(c_double) :: total_s(*), stderr_s(*)
+        end function
+
+        ! go2d snapshots written behind the steps (xt, yt, ht: T arrays)
+        integer(c_int) function nemo2d_set_output_grid(m, xt, yt, ht) &
+            bind(C, name="nemo2d_set_output_grid")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double) :: xt(*), yt(*), ht(*)
+        end function
+        integer(c_int) function nemo2d_snapshot_begin(m, istp, dir) bind(C, name="nemo2d_snapshot_begin")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: m
+            integer(c_int), value :: istp
+            character(kind=c_char) :: dir(*)
+        end function
+        integer(c_int) function nemo2d_snapshot_wait(m) bind(C, name="nemo2d_snapshot_wait")
             import :: c_int, c_ptr
             type(c_ptr), value :: m
         end function

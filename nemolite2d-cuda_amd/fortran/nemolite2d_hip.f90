@@ -8,7 +8,9 @@
 !! (continuity, momentum_u, momentum_v, bc_ssh, bc_solid_u, bc_solid_v,
 !! bc_flather_u, bc_flather_v, next) or, with fused=.true. in the optional
 !! &namhip group, the single fused step.  Per-step checksums sum|ssh|, sum|u|,
-!! sum|v| are written to the file named by csfile.
+!! sum|v| are written to the file named by csfile.  With async_output=.true.
+!! the snapshots are copied and written by the library behind the following
+!! steps (same files) instead of retrieve-then-write.
 program nemolite2d_hip
     use, intrinsic :: iso_c_binding, only: c_int, c_double, c_ptr, c_null_ptr
     use nemo2d_api
@@ -26,14 +28,16 @@ program nemolite2d_hip
 
     integer :: jpiglo, jpjglo, jpi, jpj, jphgr_msh, nit000, nitend, irecord
     real(wp) :: dx, dy, dep_const, rdt, cbfr, visc
-    logical :: fused, write_output
+    logical :: fused, write_output, async_output
     character(len=256) :: csfile
 
     type(c_ptr) :: model = c_null_ptr
     integer :: istp, nsteps, k
-    integer(c_int) :: nlogged
+    integer(c_int) :: nlogged, tcount(5)
     integer(8) :: c0, c1, crate
-    real(wp) :: t_loop, t_dev_ms, cells
+    real(wp) :: t_loop, t_dev_ms, cells, ttotal(5), terr(5)
+    character(len=32), parameter :: region(5) = [character(len=32) :: 'Continuity', 'Momentum', &
+                                                 'BCs', 'Next', 'Fused step']
 
     call setup
     call grid
@@ -47,10 +51,13 @@ program nemolite2d_hip
     nsteps = max(nitend - nit000 + 1, 0)
     call nemo2d_check(nemo2d_record_checksums(model, int(nsteps, c_int)), 'nemo2d_record_checksums')
 
+    if (write_output .and. async_output) &
+        call nemo2d_check(nemo2d_set_output_grid(model, xt, yt, ht), 'nemo2d_set_output_grid')
     istp = 0
-    if (write_output) call output
+    if (write_output) call snapshot
 
     call system_clock(c0, crate)
+    call nemo2d_check(nemo2d_stage_timers(model, 1_c_int), 'nemo2d_stage_timers')
     call nemo2d_check(nemo2d_timer_start(model), 'nemo2d_timer_start')
     do istp = nit000, nitend
         call step
@@ -60,6 +67,8 @@ program nemolite2d_hip
     t_loop = real(c1 - c0, wp)/real(crate, wp)
 
     call nemo2d_check(nemo2d_retrieve_results(model, sshn, un, vn, ua, va), 'nemo2d_retrieve_results')
+    if (write_output .and. async_output) &
+        call nemo2d_check(nemo2d_snapshot_wait(model), 'nemo2d_snapshot_wait')
 
     ! per-step checksums, then the serial model's two final lines
     allocate (cslog(3*max(nsteps, 1)))
@@ -76,12 +85,24 @@ program nemolite2d_hip
         write (*, "('va checksum = ',E16.8)") cslog(3*nsteps)
     end if
 
+    ! timing report in dl_timer's layout (shared/dl_timer/src/dl_timer.f90:713-773)
+    call nemo2d_check(nemo2d_stage_times(model, tcount, ttotal, terr), 'nemo2d_stage_times')
     cells = real(jpi, wp)*real(jpj, wp)*real(nsteps, wp)
     write (*, '(A)') '==================================== Timing report ===================================='
-    write (*, '(A)') 'Region                          Counts      Total       Average*   Avg/repeat*'
-    write (*, '(A,I10,3ES13.5)') 'Time-stepping                 ', 1, t_loop, t_loop, &
-        t_loop/real(max(nsteps, 1), wp)
-    write (*, '(A,F12.3,A,ES12.4,A)') 'device time (HIP events): ', t_dev_ms, ' ms;  ', &
+    write (*, '(A)') 'Timed using HIP events on the device. Units are seconds.'
+    write (*, "(88('-'))")
+    write (*, "('Region',26x,'Counts',6x,'Total',7x,'Average*   Avg/repeat*  Std Err')")
+    write (*, "(88('-'))")
+    ! labels left-justified in 32 columns, as dl_timer writes label(1:32) with (A)
+    write (*, '(A,1x,I6,1x,E12.5,1x,E12.5,1x,E12.5,1x,E9.2)') label32('Time-stepping'), 1, &
+        t_dev_ms*1.0e-3_wp, t_dev_ms*1.0e-3_wp, t_dev_ms*1.0e-3_wp/real(max(nsteps, 1), wp), 0.0_wp
+    do k = 1, 5
+        if (tcount(k) > 0) write (*, '(A,1x,I6,1x,E12.5,1x,E12.5,1x,E12.5,1x,E9.2)') region(k), &
+            tcount(k), ttotal(k), ttotal(k)/real(tcount(k), wp), ttotal(k)/real(tcount(k), wp), terr(k)
+    end do
+    write (*, "(88('-'))")
+    write (*, '(A)') '* per call, device event times (no systematic-error correction)'
+    write (*, '(A,F12.3,A,ES12.4,A)') 'host wall time of the loop: ', t_loop, ' s;  device rate ', &
         cells/(t_dev_ms*1.0e-3_wp), ' cell-updates/s'
     write (*, '(A)') '========================================================================================'
 
@@ -90,19 +111,27 @@ program nemolite2d_hip
 
 contains
 
+    !> A label in 32 columns, blank-padded on the right.
+    pure function label32(s) result(l)
+        character(len=*), intent(in) :: s
+        character(len=32) :: l
+        l = s
+    end function label32
+
     !> &namctl as original/nemolite2d.f90:96-141 (same defaults, same -2
     !! correction), plus the optional &namhip group of this driver.
     subroutine setup
         integer :: ios
         namelist /namctl/ jpiglo, jpjglo, jphgr_msh, dx, dy, dep_const, &
             nit000, nitend, irecord, rdt, cbfr, visc
-        namelist /namhip/ fused, write_output, csfile
+        namelist /namhip/ fused, write_output, async_output, csfile
 
         jpiglo = 258; jpjglo = 258; jphgr_msh = 1
         dx = 1000._wp; dy = 1000._wp; dep_const = 100._wp
         nit000 = 1; nitend = 1000; irecord = 1
         rdt = 10._wp; cbfr = 0.001_wp; visc = 100._wp
-        fused = .false.; write_output = .true.; csfile = 'nemo2d_checksums.txt'
+        fused = .false.; write_output = .true.; async_output = .false.
+        csfile = 'nemo2d_checksums.txt'
 
         open (1, file='namelist', status='OLD')
         rewind (1)
@@ -199,12 +228,21 @@ contains
             call nemo2d_check(nemo2d_bc_flather_v(model), 'nemo2d_bc_flather_v')
             call nemo2d_check(nemo2d_next(model), 'nemo2d_next')
         end if
-        if (write_output .and. mod(istp, irecord) == 0) then
+        if (write_output .and. mod(istp, irecord) == 0) call snapshot
+    end subroutine step
+
+    !> Snapshot of the current state: retrieved and written here, or (async)
+    !! queued in the library and written behind the following steps.
+    subroutine snapshot
+        if (async_output) then
+            call nemo2d_check(nemo2d_snapshot_begin(model, int(istp, c_int), '.'//achar(0)), &
+                              'nemo2d_snapshot_begin')
+        else
             call nemo2d_check(nemo2d_retrieve_results(model, sshn, un, vn, ua, va), &
                               'nemo2d_retrieve_results')
             call output
         end if
-    end subroutine step
+    end subroutine snapshot
 
     !> go2d_NNNNN.dat snapshot in the serial model's format
     !! (original/nemolite2d.f90:773-799).

@@ -37,7 +37,8 @@ SYMBOLS = (
     "nemo2d_nn_mse", "nemo2d_nn_adam", "nemo2d_nn_pack_weights", "nemo2d_nn_to_bf16",
     "nemo2d_dev_alloc", "nemo2d_dev_free", "nemo2d_dev_upload", "nemo2d_dev_download",
     "nemo2d_dev_zero", "nemo2d_closure_features", "nemo2d_closure_apply",
-    "nemo2d_stream_bandwidth",
+    "nemo2d_stream_bandwidth", "nemo2d_format_go2d", "nemo2d_set_output_grid",
+    "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
 )
 
 # nemo2d_set_variant choices
@@ -90,6 +91,13 @@ def lib():
         L.nemo2d_ensemble_retrieve.argtypes = [vp, i, dp, dp, dp]
         L.nemo2d_ensemble_destroy.argtypes = [vp]
         L.nemo2d_stream_bandwidth.argtypes = [ctypes.c_long, i, dp]
+        L.nemo2d_set_output_grid.argtypes = [vp, dp, dp, dp]
+        L.nemo2d_stage_timers.argtypes = [vp, i]
+        L.nemo2d_stage_times.argtypes = [vp, ip, dp, dp]
+        L.nemo2d_snapshot_begin.argtypes = [vp, i, ctypes.c_char_p]
+        L.nemo2d_snapshot_wait.argtypes = [vp]
+        L.nemo2d_format_go2d.argtypes = [i, i, dp, dp, dp, dp, dp, dp, ctypes.c_char_p, ctypes.c_long,
+                                         ctypes.POINTER(ctypes.c_long)]
         L.nemo2d_closure_features.argtypes = [vp, dp, dp, vp]
         L.nemo2d_closure_apply.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, dp]
         L.nemo2d_dev_alloc.argtypes = [ctypes.c_long, ctypes.POINTER(vp)]
@@ -319,6 +327,30 @@ class Model:
         _check(lib().nemo2d_diagnostics(self._h, _dp(out)))
         return out
 
+    REGIONS = ("Continuity", "Momentum", "BCs", "Next", "Fused step")
+
+    def stage_timers(self, on=True):
+        _check(lib().nemo2d_stage_timers(self._h, 1 if on else 0))
+
+    def stage_times(self):
+        """{region: (count, total seconds, standard error of the mean)}."""
+        c = (ctypes.c_int * 5)()
+        t, e = np.zeros(5), np.zeros(5)
+        _check(lib().nemo2d_stage_times(self._h, c, _dp(t), _dp(e)))
+        return {r: (int(c[k]), float(t[k]), float(e[k])) for k, r in enumerate(self.REGIONS)}
+
+    def set_output_grid(self, xt, yt, ht):
+        """T-point coordinates and depth for snapshot files."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xt, yt, ht)]
+        _check(lib().nemo2d_set_output_grid(self._h, *[_dp(a) for a in arrs]))
+
+    def snapshot_begin(self, istp, directory="."):
+        """Queue go2d_<istp>.dat of the current state (written behind later steps)."""
+        _check(lib().nemo2d_snapshot_begin(self._h, int(istp), str(directory).encode()))
+
+    def snapshot_wait(self):
+        _check(lib().nemo2d_snapshot_wait(self._h))
+
     def closure_features(self, mean, scale, x):
         """Closure input features of the now-state into the device array x
         (BF16 bits [jpj*jpi, 8]); mean, scale: 4 values each."""
@@ -339,6 +371,19 @@ class Model:
         ms = ctypes.c_double(0.0)
         _check(lib().nemo2d_timer_stop(self._h, ctypes.byref(ms)))
         return ms.value
+
+
+def format_go2d(xt, yt, ht, sshn, un, vn):
+    """The text of a go2d snapshot file (original/nemolite2d.f90:773-799) for
+    host arrays in the reference's layout: xt, yt, ht, sshn (jpj, jpi), un
+    (jpj, jpi+1), vn (jpj+1, jpi).  Host code only."""
+    jpj, jpi = np.shape(sshn)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xt, yt, ht, sshn, un, vn)]
+    n = ctypes.c_long(0)
+    _check(lib().nemo2d_format_go2d(jpi, jpj, *[_dp(a) for a in arrs], None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    _check(lib().nemo2d_format_go2d(jpi, jpj, *[_dp(a) for a in arrs], buf, n.value, ctypes.byref(n)))
+    return buf.raw[:n.value]
 
 
 def stream_bandwidth(n, repeats=20):

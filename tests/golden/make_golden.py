@@ -15,6 +15,9 @@ records only numbers:
 * ``ref_fields_<case>.npz`` -- a decimated sample of the ``go2d_NNNNN.dat``
   snapshots (x, y, depth, ssh, u at T, v at T written ``(6e20.7e4)`` by
   original/nemolite2d.f90:773-799) for point-wise checks.
+* ``ref_go2d_10x10_00012.dat`` -- one such snapshot file byte for byte (a
+  10x10 basin after 12 steps), the expected output of the library's
+  snapshot writer on the same fields.
 
 Usage (from the repository root)::
 
@@ -169,6 +172,15 @@ def main():
             known[case_name(r["namelist"])] = r
             save()
     save()
+
+    dat = os.path.join(HERE, "ref_go2d_10x10_00012.dat")
+    if not os.path.exists(dat):
+        c = base(jpiglo=12, jpjglo=12, nitend=12, irecord=12)
+        _, d, dats = run_case(c, keep_dat=True)
+        shutil.copyfile(os.path.join(d, "go2d_00012.dat"), dat)
+        for x in dats:
+            os.remove(os.path.join(d, x))
+        print("wrote", dat, flush=True)
 
     for tag, c in field_cases():
         npz = os.path.join(HERE, "ref_fields_%s.npz" % tag)

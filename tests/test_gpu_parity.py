@@ -321,18 +321,20 @@ NAMELIST = """&namctl
 &namhip
    fused = {fused}
    write_output = .true.
+   async_output = {asy}
    csfile = 'cs.txt'
 /
 """
 
 
-@pytest.mark.parametrize("fused", [".true.", ".false."])
-def test_fortran_driver(tmp_path, fused):
+@pytest.mark.parametrize("fused,asy", [(".true.", ".false."), (".false.", ".false."),
+                                      (".true.", ".true."), (".false.", ".true.")])
+def test_fortran_driver(tmp_path, fused, asy):
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "nemolite2d-cuda_amd", "bin", "nemolite2d_hip")
     g, n = 66, 300
-    (tmp_path / "namelist").write_text(NAMELIST.format(g=g, n=n, irec=150, fused=fused))
+    (tmp_path / "namelist").write_text(NAMELIST.format(g=g, n=n, irec=150, fused=fused, asy=asy))
     p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     o = Oracle(g - 2, g - 2)
@@ -340,6 +342,14 @@ def test_fortran_driver(tmp_path, fused):
     lines = [l for l in p.stdout.splitlines() if "checksum =" in l]
     assert lines[0].split("=")[1].strip() == e16_8(ref[-1, 1])
     assert lines[1].split("=")[1].strip() == e16_8(ref[-1, 2])
+    # timing rows as the reference's dl_timer report lays them out (what its
+    # benchmark extractor splits): label, count, total, average, avg/repeat, err
+    rows = {l.split()[0]: l.split() for l in p.stdout.splitlines()
+            if l.startswith(("Time-stepping", "Continuity", "Momentum", "BCs", "Next", "Fused"))}
+    assert len(rows["Time-stepping"]) == 6 and float(rows["Time-stepping"][2]) > 0
+    names = ("Fused",) if fused == ".true." else ("Continuity", "Momentum", "BCs", "Next")
+    for r in names:
+        assert int(rows[r][-5 if r == "Fused" else 1]) == n, rows[r]
     cs = np.loadtxt(tmp_path / "cs.txt")
     assert cs.shape == (n, 4)
     assert np.all(rel_close(cs[:, 1:], ref, 1e-12))
@@ -349,6 +359,33 @@ def test_fortran_driver(tmp_path, fused):
         assert d.shape == ((g - 2) ** 2, 6)
     d = np.loadtxt(tmp_path / "go2d_00300.dat")
     assert np.allclose(d[:, 3], o.arr("sshn").ravel(), rtol=1e-6, atol=1e-12)
+    # the text itself: the oracle's state through the library's formatter
+    # (pinned byte for byte to a reference file on the CPU)
+    txt = nemo2d.format_go2d(o.arr("xt"), o.arr("yt"), o.arr("ht"), o.arr("sshn"), o.arr("un"),
+                             o.arr("vn"))
+    assert (tmp_path / "go2d_00300.dat").read_bytes() == txt
+
+
+def test_snapshots_behind_steps_equal_synchronous_state(tmp_path):
+    """snapshot_begin queues the copy and the file; the steps queued after
+    it do not disturb it (the buffer it copies is next written two steps
+    later, which waits for the copy)."""
+    o = Oracle(150, 90)
+    with model_like(o) as m:
+        m.set_output_grid(o.arr("xt"), o.arr("yt"), o.arr("ht"))
+        for istp in range(1, 13):
+            m.step(istp)
+            o.step(istp)
+            if istp in (3, 4, 9):
+                m.snapshot_begin(istp, str(tmp_path))
+                want = nemo2d.format_go2d(o.arr("xt"), o.arr("yt"), o.arr("ht"), o.arr("sshn"),
+                                          o.arr("un"), o.arr("vn"))
+                (tmp_path / ("want_%05d.dat" % istp)).write_bytes(want)
+        m.snapshot_wait()
+        for istp in (3, 4, 9):
+            assert (tmp_path / ("go2d_%05d.dat" % istp)).read_bytes() == \
+                (tmp_path / ("want_%05d.dat" % istp)).read_bytes(), istp
+        assert_fields_equal(m, o, "after snapshots")
 
 
 @pytest.mark.parametrize("variant", ["auto", "march12_shd", "cell"])
