@@ -897,28 +897,41 @@ __global__ void k_bc_flather_v(const int* __restrict__ ptr, const long* __restri
 }
 
 // next, original/nemolite2d.f90:704-769: un=ua, vn=va, sshn=ssha, then the
-// interpolations from the new ssh (read from ssha, which equals it).
-template <class S>
-__global__ void k_next(S st, const uint16_t* __restrict__ fl, const double* __restrict__ ssha,
-                       const double* __restrict__ ua, const double* __restrict__ va,
-                       double* __restrict__ sshn, double* __restrict__ sshn_u,
-                       double* __restrict__ sshn_v, double* __restrict__ un,
-                       double* __restrict__ vn, Layout L) {
+// interpolations from the new ssh (read from ssha, which equals it).  With
+// CS the block also sums |sshn|, |un|, |vn| of the new state over
+// (1:jpi, 1:jpj) into its checksum partial.
+template <class S, bool CS>
+__global__ void __launch_bounds__(TBX * TBY)
+k_next(S st, const uint16_t* __restrict__ fl, const double* __restrict__ ssha,
+       const double* __restrict__ ua, const double* __restrict__ va, double* __restrict__ sshn,
+       double* __restrict__ sshn_u, double* __restrict__ sshn_v, double* __restrict__ un,
+       double* __restrict__ vn, Layout L, double* __restrict__ partials) {
     const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
-    if (i > L.jpi || j > L.jpj) return;
-    const long o = L.at(i, j), P = L.pitch;
-    const unsigned f = fl[o];
-    if (i >= 1 && j >= 1) sshn[o] = ssha[o];
-    if (j >= 1) {
-        un[o] = ua[o];
-        const unsigned c = sucls(f);
-        if (c != NX_SKIP) sshn_u[o] = next_su(st, c, o, ssha[o], ssha[o + 1], 0.0);
+    DD ct = {0.0, 0.0}, cu = {0.0, 0.0}, cv = {0.0, 0.0};
+    if (i <= L.jpi && j <= L.jpj) {
+        const long o = L.at(i, j), P = L.pitch;
+        const unsigned f = fl[o];
+        const double s = ssha[o];
+        if (i >= 1 && j >= 1) {
+            sshn[o] = s;
+            if (CS) dd_add(ct, fabs(s));
+        }
+        if (j >= 1) {
+            const double u = ua[o];
+            un[o] = u;
+            if (CS && i >= 1) dd_add(cu, fabs(u));
+            const unsigned c = sucls(f);
+            if (c != NX_SKIP) sshn_u[o] = next_su(st, c, o, s, ssha[o + 1], 0.0);
+        }
+        if (i >= 1) {
+            const double v = va[o];
+            vn[o] = v;
+            if (CS && j >= 1) dd_add(cv, fabs(v));
+            const unsigned c = svcls(f);
+            if (c != NX_SKIP) sshn_v[o] = next_sv(st, c, o, P, s, ssha[o + P], 0.0);
+        }
     }
-    if (i >= 1) {
-        vn[o] = va[o];
-        const unsigned c = svcls(f);
-        if (c != NX_SKIP) sshn_v[o] = next_sv(st, c, o, P, ssha[o], ssha[o + P], 0.0);
-    }
+    if (CS) block_reduce3(ct, cu, cv, partials + 6 * (long)(blockIdx.x + gridDim.x * blockIdx.y));
 }
 
 // ---- learned closure on the device state ------------------------------------------
@@ -1695,15 +1708,33 @@ int nemo2d_next(nemo2d_model* m) {
     if (grc) return grc;
     const Layout& L = m->L;
     double** s = m->S[m->cur];
-    WITH_STATICS(m, st, (k_next<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, m->flags, m->ssha, m->ua, m->va, s[F_SSHN], s[F_SSHN_U],
-                            s[F_SSHN_V], s[F_UN], s[F_VN], L)));
+    const dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    // with recording on, the checksums come from the same pass: block
+    // partials in the step's slot (reduced when the log is read) or in the
+    // scratch partials (reduced at once)
+    const bool rec = m->cs_cap > 0 && m->cs_next < m->cs_cap;
+    const int slot = m->cs_next;
+    double* part = rec && m->cs_part ? m->cs_part + slot * m->cs_slot : m->partials;
+    if (rec) {
+        WITH_STATICS(m, st, (k_next<decltype(st), true><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                                st, m->flags, m->ssha, m->ua, m->va, s[F_SSHN], s[F_SSHN_U],
+                                s[F_SSHN_V], s[F_UN], s[F_VN], L, part)));
+    } else {
+        WITH_STATICS(m, st, (k_next<decltype(st), false><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                                st, m->flags, m->ssha, m->ua, m->va, s[F_SSHN], s[F_SSHN_U],
+                                s[F_SSHN_V], s[F_UN], s[F_VN], L, nullptr)));
+    }
     LAUNCHCHK("next");
     m->a_current = true;
-    if (m->cs_cap > 0 && m->cs_next < m->cs_cap) {
-        if (!m->cs_pend.empty()) m->cs_pend[m->cs_next] = 0;  // this slot is final now
-        int rc = checksum_into(m, m->cur, m->cs_log + 3 * (long)m->cs_next);
-        if (rc) return rc;
+    if (rec) {
+        const int nparts = (int)(g.x * g.y);
+        if (m->cs_part) {
+            m->cs_pend[slot] = nparts;
+            m->pend_any = true;
+        } else {
+            k_checksum_finish<<<1, RED_THREADS, 0, m->stream>>>(part, nparts, m->cs_log + 3 * (long)slot);
+            LAUNCHCHK("checksum_finish");
+        }
         ++m->cs_next;
         if (m->cs_next > m->cs_n) m->cs_n = m->cs_next;
     }
