@@ -169,22 +169,27 @@ def form_closure(nc, runs, snaps, iters, windows):
 
 
 # the fused-step kernel each variant runs on the uniform benchmark grid
-KERNELS = {"auto": "k_step_march_sh<12, false>", "march12_shn": "k_step_march_sh<12, false>",
-           "march16_shn": "k_step_march_sh<16, false>", "march24_shn": "k_step_march_sh<24, false>",
-           "march32_shn": "k_step_march_sh<32, false>", "march16_pf": "k_step_march<16, PF>",
-           "cell": "k_step_fused"}
+DEFAULT_VARIANT = "march12_rc"
+KERNELS = {"march12_rc": "k_step_march_rc<12, false, true>",
+           "march12_rcf": "k_step_march_rc<12, true, true>",
+           "march12_rcfn": "k_step_march_rc<12, true, false>",
+           "march12_dpfb": "k_step_march_dp<12, true, true>",
+           "march12_dpb": "k_step_march_dp<12, false, true>",
+           "march12_shn": "k_step_march_sh<12, false>", "march16_shn": "k_step_march_sh<16, false>",
+           "march24_shn": "k_step_march_sh<24, false>", "march32_shn": "k_step_march_sh<32, false>",
+           "march16_pf": "k_step_march<16, PF>", "cell": "k_step_fused"}
 
 
 def load_traffic(variant, n):
-    """Measured HBM bytes per launch (profiles/traffic_step_fused_4096.json)
-    when it was measured for this kernel and size, else None."""
-    p = os.path.join(ROOT, "profiles", "traffic_step_fused_4096.json")
-    if n != 4096 or not os.path.exists(p):
+    """Measured HBM bytes per launch of this variant's kernel at this size
+    (profiles/traffic_step_<variant>_<n>.json), else None."""
+    v = DEFAULT_VARIANT if variant == "auto" else variant
+    p = os.path.join(ROOT, "profiles", "traffic_step_%s_%d.json" % (v, n))
+    if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
-    same = variant == d.get("variant") or (variant == "auto" and d.get("variant") == "march12_shn")
-    return d.get("bytes_per_launch") if same else None
+    return d.get("bytes_per_launch") if d.get("variant") == v else None
 
 
 def main():
@@ -295,7 +300,8 @@ def main():
         "checksum_last_step": [float(x) for x in log[-1]] if len(log) else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": KERNELS.get(args.variant, "fused step, variant " + args.variant),
+                     "kernel": KERNELS.get(DEFAULT_VARIANT if args.variant == "auto" else args.variant,
+                                           "fused step, variant " + args.variant),
                      "kernel_ms": ms_kernel,
                      "bytes_per_launch": BYTES_PER_CELL * cells},
     }

@@ -97,6 +97,11 @@ struct nemo2d_model {
     // checksum reduction
     double* partials = nullptr;  // 6 doubles per block
     int npart_max = 0;
+    // sshn_u/sshn_v of state set k not written by the last step (they are
+    // the interpolations of its sshn; uv_sync fills them in), and whether the
+    // current state's sshn_u/sshn_v are known to be those interpolations
+    bool uv_stale[2] = {false, false};
+    bool uv_derived = false;
     double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
     double* cs_log = nullptr;    // 3 doubles per logged step
     // Deferred checksum finish: the fused step leaves each logged step's
@@ -318,7 +323,7 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
             }
             double s_c = 0.0;
             if (j >= 1 && tcol) s_c = (f & T_CLAMP) ? tide : continuity_w(w, p.rdt);
-            const double s_e = __shfl_down(s_c, 1, 64);
+            const double s_e = lane_next(s_c);
             if (own) {
                 if (j >= 1) {
                     if (i >= 1) {
@@ -520,26 +525,48 @@ struct NorthTerms {
     double dvdyn;  // momentum_v: dvdy_n
     double qdudy;  // momentum_u: (un_n - un_c) / (e2u + e2u), dudy_n's quotient
 };
-__device__ __forceinline__ NorthTerms north_terms(const Scal& s, const Row5& c, const Row5& nr) {
+// Own-column row values of the five now-state arrays.
+struct Col5 {
+    double un, vn, ss, su, sv;
+};
+// c: this row (own column), c_vn_e / c_sv_e: its vn and sshn_v one column
+// east, nr: the row above (own column)
+// x / d for the uniform divisor d of tag T (QD_*): the IEEE division, or
+// with FD the exact division by the precomputed reciprocal (same bits)
+template <bool FD, int T>
+__device__ __forceinline__ double udiv(const Scal& s, double x, double d) {
+    if constexpr (FD) return div_rcp(x, d, s.rcp[T]);
+    else return x / d;
+}
+
+template <bool FD = false>
+__device__ __forceinline__ NorthTerms north_terms1(const Scal& s, const Col5& c, double c_vn_e,
+                                                   double c_sv_e, const Col5& nr) {
     const double h = 0.5;
     NorthTerms t;
-    t.r3 = (c.sv[1] + s.hv) * c.vn[1];
+    t.r3 = (c.sv + s.hv) * c.vn;
     // momentum_u (original/nemolite2d.f90:439-514) at this row's U point
-    t.vnc = h * (c.vn[1] + c.vn[2]);
+    t.vnc = h * (c.vn + c_vn_e);
     t.vnu = h * t.vnc * (s.e1v + s.e1v);
-    t.depnu = h * (s.hv + c.sv[1] + s.hv + c.sv[2]);
+    t.depnu = h * (s.hv + c.sv + s.hv + c_sv_e);
     const double sn = sgn_half(t.vnu);
-    const double uun = (h + sn) * c.un[1] + (h - sn) * nr.un[1];
+    const double uun = (h + sn) * c.un + (h - sn) * nr.un;
     t.kn = uun * t.vnu * t.depnu;
     // momentum_v (:527-606) at this row's V point
-    const double vn_n = h * (c.vn[1] + nr.vn[1]) * s.e1t;
-    const double depn = s.ht + nr.ss[1];
+    const double vn_n = h * (c.vn + nr.vn) * s.e1t;
+    const double depn = s.ht + nr.ss;
     const double sv = sgn_half(vn_n);
-    const double vvn = (h + sv) * c.vn[1] + (h - sv) * nr.vn[1];
+    const double vvn = (h + sv) * c.vn + (h - sv) * nr.vn;
     t.hn = vvn * vn_n * depn;
-    t.dvdyn = (nr.vn[1] - c.vn[1]) / s.e2t * (s.ht + nr.ss[1]);
-    t.qdudy = (nr.un[1] - c.un[1]) / (s.e2u + s.e2u);
+    t.dvdyn = udiv<FD, QD_E2T>(s, nr.vn - c.vn, s.e2t) * (s.ht + nr.ss);
+    t.qdudy = udiv<FD, QD_E2U2>(s, nr.un - c.un, s.e2u + s.e2u);
     return t;
+}
+__device__ __forceinline__ Col5 col5(const Row5& r, int c) {
+    return Col5{r.un[c], r.vn[c], r.ss[c], r.su[c], r.sv[c]};
+}
+__device__ __forceinline__ NorthTerms north_terms(const Scal& s, const Row5& c, const Row5& nr) {
+    return north_terms1(s, col5(c, 1), c.vn[2], c.sv[2], col5(nr, 1));
 }
 
 struct ShCarry {
@@ -600,7 +627,7 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
             s_c = B.ss[1] + (r2 - r1 + st.r3 - nt.r3) * x.p.rdt / s.e12t;
         }
     }
-    const double s_e = __shfl_down(s_c, 1, 64);
+    const double s_e = lane_next(s_c);
 
     // west-face terms of this lane's points; the east ones come from lane+1
     const double un_c = B.un[1];
@@ -617,11 +644,11 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
     const double gw = ((h - sw_v) * vn_c + (h + sw_v) * B.vn[0]) * u_wv * depw_v;
     // dvdx_w's quotient; lane+1's is dvdx_e's
     const double qdvdx_w = (vn_c - B.vn[0]) / (s.e1v + s.e1v);
-    const double fe = __shfl_down(fw, 1, 64);
-    const double dudx_e = __shfl_down(dudx_w, 1, 64);
-    const double ge = __shfl_down(gw, 1, 64);
-    const double u_ec = __shfl_down(u_wc, 1, 64);
-    const double qdvdx_e = __shfl_down(qdvdx_w, 1, 64);
+    const double fe = lane_next(fw);
+    const double dudx_e = lane_next(dudx_w);
+    const double ge = lane_next(gw);
+    const double u_ec = lane_next(u_wc);
+    const double qdvdx_e = lane_next(qdvdx_w);
 
     const double su_old = B.su[1], sv_old = B.sv[1];
     if (x.own) {
@@ -765,6 +792,509 @@ k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 s_n = (fl[on] & T_CLAMP) ? tide : continuity_at(st, n, on, P, p.rdt);
             }
             a.sshn_v[o] = next_sv(st, c, o, P, k.s_prev, s_n, k.sv_prev);
+        }
+    }
+    block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
+}
+
+// ---- fused step, own-column marching (DPP neighbours) -----------------------------
+// The shared-face-term algorithm above with each lane loading only its own
+// column of the five state arrays per row: one 512-byte coalesced load per
+// array and wave.  The west and east neighbours come from lanes l-1 and l+1
+// by DPP wavefront shifts, so lanes 1..62 own the output columns and lanes 0
+// and 63 are the west and east halos.  A row is 10 VGPRs instead of 30, so
+// the ring of four rows keeps the loads of row j+2 in flight while row j is
+// computed, within the register budget of four waves per SIMD.
+constexpr int DOUT = 62;  // output columns per wave
+
+// A row's own-column state and class word, loaded together: a wait for
+// any of them (vector loads complete in issue order) then never waits for
+// loads issued later for rows further ahead.
+struct Col5F : Col5 {
+    unsigned f;
+};
+__device__ __forceinline__ void load_col5(Col5F& r, const NState& n, const uint16_t* fl,
+                                          unsigned boff) {
+    r.un = ldb(n.un, boff);
+    r.vn = ldb(n.vn, boff);
+    r.ss = ldb(n.sshn, boff);
+    r.su = ldb(n.sshn_u, boff);
+    r.sv = ldb(n.sshn_v, boff);
+    r.f = ldb(fl, boff >> 2);
+}
+
+struct DpCarry {
+    NorthTerms south;         // the row below's north terms
+    double s_prev, sv_prev;   // new ssh and old sshn_v at (i, j-1)
+    double un_w, su_w;        // un and sshn_u one column west, this row
+    unsigned f_prev;
+    double ct, cu, cv;        // per-lane checksum sums
+};
+
+// next's interpolation (next_su / next_sv with uniform statics; E12 = QD_E12U
+// or QD_E12V)
+template <bool FD, int E12>
+__device__ __forceinline__ double dp_next(const Scal& s, unsigned cls, double s_lo, double s_hi,
+                                          double keep) {
+    switch (cls) {
+        case NX_AVG: {
+            const double r = s.e12t * s_lo + s.e12t * s_hi;
+            return udiv<FD, E12>(s, 0.5 * r, E12 == QD_E12U ? s.e12u : s.e12v);
+        }
+        case NX_HI: return s_hi;
+        case NX_LO: return s_lo;
+        default: return keep;
+    }
+}
+
+// One row j: A, B, C hold rows j-1, j, j+1 (own column); D receives row j+2.
+template <bool FD, bool BAR>
+__device__ __forceinline__ void dp_row(const ShCtx& x, DpCarry& k, const Col5F& A, const Col5F& B,
+                                       const Col5F& C, Col5F& D, int j) {
+    if constexpr (BAR) __syncthreads();  // the block's waves start the row together
+    const Layout& L = x.L;
+    const Scal& s = x.s;
+    const long P = L.pitch;
+    const double h = 0.5;
+    const unsigned f = B.f;
+    {
+        const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
+        load_col5(D, x.n, x.fl, x.rowb(jn) + x.lb8);
+    }
+    const unsigned o8 = x.rowb(j) + x.lb8;  // byte offset of (i, j)
+    // neighbours by lane shifts (every lane active here)
+    const double bun_w = k.un_w, bsu_w = k.su_w;
+    const double bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
+    const double bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv), bss_e = lane_next(B.ss);
+    const double cun_w = lane_prev(C.un), csu_w = lane_prev(C.su);
+    const NorthTerms nt = north_terms1<FD>(s, B, bvn_e, bsv_e, C);
+    const NorthTerms& st = k.south;
+
+    // continuity (original/nemolite2d.f90:399-403)
+    double s_c = 0.0;
+    if (j >= 1 && x.tcol) {
+        if (f & T_CLAMP) {
+            s_c = x.tide;
+        } else {
+            const double r1 = (B.su + s.hu) * B.un;
+            const double r2 = (bsu_w + s.hu) * bun_w;
+            s_c = B.ss + udiv<FD, QD_E12T>(s, (r2 - r1 + st.r3 - nt.r3) * x.p.rdt, s.e12t);
+        }
+    }
+    const double s_e = lane_next(s_c);
+
+    // west-face terms of this lane's points; the east ones come from lane+1
+    const double un_c = B.un;
+    const double u_w = h * (un_c + bun_w) * s.e2t;
+    const double depw_u = s.ht + B.ss;
+    const double sw_u = sgn_half(u_w);
+    const double fw = ((h - sw_u) * un_c + (h + sw_u) * bun_w) * u_w * depw_u;
+    const double dudx_w = udiv<FD, QD_E1T>(s, un_c - bun_w, s.e1t) * (s.ht + B.ss);
+    const double vn_c = B.vn;
+    const double u_wc = h * (bun_w + cun_w);
+    const double u_wv = h * u_wc * (s.e2u + s.e2u);
+    const double depw_v = h * (s.hu + bsu_w + s.hu + csu_w);
+    const double sw_v = sgn_half(u_wv);
+    const double gw = ((h - sw_v) * vn_c + (h + sw_v) * bvn_w) * u_wv * depw_v;
+    // dvdx_w's quotient; lane+1's is dvdx_e's
+    const double qdvdx_w = udiv<FD, QD_E1V2>(s, vn_c - bvn_w, s.e1v + s.e1v);
+    const double fe = lane_next(fw);
+    const double dudx_e = lane_next(dudx_w);
+    const double ge = lane_next(gw);
+    const double u_ec = lane_next(u_wc);
+    const double qdvdx_e = lane_next(qdvdx_w);
+
+    const double su_old = B.su, sv_old = B.sv;
+    if (x.own) {
+        const int i = x.i;
+        double u = 0.0, v = 0.0;
+        const unsigned cu = ucls(f), cv = vcls(f);
+        if (j >= 1 && cu == CLS_COMP) {
+            const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
+            double ts = st.kn, tn = nt.kn;
+            if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
+            if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
+            const double adv = fw - fe + ts - tn;
+            double dudy_s = 0.0, dudy_n = 0.0;
+            if (!ws) dudy_s = st.qdudy * (s.hu + B.su + s.hu + A.su);
+            if (!wn) dudy_n = nt.qdudy * (s.hu + B.su + s.hu + C.su);
+            double vis = (dudx_e - dudx_w) * s.e2u + (dudy_n - dudy_s) * s.e1u * h;
+            vis = x.p.visc * vis;
+            const double hsu = s.hu + B.su;
+            const double cor = h * (s.fcu * (st.vnc + nt.vnc)) * s.e12u * hsu;
+            const double hpg = -GRAV * hsu * s.e2u * (bss_e - B.ss);
+            const double q = udiv<FD, QD_E12U>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12u);
+            u = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HU>(s, un_c * hsu + q, s.hu + 0.0), x.p.fric);
+        }
+        if (i >= 1 && cv == CLS_COMP) {
+            const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
+            double tw = gw, te = ge;
+            if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
+            if (we) {
+                const double u_e = h * u_ec * (s.e2u + s.e2u);
+                const double depe = h * (s.hu + B.su + s.hu + C.su);
+                te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
+            }
+            const double adv = tw - te + st.hn - nt.hn;
+            double dvdx_w = 0.0, dvdx_e = 0.0;
+            if (!ww) dvdx_w = qdvdx_w * (s.hv + B.sv + s.hv + bsv_w);
+            if (!we) dvdx_e = qdvdx_e * (s.hv + B.sv + s.hv + bsv_e);
+            double vis = (nt.dvdyn - st.dvdyn) * s.e1v + (dvdx_e - dvdx_w) * s.e2v * h;
+            vis = x.p.visc * vis;
+            const double hsv = s.hv + B.sv;
+            const double cor = -h * (s.fcv * (u_ec + u_wc)) * s.e12v * hsv;
+            const double hpg = -GRAV * hsv * s.e1v * (C.ss - B.ss);
+            const double q = udiv<FD, QD_E12V>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12v);
+            v = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HV>(s, vn_c * hsv + q, s.hv + 0.0), x.p.fric);
+        }
+        if (j >= 1) {
+            if (i >= 1) {
+                stb(x.a.sshn, o8, s_c);
+                k.ct += fabs(s_c);
+            }
+            if (cu != CLS_FLATHER) {
+                stb(x.a.un, o8, u);
+                if (i >= 1) k.cu += fabs(u);
+            }
+            stb(x.a.sshn_u, o8, dp_next<FD, QD_E12U>(s, sucls(f), s_c, s_e, su_old));
+        }
+        if (i >= 1) {
+            if (cv != CLS_FLATHER) {
+                stb(x.a.vn, o8, v);
+                if (j >= 1) k.cv += fabs(v);
+            }
+            // bc_flather_v (original/nemolite2d.f90:676-690) at a row-0 point
+            // whose inner point is this one; row 0 is outside the checksums
+            if (j == 1 && vcls(k.f_prev) == CLS_FLATHER && (k.f_prev & V_FL_NORTH)) {
+                const unsigned o08 = o8 - (unsigned)(8 * P);
+                stb(x.a.vn, o08, v + ldb(x.flv, o08) * (A.sv - B.sv));
+            }
+            if (k.f_prev != 0xffffffffu)
+                stb(x.a.sshn_v, o8 - (unsigned)(8 * P),
+                    dp_next<FD, QD_E12V>(s, svcls(k.f_prev), k.s_prev, s_c, k.sv_prev));
+        }
+    }
+    k.south = nt;
+    k.s_prev = s_c;
+    k.sv_prev = sv_old;
+    k.f_prev = f;
+    k.un_w = cun_w;
+    k.su_w = csu_w;
+}
+
+template <int NJ, bool FD, bool BAR>
+__global__ void __launch_bounds__(64 * MW)
+k_step_march_dp(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
+                const double* __restrict__ flv, Layout L, Params p, double tide,
+                double* __restrict__ partials, int nbx, int ntiles) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    DpCarry k;
+    k.ct = k.cu = k.cv = 0.0;
+    if (t < ntiles) {
+        const int bx = t % nbx, by = t / nbx;
+        ShCtx x;
+        x.s = sc_;
+        x.p = p;
+        x.n = n;
+        x.a = a;
+        x.fl = fl;
+        x.flv = flv;
+        x.L = L;
+        x.tide = tide;
+        // lane 0 is column i0 - 1 (>= -1, inside the left padding)
+        x.i = (bx * MW + wave) * DOUT + lane - 1;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+        x.lb8 = (unsigned)(8 * x.il);              // wraps for -1; sums stay in range
+        x.own = lane >= 1 && lane <= DOUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = by * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        Col5F R0, R1, R2, R3;
+        load_col5(R0, n, fl, x.rowb(jb - 1) + x.lb8);
+        load_col5(R1, n, fl, x.rowb(jb) + x.lb8);
+        load_col5(R2, n, fl, x.rowb(jb + 1) + x.lb8);
+        k.f_prev = 0xffffffffu;  // no row below inside this segment
+        k.s_prev = 0.0;
+        k.sv_prev = 0.0;
+        k.south = north_terms1<FD>(sc_, R0, lane_next(R0.vn), lane_next(R0.sv), R1);
+        k.un_w = lane_prev(R1.un);
+        k.su_w = lane_prev(R1.su);
+        for (int j = jb; j <= je; j += 4) {
+            dp_row<FD, BAR>(x, k, R0, R1, R2, R3, j);
+            if (j + 1 > je) break;
+            dp_row<FD, BAR>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            dp_row<FD, BAR>(x, k, R2, R3, R0, R1, j + 2);
+            if (j + 3 > je) break;
+            dp_row<FD, BAR>(x, k, R3, R0, R1, R2, j + 3);
+        }
+        // the last row's sshn_v needs the new ssh one row up
+        if (x.own && x.i >= 1) {
+            StUniform st{sc_};
+            const long P = L.pitch;
+            const long o = L.at(x.il, je);
+            const unsigned c = svcls(k.f_prev);
+            double s_n = 0.0;
+            if (je + 1 <= L.jpj && (c == NX_AVG || c == NX_HI)) {
+                const long on = o + P;
+                s_n = (fl[on] & T_CLAMP) ? tide : continuity_at(st, n, on, P, p.rdt);
+            }
+            a.sshn_v[o] = next_sv(st, c, o, P, k.s_prev, s_n, k.sv_prev);
+        }
+    }
+    block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
+}
+
+// ---- fused step reading sshn, un, vn only (uniform statics) -----------------------
+// sshn_u and sshn_v of a state are next's interpolations of its sshn
+// (original/nemolite2d.f90:735-762) except at the points next leaves alone,
+// which keep their values.  This kernel reads sshn, un and vn (and the class
+// word), recomputes the now-state's sshn_u and sshn_v from sshn with next's
+// own expressions (reading the stored value only where next leaves a point
+// alone), and writes sshn, un and vn: 48 of the 80 bytes per cell of the
+// five-array step.  The new state's sshn_u/sshn_v arrays are not written; the
+// library fills them with the same interpolation before anything reads them.
+// The ring, the shared face terms and the lane shifts are those of
+// k_step_march_dp; since a lane's sshn_u needs the sshn one column east, the
+// east halo is two lanes wide (lanes 1..61 own the columns).
+constexpr int ROUT = 61;
+
+// The stored value of a point next leaves alone.  The load's wait is issued
+// inside the (rare) branch, so the value joining the main path carries no
+// pending load and the row's prefetched loads are not waited for.
+__device__ __forceinline__ double rc_keep(double v, bool keep, const double* arr, unsigned boff) {
+    if (keep) {
+        v = ldb(arr, boff);
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), other counters untouched
+    }
+    return v;
+}
+// sshn_u / sshn_v at a point from next's class of it (NX_SKIP where next does
+// not set it): next_su / next_sv's expressions
+template <bool FD>
+__device__ __forceinline__ double rc_su(const Scal& s, unsigned cls, double s_lo, double s_hi,
+                                        const double* keep, unsigned boff) {
+    const double r = s.e12t * s_lo + s.e12t * s_hi;
+    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<FD, QD_E12U>(s, 0.5 * r, s.e12u);
+    return rc_keep(v, cls == NX_SKIP, keep, boff);
+}
+template <bool FD>
+__device__ __forceinline__ double rc_sv(const Scal& s, unsigned cls, double s_lo, double s_hi,
+                                        const double* keep, unsigned boff) {
+    const double r = s.e12t * s_lo + s.e12t * s_hi;
+    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<FD, QD_E12V>(s, 0.5 * r, s.e12v);
+    return rc_keep(v, cls == NX_SKIP, keep, boff);
+}
+
+__device__ __forceinline__ void load_col3(Col5F& r, const NState& n, const uint16_t* fl,
+                                          unsigned boff) {
+    r.un = ldb(n.un, boff);
+    r.vn = ldb(n.vn, boff);
+    r.ss = ldb(n.sshn, boff);
+    r.f = ldb(fl, boff >> 2);
+}
+
+struct RcCtx : ShCtx {
+    bool ucol, vcol;  // this column holds next-set U / V points (0..jpi / 1..jpi)
+    // next's class of the U / V point of this lane in row r (NX_SKIP outside
+    // the rows next sets)
+    __device__ unsigned ucls_at(unsigned f, int r) const {
+        return (ucol && r >= 1 && r <= L.jpj) ? sucls(f) : NX_SKIP;
+    }
+    __device__ unsigned vcls_at(unsigned f, int r) const {
+        return (vcol && r >= 0 && r <= L.jpj) ? svcls(f) : NX_SKIP;
+    }
+};
+
+struct RcCarry {
+    NorthTerms south;         // the row below's north terms
+    double un_w, su_w;        // un and sshn_u one column west, this row
+    double ct, cu, cv;        // per-lane checksum sums
+};
+
+// One row j: A, B, C hold rows j-1, j, j+1; C has just arrived (its sshn_u is
+// formed here, and B's sshn_v, which needs C's sshn); D receives row j+2.
+template <bool FD, bool BAR>
+__device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
+                                       Col5F& C, Col5F& D, int j) {
+    if constexpr (BAR) __syncthreads();  // the block's waves start the row together
+    const Layout& L = x.L;
+    const Scal& s = x.s;
+    const long P = L.pitch;
+    const double h = 0.5;
+    const unsigned f = B.f;
+    {
+        const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
+        load_col3(D, x.n, x.fl, x.rowb(jn) + x.lb8);
+    }
+    const unsigned o8 = x.rowb(j) + x.lb8;  // byte offset of (i, j)
+    // the now-state's interpolations that this row is the first to need
+    C.su = rc_su<FD>(s, x.ucls_at(C.f, j + 1), C.ss, lane_next(C.ss), x.n.sshn_u,
+                     o8 + (unsigned)(8 * P));
+    B.sv = rc_sv<FD>(s, x.vcls_at(f, j), B.ss, C.ss, x.n.sshn_v, o8);
+    // neighbours by lane shifts (every lane active here)
+    const double bun_w = k.un_w, bsu_w = k.su_w;
+    const double bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
+    const double bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv), bss_e = lane_next(B.ss);
+    const double cun_w = lane_prev(C.un), csu_w = lane_prev(C.su);
+    const NorthTerms nt = north_terms1<FD>(s, B, bvn_e, bsv_e, C);
+    const NorthTerms& st = k.south;
+
+    // continuity (original/nemolite2d.f90:399-403)
+    double s_c = 0.0;
+    if (j >= 1 && x.tcol) {
+        if (f & T_CLAMP) {
+            s_c = x.tide;
+        } else {
+            const double r1 = (B.su + s.hu) * B.un;
+            const double r2 = (bsu_w + s.hu) * bun_w;
+            s_c = B.ss + udiv<FD, QD_E12T>(s, (r2 - r1 + st.r3 - nt.r3) * x.p.rdt, s.e12t);
+        }
+    }
+
+    // west-face terms of this lane's points; the east ones come from lane+1
+    const double un_c = B.un;
+    const double u_w = h * (un_c + bun_w) * s.e2t;
+    const double depw_u = s.ht + B.ss;
+    const double sw_u = sgn_half(u_w);
+    const double fw = ((h - sw_u) * un_c + (h + sw_u) * bun_w) * u_w * depw_u;
+    const double dudx_w = udiv<FD, QD_E1T>(s, un_c - bun_w, s.e1t) * (s.ht + B.ss);
+    const double vn_c = B.vn;
+    const double u_wc = h * (bun_w + cun_w);
+    const double u_wv = h * u_wc * (s.e2u + s.e2u);
+    const double depw_v = h * (s.hu + bsu_w + s.hu + csu_w);
+    const double sw_v = sgn_half(u_wv);
+    const double gw = ((h - sw_v) * vn_c + (h + sw_v) * bvn_w) * u_wv * depw_v;
+    // dvdx_w's quotient; lane+1's is dvdx_e's
+    const double qdvdx_w = udiv<FD, QD_E1V2>(s, vn_c - bvn_w, s.e1v + s.e1v);
+    const double fe = lane_next(fw);
+    const double dudx_e = lane_next(dudx_w);
+    const double ge = lane_next(gw);
+    const double u_ec = lane_next(u_wc);
+    const double qdvdx_e = lane_next(qdvdx_w);
+
+    if (x.own) {
+        const int i = x.i;
+        double u = 0.0, v = 0.0;
+        const unsigned cu = ucls(f), cv = vcls(f);
+        if (j >= 1 && cu == CLS_COMP) {
+            const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
+            double ts = st.kn, tn = nt.kn;
+            if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
+            if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
+            const double adv = fw - fe + ts - tn;
+            double dudy_s = 0.0, dudy_n = 0.0;
+            if (!ws) dudy_s = st.qdudy * (s.hu + B.su + s.hu + A.su);
+            if (!wn) dudy_n = nt.qdudy * (s.hu + B.su + s.hu + C.su);
+            double vis = (dudx_e - dudx_w) * s.e2u + (dudy_n - dudy_s) * s.e1u * h;
+            vis = x.p.visc * vis;
+            const double hsu = s.hu + B.su;
+            const double cor = h * (s.fcu * (st.vnc + nt.vnc)) * s.e12u * hsu;
+            const double hpg = -GRAV * hsu * s.e2u * (bss_e - B.ss);
+            const double q = udiv<FD, QD_E12U>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12u);
+            u = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HU>(s, un_c * hsu + q, s.hu + 0.0), x.p.fric);
+        }
+        if (i >= 1 && cv == CLS_COMP) {
+            const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
+            double tw = gw, te = ge;
+            if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
+            if (we) {
+                const double u_e = h * u_ec * (s.e2u + s.e2u);
+                const double depe = h * (s.hu + B.su + s.hu + C.su);
+                te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
+            }
+            const double adv = tw - te + st.hn - nt.hn;
+            double dvdx_w = 0.0, dvdx_e = 0.0;
+            if (!ww) dvdx_w = qdvdx_w * (s.hv + B.sv + s.hv + bsv_w);
+            if (!we) dvdx_e = qdvdx_e * (s.hv + B.sv + s.hv + bsv_e);
+            double vis = (nt.dvdyn - st.dvdyn) * s.e1v + (dvdx_e - dvdx_w) * s.e2v * h;
+            vis = x.p.visc * vis;
+            const double hsv = s.hv + B.sv;
+            const double cor = -h * (s.fcv * (u_ec + u_wc)) * s.e12v * hsv;
+            const double hpg = -GRAV * hsv * s.e1v * (C.ss - B.ss);
+            const double q = udiv<FD, QD_E12V>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12v);
+            v = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HV>(s, vn_c * hsv + q, s.hv + 0.0), x.p.fric);
+        }
+        if (j >= 1) {
+            if (i >= 1) {
+                stb(x.a.sshn, o8, s_c);
+                k.ct += fabs(s_c);
+            }
+            if (cu != CLS_FLATHER) {
+                stb(x.a.un, o8, u);
+                if (i >= 1) k.cu += fabs(u);
+            }
+        }
+        if (i >= 1) {
+            if (cv != CLS_FLATHER) {
+                stb(x.a.vn, o8, v);
+                if (j >= 1) k.cv += fabs(v);
+            }
+            // bc_flather_v (original/nemolite2d.f90:676-690) at a row-0 point
+            // whose inner point is this one; row 0 is outside the checksums
+            if (j == 1 && vcls(A.f) == CLS_FLATHER && (A.f & V_FL_NORTH)) {
+                const unsigned o08 = o8 - (unsigned)(8 * P);
+                stb(x.a.vn, o08, v + ldb(x.flv, o08) * (A.sv - B.sv));
+            }
+        }
+    }
+    k.south = nt;
+    k.un_w = cun_w;
+    k.su_w = csu_w;
+}
+
+template <int NJ, bool FD, bool BAR>
+__global__ void __launch_bounds__(64 * MW)
+k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
+                const double* __restrict__ flv, Layout L, Params p, double tide,
+                double* __restrict__ partials, int nbx, int ntiles) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    RcCarry k;
+    k.ct = k.cu = k.cv = 0.0;
+    if (t < ntiles) {
+        const int bx = t % nbx, by = t / nbx;
+        RcCtx x;
+        x.s = sc_;
+        x.p = p;
+        x.n = n;
+        x.a = a;
+        x.fl = fl;
+        x.flv = flv;
+        x.L = L;
+        x.tide = tide;
+        // lane 0 is column i0 - 1 (>= -1, inside the left padding)
+        x.i = (bx * MW + wave) * ROUT + lane - 1;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+        x.lb8 = (unsigned)(8 * x.il);              // wraps for -1; sums stay in range
+        x.own = lane >= 1 && lane <= ROUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        x.ucol = x.i >= 0 && x.i <= L.jpi;
+        x.vcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = by * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        Col5F R0, R1, R2, R3;
+        load_col3(R0, n, fl, x.rowb(jb - 1) + x.lb8);
+        load_col3(R1, n, fl, x.rowb(jb) + x.lb8);
+        load_col3(R2, n, fl, x.rowb(jb + 1) + x.lb8);
+        R0.su = rc_su<FD>(sc_, x.ucls_at(R0.f, jb - 1), R0.ss, lane_next(R0.ss), n.sshn_u,
+                          x.rowb(jb - 1) + x.lb8);
+        R1.su = rc_su<FD>(sc_, x.ucls_at(R1.f, jb), R1.ss, lane_next(R1.ss), n.sshn_u,
+                          x.rowb(jb) + x.lb8);
+        R0.sv = rc_sv<FD>(sc_, x.vcls_at(R0.f, jb - 1), R0.ss, R1.ss, n.sshn_v,
+                          x.rowb(jb - 1) + x.lb8);
+        k.south = north_terms1<FD>(sc_, R0, lane_next(R0.vn), lane_next(R0.sv), R1);
+        k.un_w = lane_prev(R1.un);
+        k.su_w = lane_prev(R1.su);
+        for (int j = jb; j <= je; j += 4) {
+            rc_row<FD, BAR>(x, k, R0, R1, R2, R3, j);
+            if (j + 1 > je) break;
+            rc_row<FD, BAR>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            rc_row<FD, BAR>(x, k, R2, R3, R0, R1, j + 2);
+            if (j + 3 > je) break;
+            rc_row<FD, BAR>(x, k, R3, R0, R1, R2, j + 3);
         }
     }
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
@@ -974,10 +1504,11 @@ k_closure_add(const float* __restrict__ z, int zs, Scale3 sc, Layout L, double* 
     if (j < L.jpj) vn[o] = vn[o] + (double)zp[2] * sc.s[2];
 }
 
-// sshn_u, sshn_v from the (new) sshn by the next-stage rules.
+// sshn_u, sshn_v from sshn by the next-stage rules (original/nemolite2d.f90:
+// 735-762); the points next leaves alone keep their values.
 template <class S>
 __global__ void __launch_bounds__(TBX * TBY)
-k_closure_interp(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn, Layout L,
+k_interp(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn, Layout L,
                  double* __restrict__ sshn_u, double* __restrict__ sshn_v) {
     const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
     if (i > L.jpi || j > L.jpj) return;
@@ -1032,6 +1563,19 @@ StArrays starrays(const nemo2d_model* m) {
             BODY;                           \
         }                                   \
     } while (0)
+
+// Fill in sshn_u/sshn_v of the current state set when the last fused step
+// left them unwritten (k_step_march_rc): next's interpolation of its sshn.
+int uv_sync(nemo2d_model* m) {
+    if (m->mixed || !m->uv_stale[m->cur]) return NEMO2D_OK;
+    const Layout& L = m->L;
+    double** s = m->S[m->cur];
+    WITH_STATICS(m, st, (k_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                            st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
+    LAUNCHCHK("interp");
+    m->uv_stale[m->cur] = false;
+    return NEMO2D_OK;
+}
 
 
 void dfree(void* p) {
@@ -1420,10 +1964,12 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
     dim3 g = grid2d(jpi + 1, jpj + 1);
     // partial slots: the most blocks any step kernel launches (the
-    // thread-per-cell grid, or the 16-row marching grid padded to 8 XCDs)
+    // thread-per-cell grid, or a marching grid of segments of at least 8 rows
+    // padded to 8 XCDs)
     {
-        // marching grids: segments of at least 8 rows
-        const long mb = (long)((jpi + 1 + MW * MOUT - 1) / (MW * MOUT)) * ((jpj + 1 + 7) / 8);
+        // marching grids: segments of at least 6 rows
+        const long mb = (long)((jpi + 1 + MW * ROUT - 1) / (MW * ROUT)) * ((jpj + 1 + 7) / 8);
+        static_assert(ROUT <= DOUT && DOUT <= MOUT && ROUT <= GOUT, "the narrowest strips");
         const long mbp = (mb + 7) / 8 * 8;
         m->npart_max = (int)((long)(g.x * g.y) > mbp ? (long)(g.x * g.y) : mbp);
     }
@@ -1596,6 +2142,9 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     for (int f = 0; f < NF; ++f)
         HIPCHK(hipMemcpyAsync(m->S[1][f], m->S[0][f], bytes, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
+    // sshn_u/sshn_v as given (not known to be sshn's interpolations)
+    m->uv_stale[0] = m->uv_stale[1] = false;
+    m->uv_derived = false;
     m->a_current = true;
     m->state_set = true;
     m->cs_next = 0;
@@ -1617,6 +2166,8 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
         if ((m)->mixed)                                                                    \
             return fail(NEMO2D_ERR_STATE, std::string(who) +                               \
                                               ": the single-precision form has the fused step only"); \
+        const int urc_ = uv_sync(m);                                                       \
+        if (urc_) return urc_;                                                             \
     } while (0)
 
 int nemo2d_continuity(nemo2d_model* m) {
@@ -1726,6 +2277,7 @@ int nemo2d_next(nemo2d_model* m) {
     }
     LAUNCHCHK("next");
     m->a_current = true;
+    m->uv_derived = true;
     if (rec) {
         const int nparts = (int)(g.x * g.y);
         if (m->cs_part) {
@@ -1776,7 +2328,14 @@ enum {
     VAR_MARCH24_SHN = 15,
     VAR_MARCH32_SHN = 16,
     VAR_MARCH12_SHD = 17,  // shared face terms, 12 rows, per-point depths
-    VAR_LAST = 17
+    VAR_MARCH12_DP = 18,    // shared face terms, own-column rows, DPP neighbours
+    VAR_MARCH12_DPF = 19,   //   the same dividing by precomputed reciprocals
+    VAR_MARCH12_DPB = 20,   // 18 and 19 with the block's waves in step per row
+    VAR_MARCH12_DPFB = 21,
+    VAR_MARCH12_RC = 22,    // reads sshn, un, vn only (sshn_u/v recomputed), in step per row
+    VAR_MARCH12_RCF = 23,   //   the same dividing by precomputed reciprocals
+    VAR_MARCH12_RCFN = 24,  //   the same without the per-row barrier
+    VAR_LAST = 24
 };
 
 template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
@@ -1809,6 +2368,40 @@ static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, doub
     k_step_march_sh<NJ, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march_sh");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+template <int NJ, bool FD, bool BAR>
+static int launch_march_dp(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    const Layout& L = m->L;
+    if (!sh_fits(L)) return launch_march<16, false, false, true>(m, n, a, tide, nparts);
+    const int nbx = (L.jpi + 1 + MW * DOUT - 1) / (MW * DOUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    if (FD && !m->scal.fast_div)
+        return launch_march_dp<NJ, false, BAR>(m, n, a, tide, nparts);
+    k_step_march_dp<NJ, FD, BAR><<<nblocks, 64 * MW, 0, m->stream>>>(
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
+    LAUNCHCHK("step_march_dp");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+template <int NJ, bool FD, bool BAR>
+static int launch_march_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+    const Layout& L = m->L;
+    if (FD && !m->scal.fast_div) return launch_march_rc<NJ, false, BAR>(m, n, a, tide, nparts);
+    const int nbx = (L.jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    k_step_march_rc<NJ, FD, BAR><<<nblocks, 64 * MW, 0, m->stream>>>(
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
+    LAUNCHCHK("step_march_rc");
     nparts = nblocks;
     return NEMO2D_OK;
 }
@@ -1951,11 +2544,20 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO)
-        var = m->uniform ? VAR_MARCH12_SHN : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
+        var = m->uniform ? VAR_MARCH12_RC : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
     const bool depth_var = var == VAR_MARCH12_SHD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
+    // the kernels that recompute sshn_u/sshn_v need them to be sshn's
+    // interpolations (true after any step) and no Flather points off row 0
+    const bool rc_var = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCFN;
+    if (rc_var && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
+    const bool rc_kernel = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCFN;
+    if (!rc_kernel && m->uv_stale[k]) {
+        const int urc = uv_sync(m);
+        if (urc) return urc;
+    }
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
     const int slot = istp - m->nit000;
@@ -1986,9 +2588,18 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         case VAR_MARCH24_SHN: rc = launch_march_sh<24, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_SHN: rc = launch_march_sh<32, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_SHD: rc = launch_march_depths(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_DP: rc = launch_march_dp<12, false, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_DPF: rc = launch_march_dp<12, true, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_DPFB: rc = launch_march_dp<12, true, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RC: rc = launch_march_rc<12, false, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RCF: rc = launch_march_rc<12, true, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RCFN: rc = launch_march_rc<12, true, false>(m, n, a, tide, nparts); break;
         default: rc = launch_march<32, false>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
+    m->uv_stale[1 - k] = rc_kernel;
+    m->uv_derived = true;
     m->cur = 1 - k;
     m->a_current = false;
     // the shared-term kernels set the row-0 Flather v points themselves
@@ -2194,11 +2805,12 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
     k_closure_add<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(z, zstride, sc, L, s[F_SSHN],
                                                                           s[F_UN], s[F_VN]);
     LAUNCHCHK("closure_add");
-    WITH_STATICS(m, st, (k_closure_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+    WITH_STATICS(m, st, (k_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
                             st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
     LAUNCHCHK("closure_interp");
     HIPCHK(hipStreamSynchronize(m->stream));
     m->a_current = false;
+    m->uv_derived = true;
     return NEMO2D_OK;
 }
 
@@ -2249,8 +2861,9 @@ int nemo2d_retrieve_aux(nemo2d_model* m, double* sshn_u, double* sshn_v, double*
         }
         return NEMO2D_OK;
     }
+    int rc = uv_sync(m);
+    if (rc) return rc;
     double** s = m->S[m->cur];
-    int rc = NEMO2D_OK;
     if (sshn_u && (rc = download(m, sshn_u, s[F_SSHN_U], 0, jpi, 1, jpj))) return rc;
     if (sshn_v && (rc = download(m, sshn_v, s[F_SSHN_V], 1, jpi, 0, jpj))) return rc;
     // after a fused step the new ssh is the now-state's sshn

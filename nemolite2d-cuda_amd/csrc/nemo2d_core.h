@@ -157,21 +157,44 @@ __device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b);
 // (Markstein: y within half an ulp of 1/d and a faithful quotient make
 // RN(q + (x - d*q)*y) the correctly rounded quotient; the first correction
 // makes the quotient faithful, the remainder x - d*q is exact by FMA while
-// nothing underflows).  Zeros keep their sign through x*y; |x| outside
-// [2^-960, 2^960] (where the remainder could underflow) and non-finite x take
-// the IEEE division.  Same bits as x / d for every x.
+// nothing underflows).  The quotient takes x's sign, which keeps the sign of
+// a zero x.  |x| outside [2^-960, 2^960) other than zero (where a remainder
+// could underflow), and non-finite x, take the IEEE division; the test is on
+// the biased exponent, 63 <= e <= 1982.  Same bits as x / d for every x.
 __device__ __forceinline__ double div_rcp(double x, double d, double y) {
     double q = x * y;
     double r = fma(-d, q, x);
     q = fma(r, y, q);
     r = fma(-d, q, x);
-    const double q2 = fma(r, y, q);
-    const double ax = fabs(x);
-    double res = ax == 0.0 ? x * y : q2;
-    if (__builtin_expect(!(ax >= 0x1p-960 && ax <= 0x1p960) && ax != 0.0, 0)) res = x / d;
-    return res;
+    q = copysign(fma(r, y, q), x);
+    const unsigned e = ((unsigned)__double2hiint(x) >> 20) & 0x7ffu;
+    if (__builtin_expect(e - 63u >= 1920u && x != 0.0, 0)) q = x / d;
+    return q;
 }
 
+
+// Value of x held by lane l+1 of the wave, by a DPP wavefront shift (one
+// v_mov per 32-bit half, no LDS round trip).  Lane 63 receives 0 (bound
+// control on, every lane written, so no initial value is needed).  Every lane
+// of the wave must be active.
+__device__ __forceinline__ float lane_next(float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double lane_next(double x) {
+    int2 v = __builtin_bit_cast(int2, x);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x130, 0xf, 0xf, true);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x130, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, v);
+}
+// Value of x held by lane l-1 (DPP wavefront shift the other way); lane 0
+// receives 0.  Every lane of the wave must be active.
+__device__ __forceinline__ double lane_prev(double x) {
+    int2 v = __builtin_bit_cast(int2, x);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xf, 0xf, true);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, v);
+}
 
 // Upwind weight SIGN(0.5, x) with Fortran's signed-zero rule.
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }

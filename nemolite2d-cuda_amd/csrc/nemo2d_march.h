@@ -197,7 +197,7 @@ __device__ __forceinline__ void row_g(const CtxG<R, SS, DARR>& x, CarryG<R, SS>&
             s_c = B.ss[1] + SS(dv((r2 - r1 + st.r3 - nt.r3) * x.p.rdt, m.e12t));
         }
     }
-    const SS s_e = __shfl_down(s_c, 1, 64);
+    const SS s_e = lane_next(s_c);
 
     // west-face terms; the east ones come from lane+1
     const R un_c = B.un[1];
@@ -213,11 +213,11 @@ __device__ __forceinline__ void row_g(const CtxG<R, SS, DARR>& x, CarryG<R, SS>&
     const R sw_v = sgn_half(u_wv);
     const R gw = ((h - sw_v) * vn_c + (h + sw_v) * B.vn[0]) * u_wv * depw_v;
     const R qdvdx_w = dv(vn_c - B.vn[0], m.e1v + m.e1v);
-    const R fe = __shfl_down(fw, 1, 64);
-    const R dudx_e = __shfl_down(dudx_w, 1, 64);
-    const R ge = __shfl_down(gw, 1, 64);
-    const R u_ec = __shfl_down(u_wc, 1, 64);
-    const R qdvdx_e = __shfl_down(qdvdx_w, 1, 64);
+    const R fe = lane_next(fw);
+    const R dudx_e = lane_next(dudx_w);
+    const R ge = lane_next(gw);
+    const R u_ec = lane_next(u_wc);
+    const R qdvdx_e = lane_next(qdvdx_w);
 
     const R su_old = B.su[1], sv_old = B.sv[1];
     if (x.own) {

@@ -45,7 +45,9 @@ SYMBOLS = (
 VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_flin": 5,
             "cell_flin": 6, "march16_fd": 7, "march32_fd": 8, "march16_pf": 9, "march32_pf": 10,
             "march16_sh": 11, "march32_sh": 12, "march12_shn": 13, "march16_shn": 14,
-            "march24_shn": 15, "march32_shn": 16, "march12_shd": 17}
+            "march24_shn": 15, "march32_shn": 16, "march12_shd": 17,
+            "march12_dp": 18, "march12_dpf": 19, "march12_dpb": 20, "march12_dpfb": 21,
+            "march12_rc": 22, "march12_rcf": 23, "march12_rcfn": 24}
 
 
 class Nemo2dError(RuntimeError):

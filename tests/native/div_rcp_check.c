@@ -1,0 +1,87 @@
+/* Host restatement of the device's division by a precomputed reciprocal
+ * (div_rcp in nemolite2d-cuda_amd/csrc/nemo2d_core.h): q0 = RN(x*y), two FMA
+ * remainder corrections, the quotient given x's sign, and the IEEE division
+ * for |x| outside [2^-960, 2^960) other than zero.  Checked bit for bit
+ * against x / d on random operands over the whole exponent range, on
+ * quotients within 2^-64 relative of a rounding midpoint (the hard cases of
+ * correct rounding), and on zeros, subnormals, infinities and NaN.
+ * Usage: div_rcp_check <n> <seed>; prints the number of mismatches. */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static double div_rcp(double x, double d, double y) {
+    double q = x * y;
+    double r = fma(-d, q, x);
+    q = fma(r, y, q);
+    r = fma(-d, q, x);
+    q = copysign(fma(r, y, q), x);
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    const unsigned e = (unsigned)(b >> 52) & 0x7ffu;
+    if (e - 63u >= 1920u && x != 0.0) q = x / d;
+    return q;
+}
+
+static uint64_t s_state;
+static uint64_t rnd(void) { /* splitmix64 */
+    uint64_t z = (s_state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+static double mk(int exp, uint64_t mant, int neg) {
+    uint64_t b = ((uint64_t)(neg ? 1 : 0) << 63) | ((uint64_t)(exp + 1023) << 52) |
+                 (mant & 0xfffffffffffffull);
+    double v;
+    memcpy(&v, &b, 8);
+    return v;
+}
+static long bad = 0, tried = 0;
+static void check(double x, double d) {
+    const double y = 1.0 / d;
+    const double a = div_rcp(x, d, y), b = x / d;
+    ++tried;
+    if (memcmp(&a, &b, 8) != 0 && !(isnan(a) && isnan(b))) {
+        if (bad < 10) fprintf(stderr, "mismatch x=%a d=%a: %a vs %a\n", x, d, a, b);
+        ++bad;
+    }
+}
+
+int main(int argc, char** argv) {
+    const long n = argc > 1 ? atol(argv[1]) : 1000000;
+    s_state = argc > 2 ? strtoull(argv[2], 0, 10) : 1;
+    /* divisors of the model's uniform grids (cell areas, spacings, doubled
+     * spacings, depths, the friction factor) and random ones in range */
+    const double fixed[] = {1.0e6, 1.0e3, 2.0e3, 100.0, 100.0 + 0.0, 1.0 + 0.0015 * 20.0,
+                            1.0 + 0.00015 * 20.0, 4.0e6, 2.5e5, 1.0 + 0.0015 * 12.0, 50.0, 7.0};
+    const int nf = (int)(sizeof fixed / sizeof fixed[0]);
+    for (long i = 0; i < n; ++i) {
+        double d;
+        if (i % 4 != 3) d = fixed[rnd() % nf];
+        else d = mk((int)(rnd() % 121) - 60, rnd(), 0);
+        /* random x over the full finite range */
+        check(mk((int)(rnd() % 2046) - 1022, rnd(), (int)(rnd() & 1)), d);
+        /* x inside the fast range */
+        check(mk((int)(rnd() % 1920) - 960, rnd(), (int)(rnd() & 1)), d);
+        /* near-midpoint quotient: x = RN(d * (q + ulp(q)/2)) in long double */
+        {
+            const double q = mk((int)(rnd() % 1800) - 900, rnd(), (int)(rnd() & 1));
+            const long double mid = (long double)q + (long double)(nextafter(q, INFINITY) - q) / 2;
+            const double x = (double)(mid * (long double)d);
+            check(x, d);
+            check(nextafter(x, INFINITY), d);
+            check(nextafter(x, -INFINITY), d);
+        }
+    }
+    /* specials and the edges of the fast range */
+    const double sp[] = {0.0, -0.0, 0x1p-1074, -0x1p-1074, 0x1p-1022, 0x1p-960, -0x1p-960,
+                         0x1.fffffffffffffp-961, 0x1p960, 0x1.fffffffffffffp959, -0x1p960,
+                         INFINITY, -INFINITY, NAN, 1.0, -1.0, 3.0, 0x1.fffffffffffffp1023};
+    for (int k = 0; k < (int)(sizeof sp / sizeof sp[0]); ++k)
+        for (int j = 0; j < nf; ++j) check(sp[k], fixed[j]);
+    printf("%ld %ld\n", tried, bad);
+    return 0;
+}

@@ -407,6 +407,46 @@ def test_per_point_depths_bitwise_vs_oracle(variant, islands):
         assert np.all(np.isfinite(m.checksum_log(25)))
 
 
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcf"])
+@pytest.mark.parametrize("islands", [False, True])
+def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
+    """The fused kernel that reads sshn, un and vn only (sshn_u and sshn_v
+    recomputed from sshn with next's expressions, the stored value kept where
+    next leaves a point alone, original/nemolite2d.f90:735-762): uniform
+    statics, the reference basin's open edge, inner walls from islands;
+    interleaved with stage steps and with state reads (both fill the
+    interpolated arrays in first)."""
+    o = random_grid_oracle(203, 61, 31 + islands, islands=islands, open_edges="", vary_metrics=False)
+    with model_like(o) as m:
+        m.set_variant(variant)
+        m.record_checksums(40)
+        for istp in range(1, 41):
+            (m.step_stages if istp % 13 == 0 else m.step)(istp)
+            o.step(istp)
+            if istp % 7 == 0:
+                assert_fields_equal(m, o, "%s step %d" % (variant, istp))
+        assert_fields_equal(m, o, "%s end" % variant)
+        assert np.all(np.isfinite(m.checksum_log(40)))
+
+
+@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
+def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
+    """A state whose sshn_u/sshn_v are not sshn's interpolations (set by the
+    caller) must be stepped from the stored values: the kernel that recomputes
+    them is used only once a step has made them so."""
+    o = Oracle(150, 70)
+    rng = np.random.default_rng(5)
+    for k in ("sshn", "sshn_u", "sshn_v"):
+        a = o.arr(k)
+        a[:] = 0.01 * rng.standard_normal(a.shape)
+    with model_like(o) as m:
+        m.set_variant(variant)
+        for istp in range(1, 6):
+            m.step(istp)
+            o.step(istp)
+            assert_fields_equal(m, o, "%s step %d" % (variant, istp))
+
+
 @pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
 def test_every_kernel_variant_is_bitwise_equal(variant):
     """Default grid (uniform statics: all variants apply) and a general mask
