@@ -169,10 +169,12 @@ def form_closure(nc, runs, snaps, iters, windows):
 
 
 # the fused-step kernel each variant runs on the uniform benchmark grid
-DEFAULT_VARIANT = "march12_rc"
-KERNELS = {"march12_rc": "k_step_march_rc<12, false, true>",
-           "march12_rcf": "k_step_march_rc<12, true, true>",
-           "march12_rcfn": "k_step_march_rc<12, true, false>",
+DEFAULT_VARIANT = "march12_rcg2np"
+KERNELS = {"march12_rcg2np": "k_step_march_rc<12, 2, false, 2, true>",
+           "march12_rc2np": "k_step_march_rc<12, 0, false, 2, true>",
+           "march12_rcg2n": "k_step_march_rc<12, 2, false, 2, false>",
+           "march12_rc2n": "k_step_march_rc<12, 0, false, 2, false>",
+           "march12_rc": "k_step_march_rc<12, 0, true, 1, false>",
            "march12_dpfb": "k_step_march_dp<12, true, true>",
            "march12_dpb": "k_step_march_dp<12, false, true>",
            "march12_shn": "k_step_march_sh<12, false>", "march16_shn": "k_step_march_sh<16, false>",

@@ -111,7 +111,7 @@ int nemo2d_step(nemo2d_model *m, int istp);
 int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default (22 on grids with uniform metrics and depths, else 1),
+ * choice): 0 default (26 on grids with uniform metrics and depths, else 1),
  * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
  * 5 marching and 6 per-cell with Flather points recomputed inline instead
  * of in the epilogue, 7/8 marching (16/32 rows) dividing by precomputed
@@ -126,9 +126,12 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
  * 20/21 as 18/19 with the block's waves kept in step row by row, 22 the
  * same reading sshn, un and vn only (sshn_u and sshn_v recomputed from sshn
  * as next forms them; used once a step has made them so and only with
- * Flather points on row 0 alone, else 21 runs), 23 as 22 dividing by
- * reciprocals, 24 as 23 without the per-row step-keeping.
- * Marching needs uniform metrics, 2-16 and 18-24 also uniform depths, and
+ * Flather points on row 0 alone, else 21 runs), 23 as 22 with loads two rows
+ * ahead and no step-keeping, 24 as 23 dividing by reciprocals under one
+ * guard per row (the row is done again with IEEE divisions when any
+ * numerator is outside the exact range), 25/26 as 23/24 running class-free
+ * code on the tiles whose points are all plain interior points.
+ * Marching needs uniform metrics, 2-16 and 18-26 also uniform depths, and
  * 17 Flather points only on row 0 with the inner point north; other grids
  * use 1 (or 6).  The default for grids with uniform metrics but varying
  * depths is 17 where it applies.  The single-precision form marches (as 17)

@@ -102,6 +102,9 @@ struct nemo2d_model {
     // current state's sshn_u/sshn_v are known to be those interpolations
     bool uv_stale[2] = {false, false};
     bool uv_derived = false;
+    // per (wave strip, 12-row segment) of k_step_march_rc's tiling: 1 where
+    // every point the wave touches has the plain interior class word
+    uint8_t* rc_plain = nullptr;
     double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
     double* cs_log = nullptr;    // 3 doubles per logged step
     // Deferred checksum finish: the fused step leaves each logged step's
@@ -531,17 +534,21 @@ struct Col5 {
 };
 // c: this row (own column), c_vn_e / c_sv_e: its vn and sshn_v one column
 // east, nr: the row above (own column)
-// x / d for the uniform divisor d of tag T (QD_*): the IEEE division, or
-// with FD the exact division by the precomputed reciprocal (same bits)
-template <bool FD, int T>
-__device__ __forceinline__ double udiv(const Scal& s, double x, double d) {
-    if constexpr (FD) return div_rcp(x, d, s.rcp[T]);
+// x / d for the uniform divisor d of tag T (QD_*).  DM 0: the IEEE
+// division; 1: the guarded division by the precomputed reciprocal (div_rcp);
+// 2: the unguarded one, its guard gathered in *ck (div_rcp_acc).  All give
+// x / d's bits (2 when the gathered guard holds).
+template <int DM, int T>
+__device__ __forceinline__ double udiv(const Scal& s, double x, double d, DivCheck* ck = nullptr) {
+    if constexpr (DM == 2) return div_rcp_acc(x, d, s.rcp[T], *ck);
+    else if constexpr (DM == 1) return div_rcp(x, d, s.rcp[T]);
     else return x / d;
 }
 
-template <bool FD = false>
+template <int FD = 0>
 __device__ __forceinline__ NorthTerms north_terms1(const Scal& s, const Col5& c, double c_vn_e,
-                                                   double c_sv_e, const Col5& nr) {
+                                                   double c_sv_e, const Col5& nr,
+                                                   DivCheck* ck = nullptr) {
     const double h = 0.5;
     NorthTerms t;
     t.r3 = (c.sv + s.hv) * c.vn;
@@ -558,8 +565,8 @@ __device__ __forceinline__ NorthTerms north_terms1(const Scal& s, const Col5& c,
     const double sv = sgn_half(vn_n);
     const double vvn = (h + sv) * c.vn + (h - sv) * nr.vn;
     t.hn = vvn * vn_n * depn;
-    t.dvdyn = udiv<FD, QD_E2T>(s, nr.vn - c.vn, s.e2t) * (s.ht + nr.ss);
-    t.qdudy = udiv<FD, QD_E2U2>(s, nr.un - c.un, s.e2u + s.e2u);
+    t.dvdyn = udiv<FD, QD_E2T>(s, nr.vn - c.vn, s.e2t, ck) * (s.ht + nr.ss);
+    t.qdudy = udiv<FD, QD_E2U2>(s, nr.un - c.un, s.e2u + s.e2u, ck);
     return t;
 }
 __device__ __forceinline__ Col5 col5(const Row5& r, int c) {
@@ -1059,6 +1066,11 @@ k_step_march_dp(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
 // k_step_march_dp; since a lane's sshn_u needs the sshn one column east, the
 // east halo is two lanes wide (lanes 1..61 own the columns).
 constexpr int ROUT = 61;
+// The class word of an interior point that nothing special touches: U and V
+// computed, no walls, no clamp, next averaging both interpolations.
+constexpr unsigned RC_PLAIN = (CLS_COMP << U_CLS_SHIFT) | (CLS_COMP << V_CLS_SHIFT) |
+                              (NX_AVG << SU_SHIFT) | (NX_AVG << SV_SHIFT);
+constexpr int RC_NJ = 12, RC_LA = 2;  // the tiling the plain-tile table is made for
 
 // The stored value of a point next leaves alone.  The load's wait is issued
 // inside the (rare) branch, so the value joining the main path carries no
@@ -1072,27 +1084,31 @@ __device__ __forceinline__ double rc_keep(double v, bool keep, const double* arr
 }
 // sshn_u / sshn_v at a point from next's class of it (NX_SKIP where next does
 // not set it): next_su / next_sv's expressions
-template <bool FD>
+template <int DM, bool PLAIN = false>
 __device__ __forceinline__ double rc_su(const Scal& s, unsigned cls, double s_lo, double s_hi,
-                                        const double* keep, unsigned boff) {
+                                        const double* keep, unsigned boff, DivCheck* ck) {
     const double r = s.e12t * s_lo + s.e12t * s_hi;
-    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<FD, QD_E12U>(s, 0.5 * r, s.e12u);
+    if constexpr (PLAIN) return udiv<DM, QD_E12U>(s, 0.5 * r, s.e12u, ck);
+    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<DM, QD_E12U>(s, 0.5 * r, s.e12u, ck);
     return rc_keep(v, cls == NX_SKIP, keep, boff);
 }
-template <bool FD>
+template <int DM, bool PLAIN = false>
 __device__ __forceinline__ double rc_sv(const Scal& s, unsigned cls, double s_lo, double s_hi,
-                                        const double* keep, unsigned boff) {
+                                        const double* keep, unsigned boff, DivCheck* ck) {
     const double r = s.e12t * s_lo + s.e12t * s_hi;
-    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<FD, QD_E12V>(s, 0.5 * r, s.e12v);
+    if constexpr (PLAIN) return udiv<DM, QD_E12V>(s, 0.5 * r, s.e12v, ck);
+    double v = cls == NX_HI ? s_hi : cls == NX_LO ? s_lo : udiv<DM, QD_E12V>(s, 0.5 * r, s.e12v, ck);
     return rc_keep(v, cls == NX_SKIP, keep, boff);
 }
 
+template <bool PLAIN = false>
 __device__ __forceinline__ void load_col3(Col5F& r, const NState& n, const uint16_t* fl,
                                           unsigned boff) {
     r.un = ldb(n.un, boff);
     r.vn = ldb(n.vn, boff);
     r.ss = ldb(n.sshn, boff);
-    r.f = ldb(fl, boff >> 2);
+    if constexpr (PLAIN) r.f = RC_PLAIN;
+    else r.f = ldb(fl, boff >> 2);
 }
 
 struct RcCtx : ShCtx {
@@ -1113,45 +1129,55 @@ struct RcCarry {
     double ct, cu, cv;        // per-lane checksum sums
 };
 
-// One row j: A, B, C hold rows j-1, j, j+1; C has just arrived (its sshn_u is
-// formed here, and B's sshn_v, which needs C's sshn); D receives row j+2.
-template <bool FD, bool BAR>
-__device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
-                                       Col5F& C, Col5F& D, int j) {
-    if constexpr (BAR) __syncthreads();  // the block's waves start the row together
-    const Layout& L = x.L;
+// What row j computes: the row's north terms, C's sshn_u and B's sshn_v
+// (the now-state's interpolations first needed here), the west values
+// carried to the next row, and the new sshn, u and v.
+struct RcOut {
+    NorthTerms nt;
+    double csu, bsv, cun_w, csu_w;
+    double s_c, u, v;
+};
+
+// Row j's arithmetic, no stores: A, B, C hold rows j-1, j, j+1 (C has just
+// arrived).  With DM 2 the divisions' guard is gathered in *ck.
+// With PLAIN every point the wave touches has the class word RC_PLAIN (the
+// tile table says so), and the class tests drop out.
+template <int DM, bool PLAIN = false>
+__device__ __forceinline__ RcOut rc_compute(const RcCtx& x, const RcCarry& k, const Col5F& A,
+                                            const Col5F& B0, const Col5F& C0, int j, DivCheck* ck) {
     const Scal& s = x.s;
-    const long P = L.pitch;
+    const long P = x.L.pitch;
     const double h = 0.5;
-    const unsigned f = B.f;
-    {
-        const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
-        load_col3(D, x.n, x.fl, x.rowb(jn) + x.lb8);
-    }
+    const unsigned f = B0.f;
     const unsigned o8 = x.rowb(j) + x.lb8;  // byte offset of (i, j)
-    // the now-state's interpolations that this row is the first to need
-    C.su = rc_su<FD>(s, x.ucls_at(C.f, j + 1), C.ss, lane_next(C.ss), x.n.sshn_u,
-                     o8 + (unsigned)(8 * P));
-    B.sv = rc_sv<FD>(s, x.vcls_at(f, j), B.ss, C.ss, x.n.sshn_v, o8);
+    RcOut o;
+    Col5F B = B0, C = C0;
+    C.su = o.csu = rc_su<DM, PLAIN>(s, x.ucls_at(C.f, j + 1), C.ss, lane_next(C.ss), x.n.sshn_u,
+                                    o8 + (unsigned)(8 * P), ck);
+    B.sv = o.bsv = rc_sv<DM, PLAIN>(s, x.vcls_at(f, j), B.ss, C.ss, x.n.sshn_v, o8, ck);
     // neighbours by lane shifts (every lane active here)
     const double bun_w = k.un_w, bsu_w = k.su_w;
     const double bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
     const double bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv), bss_e = lane_next(B.ss);
-    const double cun_w = lane_prev(C.un), csu_w = lane_prev(C.su);
-    const NorthTerms nt = north_terms1<FD>(s, B, bvn_e, bsv_e, C);
+    o.cun_w = lane_prev(C.un);
+    o.csu_w = lane_prev(C.su);
+    const double cun_w = o.cun_w, csu_w = o.csu_w;
+    o.nt = north_terms1<DM>(s, B, bvn_e, bsv_e, C, ck);
+    const NorthTerms& nt = o.nt;
     const NorthTerms& st = k.south;
 
     // continuity (original/nemolite2d.f90:399-403)
     double s_c = 0.0;
-    if (j >= 1 && x.tcol) {
-        if (f & T_CLAMP) {
+    if (PLAIN || (j >= 1 && x.tcol)) {
+        if (!PLAIN && (f & T_CLAMP)) {
             s_c = x.tide;
         } else {
             const double r1 = (B.su + s.hu) * B.un;
             const double r2 = (bsu_w + s.hu) * bun_w;
-            s_c = B.ss + udiv<FD, QD_E12T>(s, (r2 - r1 + st.r3 - nt.r3) * x.p.rdt, s.e12t);
+            s_c = B.ss + udiv<DM, QD_E12T>(s, (r2 - r1 + st.r3 - nt.r3) * x.p.rdt, s.e12t, ck);
         }
     }
+    o.s_c = s_c;
 
     // west-face terms of this lane's points; the east ones come from lane+1
     const double un_c = B.un;
@@ -1159,7 +1185,7 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
     const double depw_u = s.ht + B.ss;
     const double sw_u = sgn_half(u_w);
     const double fw = ((h - sw_u) * un_c + (h + sw_u) * bun_w) * u_w * depw_u;
-    const double dudx_w = udiv<FD, QD_E1T>(s, un_c - bun_w, s.e1t) * (s.ht + B.ss);
+    const double dudx_w = udiv<DM, QD_E1T>(s, un_c - bun_w, s.e1t, ck) * (s.ht + B.ss);
     const double vn_c = B.vn;
     const double u_wc = h * (bun_w + cun_w);
     const double u_wv = h * u_wc * (s.e2u + s.e2u);
@@ -1167,19 +1193,19 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
     const double sw_v = sgn_half(u_wv);
     const double gw = ((h - sw_v) * vn_c + (h + sw_v) * bvn_w) * u_wv * depw_v;
     // dvdx_w's quotient; lane+1's is dvdx_e's
-    const double qdvdx_w = udiv<FD, QD_E1V2>(s, vn_c - bvn_w, s.e1v + s.e1v);
+    const double qdvdx_w = udiv<DM, QD_E1V2>(s, vn_c - bvn_w, s.e1v + s.e1v, ck);
     const double fe = lane_next(fw);
     const double dudx_e = lane_next(dudx_w);
     const double ge = lane_next(gw);
     const double u_ec = lane_next(u_wc);
     const double qdvdx_e = lane_next(qdvdx_w);
 
+    double u = 0.0, v = 0.0;
     if (x.own) {
         const int i = x.i;
-        double u = 0.0, v = 0.0;
         const unsigned cu = ucls(f), cv = vcls(f);
-        if (j >= 1 && cu == CLS_COMP) {
-            const bool ws = (f & U_WALL_S) != 0, wn = (f & U_WALL_N) != 0;
+        if (PLAIN || (j >= 1 && cu == CLS_COMP)) {
+            const bool ws = !PLAIN && (f & U_WALL_S) != 0, wn = !PLAIN && (f & U_WALL_N) != 0;
             double ts = st.kn, tn = nt.kn;
             if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
             if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
@@ -1192,11 +1218,11 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
             const double hsu = s.hu + B.su;
             const double cor = h * (s.fcu * (st.vnc + nt.vnc)) * s.e12u * hsu;
             const double hpg = -GRAV * hsu * s.e2u * (bss_e - B.ss);
-            const double q = udiv<FD, QD_E12U>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12u);
-            u = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HU>(s, un_c * hsu + q, s.hu + 0.0), x.p.fric);
+            const double q = udiv<DM, QD_E12U>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12u, ck);
+            u = udiv<DM, QD_FRIC>(s, udiv<DM, QD_HU>(s, un_c * hsu + q, s.hu + 0.0, ck), x.p.fric, ck);
         }
-        if (i >= 1 && cv == CLS_COMP) {
-            const bool ww = (f & V_WALL_W) != 0, we = (f & V_WALL_E) != 0;
+        if (PLAIN || (i >= 1 && cv == CLS_COMP)) {
+            const bool ww = !PLAIN && (f & V_WALL_W) != 0, we = !PLAIN && (f & V_WALL_E) != 0;
             double tw = gw, te = ge;
             if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
             if (we) {
@@ -1213,42 +1239,145 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
             const double hsv = s.hv + B.sv;
             const double cor = -h * (s.fcv * (u_ec + u_wc)) * s.e12v * hsv;
             const double hpg = -GRAV * hsv * s.e1v * (C.ss - B.ss);
-            const double q = udiv<FD, QD_E12V>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12v);
-            v = udiv<FD, QD_FRIC>(s, udiv<FD, QD_HV>(s, vn_c * hsv + q, s.hv + 0.0), x.p.fric);
+            const double q = udiv<DM, QD_E12V>(s, x.p.rdt * (adv + vis + cor + hpg), s.e12v, ck);
+            v = udiv<DM, QD_FRIC>(s, udiv<DM, QD_HV>(s, vn_c * hsv + q, s.hv + 0.0, ck), x.p.fric, ck);
         }
+    }
+    o.u = u;
+    o.v = v;
+    return o;
+}
+
+// Row j's stores and carries.
+template <bool PLAIN = false>
+__device__ __forceinline__ void rc_commit(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
+                                          Col5F& C, const RcOut& o, int j) {
+    const long P = x.L.pitch;
+    const unsigned f = B.f;
+    const unsigned o8 = x.rowb(j) + x.lb8;
+    if (PLAIN && x.own) {
+        stb(x.a.sshn, o8, o.s_c);
+        k.ct += fabs(o.s_c);
+        stb(x.a.un, o8, o.u);
+        k.cu += fabs(o.u);
+        stb(x.a.vn, o8, o.v);
+        k.cv += fabs(o.v);
+    } else if (!PLAIN && x.own) {
+        const int i = x.i;
+        const unsigned cu = ucls(f), cv = vcls(f);
         if (j >= 1) {
             if (i >= 1) {
-                stb(x.a.sshn, o8, s_c);
-                k.ct += fabs(s_c);
+                stb(x.a.sshn, o8, o.s_c);
+                k.ct += fabs(o.s_c);
             }
             if (cu != CLS_FLATHER) {
-                stb(x.a.un, o8, u);
-                if (i >= 1) k.cu += fabs(u);
+                stb(x.a.un, o8, o.u);
+                if (i >= 1) k.cu += fabs(o.u);
             }
         }
         if (i >= 1) {
             if (cv != CLS_FLATHER) {
-                stb(x.a.vn, o8, v);
-                if (j >= 1) k.cv += fabs(v);
+                stb(x.a.vn, o8, o.v);
+                if (j >= 1) k.cv += fabs(o.v);
             }
             // bc_flather_v (original/nemolite2d.f90:676-690) at a row-0 point
             // whose inner point is this one; row 0 is outside the checksums
             if (j == 1 && vcls(A.f) == CLS_FLATHER && (A.f & V_FL_NORTH)) {
                 const unsigned o08 = o8 - (unsigned)(8 * P);
-                stb(x.a.vn, o08, v + ldb(x.flv, o08) * (A.sv - B.sv));
+                stb(x.a.vn, o08, o.v + ldb(x.flv, o08) * (A.sv - o.bsv));
             }
         }
     }
-    k.south = nt;
-    k.un_w = cun_w;
-    k.su_w = csu_w;
+    k.south = o.nt;
+    k.un_w = o.cun_w;
+    k.su_w = o.csu_w;
+    C.su = o.csu;
+    B.sv = o.bsv;
 }
 
-template <int NJ, bool FD, bool BAR>
+// One row j: A, B, C hold rows j-1, j, j+1; D receives row j+1+LA (the
+// rows between are in flight).  With DM 2 the row's divisions go by
+// reciprocal with one guard for the whole row; if it fails on any lane, the
+// wave does the row again with IEEE divisions.
+template <int DM, bool BAR, int LA, bool PLAIN = false>
+__device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
+                                       Col5F& C, Col5F& D, int j) {
+    if constexpr (BAR) __syncthreads();  // the block's waves start the row together
+    {
+        const int jn = j + 1 + LA <= x.L.jpj + 2 ? j + 1 + LA : x.L.jpj + 2;
+        load_col3<PLAIN>(D, x.n, x.fl, x.rowb(jn) + x.lb8);
+    }
+    if constexpr (DM == 2) {
+        DivCheck ck;
+        div_check_reset(ck);
+        RcOut o = rc_compute<2, PLAIN>(x, k, A, B, C, j, &ck);
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(!div_check_ok(ck)) != 0, 0))
+            o = rc_compute<0, PLAIN>(x, k, A, B, C, j, nullptr);
+        rc_commit<PLAIN>(x, k, A, B, C, o, j);
+    } else {
+        const RcOut o = rc_compute<DM, PLAIN>(x, k, A, B, C, j, nullptr);
+        rc_commit<PLAIN>(x, k, A, B, C, o, j);
+    }
+}
+
+// One segment rows jb..je of a wave's strip.
+template <int DM, bool BAR, int LA, bool PLAIN>
+__device__ __forceinline__ void rc_march(const RcCtx& x, RcCarry& k, int jb, int je) {
+    const Layout& L = x.L;
+    const NState& n = x.n;
+    const uint16_t* fl = x.fl;
+    const Scal& sc_ = x.s;
+    Col5F R0, R1, R2, R3, R4;
+    load_col3<PLAIN>(R0, n, fl, x.rowb(jb - 1) + x.lb8);
+    load_col3<PLAIN>(R1, n, fl, x.rowb(jb) + x.lb8);
+    load_col3<PLAIN>(R2, n, fl, x.rowb(jb + 1) + x.lb8);
+    if constexpr (LA == 2)
+        load_col3<PLAIN>(R3, n, fl, x.rowb(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lb8);
+    // the segment's first interpolations and south terms, IEEE divisions
+    R0.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R0.f, jb - 1), R0.ss, lane_next(R0.ss), n.sshn_u,
+                            x.rowb(jb - 1) + x.lb8, nullptr);
+    R1.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R1.f, jb), R1.ss, lane_next(R1.ss), n.sshn_u,
+                            x.rowb(jb) + x.lb8, nullptr);
+    R0.sv = rc_sv<0, PLAIN>(sc_, x.vcls_at(R0.f, jb - 1), R0.ss, R1.ss, n.sshn_v,
+                            x.rowb(jb - 1) + x.lb8, nullptr);
+    k.south = north_terms1<0>(sc_, R0, lane_next(R0.vn), lane_next(R0.sv), R1);
+    k.un_w = lane_prev(R1.un);
+    k.su_w = lane_prev(R1.su);
+    // slot roles rotate, nothing is copied: rows j-1, j, j+1 held, the next
+    // LA rows in flight or loading
+    if constexpr (LA == 1) {
+        for (int j = jb; j <= je; j += 4) {
+            rc_row<DM, BAR, 1, PLAIN>(x, k, R0, R1, R2, R3, j);
+            if (j + 1 > je) break;
+            rc_row<DM, BAR, 1, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            rc_row<DM, BAR, 1, PLAIN>(x, k, R2, R3, R0, R1, j + 2);
+            if (j + 3 > je) break;
+            rc_row<DM, BAR, 1, PLAIN>(x, k, R3, R0, R1, R2, j + 3);
+        }
+    } else {
+        for (int j = jb; j <= je; j += 5) {
+            rc_row<DM, BAR, 2, PLAIN>(x, k, R0, R1, R2, R4, j);
+            if (j + 1 > je) break;
+            rc_row<DM, BAR, 2, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            rc_row<DM, BAR, 2, PLAIN>(x, k, R2, R3, R4, R1, j + 2);
+            if (j + 3 > je) break;
+            rc_row<DM, BAR, 2, PLAIN>(x, k, R3, R4, R0, R2, j + 3);
+            if (j + 4 > je) break;
+            rc_row<DM, BAR, 2, PLAIN>(x, k, R4, R0, R1, R3, j + 4);
+        }
+    }
+}
+
+// With PL, a wave whose tile the table marks plain (every point it touches,
+// halos included, has the class word RC_PLAIN) runs the class-free code.
+template <int NJ, int DM, bool BAR, int LA = 1, bool PL = false>
 __global__ void __launch_bounds__(64 * MW)
 k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 const double* __restrict__ flv, Layout L, Params p, double tide,
-                double* __restrict__ partials, int nbx, int ntiles) {
+                double* __restrict__ partials, int nbx, int ntiles,
+                const uint8_t* __restrict__ plain) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     RcCarry k;
@@ -1274,28 +1403,11 @@ k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         x.vcol = x.i >= 1 && x.i <= L.jpi;
         const int jb = by * NJ;
         const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
-        Col5F R0, R1, R2, R3;
-        load_col3(R0, n, fl, x.rowb(jb - 1) + x.lb8);
-        load_col3(R1, n, fl, x.rowb(jb) + x.lb8);
-        load_col3(R2, n, fl, x.rowb(jb + 1) + x.lb8);
-        R0.su = rc_su<FD>(sc_, x.ucls_at(R0.f, jb - 1), R0.ss, lane_next(R0.ss), n.sshn_u,
-                          x.rowb(jb - 1) + x.lb8);
-        R1.su = rc_su<FD>(sc_, x.ucls_at(R1.f, jb), R1.ss, lane_next(R1.ss), n.sshn_u,
-                          x.rowb(jb) + x.lb8);
-        R0.sv = rc_sv<FD>(sc_, x.vcls_at(R0.f, jb - 1), R0.ss, R1.ss, n.sshn_v,
-                          x.rowb(jb - 1) + x.lb8);
-        k.south = north_terms1<FD>(sc_, R0, lane_next(R0.vn), lane_next(R0.sv), R1);
-        k.un_w = lane_prev(R1.un);
-        k.su_w = lane_prev(R1.su);
-        for (int j = jb; j <= je; j += 4) {
-            rc_row<FD, BAR>(x, k, R0, R1, R2, R3, j);
-            if (j + 1 > je) break;
-            rc_row<FD, BAR>(x, k, R1, R2, R3, R0, j + 1);
-            if (j + 2 > je) break;
-            rc_row<FD, BAR>(x, k, R2, R3, R0, R1, j + 2);
-            if (j + 3 > je) break;
-            rc_row<FD, BAR>(x, k, R3, R0, R1, R2, j + 3);
-        }
+        const int nseg = ntiles / nbx;
+        if (PL && plain[(long)(bx * MW + wave) * nseg + by])
+            rc_march<DM, BAR, LA, true>(x, k, jb, je);
+        else
+            rc_march<DM, BAR, LA, false>(x, k, jb, je);
     }
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
@@ -2047,6 +2159,31 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
         if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync(m->flags, fl.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, m->stream));
+    {
+        // plain tiles of k_step_march_rc (strips of ROUT columns with lanes
+        // at c0-1 .. c0+62, segments of RC_NJ rows with rows jb-1 .. je+1+RC_LA
+        // loaded): every class word there equal to RC_PLAIN
+        const int nbx = (jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
+        const int nstrip = nbx * MW, nseg = (jpj + 1 + RC_NJ - 1) / RC_NJ;
+        std::vector<uint8_t> tab((size_t)nstrip * nseg, 0);
+        for (int w = 0; w < nstrip; ++w) {
+            const int i0 = w * ROUT - 1, i1 = w * ROUT + 62;
+            if (i0 < 0 || i1 > jpi + 1) continue;
+            for (int q = 0; q < nseg; ++q) {
+                const int jb = q * RC_NJ, je = jb + RC_NJ - 1 < jpj ? jb + RC_NJ - 1 : jpj;
+                const int j0 = jb - 1, j1 = je + 1 + RC_LA;
+                if (j0 < 0 || j1 > jpj + 1) continue;
+                bool ok = true;
+                for (int j = j0; j <= j1 && ok; ++j)
+                    for (int i = i0; i <= i1 && ok; ++i) ok = fl[L.at(i, j)] == RC_PLAIN;
+                tab[(size_t)w * nseg + q] = ok ? 1 : 0;
+            }
+        }
+        dfree(m->rc_plain);
+        m->rc_plain = nullptr;
+        HIPCHK(hipMalloc((void**)&m->rc_plain, tab.size()));
+        HIPCHK(hipMemcpyAsync(m->rc_plain, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream));
+    }
     HIPCHK(hipMemcpyAsync(m->flu, hflu.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(m->flv, hflv.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
@@ -2332,10 +2469,12 @@ enum {
     VAR_MARCH12_DPF = 19,   //   the same dividing by precomputed reciprocals
     VAR_MARCH12_DPB = 20,   // 18 and 19 with the block's waves in step per row
     VAR_MARCH12_DPFB = 21,
-    VAR_MARCH12_RC = 22,    // reads sshn, un, vn only (sshn_u/v recomputed), in step per row
-    VAR_MARCH12_RCF = 23,   //   the same dividing by precomputed reciprocals
-    VAR_MARCH12_RCFN = 24,  //   the same without the per-row barrier
-    VAR_LAST = 24
+    VAR_MARCH12_RC = 22,     // reads sshn, un, vn only (sshn_u/v recomputed), in step per row
+    VAR_MARCH12_RC2N = 23,   //   loads two rows ahead, no per-row step-keeping
+    VAR_MARCH12_RCG2N = 24,  //   and reciprocal divisions under one guard per row
+    VAR_MARCH12_RC2NP = 25,  // 23 and 24 with class-free code on plain interior tiles
+    VAR_MARCH12_RCG2NP = 26,
+    VAR_LAST = 26
 };
 
 template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
@@ -2390,17 +2529,18 @@ static int launch_march_dp(nemo2d_model* m, const NState& n, const AOut& a, doub
     return NEMO2D_OK;
 }
 
-template <int NJ, bool FD, bool BAR>
+template <int NJ, int DM, bool BAR, int LA = 1, bool PL = false>
 static int launch_march_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (FD && !m->scal.fast_div) return launch_march_rc<NJ, false, BAR>(m, n, a, tide, nparts);
+    if (DM != 0 && !m->scal.fast_div) return launch_march_rc<NJ, 0, BAR, LA, PL>(m, n, a, tide, nparts);
+    static_assert(!PL || (NJ == RC_NJ && LA <= RC_LA), "the plain-tile table's tiling");
     const int nbx = (L.jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
     const int nby = (L.jpj + 1 + NJ - 1) / NJ;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_rc<NJ, FD, BAR><<<nblocks, 64 * MW, 0, m->stream>>>(
-        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
+    k_step_march_rc<NJ, DM, BAR, LA, PL><<<nblocks, 64 * MW, 0, m->stream>>>(
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles, m->rc_plain);
     LAUNCHCHK("step_march_rc");
     nparts = nblocks;
     return NEMO2D_OK;
@@ -2544,16 +2684,16 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO)
-        var = m->uniform ? VAR_MARCH12_RC : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
+        var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
     const bool depth_var = var == VAR_MARCH12_SHD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
     // the kernels that recompute sshn_u/sshn_v need them to be sshn's
     // interpolations (true after any step) and no Flather points off row 0
-    const bool rc_var = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCFN;
+    const bool rc_var = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP;
     if (rc_var && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
-    const bool rc_kernel = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCFN;
+    const bool rc_kernel = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP;
     if (!rc_kernel && m->uv_stale[k]) {
         const int urc = uv_sync(m);
         if (urc) return urc;
@@ -2592,9 +2732,11 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         case VAR_MARCH12_DPF: rc = launch_march_dp<12, true, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPFB: rc = launch_march_dp<12, true, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RC: rc = launch_march_rc<12, false, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RCF: rc = launch_march_rc<12, true, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RCFN: rc = launch_march_rc<12, true, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RC: rc = launch_march_rc<12, 0, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RC2N: rc = launch_march_rc<12, 0, false, 2>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RCG2N: rc = launch_march_rc<12, 2, false, 2>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RC2NP: rc = launch_march_rc<12, 0, false, 2, true>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RCG2NP: rc = launch_march_rc<12, 2, false, 2, true>(m, n, a, tide, nparts); break;
         default: rc = launch_march<32, false>(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
@@ -3054,7 +3196,7 @@ int nemo2d_destroy(nemo2d_model* m) {
         for (int f = 0; f < NF; ++f) dfree(m->S[k][f]);
     for (int s = 0; s < NS; ++s) dfree(m->st[s]);
     dfree(m->ssha); dfree(m->ua); dfree(m->va);
-    dfree(m->flags); dfree(m->flu); dfree(m->flv);
+    dfree(m->flags); dfree(m->flu); dfree(m->flv); dfree(m->rc_plain);
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
     dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log); dfree(m->cs_part); dfree(m->d_pend);
     for (int k = 0; k < 2; ++k)

@@ -173,6 +173,39 @@ __device__ __forceinline__ double div_rcp(double x, double d, double y) {
 }
 
 
+// The same quotient without the per-division guard, for code that
+// accumulates the guard over many divisions and redoes them all with the
+// IEEE division when it fails: the corrections are written q - (d*q - x)*y
+// (the same rounded values as q + (x - d*q)*y), which keeps the sign of a
+// zero x without a copysign.  DivCheck gathers the frexp exponents of the
+// numerators and whether one was NaN, infinite or subnormal; the quotients
+// are x / d's bits when div_check_ok holds (every numerator zero or in
+// [2^-960, 2^960), exponents -959..960).
+struct DivCheck {
+    int emin, emax;
+    bool odd;
+};
+__device__ __forceinline__ void div_check_reset(DivCheck& c) {
+    c.emin = 0;
+    c.emax = 0;
+    c.odd = false;
+}
+__device__ __forceinline__ double div_rcp_acc(double x, double d, double y, DivCheck& c) {
+    double q = x * y;
+    double r = fma(d, q, -x);
+    q = fma(-r, y, q);
+    r = fma(d, q, -x);
+    q = fma(-r, y, q);
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    c.emin = e < c.emin ? e : c.emin;
+    c.emax = e > c.emax ? e : c.emax;
+    c.odd = c.odd || __builtin_amdgcn_class(x, 0x297);  // NaN, infinity, subnormal
+    return q;
+}
+__device__ __forceinline__ bool div_check_ok(const DivCheck& c) {
+    return !c.odd && c.emin >= -959 && c.emax <= 960;
+}
+
 // Value of x held by lane l+1 of the wave, by a DPP wavefront shift (one
 // v_mov per 32-bit half, no LDS round trip).  Lane 63 receives 0 (bound
 // control on, every lane written, so no initial value is needed).  Every lane

@@ -47,7 +47,8 @@ VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "mar
             "march16_sh": 11, "march32_sh": 12, "march12_shn": 13, "march16_shn": 14,
             "march24_shn": 15, "march32_shn": 16, "march12_shd": 17,
             "march12_dp": 18, "march12_dpf": 19, "march12_dpb": 20, "march12_dpfb": 21,
-            "march12_rc": 22, "march12_rcf": 23, "march12_rcfn": 24}
+            "march12_rc": 22, "march12_rc2n": 23, "march12_rcg2n": 24,
+            "march12_rc2np": 25, "march12_rcg2np": 26}
 
 
 class Nemo2dError(RuntimeError):

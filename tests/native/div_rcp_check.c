@@ -1,7 +1,8 @@
 /* Host restatement of the device's division by a precomputed reciprocal
- * (div_rcp in nemolite2d-cuda_amd/csrc/nemo2d_core.h): q0 = RN(x*y), two FMA
- * remainder corrections, the quotient given x's sign, and the IEEE division
- * for |x| outside [2^-960, 2^960) other than zero.  Checked bit for bit
+ * (div_rcp and div_rcp_acc in nemolite2d-cuda_amd/csrc/nemo2d_core.h):
+ * q0 = RN(x*y), two FMA remainder corrections, the quotient given x's sign,
+ * and the IEEE division for |x| outside [2^-960, 2^960) other than zero; the
+ * unguarded form's quotient wherever its guard holds.  Checked bit for bit
  * against x / d on random operands over the whole exponent range, on
  * quotients within 2^-64 relative of a rounding midpoint (the hard cases of
  * correct rounding), and on zeros, subnormals, infinities and NaN.
@@ -25,6 +26,22 @@ static double div_rcp(double x, double d, double y) {
     return q;
 }
 
+/* The unguarded form (div_rcp_acc): corrections written q - (d*q - x)*y,
+ * exact when the numerator is zero or normal with frexp exponent -959..960;
+ * returns 1 and the quotient when that holds, else 0. */
+static int div_rcp_acc(double x, double d, double y, double* out) {
+    double q = x * y;
+    double r = fma(d, q, -x);
+    q = fma(-r, y, q);
+    r = fma(d, q, -x);
+    q = fma(-r, y, q);
+    *out = q;
+    if (isnan(x) || isinf(x) || (x != 0.0 && fabs(x) < 0x1p-1022)) return 0;
+    int e;
+    frexp(x, &e);
+    return e >= -959 && e <= 960;
+}
+
 static uint64_t s_state;
 static uint64_t rnd(void) { /* splitmix64 */
     uint64_t z = (s_state += 0x9e3779b97f4a7c15ull);
@@ -39,7 +56,7 @@ static double mk(int exp, uint64_t mant, int neg) {
     memcpy(&v, &b, 8);
     return v;
 }
-static long bad = 0, tried = 0;
+static long bad = 0, tried = 0, guarded = 0;
 static void check(double x, double d) {
     const double y = 1.0 / d;
     const double a = div_rcp(x, d, y), b = x / d;
@@ -47,6 +64,14 @@ static void check(double x, double d) {
     if (memcmp(&a, &b, 8) != 0 && !(isnan(a) && isnan(b))) {
         if (bad < 10) fprintf(stderr, "mismatch x=%a d=%a: %a vs %a\n", x, d, a, b);
         ++bad;
+    }
+    double c;
+    if (div_rcp_acc(x, d, y, &c)) {
+        ++guarded;
+        if (memcmp(&c, &b, 8) != 0) {
+            if (bad < 10) fprintf(stderr, "unguarded mismatch x=%a d=%a: %a vs %a\n", x, d, c, b);
+            ++bad;
+        }
     }
 }
 
@@ -82,6 +107,6 @@ int main(int argc, char** argv) {
                          INFINITY, -INFINITY, NAN, 1.0, -1.0, 3.0, 0x1.fffffffffffffp1023};
     for (int k = 0; k < (int)(sizeof sp / sizeof sp[0]); ++k)
         for (int j = 0; j < nf; ++j) check(sp[k], fixed[j]);
-    printf("%ld %ld\n", tried, bad);
+    printf("%ld %ld %ld\n", tried, bad, guarded);
     return 0;
 }

@@ -1,4 +1,5 @@
-"""The device's exact division by a precomputed reciprocal (div_rcp in
+"""The device's exact division by a precomputed reciprocal (div_rcp, and
+div_rcp_acc wherever its gathered guard holds, in
 nemolite2d-cuda_amd/csrc/nemo2d_core.h), restated on the host in
 tests/native/div_rcp_check.c, gives the bits of the IEEE quotient x / d on
 random, near-midpoint and special operands.  The reference divides with
@@ -20,6 +21,6 @@ def test_division_by_reciprocal_is_exact(tmp_path):
                            os.path.join(HERE, "native", "div_rcp_check.c"), "-lm"])
     for seed in (1, 2, 3):
         out = subprocess.check_output([exe, "400000", str(seed)], text=True).split()
-        tried, bad = int(out[0]), int(out[1])
-        assert tried > 2_000_000
+        tried, bad, guarded = int(out[0]), int(out[1]), int(out[2])
+        assert tried > 2_000_000 and guarded > tried // 2
         assert bad == 0, "%d of %d quotients differ from IEEE division" % (bad, tried)

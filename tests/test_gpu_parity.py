@@ -407,7 +407,7 @@ def test_per_point_depths_bitwise_vs_oracle(variant, islands):
         assert np.all(np.isfinite(m.checksum_log(25)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcf"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2n", "march12_rc2np"])
 @pytest.mark.parametrize("islands", [False, True])
 def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
     """The fused kernel that reads sshn, un and vn only (sshn_u and sshn_v
@@ -442,6 +442,26 @@ def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
     with model_like(o) as m:
         m.set_variant(variant)
         for istp in range(1, 6):
+            m.step(istp)
+            o.step(istp)
+            assert_fields_equal(m, o, "%s step %d" % (variant, istp))
+
+
+@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_dpf"])
+def test_tiny_values_take_the_ieee_divisions(variant):
+    """Numerators below 2^-960 (as in the subnormal band that trails the wave
+    front of a run from rest) make the kernels that divide by reciprocals
+    fall back to IEEE division (per division, or by redoing the row): a state
+    full of such values, and of zeros, must still match the oracle bit for
+    bit."""
+    o = Oracle(300, 70)
+    rng = np.random.default_rng(9)
+    for k, scale in (("sshn", 1e-300), ("un", 1e-305), ("vn", 1e-310)):
+        a = o.arr(k)
+        a[:] = rng.standard_normal(a.shape) * scale * (rng.random(a.shape) < 0.6)
+    with model_like(o) as m:
+        m.set_variant(variant)
+        for istp in range(1, 9):
             m.step(istp)
             o.step(istp)
             assert_fields_equal(m, o, "%s step %d" % (variant, istp))
