@@ -80,7 +80,7 @@ def form_fp32(n, steps, seed=1):
             "energy": {"fp64": out["fp64"][2][1], "fp32": out["fp32"][2][1]}}
 
 
-def form_ensemble(nmem, steps, seed=7):
+def form_ensemble(nmem, steps, seed=7, graph=True):
     """Batched basins: one persistent kernel per step, graph-captured loop,
     autotuned tiling; FP64 and FP32 on the same members."""
     import nemo2d
@@ -90,7 +90,7 @@ def form_ensemble(nmem, steps, seed=7):
     for prec in (64, 32):
         e = nemo2d.Ensemble(mem, rdt=12.0, precision=prec)
         choice, _ = e.tune(nsteps=10)
-        ms = e.run(1, steps, graph=True)
+        ms = e.run(1, steps, graph=graph)
         res[prec] = {"cell_updates_per_s": e.cells * steps / (ms * 1e-3), "ms": ms,
                      "tiling": nemo2d.Ensemble.tiling_shape(choice)}
         states[prec] = [e.retrieve(k) for k in range(0, nmem, max(1, nmem // 16))]
@@ -194,6 +194,17 @@ def load_traffic(variant, n):
     return d.get("bytes_per_launch") if d.get("variant") == v else None
 
 
+def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True):
+    out = {}
+    if "fp32_mixed" in which:
+        out["fp32_mixed"] = form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc)))
+    if "ensemble" in which:
+        out["ensemble"] = form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc)), graph=graph)
+    if "closure" in which:
+        out["closure"] = form_closure(max(32, int(256 * sc)), 4, 16, 300, 20)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +224,12 @@ def main():
     ap.add_argument("--forms-scale", type=float, default=1.0,
                     help="scale of the forms' workloads (1.0: 8192^2 x 5000 FP32, 256-basin "
                          "ensemble x 3000, closure at 256^2)")
+    ap.add_argument("--form", choices=("fp32_mixed", "ensemble", "closure"),
+                    help="run only this form (at --forms-scale) and print its JSON line; "
+                         "for per-form kernel traces")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="with --form ensemble: launch the steps one by one instead of "
+                         "replaying the captured graph")
     ap.add_argument("--sweep", action="store_true",
                     help="time every fused-step variant (200 steps each) and report them "
                          "under 'variants'")
@@ -222,6 +239,11 @@ def main():
               "multi-GPU decomposition is a design, not an implementation" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.form:
+        print(json.dumps({args.form: run_forms(args.forms_scale, (args.form,),
+                                               graph=not args.no_graph)[args.form]}), flush=True)
+        return
 
     import nemo2d
     n, K, W = args.n, args.steps, args.warmup
@@ -315,12 +337,7 @@ def main():
     if variants:
         out["variants_ms_per_step"] = variants
     if not args.no_forms:
-        sc = args.forms_scale
-        out["forms"] = {
-            "fp32_mixed": form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
-            "ensemble": form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc))),
-            "closure": form_closure(max(32, int(256 * sc)), 4, 16, 300, 20),
-        }
+        out["forms"] = run_forms(args.forms_scale)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     else:

@@ -185,21 +185,35 @@ __global__ void k_conv_wsum(const float* __restrict__ work, int nchunks, int cin
     dw[((long)tap * cin + ci) * cout + co] = s;
 }
 
-__global__ void k_bias_grad(const uint16_t* __restrict__ dz, long P, int coutp, int cout,
-                            float* __restrict__ db) {
-    // one block; thread t sums channel t%32 over its strided pixel set
+// Bias gradient, pass 1: block b sums each channel (< 32) over pixels
+// [b*WCHUNK, (b+1)*WCHUNK); thread (g, c) takes pixels g, g+8, ... of the
+// chunk, so a row of the block reads 8 consecutive pixels' channels at once.
+__global__ void __launch_bounds__(NB)
+k_bias_grad_part(const uint16_t* __restrict__ dz, long P, int coutp, int cout,
+                 float* __restrict__ part) {
     __shared__ float red[NB];
-    const int co = threadIdx.x & 31, lanegrp = threadIdx.x >> 5;  // 8 groups
+    const int co = threadIdx.x & 31, g = threadIdx.x >> 5;  // NB/32 pixel groups
+    const long p0 = (long)blockIdx.x * WCHUNK;
+    const long p1 = p0 + WCHUNK < P ? p0 + WCHUNK : P;
     float s = 0.f;
     if (co < cout)
-        for (long p = lanegrp; p < P; p += NB / 32) s += bf2f(dz[p * coutp + co]);
+        for (long p = p0 + g; p < p1; p += NB / 32) s += bf2f(dz[p * coutp + co]);
     red[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x < 32 && co < cout) {
+    if (threadIdx.x < 32) {
         float t = 0.f;
-        for (int g = 0; g < NB / 32; ++g) t += red[g * 32 + co];
-        db[co] = t;
+        for (int q = 0; q < NB / 32; ++q) t += red[q * 32 + co];
+        part[(long)blockIdx.x * 32 + co] = t;
     }
+}
+// pass 2: the chunks' sums in chunk order
+__global__ void k_bias_grad_sum(const float* __restrict__ part, int nchunks, int cout,
+                                float* __restrict__ db) {
+    const int co = threadIdx.x;
+    if (co >= cout) return;
+    float t = 0.f;
+    for (int b = 0; b < nchunks; ++b) t += part[(long)b * 32 + co];
+    db[co] = t;
 }
 
 __global__ void k_gelu_bwd(const float* __restrict__ dact, const float* __restrict__ z, long count,
@@ -325,8 +339,12 @@ int nemo2d_nn_conv_bwd_weight(const void* x, const void* dz, int n, int h, int w
     if (rc) return rc;
     k_conv_wsum<<<(9 * 1024 + 255) / 256, 256, 0, s>>>(work, nchunks, cin, cout, dw);
     if ((rc = launched("conv_wsum"))) return rc;
-    k_bias_grad<<<1, NB, 0, s>>>((const uint16_t*)dz, P, pad8(cout), cout, db);
-    return launched("bias_grad");
+    // the weight partials are consumed; the work buffer takes the bias
+    // partials (nchunks x 32 floats, within its nchunks x 9 x 1024)
+    k_bias_grad_part<<<nchunks, NB, 0, s>>>((const uint16_t*)dz, P, pad8(cout), cout, work);
+    if ((rc = launched("bias_grad_part"))) return rc;
+    k_bias_grad_sum<<<1, 32, 0, s>>>(work, nchunks, cout, db);
+    return launched("bias_grad_sum");
 }
 
 int nemo2d_nn_gelu_bwd(const float* dact, const float* z, long count, void* dz, void* stream) {
