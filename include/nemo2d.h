@@ -130,13 +130,16 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
  * ahead and no step-keeping, 24 as 23 dividing by reciprocals under one
  * guard per row (the row is done again with IEEE divisions when any
  * numerator is outside the exact range), 25/26 as 23/24 running class-free
- * code on the tiles whose points are all plain interior points.
+ * code on the tiles whose points are all plain interior points, 27 as 17
+ * reading sshn, un, vn and the depths only (sshn_u/sshn_v recomputed, plain
+ * tiles class-free; 17 runs where 27 cannot, as for 22).
  * Marching needs uniform metrics, 2-16 and 18-26 also uniform depths, and
- * 17 Flather points only on row 0 with the inner point north; other grids
- * use 1 (or 6).  The default for grids with uniform metrics but varying
- * depths is 17 where it applies.  The single-precision form marches (as 17)
- * where it applies unless 1 is chosen.  No counterpart in the reference: a
- * tuning hook. */
+ * 17 and 27 Flather points only on row 0 with the inner point north; other
+ * grids use 1 (or 6).  The default for grids with uniform metrics but
+ * varying depths is 27 where it applies.  The single-precision form marches
+ * where it applies (reading sshn, un, vn only, as 27, once a step has made
+ * sshn_u/sshn_v sshn's interpolations; 17 chooses its five-array kernel)
+ * unless 1 is chosen.  No counterpart in the reference: a tuning hook. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
 
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the

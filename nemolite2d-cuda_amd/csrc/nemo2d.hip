@@ -1636,6 +1636,26 @@ k_interp(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn,
     }
 }
 
+// The same for the single-precision form's state (FP32 sshn_u/sshn_v from an
+// FP64 or FP32 sshn, FP32 arithmetic and areas, as its step forms them).
+template <class SS>
+__global__ void __launch_bounds__(TBX * TBY)
+k_interp_mixed(const uint16_t* __restrict__ fl, const SS* __restrict__ sshn, Layout L, float e12t,
+               float e12u, float e12v, float* __restrict__ sshn_u, float* __restrict__ sshn_v) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j), P = L.pitch;
+    const unsigned f = fl[o];
+    if (j >= 1) {
+        const unsigned c = sucls(f);
+        if (c != NX_SKIP) sshn_u[o] = next_g<float>(c, (float)sshn[o], (float)sshn[o + 1], 0.f, e12t, e12u);
+    }
+    if (i >= 1) {
+        const unsigned c = svcls(f);
+        if (c != NX_SKIP) sshn_v[o] = next_g<float>(c, (float)sshn[o], (float)sshn[o + P], 0.f, e12t, e12v);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -1677,13 +1697,28 @@ StArrays starrays(const nemo2d_model* m) {
     } while (0)
 
 // Fill in sshn_u/sshn_v of the current state set when the last fused step
-// left them unwritten (k_step_march_rc): next's interpolation of its sshn.
+// left them unwritten (k_step_march_rc, k_step_march_gr): next's
+// interpolation of its sshn, in the state's own arithmetic.
 int uv_sync(nemo2d_model* m) {
-    if (m->mixed || !m->uv_stale[m->cur]) return NEMO2D_OK;
+    if (!m->uv_stale[m->cur]) return NEMO2D_OK;
     const Layout& L = m->L;
-    double** s = m->S[m->cur];
-    WITH_STATICS(m, st, (k_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
+    const dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
+    if (m->mixed) {
+        void** x = m->X[m->cur];
+        const Scal& d = m->scal;
+        if (m->compensated)
+            k_interp_mixed<double><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                m->flags, (const double*)x[F_SSHN], L, (float)d.e12t, (float)d.e12u, (float)d.e12v,
+                (float*)x[F_SSHN_U], (float*)x[F_SSHN_V]);
+        else
+            k_interp_mixed<float><<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                m->flags, (const float*)x[F_SSHN], L, (float)d.e12t, (float)d.e12u, (float)d.e12v,
+                (float*)x[F_SSHN_U], (float*)x[F_SSHN_V]);
+    } else {
+        double** s = m->S[m->cur];
+        WITH_STATICS(m, st, (k_interp<<<g, dim3(TBX, TBY), 0, m->stream>>>(
+                                st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
+    }
     LAUNCHCHK("interp");
     m->uv_stale[m->cur] = false;
     return NEMO2D_OK;
@@ -2255,6 +2290,8 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
         for (int f = 0; f < NF; ++f)
             HIPCHK(hipMemcpyAsync(m->X[1][f], m->X[0][f], nb[f], hipMemcpyDeviceToDevice, m->stream));
         HIPCHK(hipStreamSynchronize(m->stream));
+        m->uv_stale[0] = m->uv_stale[1] = false;
+        m->uv_derived = false;
         m->a_current = false;
         m->state_set = true;
         m->cs_next = 0;
@@ -2474,7 +2511,8 @@ enum {
     VAR_MARCH12_RCG2N = 24,  //   and reciprocal divisions under one guard per row
     VAR_MARCH12_RC2NP = 25,  // 23 and 24 with class-free code on plain interior tiles
     VAR_MARCH12_RCG2NP = 26,
-    VAR_LAST = 26
+    VAR_MARCH12_GRD = 27,    // 17 reading sshn, un, vn (and depths) only, plain tiles
+    VAR_LAST = 27
 };
 
 template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
@@ -2575,6 +2613,35 @@ static MetG<double> metg(const Scal& d) {
 // FP64 fused step by the generic marching kernel with per-point depths
 // (within 168 VGPRs: 3 waves per SIMD; measured 3% faster than 2 waves at
 // 170 VGPRs despite a 12-byte spill)
+// The generic step that reads sshn, un, vn (and depths) only; the
+// plain-tile table of k_step_march_rc serves it (same strips, segments and
+// rows ahead).
+template <class R, class SS, bool DARR, int NJ, int LA>
+static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& p,
+                           const StateG<R, SS, DARR>& n, const OutG<R, SS>& a, double tide,
+                           int& nparts) {
+    static_assert(GROUT == ROUT && GW == MW && NJ == RC_NJ && LA <= RC_LA, "the plain-tile table");
+    const Layout& L = m->L;
+    const int nbx = (L.jpi + 1 + GW * GROUT - 1) / (GW * GROUT);
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int ntiles = nbx * nby;
+    const int nblocks = (ntiles + 7) / 8 * 8;
+    if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
+    k_step_march_gr<R, SS, DARR, NJ, LA><<<nblocks, 64 * GW, 0, m->stream>>>(
+        mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, m->rc_plain);
+    LAUNCHCHK("step_march_gr");
+    nparts = nblocks;
+    return NEMO2D_OK;
+}
+
+static int launch_march_depths_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide,
+                                  int& nparts) {
+    StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
+                                    m->st[S_HV]};
+    OutG<double, double> ag{a.sshn, a.sshn_u, a.sshn_v, a.un, a.vn};
+    return launch_march_gr<double, double, true, 12, 2>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
+}
+
 static int launch_march_depths(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
                                     m->st[S_HV]};
@@ -2610,6 +2677,13 @@ static int step_mixed(nemo2d_model* m, int istp) {
     void** x = m->X[k];
     void** y = m->X[1 - k];
     if (m->variant != VAR_CELL && flather_row0_only(m) && sh_fits(L)) {
+        // the kernel that reads sshn, un, vn only once sshn_u/sshn_v are sshn's
+        // interpolations (after any step)
+        const bool gr = m->uv_derived && m->variant != VAR_MARCH12_SHD;
+        if (!gr) {
+            const int urc = uv_sync(m);
+            if (urc) return urc;
+        }
         const MetG<float> mt{sf.e1t, sf.e2t, sf.e1u, sf.e2u, sf.e1v, sf.e2v, sf.e12t, sf.e12u, sf.e12v,
                              sf.fcu, sf.fcv, sf.ht, sf.hu, sf.hv};
         int nparts = 0, rc;
@@ -2620,16 +2694,20 @@ static int step_mixed(nemo2d_model* m, int istp) {
                                           (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
             OutG<float, double> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                   (float*)y[F_UN], (float*)y[F_VN]};
-            rc = launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts);
+            rc = gr ? launch_march_gr<float, double, true, 12, 2>(m, mt, pf, n, a, tide, nparts)
+                    : launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts);
         } else {
             StateG<float, float, true> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U],
                                          (const float*)x[F_SSHN_V], (const float*)x[F_UN],
                                          (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
             OutG<float, float> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                  (float*)y[F_UN], (float*)y[F_VN]};
-            rc = launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts);
+            rc = gr ? launch_march_gr<float, float, true, 12, 2>(m, mt, pf, n, a, tide, nparts)
+                    : launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts);
         }
         if (rc) return rc;
+        m->uv_stale[1 - k] = gr;
+        m->uv_derived = true;
         m->cur = 1 - k;
         const int slot = istp - m->nit000;
         if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
@@ -2639,6 +2717,10 @@ static int step_mixed(nemo2d_model* m, int istp) {
             m->cs_next = slot + 1;
         }
         return NEMO2D_OK;
+    }
+    {
+        const int urc = uv_sync(m);
+        if (urc) return urc;
     }
     if (m->compensated) {
         MixedState<true> n{(const double*)x[F_SSHN], (const float*)x[F_SSHN_U], (const float*)x[F_SSHN_V],
@@ -2658,6 +2740,8 @@ static int step_mixed(nemo2d_model* m, int istp) {
                                                                  m->partials);
     }
     LAUNCHCHK("step_mixed");
+    m->uv_stale[1 - k] = false;
+    m->uv_derived = true;
     m->cur = 1 - k;
     const int slot = istp - m->nit000;
     if (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) {
@@ -2684,8 +2768,8 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO)
-        var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_SHD : VAR_CELL);
-    const bool depth_var = var == VAR_MARCH12_SHD;
+        var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
+    const bool depth_var = var == VAR_MARCH12_SHD || var == VAR_MARCH12_GRD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
@@ -2693,7 +2777,8 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     // interpolations (true after any step) and no Flather points off row 0
     const bool rc_var = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP;
     if (rc_var && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
-    const bool rc_kernel = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP;
+    if (var == VAR_MARCH12_GRD && !m->uv_derived) var = VAR_MARCH12_SHD;
+    const bool rc_kernel = (var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP) || var == VAR_MARCH12_GRD;
     if (!rc_kernel && m->uv_stale[k]) {
         const int urc = uv_sync(m);
         if (urc) return urc;
@@ -2728,6 +2813,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
         case VAR_MARCH24_SHN: rc = launch_march_sh<24, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH32_SHN: rc = launch_march_sh<32, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_SHD: rc = launch_march_depths(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_GRD: rc = launch_march_depths_rc(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DP: rc = launch_march_dp<12, false, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPF: rc = launch_march_dp<12, true, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
@@ -2746,7 +2832,7 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     m->a_current = false;
     // the shared-term kernels set the row-0 Flather v points themselves
     const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L) &&
-                    (var != VAR_MARCH12_SHD || depth_march_ok);
+                    ((var != VAR_MARCH12_SHD && var != VAR_MARCH12_GRD) || depth_march_ok);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && has_fl) {
         // Flather points of the new state, then (if recording) the checksums
@@ -2992,6 +3078,8 @@ int nemo2d_retrieve_aux(nemo2d_model* m, double* sshn_u, double* sshn_v, double*
     READY(m, "nemo2d_retrieve_aux");
     const int jpi = m->L.jpi, jpj = m->L.jpj;
     if (m->mixed) {
+        const int urc = uv_sync(m);
+        if (urc) return urc;
         void** x = m->X[m->cur];
         int rc = NEMO2D_OK;
         if (sshn_u && (rc = download_as<float>(m, sshn_u, (const float*)x[F_SSHN_U], 0, jpi, 1, jpj))) return rc;

@@ -222,6 +222,10 @@ __device__ __forceinline__ double lane_next(double x) {
 }
 // Value of x held by lane l-1 (DPP wavefront shift the other way); lane 0
 // receives 0.  Every lane of the wave must be active.
+__device__ __forceinline__ float lane_prev(float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, true));
+}
 __device__ __forceinline__ double lane_prev(double x) {
     int2 v = __builtin_bit_cast(int2, x);
     v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xf, 0xf, true);

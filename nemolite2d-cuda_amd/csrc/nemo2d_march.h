@@ -378,6 +378,331 @@ k_step_march_g(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
 
+// ---- the same step reading sshn, un, vn (and the depths) only ----------------
+// As k_step_march_rc of the FP64 drop-in, generic like the kernel above:
+// sshn_u and sshn_v of the now-state are formed from sshn with next's own
+// expression in R arithmetic (next_g of R(sshn) values, as next formed the
+// stored ones), the stored value is read only where next leaves a point
+// alone, and only sshn, un and vn are written; the library fills the new
+// state's sshn_u/sshn_v in (the same interpolation) before anything reads
+// them.  Each lane loads its own column; neighbours come by DPP lane
+// shifts; lanes 1..GROUT own the columns (lane 0 and lanes GROUT+1..63 are
+// halos); the ring of LA+3 rows keeps the next LA rows' loads in flight;
+// with PLAIN (a tile whose every point has the plain interior class word)
+// the class tests drop out.
+constexpr int GROUT = 61;
+constexpr unsigned G_PLAIN = (CLS_COMP << U_CLS_SHIFT) | (CLS_COMP << V_CLS_SHIFT) |
+                             (NX_AVG << SU_SHIFT) | (NX_AVG << SV_SHIFT);
+
+template <class R, class SS>
+struct ColG {
+    R un, vn, su, sv;
+    SS ss;
+    R ht, hu, hv;  // per-point depths (DARR)
+    unsigned f;
+};
+
+template <class R, class SS, bool DARR, bool PLAIN>
+__device__ __forceinline__ void load_colg(ColG<R, SS>& r, const StateG<R, SS, DARR>& n,
+                                          const uint16_t* fl, unsigned k) {
+    r.un = ldg_(n.un, k);
+    r.vn = ldg_(n.vn, k);
+    r.ss = ldg_(n.sshn, k);
+    if constexpr (DARR) {
+        r.ht = ldg_(n.ht, k);
+        r.hu = ldg_(n.hu, k);
+        r.hv = ldg_(n.hv, k);
+    }
+    if constexpr (PLAIN) r.f = G_PLAIN;
+    else r.f = ldg_(fl, k);
+}
+
+// the stored value where next leaves a point alone (wait inside the branch)
+template <class R>
+__device__ __forceinline__ R keep_g(R v, bool keep, const R* arr, unsigned k) {
+    if (keep) {
+        v = ldg_(arr, k);
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), other counters untouched
+    }
+    return v;
+}
+
+template <class R, class SS, bool DARR>
+struct CtxGR : CtxG<R, SS, DARR> {
+    bool ucol, vcol;
+    __device__ unsigned ucls_at(unsigned f, int r) const {
+        return (ucol && r >= 1 && r <= this->L.jpj) ? sucls(f) : NX_SKIP;
+    }
+    __device__ unsigned vcls_at(unsigned f, int r) const {
+        return (vcol && r >= 0 && r <= this->L.jpj) ? svcls(f) : NX_SKIP;
+    }
+};
+
+template <class R>
+struct CarryGR {
+    NorthG<R> south;
+    R un_w, su_w, hu_w;  // un, sshn_u, hu one column west, this row
+    double ct, cu, cv;
+};
+
+template <class R, class SS, bool DARR, bool PLAIN>
+__device__ __forceinline__ R su_at_g(const CtxGR<R, SS, DARR>& x, const ColG<R, SS>& c, SS ss_e,
+                                     int r, unsigned k) {
+    const unsigned cls = PLAIN ? NX_AVG : x.ucls_at(c.f, r);
+    const R v = next_g<R>(cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_e), R(0), x.m.e12t, x.m.e12u);
+    return PLAIN ? v : keep_g(v, cls == NX_SKIP, x.n.sshn_u, k);
+}
+template <class R, class SS, bool DARR, bool PLAIN>
+__device__ __forceinline__ R sv_at_g(const CtxGR<R, SS, DARR>& x, const ColG<R, SS>& c, SS ss_n,
+                                     int r, unsigned k) {
+    const unsigned cls = PLAIN ? NX_AVG : x.vcls_at(c.f, r);
+    const R v = next_g<R>(cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_n), R(0), x.m.e12t, x.m.e12v);
+    return PLAIN ? v : keep_g(v, cls == NX_SKIP, x.n.sshn_v, k);
+}
+
+// north terms of row c (own column; vn, sshn_v and hv one column east) with
+// row nr above
+template <class R, class SS, bool DARR>
+__device__ __forceinline__ NorthG<R> north_gc(const MetG<R>& m, const ColG<R, SS>& c, R vn_e,
+                                              R sv_e, R hv_e, const ColG<R, SS>& nr) {
+    const R h = R(0.5);
+    const R hv_c = DARR ? c.hv : m.hv, ht_n = DARR ? nr.ht : m.ht;
+    NorthG<R> t;
+    t.r3 = (c.sv + hv_c) * c.vn;
+    t.vnc = h * (c.vn + vn_e);
+    t.vnu = h * t.vnc * (m.e1v + m.e1v);
+    t.depnu = h * (hv_c + c.sv + (DARR ? hv_e : m.hv) + sv_e);
+    const R sn = sgn_half(t.vnu);
+    t.kn = ((h + sn) * c.un + (h - sn) * nr.un) * t.vnu * t.depnu;
+    const R vn_n = h * (c.vn + nr.vn) * m.e1t;
+    const R depn = ht_n + R(nr.ss);
+    const R sv = sgn_half(vn_n);
+    t.hn = ((h + sv) * c.vn + (h - sv) * nr.vn) * vn_n * depn;
+    t.dvdyn = dv(nr.vn - c.vn, m.e2t) * (ht_n + R(nr.ss));
+    t.qdudy = dv(nr.un - c.un, m.e2u + m.e2u);
+    return t;
+}
+
+// One row j: A, B, C hold rows j-1, j, j+1 (C just arrived: its sshn_u is
+// formed here, and B's sshn_v); D receives row j+1+LA.
+template <class R, class SS, bool DARR, int LA, bool PLAIN>
+__device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k,
+                                        const ColG<R, SS>& A, ColG<R, SS>& B, ColG<R, SS>& C,
+                                        ColG<R, SS>& D, int j) {
+    const Layout& L = x.L;
+    const MetG<R>& m = x.m;
+    const R h = R(0.5);
+    const unsigned kP = (unsigned)L.pitch;
+    {
+        const int jn = j + 1 + LA <= L.jpj + 2 ? j + 1 + LA : L.jpj + 2;
+        load_colg<R, SS, DARR, PLAIN>(D, x.n, x.fl, x.rowk(jn) + x.lk);
+    }
+    const unsigned f = B.f;
+    const unsigned ko = x.rowk(j) + x.lk;  // element offset of (i, j)
+    C.su = su_at_g<R, SS, DARR, PLAIN>(x, C, lane_next(C.ss), j + 1, ko + kP);
+    B.sv = sv_at_g<R, SS, DARR, PLAIN>(x, B, C.ss, j, ko);
+    // neighbours by lane shifts (every lane active here)
+    const R bun_w = k.un_w, bsu_w = k.su_w;
+    const R bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
+    const R bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv);
+    const SS bss_e = lane_next(B.ss);
+    const R cun_w = lane_prev(C.un), csu_w = lane_prev(C.su);
+    const R hu_b = DARR ? B.hu : m.hu, hv_b = DARR ? B.hv : m.hv, ht_b = DARR ? B.ht : m.ht;
+    const R hu_a = DARR ? A.hu : m.hu, hu_c = DARR ? C.hu : m.hu;
+    const R bhu_w = DARR ? k.hu_w : m.hu;
+    const R chu_w = DARR ? lane_prev(C.hu) : m.hu;
+    const R bhv_w = DARR ? lane_prev(B.hv) : m.hv, bhv_e = DARR ? lane_next(B.hv) : m.hv;
+    const NorthG<R> nt = north_gc<R, SS, DARR>(m, B, bvn_e, bsv_e, bhv_e, C);
+    const NorthG<R>& st = k.south;
+
+    // continuity
+    SS s_c = SS(0);
+    if (PLAIN || (j >= 1 && x.tcol)) {
+        if (!PLAIN && (f & T_CLAMP)) {
+            s_c = SS(x.tide);
+        } else {
+            const R r1 = (B.su + hu_b) * B.un;
+            const R r2 = (bsu_w + bhu_w) * bun_w;
+            s_c = B.ss + SS(dv((r2 - r1 + st.r3 - nt.r3) * x.p.rdt, m.e12t));
+        }
+    }
+
+    // west-face terms; the east ones come from lane+1
+    const R un_c = B.un;
+    const R u_w = h * (un_c + bun_w) * m.e2t;
+    const R depw_u = ht_b + R(B.ss);
+    const R sw_u = sgn_half(u_w);
+    const R fw = ((h - sw_u) * un_c + (h + sw_u) * bun_w) * u_w * depw_u;
+    const R dudx_w = dv(un_c - bun_w, m.e1t) * (ht_b + R(B.ss));
+    const R vn_c = B.vn;
+    const R u_wc = h * (bun_w + cun_w);
+    const R u_wv = h * u_wc * (m.e2u + m.e2u);
+    const R depw_v = h * (bhu_w + bsu_w + chu_w + csu_w);
+    const R sw_v = sgn_half(u_wv);
+    const R gw = ((h - sw_v) * vn_c + (h + sw_v) * bvn_w) * u_wv * depw_v;
+    const R qdvdx_w = dv(vn_c - bvn_w, m.e1v + m.e1v);
+    const R fe = lane_next(fw);
+    const R dudx_e = lane_next(dudx_w);
+    const R ge = lane_next(gw);
+    const R u_ec = lane_next(u_wc);
+    const R qdvdx_e = lane_next(qdvdx_w);
+
+    if (x.own) {
+        const int i = x.i;
+        R u = R(0), v = R(0);
+        const unsigned cu = ucls(f), cv = vcls(f);
+        if (PLAIN || (j >= 1 && cu == CLS_COMP)) {
+            const bool ws = !PLAIN && (f & U_WALL_S) != 0, wn = !PLAIN && (f & U_WALL_N) != 0;
+            R ts = st.kn, tn = nt.kn;
+            if (ws) ts = ((h - sgn_half(st.vnu)) * un_c) * st.vnu * st.depnu;
+            if (wn) tn = ((h + sgn_half(nt.vnu)) * un_c) * nt.vnu * nt.depnu;
+            const R adv = fw - fe + ts - tn;
+            R dudy_s = R(0), dudy_n = R(0);
+            if (!ws) dudy_s = st.qdudy * (hu_b + B.su + hu_a + A.su);
+            if (!wn) dudy_n = nt.qdudy * (hu_b + B.su + hu_c + C.su);
+            R vis = (dudx_e - dudx_w) * m.e2u + (dudy_n - dudy_s) * m.e1u * h;
+            vis = x.p.visc * vis;
+            const R hsu = hu_b + B.su;
+            const R cor = h * (m.fcu * (st.vnc + nt.vnc)) * m.e12u * hsu;
+            const R hpg = -R(GRAV) * hsu * m.e2u * (R(bss_e) - R(B.ss));
+            u = dv(dv(un_c * hsu + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12u), hu_b + R(0)),
+                   x.p.fric);
+        }
+        if (PLAIN || (i >= 1 && cv == CLS_COMP)) {
+            const bool ww = !PLAIN && (f & V_WALL_W) != 0, we = !PLAIN && (f & V_WALL_E) != 0;
+            R tw = gw, te = ge;
+            if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
+            if (we) {
+                const R u_e = h * u_ec * (m.e2u + m.e2u);
+                const R depe = h * (hu_b + B.su + hu_c + C.su);
+                te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
+            }
+            const R adv = tw - te + st.hn - nt.hn;
+            R dvdx_w = R(0), dvdx_e = R(0);
+            if (!ww) dvdx_w = qdvdx_w * (hv_b + B.sv + bhv_w + bsv_w);
+            if (!we) dvdx_e = qdvdx_e * (hv_b + B.sv + bhv_e + bsv_e);
+            R vis = (nt.dvdyn - st.dvdyn) * m.e1v + (dvdx_e - dvdx_w) * m.e2v * h;
+            vis = x.p.visc * vis;
+            const R hsv = hv_b + B.sv;
+            const R cor = -h * (m.fcv * (u_ec + u_wc)) * m.e12v * hsv;
+            const R hpg = -R(GRAV) * hsv * m.e1v * (R(C.ss) - R(B.ss));
+            v = dv(dv(vn_c * hsv + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12v), hv_b + R(0)),
+                   x.p.fric);
+        }
+        if (PLAIN || j >= 1) {
+            if (PLAIN || i >= 1) {
+                stg_(x.a.sshn, ko, s_c);
+                k.ct += fabs((double)s_c);
+            }
+            if (PLAIN || cu != CLS_FLATHER) {
+                stg_(x.a.un, ko, u);
+                if (PLAIN || i >= 1) k.cu += fabs((double)u);
+            }
+        }
+        if (PLAIN || i >= 1) {
+            if (PLAIN || cv != CLS_FLATHER) {
+                stg_(x.a.vn, ko, v);
+                if (PLAIN || j >= 1) k.cv += fabs((double)v);
+            }
+            // bc_flather_v at a row-0 point whose inner point is this one
+            // (row 0 is outside the checksums)
+            if (!PLAIN && j == 1 && vcls(A.f) == CLS_FLATHER && (A.f & V_FL_NORTH))
+                stg_(x.a.vn, ko - kP, v + R(ldg_(x.flv, ko - kP)) * (A.sv - B.sv));
+        }
+    }
+    k.south = nt;
+    k.un_w = cun_w;
+    k.su_w = csu_w;
+    k.hu_w = chu_w;
+}
+
+template <class R, class SS, bool DARR, int LA, bool PLAIN>
+__device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k, int jb,
+                                          int je) {
+    const Layout& L = x.L;
+    ColG<R, SS> R0, R1, R2, R3, R4;
+    load_colg<R, SS, DARR, PLAIN>(R0, x.n, x.fl, x.rowk(jb - 1) + x.lk);
+    load_colg<R, SS, DARR, PLAIN>(R1, x.n, x.fl, x.rowk(jb) + x.lk);
+    load_colg<R, SS, DARR, PLAIN>(R2, x.n, x.fl, x.rowk(jb + 1) + x.lk);
+    if constexpr (LA == 2)
+        load_colg<R, SS, DARR, PLAIN>(R3, x.n, x.fl,
+                                      x.rowk(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lk);
+    R0.su = su_at_g<R, SS, DARR, PLAIN>(x, R0, lane_next(R0.ss), jb - 1, x.rowk(jb - 1) + x.lk);
+    R1.su = su_at_g<R, SS, DARR, PLAIN>(x, R1, lane_next(R1.ss), jb, x.rowk(jb) + x.lk);
+    R0.sv = sv_at_g<R, SS, DARR, PLAIN>(x, R0, R1.ss, jb - 1, x.rowk(jb - 1) + x.lk);
+    k.south = north_gc<R, SS, DARR>(x.m, R0, lane_next(R0.vn), lane_next(R0.sv),
+                                    DARR ? lane_next(R0.hv) : x.m.hv, R1);
+    k.un_w = lane_prev(R1.un);
+    k.su_w = lane_prev(R1.su);
+    k.hu_w = DARR ? lane_prev(R1.hu) : x.m.hu;
+    if constexpr (LA == 1) {
+        for (int j = jb; j <= je; j += 4) {
+            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R0, R1, R2, R3, j);
+            if (j + 1 > je) break;
+            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R2, R3, R0, R1, j + 2);
+            if (j + 3 > je) break;
+            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R3, R0, R1, R2, j + 3);
+        }
+    } else {
+        for (int j = jb; j <= je; j += 5) {
+            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R0, R1, R2, R4, j);
+            if (j + 1 > je) break;
+            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            if (j + 2 > je) break;
+            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R2, R3, R4, R1, j + 2);
+            if (j + 3 > je) break;
+            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R3, R4, R0, R2, j + 3);
+            if (j + 4 > je) break;
+            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R4, R0, R1, R3, j + 4);
+        }
+    }
+}
+
+// plain: per (wave strip of GROUT columns, NJ-row segment) 1 where every
+// class word the wave touches (with LA rows loaded ahead) is G_PLAIN; null
+// for none
+template <class R, class SS, bool DARR, int NJ, int LA>
+__global__ void __launch_bounds__(64 * GW)
+k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
+                const uint16_t* __restrict__ fl, const double* __restrict__ flv, Layout L,
+                double tide, double* __restrict__ partials, int nbx, int ntiles,
+                const uint8_t* __restrict__ plain) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    CarryGR<R> k;
+    k.ct = k.cu = k.cv = 0.0;
+    if (t < ntiles) {
+        const int strip = (t % nbx) * GW + wave, seg = t / nbx;
+        CtxGR<R, SS, DARR> x;
+        x.m = m;
+        x.p = p;
+        x.n = n;
+        x.a = a;
+        x.fl = fl;
+        x.flv = flv;
+        x.L = L;
+        x.tide = tide;
+        // lane 0 is column i0 - 1 (>= -1, inside the left padding)
+        x.i = strip * GROUT + lane - 1;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
+        x.lk = (unsigned)x.il;                     // wraps for -1; sums stay in range
+        x.own = lane >= 1 && lane <= GROUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        x.ucol = x.i >= 0 && x.i <= L.jpi;
+        x.vcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = seg * NJ;
+        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        if (plain && plain[(long)strip * (ntiles / nbx) + seg])
+            rcg_march<R, SS, DARR, LA, true>(x, k, jb, je);
+        else
+            rcg_march<R, SS, DARR, LA, false>(x, k, jb, je);
+    }
+    block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
+}
+
 // Sums of the wave's lanes (fixed order), written by lane 0 to out[0..5].
 __device__ __forceinline__ void wave_reduce3(DD a0, DD a1, DD a2, double* out) {
     for (int mk = 32; mk >= 1; mk >>= 1) {

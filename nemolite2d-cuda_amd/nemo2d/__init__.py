@@ -48,7 +48,7 @@ VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "mar
             "march24_shn": 15, "march32_shn": 16, "march12_shd": 17,
             "march12_dp": 18, "march12_dpf": 19, "march12_dpb": 20, "march12_dpfb": 21,
             "march12_rc": 22, "march12_rc2n": 23, "march12_rcg2n": 24,
-            "march12_rc2np": 25, "march12_rcg2np": 26}
+            "march12_rc2np": 25, "march12_rcg2np": 26, "march12_grd": 27}
 
 
 class Nemo2dError(RuntimeError):
