@@ -218,7 +218,10 @@ int nemo2d_ensemble_tune(nemo2d_ensemble *e, int nsteps, int *choice, double *be
 int nemo2d_ensemble_set_tiling(nemo2d_ensemble *e, int choice);
 /* Work-item shape (columns x rows of cells) of tiling `choice`: choices
  * 0..7 are blocks of 256 cells, one per thread; choice 8 is one wave's
- * strip of 63 columns by 12 rows, column-marching with shared face terms. */
+ * strip of 63 columns by 12 rows, column-marching with shared face terms;
+ * choice 9 (the default) the same over strips of 61 columns reading sshn,
+ * un, vn only (sshn_u/sshn_v formed from sshn as next forms them, and filled
+ * in before a step of another choice), class-free on plain work items. */
 int nemo2d_ensemble_tiling(int choice, int *tx, int *ty);
 /* Steps nit000..nitend; use_graph != 0 captures the whole loop into a HIP
  * graph first (capture not timed) and replays it.  *ms = device time of the

@@ -51,16 +51,21 @@ struct EnsMember {
 };
 
 // Candidate tilings: TX columns x (EB/TX)*ROWS rows per work item of a
-// block, one point per thread; the last is column-marching: a work item is
-// one wave's strip of GOUT columns over EMJ rows (shared face terms).
+// block, one point per thread; the last two are column-marching: a work item
+// is one wave's strip of GOUT columns over EMJ rows (shared face terms), or
+// of GROUT columns read as sshn, un, vn only (sshn_u/sshn_v formed from
+// sshn, plain tiles class-free).
 struct Tiling {
     int tx, rows;
 };
 constexpr int EMJ = 12;  // rows per marching work item
 constexpr Tiling TILINGS[] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 1}, {32, 2},
-                              {GOUT, EMJ}};
+                              {GOUT, EMJ}, {GROUT, EMJ}};
 constexpr int NTILINGS = sizeof(TILINGS) / sizeof(TILINGS[0]);
-constexpr int MARCH_TILING = NTILINGS - 1;
+constexpr int MARCH_TILING = NTILINGS - 2;
+constexpr int MARCH_RC_TILING = NTILINGS - 1;
+constexpr int ERC_LA = 2;  // rows loaded ahead by the sshn-only marching item
+constexpr bool is_march(int t) { return t == MARCH_TILING || t == MARCH_RC_TILING; }
 
 template <int TX, int ROWS>
 __global__ void __launch_bounds__(EB)
@@ -154,6 +159,87 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
     }
 }
 
+// The same step reading sshn, un, vn only: a work item (member, strip,
+// segment, plain) marches one wave's strip of GROUT columns; the new state's
+// sshn_u/sshn_v are not written (k_ens_interp fills them in when a kernel
+// that reads them comes next).
+template <class R, class SS>
+__global__ void __launch_bounds__(EB)
+k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
+               StateG<R, SS, false> n0, OutG<R, SS> a0, const uint16_t* __restrict__ flp,
+               const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
+               int tide_stride, int step_idx, double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    for (int w = blockIdx.x * (EB / 64) + (threadIdx.x >> 6); w < nitems; w += gridDim.x * (EB / 64)) {
+        const int4 it = items[w];
+        const EnsMember& M = mem[it.x];
+        const long off = M.off;
+        CtxGR<R, SS, false> x;
+        if constexpr (sizeof(R) == sizeof(double)) {
+            const Scal& d = M.s;
+            x.m = MetG<R>{d.e1t, d.e2t, d.e1u, d.e2u, d.e1v, d.e2v, d.e12t, d.e12u, d.e12v,
+                          d.fcu, d.fcv, d.ht, d.hu, d.hv};
+        } else {
+            const ScalF& d = M.sf;
+            x.m = MetG<R>{d.e1t, d.e2t, d.e1u, d.e2u, d.e1v, d.e2v, d.e12t, d.e12u, d.e12v,
+                          d.fcu, d.fcv, d.ht, d.hu, d.hv};
+        }
+        x.p = p;
+        x.n = StateG<R, SS, false>{n0.sshn + off, n0.sshn_u + off, n0.sshn_v + off, n0.un + off,
+                                   n0.vn + off, nullptr, nullptr, nullptr};
+        x.a = OutG<R, SS>{a0.sshn + off, a0.sshn_u + off, a0.sshn_v + off, a0.un + off, a0.vn + off};
+        x.fl = flp + off;
+        x.flv = flvp + off;
+        x.L = M.L;
+        x.tide = tide_tab[(long)it.x * tide_stride + step_idx];
+        const Layout& L = M.L;
+        x.i = it.y * GROUT + lane - 1;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;
+        x.lk = (unsigned)x.il;
+        x.own = lane >= 1 && lane <= GROUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        x.ucol = x.i >= 0 && x.i <= L.jpi;
+        x.vcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = it.z * EMJ;
+        const int je = jb + EMJ - 1 < L.jpj ? jb + EMJ - 1 : L.jpj;
+        CarryGR<R> k;
+        k.ct = k.cu = k.cv = 0.0;
+        if (it.w) rcg_march<R, SS, false, ERC_LA, true>(x, k, jb, je);
+        else rcg_march<R, SS, false, ERC_LA, false>(x, k, jb, je);
+        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)w);
+    }
+}
+
+// sshn_u, sshn_v of every member from its sshn by next's rules (the points
+// next leaves alone keep their values), in the state's arithmetic.
+template <class R, class SS>
+__global__ void __launch_bounds__(256)
+k_ens_interp(const EnsMember* __restrict__ mem, const SS* __restrict__ sshn, R* __restrict__ sshn_u,
+             R* __restrict__ sshn_v, const uint16_t* __restrict__ fl) {
+    const EnsMember& M = mem[blockIdx.y];
+    const Layout& L = M.L;
+    R e12t, e12u, e12v;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        e12t = M.s.e12t; e12u = M.s.e12u; e12v = M.s.e12v;
+    } else {
+        e12t = M.sf.e12t; e12u = M.sf.e12u; e12v = M.sf.e12v;
+    }
+    const long npts = (long)(L.jpi + 1) * (L.jpj + 1);
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < npts; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % (L.jpi + 1)), j = (int)(q / (L.jpi + 1));
+        const long o = M.off + L.at(i, j), P = L.pitch;
+        const unsigned f = fl[o];
+        if (j >= 1) {
+            const unsigned c = sucls(f);
+            if (c != NX_SKIP) sshn_u[o] = next_g<R>(c, R(sshn[o]), R(sshn[o + 1]), R(0), e12t, e12u);
+        }
+        if (i >= 1) {
+            const unsigned c = svcls(f);
+            if (c != NX_SKIP) sshn_v[o] = next_g<R>(c, R(sshn[o]), R(sshn[o + P]), R(0), e12t, e12v);
+        }
+    }
+}
+
 // Per-member checksums: block m reduces its members' tile partials in order.
 __global__ void __launch_bounds__(256)
 k_ens_finish(const EnsMember* __restrict__ mem, const double* __restrict__ partials,
@@ -179,6 +265,8 @@ struct nemo2d_ensemble {
     uint16_t* fl = nullptr;
     double *flu = nullptr, *flv = nullptr;
     int tiling = 0;
+    bool uv_stale = false;       // sshn_u/sshn_v of the current state not written
+    std::vector<uint16_t> hfl;   // class words (pooled), for the plain work items
     int4* tiles = nullptr;
     int ntiles = 0, ntiles_cap = 0;
     double* partials = nullptr;
@@ -195,7 +283,7 @@ namespace {
 int build_tiles(nemo2d_ensemble* e, int tiling) {
     const Tiling T = TILINGS[tiling];
     // rows per work item (a marching item is one wave's strip)
-    const int th = tiling == MARCH_TILING ? T.rows : (EB / T.tx) * T.rows;
+    const int th = is_march(tiling) ? T.rows : (EB / T.tx) * T.rows;
     std::vector<int4> tl;
     for (int k = 0; k < e->nmem; ++k) {
         EnsMember& M = e->hmem[k];
@@ -203,7 +291,23 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
         M.tile0 = (int)tl.size();
         M.ntiles = nbx * nby;
         for (int by = 0; by < nby; ++by)
-            for (int bx = 0; bx < nbx; ++bx) tl.push_back(make_int4(k, bx, by, 0));
+            for (int bx = 0; bx < nbx; ++bx) {
+                int plain = 0;
+                if (tiling == MARCH_RC_TILING) {
+                    // every class word the wave touches (lanes at columns
+                    // bx*GROUT-1 .. +63, rows jb-1 .. je+1+ERC_LA) plain
+                    const int i0 = bx * GROUT - 1, i1 = bx * GROUT + 62;
+                    const int jb = by * EMJ, je = std::min(jb + EMJ - 1, M.L.jpj);
+                    const int j0 = jb - 1, j1 = je + 1 + ERC_LA;
+                    if (i0 >= 0 && i1 <= M.L.jpi + 1 && j0 >= 0 && j1 <= M.L.jpj + 1) {
+                        plain = 1;
+                        for (int j = j0; j <= j1 && plain; ++j)
+                            for (int i = i0; i <= i1 && plain; ++i)
+                                plain = e->hfl[(size_t)(M.off + M.L.at(i, j))] == G_PLAIN;
+                    }
+                }
+                tl.push_back(make_int4(k, bx, by, plain));
+            }
     }
     e->ntiles = (int)tl.size();
     if (e->ntiles > e->ntiles_cap) {
@@ -246,6 +350,7 @@ void launch_step_t(nemo2d_ensemble* e, int step_idx) {
     }
 }
 
+template <bool RC>
 void launch_march(nemo2d_ensemble* e, int step_idx) {
     const int k = e->cur;
     if (!e->fp32) {
@@ -253,9 +358,14 @@ void launch_march(nemo2d_ensemble* e, int step_idx) {
         double** y = (double**)e->X[1 - k];
         StateG<double, double, false> n{x[0], x[1], x[2], x[3], x[4], nullptr, nullptr, nullptr};
         OutG<double, double> a{y[0], y[1], y[2], y[3], y[4]};
-        k_ens_march<double, double><<<e->grid, EB, 0, e->stream>>>(
-            e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
-            step_idx, e->partials);
+        if constexpr (RC)
+            k_ens_march_rc<double, double><<<e->grid, EB, 0, e->stream>>>(
+                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
+                step_idx, e->partials);
+        else
+            k_ens_march<double, double><<<e->grid, EB, 0, e->stream>>>(
+                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
+                step_idx, e->partials);
     } else {
         void** x = e->X[k];
         void** y = e->X[1 - k];
@@ -264,15 +374,39 @@ void launch_march(nemo2d_ensemble* e, int step_idx) {
         OutG<float, double> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
-        k_ens_march<float, double><<<e->grid, EB, 0, e->stream>>>(
-            e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
-            step_idx, e->partials);
+        if constexpr (RC)
+            k_ens_march_rc<float, double><<<e->grid, EB, 0, e->stream>>>(
+                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
+                step_idx, e->partials);
+        else
+            k_ens_march<float, double><<<e->grid, EB, 0, e->stream>>>(
+                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
+                step_idx, e->partials);
+    }
+}
+
+// fill in the current state's sshn_u/sshn_v (after sshn-only steps)
+void launch_interp(nemo2d_ensemble* e) {
+    const int k = e->cur;
+    const dim3 g(64, e->nmem);
+    if (!e->fp32) {
+        double** x = (double**)e->X[k];
+        k_ens_interp<double, double><<<g, 256, 0, e->stream>>>(e->dmem, x[0], x[1], x[2], e->fl);
+    } else {
+        void** x = e->X[k];
+        k_ens_interp<float, double><<<g, 256, 0, e->stream>>>(e->dmem, (const double*)x[0], (float*)x[1],
+                                                              (float*)x[2], e->fl);
     }
 }
 
 int launch_step(nemo2d_ensemble* e, int step_idx, int slot) {
+    if (e->uv_stale && e->tiling != MARCH_RC_TILING) {
+        launch_interp(e);
+        e->uv_stale = false;
+    }
     switch (e->tiling) {
-        case MARCH_TILING: launch_march(e, step_idx); break;
+        case MARCH_TILING: launch_march<false>(e, step_idx); break;
+        case MARCH_RC_TILING: launch_march<true>(e, step_idx); e->uv_stale = true; break;
         case 0: launch_step_t<64, 1>(e, step_idx); break;
         case 1: launch_step_t<64, 2>(e, step_idx); break;
         case 2: launch_step_t<64, 4>(e, step_idx); break;
@@ -333,6 +467,7 @@ int nemo2d_ensemble_reset(nemo2d_ensemble* e) {
         }
     EHIP(hipStreamSynchronize(e->stream));
     e->cur = 0;
+    e->uv_stale = false;  // the rest state's interpolations are zero, as stored
     return NEMO2D_OK;
 }
 
@@ -430,7 +565,8 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
     }
     e->grid = 4 * cus;
     int rc;
-    if ((rc = build_tiles(e, MARCH_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
+    e->hfl = std::move(hfl);
+    if ((rc = build_tiles(e, MARCH_RC_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
     *out = e;
     return NEMO2D_OK;
 }
@@ -446,7 +582,7 @@ int nemo2d_ensemble_tiling(int choice, int* tx, int* ty) {
     if (choice < 0 || choice >= NTILINGS || !tx || !ty)
         return set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_tiling: bad argument");
     *tx = TILINGS[choice].tx;
-    *ty = choice == MARCH_TILING ? TILINGS[choice].rows : (EB / TILINGS[choice].tx) * TILINGS[choice].rows;
+    *ty = is_march(choice) ? TILINGS[choice].rows : (EB / TILINGS[choice].tx) * TILINGS[choice].rows;
     return NEMO2D_OK;
 }
 
@@ -460,10 +596,13 @@ int nemo2d_ensemble_run(nemo2d_ensemble* e, int nit000, int nitend, int use_grap
     hipGraph_t graph = nullptr;
     if (use_graph) {
         const int cur0 = e->cur;
+        const bool stale0 = e->uv_stale;
         EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < nsteps && !rc; ++s) rc = launch_step(e, s, s);
         hipError_t ce = hipStreamEndCapture(e->stream, &graph);
         e->cur = cur0;  // capture did not run anything
+        // the replay leaves the interpolations as the capture's last step did
+        if (rc || ce != hipSuccess) e->uv_stale = stale0;
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
             return rc;

@@ -30,7 +30,7 @@ def single(k, m, nsteps):
         return s.retrieve(), s.checksum_log(nsteps)
 
 
-@pytest.mark.parametrize("tiling", [0, 8])
+@pytest.mark.parametrize("tiling", [0, 8, 9])
 def test_fp64_members_bitwise_equal_single_model_and_oracle(tiling):
     m = small_members()
     nsteps = 80
@@ -60,13 +60,13 @@ def test_graph_plain_and_tilings_agree():
     results = []
     with nemo2d.Ensemble(m, rdt=RDT) as e:
         for tiling, graph in ((0, True), (0, False), (3, True), (5, True), (7, False), (2, True),
-                              (8, True), (8, False)):
+                              (8, True), (8, False), (9, True), (9, False)):
             e.set_tiling(tiling)
             e.reset()
             e.run(1, nsteps, graph=graph)
             results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
         choice, ms = e.tune(nsteps=5)
-        assert 0 <= choice < 9 and ms > 0
+        assert 0 <= choice < 10 and ms > 0
         e.run(1, nsteps)
         results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
     for other in results[1:]:
@@ -95,14 +95,36 @@ def test_fp32_marching_tiling_equals_per_cell_tiling():
     m = nemo2d.ensemble_members(7, seed=9, size=(20, 150))
     out = []
     with nemo2d.Ensemble(m, rdt=RDT, precision=32) as e:
-        for tiling in (0, 8):
+        for tiling in (0, 8, 9):
             e.set_tiling(tiling)
             e.reset()
             e.run(1, 120)
             out.append([e.retrieve(k) for k in range(7)])
-    for a, b in zip(*out):
-        for f in ("sshn", "un", "vn"):
-            assert np.array_equal(a[f], b[f]), f
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            for f in ("sshn", "un", "vn"):
+                assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_switching_tilings_mid_run(precision):
+    """Steps of the sshn-only marching tiling leave sshn_u/sshn_v unwritten;
+    a following step of a tiling that reads them must see them filled in
+    (graph or not), and the large members have plain work items."""
+    m = nemo2d.ensemble_members(5, seed=11, size=(130, 260))
+    with nemo2d.Ensemble(m, rdt=RDT, precision=precision) as e:
+        e.set_tiling(0)
+        e.run(1, 60)
+        ref = [e.retrieve(k) for k in range(5)]
+        e.reset()
+        for tiling, (a, b), graph in ((9, (1, 20), True), (0, (21, 30), True), (9, (31, 45), False),
+                                      (8, (46, 50), False), (9, (51, 55), True), (3, (56, 60), True)):
+            e.set_tiling(tiling)
+            e.run(a, b, graph=graph)
+        for k in range(5):
+            r = e.retrieve(k)
+            for f in ("sshn", "un", "vn"):
+                assert np.array_equal(r[f], ref[k][f]), (k, f)
 
 
 def test_ensemble_rejects_bad_members():
