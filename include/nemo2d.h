@@ -110,6 +110,16 @@ int nemo2d_step(nemo2d_model *m, int istp);
  * have no counterpart in the reference (ensemble members). */
 int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 
+/* The nine stage calls of a step (nemo2d_continuity .. nemo2d_bc_flather_v,
+ * in the serial model's order) are recorded rather than run, and
+ * nemo2d_next then runs them with next as one fused step (the same results
+ * bit for bit, the same checksum logging); a call out of that order, or any
+ * other entry point (retrieve, checksums, snapshot, ...), first runs what was
+ * recorded, stage by stage.  on = 0 runs every stage call at once (a kernel
+ * per stage, as before this deferral); default on.  No counterpart in the
+ * reference. */
+int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
+
 /* Choose the kernel of nemo2d_step (results are identical for every
  * choice): 0 default (26 on grids with uniform metrics and depths, else 1),
  * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
@@ -291,7 +301,9 @@ int nemo2d_dev_zero(void *p, long bytes);
  * each region on the model's stream (and clears the sums); stage_times
  * fills, for regions 0..4 = Continuity, Momentum (momentum_u..momentum_v),
  * BCs (bc_ssh..bc_flather_v), Next, fused step, the call count, the total
- * seconds and the standard error of the mean per-call time.  No
+ * seconds and the standard error of the mean per-call time.  A stage call
+ * recorded for the fused pass (nemo2d_set_stage_fusion) counts with no
+ * time: its work is timed in the Next region that runs it.  No
  * counterpart in the reference's C-ABI (its driver times on the host). */
 int nemo2d_stage_timers(nemo2d_model *m, int on);
 int nemo2d_stage_times(nemo2d_model *m, int *counts, double *total_s, double *stderr_s);

@@ -102,6 +102,11 @@ struct nemo2d_model {
     // current state's sshn_u/sshn_v are known to be those interpolations
     bool uv_stale[2] = {false, false};
     bool uv_derived = false;
+    // the stage calls of a step recorded so far (in order) for one fused
+    // pass at nemo2d_next, and bc_ssh's time
+    int lz_n = 0;
+    double lz_rtime = 0.0;
+    bool lz_on = true;
     // per (wave strip, 12-row segment) of k_step_march_rc's tiling: 1 where
     // every point the wave touches has the plain interior class word
     uint8_t* rc_plain = nullptr;
@@ -1751,14 +1756,16 @@ int snap_join(nemo2d_model* m) {
 // ---- region timers: Continuity, Momentum, BCs, Next (the serial model's
 // dl_timer regions, original/nemolite2d.f90:391-769) and the fused step
 enum { TR_CONT = 0, TR_MOM, TR_BCS, TR_NEXT, TR_FUSED, TR_N };
+// a region opened by a stage call that was recorded for the fused pass
+constexpr int TEV_DEFER = -2;
 
 // fold the finished calls into the running sums and recycle their events
 void timers_fold(nemo2d_model* m) {
     if (m->tcalls.empty()) return;
     (void)hipStreamSynchronize(m->stream);
     for (const auto& c : m->tcalls) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, m->tev[c.b], m->tev[c.e]) != hipSuccess) continue;
+        float ms = 0.f;  // a recorded (deferred) call did no GPU work of its own
+        if (c.b != TEV_DEFER && hipEventElapsedTime(&ms, m->tev[c.b], m->tev[c.e]) != hipSuccess) continue;
         m->tcount[c.region] += 1;
         m->tsum[c.region] += ms;
         m->tsq[c.region] += (double)ms * ms;
@@ -1778,18 +1785,24 @@ int timer_event(nemo2d_model* m) {
     return k;
 }
 
-void region_begin(nemo2d_model* m, int r) {
+// deferred: the call is only recorded (no event; the region counts the
+// call with no time unless a call that does run work ends it)
+void region_begin(nemo2d_model* m, int r, bool deferred) {
     if (!m->timing) return;
     bool open = false;
     for (int k = 0; k < TR_N; ++k) open = open || m->topen[k] >= 0;
     if (!open && m->tev_used > 8192) timers_fold(m);
-    m->topen[r] = timer_event(m);
+    m->topen[r] = deferred ? TEV_DEFER : timer_event(m);
 }
 
-void region_end(nemo2d_model* m, int r) {
-    if (!m->timing || m->topen[r] < 0) return;
-    const int e = timer_event(m);
-    if (e >= 0) m->tcalls.push_back({r, m->topen[r], e});
+void region_end(nemo2d_model* m, int r, bool deferred) {
+    if (!m->timing || m->topen[r] == -1) return;
+    if (deferred && m->topen[r] == TEV_DEFER) {
+        m->tcalls.push_back({r, TEV_DEFER, TEV_DEFER});
+    } else if (m->topen[r] >= 0) {
+        const int e = timer_event(m);
+        if (e >= 0) m->tcalls.push_back({r, m->topen[r], e});
+    }
     m->topen[r] = -1;
 }
 
@@ -1797,12 +1810,17 @@ void region_end(nemo2d_model* m, int r) {
 struct RegionScope {
     nemo2d_model* m;
     int r;
-    bool end;
-    RegionScope(nemo2d_model* m_, int r_, bool begin, bool end_) : m(m_), r(r_), end(end_) {
-        if (begin) region_begin(m, r);
+    bool end, deferred;
+    RegionScope(nemo2d_model* m_, int r_, bool begin, bool end_, bool deferred_ = false)
+        : m(m_), r(r_), end(end_), deferred(deferred_) {
+        if (begin) {
+            region_begin(m, r, deferred);
+        } else if (!deferred && m->timing && m->topen[r] == TEV_DEFER) {
+            m->topen[r] = timer_event(m);  // opened by a recorded call: time from here
+        }
     }
     ~RegionScope() {
-        if (end) region_end(m, r);
+        if (end) region_end(m, r, deferred);
     }
 };
 
@@ -2130,6 +2148,9 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
     return NEMO2D_OK;
 }
 
+// Runs the stage calls recorded for the fused pass (defined below).
+static int lz_flush(nemo2d_model* m);
+
 int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const double* e2t,
                     const double* e1u, const double* e2u, const double* e1v, const double* e2v,
                     const double* e12t, const double* e12u, const double* e12v,
@@ -2140,6 +2161,10 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     for (const double* a : arrs)
         if (!a) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null array");
     if (!pt) return fail(NEMO2D_ERR_ARG, "nemo2d_set_grid: null mask");
+    if (m->lz_n > 0) {  // recorded stages act on the grid this replaces
+        const int frc = lz_flush(m);
+        if (frc) return frc;
+    }
     const Layout& L = m->L;
     const int jpi = L.jpi, jpj = L.jpj;
     const long nu = (long)(jpi + 1) * jpj, nv = (long)jpi * (jpj + 1), nt = (long)jpi * jpj;
@@ -2292,6 +2317,7 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
         HIPCHK(hipStreamSynchronize(m->stream));
         m->uv_stale[0] = m->uv_stale[1] = false;
         m->uv_derived = false;
+        m->lz_n = 0;
         m->a_current = false;
         m->state_set = true;
         m->cs_next = 0;
@@ -2319,6 +2345,7 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     // sshn_u/sshn_v as given (not known to be sshn's interpolations)
     m->uv_stale[0] = m->uv_stale[1] = false;
     m->uv_derived = false;
+    m->lz_n = 0;  // recorded stage calls act on the state this replaces
     m->a_current = true;
     m->state_set = true;
     m->cs_next = 0;
@@ -2327,108 +2354,196 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     return NEMO2D_OK;
 }
 
-#define READY(m, who)                                                                      \
+
+#define READY_NF(m, who)                                                                   \
     do {                                                                                   \
         if (!(m)) return fail(NEMO2D_ERR_ARG, std::string(who) + ": null model");          \
         if (!(m)->grid_set || !(m)->state_set)                                             \
             return fail(NEMO2D_ERR_STATE, std::string(who) + ": grid and state must be set first"); \
     } while (0)
 
+// every entry point but the stage calls first runs the stage calls that
+// were recorded for a fused pass
+#define READY(m, who)                                                                      \
+    do {                                                                                   \
+        READY_NF(m, who);                                                                  \
+        const int frc_ = lz_flush(m);                                                      \
+        if (frc_) return frc_;                                                             \
+    } while (0)
+
 #define READY_STAGE(m, who)                                                                \
     do {                                                                                   \
-        READY(m, who);                                                                     \
+        READY_NF(m, who);                                                                  \
         if ((m)->mixed)                                                                    \
             return fail(NEMO2D_ERR_STATE, std::string(who) +                               \
                                               ": the single-precision form has the fused step only"); \
-        const int urc_ = uv_sync(m);                                                       \
-        if (urc_) return urc_;                                                             \
     } while (0)
+
+// ---- the nine stage calls -------------------------------------------------------
+// Each stage's kernel (the eager path).  original/nemolite2d.f90:85-93 calls
+// them in this order every step; made in that order on a grid the fused step
+// handles, they are recorded instead (stage_defer) and nemo2d_next runs the
+// fused step, which gives the nine stages' and next's results bit for bit.
+// A call out of order, or any other entry point, first runs the recorded
+// stages one by one (lz_flush).
+enum { LZ_CONT = 0, LZ_MOMU, LZ_MOMV, LZ_SSH, LZ_SOLU, LZ_SOLV, LZ_FLU, LZ_FLV, LZ_N };
+
+static int stage_run(nemo2d_model* m, int stage) {
+    int rc = uv_sync(m);  // the stages read sshn_u/sshn_v
+    if (rc) return rc;
+    const Layout& L = m->L;
+    NState n = nstate(m, m->cur);
+    switch (stage) {
+        case LZ_CONT:
+            WITH_STATICS(m, st, (k_continuity<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                                    st, n, m->ssha, L, m->prm.rdt)));
+            LAUNCHCHK("continuity");
+            break;
+        case LZ_MOMU:
+            WITH_STATICS(m, st, (k_momentum_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                                    st, n, m->flags, m->ua, L, m->prm)));
+            LAUNCHCHK("momentum_u");
+            m->a_current = true;
+            break;
+        case LZ_MOMV:
+            WITH_STATICS(m, st, (k_momentum_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                                    st, n, m->flags, m->va, L, m->prm)));
+            LAUNCHCHK("momentum_v");
+            m->a_current = true;
+            break;
+        case LZ_SSH:
+            k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                m->flags, m->ssha, L, tide_eval(m->tide_amp, m->tide_omega, m->lz_rtime));
+            LAUNCHCHK("bc_ssh");
+            break;
+        case LZ_SOLU:
+            k_bc_solid_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ua, L);
+            LAUNCHCHK("bc_solid_u");
+            break;
+        case LZ_SOLV:
+            k_bc_solid_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->va, L);
+            LAUNCHCHK("bc_solid_v");
+            break;
+        case LZ_FLU:
+            if (m->nflu > 0) {
+                const int lines = L.jpj;
+                k_bc_flather_u<<<(lines + 255) / 256, 256, 0, m->stream>>>(
+                    m->flu_ptr, m->flu_pts, lines, m->flags, m->flu, m->S[m->cur][F_SSHN_U], m->ua);
+                LAUNCHCHK("bc_flather_u");
+            }
+            break;
+        case LZ_FLV:
+            if (m->nflv > 0) {
+                const int lines = L.jpi;
+                k_bc_flather_v<<<(lines + 255) / 256, 256, 0, m->stream>>>(
+                    m->flv_ptr, m->flv_pts, lines, m->flags, m->flv, m->S[m->cur][F_SSHN_V], m->va, L.pitch);
+                LAUNCHCHK("bc_flather_v");
+            }
+            break;
+        default: break;
+    }
+    return NEMO2D_OK;
+}
+
+static int lz_flush(nemo2d_model* m) {
+    const int n = m->lz_n;
+    m->lz_n = 0;
+    for (int st = 0; st < n; ++st) {
+        const int rc = stage_run(m, st);
+        if (rc) return rc;
+    }
+    return NEMO2D_OK;
+}
+
+// True when a call of this stage would be recorded for the fused pass.
+static bool stage_defers(const nemo2d_model* m, int stage) {
+    return m->lz_on && m->fused_ok && !m->mixed && m->lz_n == stage;
+}
+
+// Records the call for the fused pass when it comes in order; else runs the
+// recorded stages and then this one.
+static int stage_call(nemo2d_model* m, int stage) {
+    if (stage_defers(m, stage)) {
+        m->lz_n = stage + 1;
+        return NEMO2D_OK;
+    }
+    int rc = lz_flush(m);
+    if (rc) return rc;
+    return stage_run(m, stage);
+}
 
 int nemo2d_continuity(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_continuity");
-    RegionScope rs_(m, TR_CONT, true, true);
-    const Layout& L = m->L;
-    NState n = nstate(m, m->cur);
-    WITH_STATICS(m, st, (k_continuity<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, n, m->ssha, L, m->prm.rdt)));
-    LAUNCHCHK("continuity");
-    return NEMO2D_OK;
+    RegionScope rs_(m, TR_CONT, true, true, stage_defers(m, LZ_CONT));
+    return stage_call(m, LZ_CONT);
 }
 
 int nemo2d_momentum_u(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_momentum_u");
-    RegionScope rs_(m, TR_MOM, true, false);
-    const Layout& L = m->L;
-    NState n = nstate(m, m->cur);
-    WITH_STATICS(m, st, (k_momentum_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, n, m->flags, m->ua, L, m->prm)));
-    LAUNCHCHK("momentum_u");
-    m->a_current = true;
-    return NEMO2D_OK;
+    RegionScope rs_(m, TR_MOM, true, false, stage_defers(m, LZ_MOMU));
+    return stage_call(m, LZ_MOMU);
 }
 
 int nemo2d_momentum_v(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_momentum_v");
-    RegionScope rs_(m, TR_MOM, false, true);
-    const Layout& L = m->L;
-    NState n = nstate(m, m->cur);
-    WITH_STATICS(m, st, (k_momentum_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, n, m->flags, m->va, L, m->prm)));
-    LAUNCHCHK("momentum_v");
-    m->a_current = true;
-    return NEMO2D_OK;
+    RegionScope rs_(m, TR_MOM, false, true, stage_defers(m, LZ_MOMV));
+    return stage_call(m, LZ_MOMV);
 }
 
 int nemo2d_bc_ssh(nemo2d_model* m, double rtime) {
     READY_STAGE(m, "nemo2d_bc_ssh");
-    RegionScope rs_(m, TR_BCS, true, false);
-    const Layout& L = m->L;
-    k_bc_ssh<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ssha, L,
-                                                                            tide_eval(m->tide_amp, m->tide_omega, rtime));
-    LAUNCHCHK("bc_ssh");
-    return NEMO2D_OK;
+    RegionScope rs_(m, TR_BCS, true, false, stage_defers(m, LZ_SSH));
+    // a recorded bc_ssh keeps its time; an earlier recorded one has been run
+    // (in order) before this call replaces it
+    if (m->lz_n != LZ_SSH) {
+        const int rc = lz_flush(m);
+        if (rc) return rc;
+    }
+    m->lz_rtime = rtime;
+    return stage_call(m, LZ_SSH);
 }
 
 int nemo2d_bc_solid_u(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_bc_solid_u");
-    const Layout& L = m->L;
-    k_bc_solid_u<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->ua, L);
-    LAUNCHCHK("bc_solid_u");
-    return NEMO2D_OK;
+    return stage_call(m, LZ_SOLU);
 }
 
 int nemo2d_bc_solid_v(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_bc_solid_v");
-    const Layout& L = m->L;
-    k_bc_solid_v<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(m->flags, m->va, L);
-    LAUNCHCHK("bc_solid_v");
-    return NEMO2D_OK;
+    return stage_call(m, LZ_SOLV);
 }
 
 int nemo2d_bc_flather_u(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_bc_flather_u");
-    if (m->nflu == 0) return NEMO2D_OK;
-    const int lines = m->L.jpj;
-    k_bc_flather_u<<<(lines + 255) / 256, 256, 0, m->stream>>>(
-        m->flu_ptr, m->flu_pts, lines, m->flags, m->flu, m->S[m->cur][F_SSHN_U], m->ua);
-    LAUNCHCHK("bc_flather_u");
-    return NEMO2D_OK;
+    return stage_call(m, LZ_FLU);
 }
 
 int nemo2d_bc_flather_v(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_bc_flather_v");
-    RegionScope rs_(m, TR_BCS, false, true);
-    if (m->nflv == 0) return NEMO2D_OK;
-    const int lines = m->L.jpi;
-    k_bc_flather_v<<<(lines + 255) / 256, 256, 0, m->stream>>>(
-        m->flv_ptr, m->flv_pts, lines, m->flags, m->flv, m->S[m->cur][F_SSHN_V], m->va, m->L.pitch);
-    LAUNCHCHK("bc_flather_v");
-    return NEMO2D_OK;
+    RegionScope rs_(m, TR_BCS, false, true, stage_defers(m, LZ_FLV));
+    return stage_call(m, LZ_FLV);
 }
+
+static int fused_step(nemo2d_model* m, double tide, int slot);
 
 int nemo2d_next(nemo2d_model* m) {
     READY_STAGE(m, "nemo2d_next");
     RegionScope rs_(m, TR_NEXT, true, true);
+    if (m->lz_n == LZ_N) {
+        // the step's nine stage calls were recorded in order: one fused pass,
+        // logging into the stage path's next slot
+        m->lz_n = 0;
+        const int slot = (m->cs_cap > 0 && m->cs_next < m->cs_cap) ? m->cs_next : -1;
+        return fused_step(m, tide_eval(m->tide_amp, m->tide_omega, m->lz_rtime), slot);
+    }
+    {
+        const int rc = lz_flush(m);
+        if (rc) return rc;
+    }
+    {
+        const int rc = uv_sync(m);
+        if (rc) return rc;
+    }
     int grc = snap_guard(m, m->cur);
     if (grc) return grc;
     const Layout& L = m->L;
@@ -2759,12 +2874,18 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     RegionScope rs_(m, TR_FUSED, m->fused_ok || m->mixed, m->fused_ok || m->mixed);
     if (m->mixed) return step_mixed(m, istp);
     if (!m->fused_ok) return run_stages(m, istp);
+    return fused_step(m, tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt),
+                      istp - m->nit000);
+}
+
+// One fused step of the FP64 model with the given tide; logs the checksums
+// into slot `slot` when recording covers it.
+static int fused_step(nemo2d_model* m, double tide, int slot) {
     const Layout& L = m->L;
     const int k = m->cur;
     NState n = nstate(m, k);
     AOut a{m->S[1 - k][F_SSHN], m->S[1 - k][F_SSHN_U], m->S[1 - k][F_SSHN_V], m->S[1 - k][F_UN],
            m->S[1 - k][F_VN]};
-    const double tide = tide_eval(m->tide_amp, m->tide_omega, (double)istp * m->prm.rdt);
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO)
@@ -2785,7 +2906,6 @@ int nemo2d_step(nemo2d_model* m, int istp) {
     }
     int nparts = 0, rc = NEMO2D_OK;
     bool flin = false;
-    const int slot = istp - m->nit000;
     double* out = (m->cs_cap > 0 && slot >= 0 && slot < m->cs_cap) ? m->cs_log + 3 * (long)slot : nullptr;
     const bool defer = out && m->cs_part;
     m->part_dst = defer ? m->cs_part + slot * m->cs_slot : m->partials;
@@ -2894,8 +3014,23 @@ int nemo2d_set_tide(nemo2d_model* m, double amp, double period_hours) {
     if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_tide: null model");
     if (!isfinite(amp) || !(period_hours > 0.0))
         return fail(NEMO2D_ERR_ARG, "nemo2d_set_tide: amplitude must be finite, period > 0");
+    if (m->lz_n > 0) {  // a recorded bc_ssh keeps the tide it was called with
+        const int rc = lz_flush(m);
+        if (rc) return rc;
+    }
     m->tide_amp = amp;
     m->tide_omega = tide_omega_of(period_hours);
+    return NEMO2D_OK;
+}
+
+int nemo2d_set_stage_fusion(nemo2d_model* m, int on) {
+    if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_set_stage_fusion: null model");
+    if (m->grid_set && m->state_set) {
+        const int rc = lz_flush(m);
+        if (rc) return rc;
+    }
+    m->lz_n = 0;
+    m->lz_on = on != 0;
     return NEMO2D_OK;
 }
 
@@ -3023,6 +3158,10 @@ int nemo2d_closure_features(nemo2d_model* m, const double* mean, const double* s
 int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const double* scale) {
     READY_STAGE(m, "nemo2d_closure_apply");
     if (!z || !scale || zstride < 3) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_apply: bad argument");
+    {
+        const int frc = lz_flush(m);  // recorded stages act on the state before the increments
+        if (frc) return frc;
+    }
     // the increments come from work on the default stream
     HIPCHK(hipStreamSynchronize(nullptr));
     int grc = snap_guard(m, m->cur);

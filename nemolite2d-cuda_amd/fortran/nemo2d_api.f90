@@ -16,7 +16,7 @@ module nemo2d_api
               nemo2d_checksums, nemo2d_retrieve_results, nemo2d_retrieve_aux, &
               nemo2d_timer_start, nemo2d_timer_stop, nemo2d_destroy, &
               nemo2d_set_output_grid, nemo2d_snapshot_begin, nemo2d_snapshot_wait, &
-              nemo2d_stage_timers, nemo2d_stage_times, nemo2d_check
+              nemo2d_stage_timers, nemo2d_stage_times, nemo2d_set_stage_fusion, nemo2d_check
 
     interface
         integer(c_int) function nemo2d_create(m, jpi, jpj, dx, dy, dep_const, nit000, nitend, &
@@ -140,6 +140,13 @@ module nemo2d_api
             type(c_ptr), value :: m
         end function
 
+        ! record the nine stage calls and run them with next as one fused
+        ! pass (on /= 0, the default) or run each call at once
+        integer(c_int) function nemo2d_set_stage_fusion(m, on) bind(C, name="nemo2d_set_stage_fusion")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), value :: on
+        end function
         ! device timers of the Continuity/Momentum/BCs/Next regions and the
         ! fused step (regions 1..5 of the arrays)
         integer(c_int) function nemo2d_stage_timers(m, on) bind(C, name="nemo2d_stage_timers")

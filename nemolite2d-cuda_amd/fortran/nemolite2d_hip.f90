@@ -7,7 +7,9 @@
 !! calls the library's nine stage entry points in the serial model's order
 !! (continuity, momentum_u, momentum_v, bc_ssh, bc_solid_u, bc_solid_v,
 !! bc_flather_u, bc_flather_v, next) or, with fused=.true. in the optional
-!! &namhip group, the single fused step.  Per-step checksums sum|ssh|, sum|u|,
+!! &namhip group, the single fused step.  The library records the nine calls
+!! and runs them with next as one fused pass (stage_fusion=.false. runs a
+!! kernel per call instead; the results are the same).  Per-step checksums sum|ssh|, sum|u|,
 !! sum|v| are written to the file named by csfile.  With async_output=.true.
 !! the snapshots are copied and written by the library behind the following
 !! steps (same files) instead of retrieve-then-write.
@@ -28,7 +30,7 @@ program nemolite2d_hip
 
     integer :: jpiglo, jpjglo, jpi, jpj, jphgr_msh, nit000, nitend, irecord
     real(wp) :: dx, dy, dep_const, rdt, cbfr, visc
-    logical :: fused, write_output, async_output
+    logical :: fused, write_output, async_output, stage_fusion
     character(len=256) :: csfile
 
     type(c_ptr) :: model = c_null_ptr
@@ -57,6 +59,8 @@ program nemolite2d_hip
     if (write_output) call snapshot
 
     call system_clock(c0, crate)
+    call nemo2d_check(nemo2d_set_stage_fusion(model, merge(1_c_int, 0_c_int, stage_fusion)), &
+                      'nemo2d_set_stage_fusion')
     call nemo2d_check(nemo2d_stage_timers(model, 1_c_int), 'nemo2d_stage_timers')
     call nemo2d_check(nemo2d_timer_start(model), 'nemo2d_timer_start')
     do istp = nit000, nitend
@@ -124,13 +128,13 @@ contains
         integer :: ios
         namelist /namctl/ jpiglo, jpjglo, jphgr_msh, dx, dy, dep_const, &
             nit000, nitend, irecord, rdt, cbfr, visc
-        namelist /namhip/ fused, write_output, async_output, csfile
+        namelist /namhip/ fused, write_output, async_output, stage_fusion, csfile
 
         jpiglo = 258; jpjglo = 258; jphgr_msh = 1
         dx = 1000._wp; dy = 1000._wp; dep_const = 100._wp
         nit000 = 1; nitend = 1000; irecord = 1
         rdt = 10._wp; cbfr = 0.001_wp; visc = 100._wp
-        fused = .false.; write_output = .true.; async_output = .false.
+        fused = .false.; write_output = .true.; async_output = .false.; stage_fusion = .true.
         csfile = 'nemo2d_checksums.txt'
 
         open (1, file='namelist', status='OLD')

@@ -39,6 +39,7 @@ SYMBOLS = (
     "nemo2d_dev_zero", "nemo2d_closure_features", "nemo2d_closure_apply",
     "nemo2d_stream_bandwidth", "nemo2d_format_go2d", "nemo2d_set_output_grid",
     "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
+    "nemo2d_set_stage_fusion",
 )
 
 # nemo2d_set_variant choices
@@ -290,6 +291,11 @@ class Model:
     def set_tide(self, amp, period_hours):
         """Tide amplitude (m) and period (hours) of the clamped-ssh boundary."""
         _check(lib().nemo2d_set_tide(self._h, float(amp), float(period_hours)))
+
+    def set_stage_fusion(self, on):
+        """Record the nine stage calls of a step and run them with next as one
+        fused step (on, the default), or run each stage call at once (off)."""
+        _check(lib().nemo2d_set_stage_fusion(self._h, 1 if on else 0))
 
     def set_variant(self, name):
         """Kernel of the fused step (see VARIANTS); results do not depend on it."""

@@ -53,11 +53,15 @@ def assert_fields_equal(m, o, what=""):
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("jpi,jpj", [(1, 1), (2, 2), (1, 5), (5, 1), (3, 7), (16, 10), (65, 9),
                                      (128, 128)])
-@pytest.mark.parametrize("path", ["fused", "stages"])
+@pytest.mark.parametrize("path", ["fused", "stages", "stages_eager"])
 def test_bitwise_vs_oracle_default_grid(jpi, jpj, path):
+    """'stages' records the nine stage calls and runs them with next as one
+    fused pass; 'stages_eager' runs a kernel per stage call."""
     nsteps = 120 if jpi * jpj < 5000 else 60
     o = Oracle(jpi, jpj)
     with model_like(o) as m:
+        if path == "stages_eager":
+            m.set_stage_fusion(False)
         m.record_checksums(nsteps)
         for istp in range(1, nsteps + 1):
             (m.step if path == "fused" else m.step_stages)(istp)
@@ -102,6 +106,88 @@ def test_stage_by_stage_vs_oracle():
             assert np.array_equal(m.retrieve()["va"], o.arr("va"))
             m.next(); o.next()
             assert_fields_equal(m, o, "next %d" % istp)
+
+
+def test_stage_calls_out_of_order_and_interrupted():
+    """Stage calls that leave the serial model's order, or that another
+    entry point interrupts (a retrieve, a tide change), run as called: the
+    recorded calls are run first, stage by stage.  Each pattern gives the
+    oracle's fields doing the same calls, bit for bit, with the recording
+    on and off."""
+    o = Oracle(48, 30, cbfr=0.00015)
+    o.run(1, 60)
+    start = state_from_oracle(o)
+    rdt = o.params["rdt"]
+
+    def step_calls(x, istp, pattern, extra):
+        rt = istp * rdt
+        calls = [x.continuity, x.momentum_u, x.momentum_v, lambda: x.bc_ssh(rt), x.bc_solid_u,
+                 x.bc_solid_v, x.bc_flather_u, x.bc_flather_v]
+        if pattern == "swap_solid":
+            calls[4], calls[5] = calls[5], calls[4]
+        if pattern == "momv_first":
+            calls[1], calls[2] = calls[2], calls[1]
+        for k, c in enumerate(calls):
+            c()
+            if extra and k == 3:
+                extra()
+        x.next()
+
+    for pattern in ("swap_solid", "momv_first", "retrieve_mid", "in_order"):
+        results = []
+        for fusion in (True, False):
+            oo = Oracle(48, 30, cbfr=0.00015)
+            oo.run(1, 60)
+            with model_like(oo) as m:
+                m.set_stage_fusion(fusion)
+                m.set_state(start)
+                for istp in range(61, 71):
+                    extra = (lambda: m.retrieve()) if pattern == "retrieve_mid" else None
+                    step_calls(m, istp, pattern, extra)
+                    step_calls(oo, istp, pattern, None)
+                assert_fields_equal(m, oo, "%s fusion=%s" % (pattern, fusion))
+                results.append(m.retrieve())
+        for k in FIELDS:
+            assert np.array_equal(results[0][k], results[1][k]), (pattern, k)
+
+
+def test_stage_fusion_tide_change_mid_step():
+    """A tide change between a recorded bc_ssh and next leaves the recorded
+    call with the tide it was called with: the same fields as running each
+    call at once."""
+    o = Oracle(64, 40)
+    o.run(1, 30)
+    out = []
+    for fusion in (True, False):
+        with model_like(o) as m:
+            m.set_stage_fusion(fusion)
+            m.set_tide(1.0, 12.42)
+            for istp in range(31, 41):
+                rt = istp * o.params["rdt"]
+                m.continuity(); m.momentum_u(); m.momentum_v(); m.bc_ssh(rt)
+                m.set_tide(1.0 + 0.05 * (istp - 30), 12.42)
+                m.bc_solid_u(); m.bc_solid_v(); m.bc_flather_u(); m.bc_flather_v(); m.next()
+            out.append(m.retrieve())
+    for k in FIELDS:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_stage_timers_with_recorded_stage_calls():
+    """Recorded stage calls count in their regions with no time; Next holds
+    the fused pass.  Run at once, every region takes time."""
+    o = Oracle(96, 64)
+    for fusion in (True, False):
+        with model_like(o) as m:
+            m.set_stage_fusion(fusion)
+            m.stage_timers(True)
+            for istp in range(1, 6):
+                m.step_stages(istp)
+            t = m.stage_times()
+            for r in ("Continuity", "Momentum", "BCs", "Next"):
+                assert t[r][0] == 5, (fusion, r, t[r])
+            assert t["Next"][1] > 0.0
+            for r in ("Continuity", "Momentum", "BCs"):
+                assert (t[r][1] == 0.0) if fusion else (t[r][1] > 0.0), (fusion, r, t[r])
 
 
 def test_fused_and_stages_mix():
