@@ -134,7 +134,10 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
             const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
             int tide_stride, int step_idx, double* __restrict__ partials) {
     const int lane = threadIdx.x & 63;
-    for (int w = blockIdx.x * (EB / 64) + (threadIdx.x >> 6); w < nitems; w += gridDim.x * (EB / 64)) {
+    // the wave index made wave-uniform (an SGPR): the work item, the member's
+    // record and its metrics are then scalar loads held in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
         const EnsMember& M = mem[it.x];
         const long off = M.off;
@@ -170,7 +173,10 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
                const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
                int tide_stride, int step_idx, double* __restrict__ partials) {
     const int lane = threadIdx.x & 63;
-    for (int w = blockIdx.x * (EB / 64) + (threadIdx.x >> 6); w < nitems; w += gridDim.x * (EB / 64)) {
+    // the wave index made wave-uniform (an SGPR): the work item, the member's
+    // record and its metrics are then scalar loads held in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
         const EnsMember& M = mem[it.x];
         const long off = M.off;
@@ -270,7 +276,9 @@ struct nemo2d_ensemble {
     int4* tiles = nullptr;
     int ntiles = 0, ntiles_cap = 0;
     double* partials = nullptr;
-    int grid = 0;
+    int grid = 0;     // blocks of the per-cell tilings
+    int cus = 256;
+    int mgrid[2][2] = {};  // blocks of the marching kernels [fp32][rc]: as many as stay resident
     double* tide_tab = nullptr;
     int tide_stride = 0;
     double* log = nullptr;
@@ -350,20 +358,33 @@ void launch_step_t(nemo2d_ensemble* e, int step_idx) {
     }
 }
 
+// Blocks of a persistent marching kernel: every block resident at once (a
+// block that waits for a slot would start its fixed share of the items late).
+template <class K>
+int resident_grid(const nemo2d_ensemble* e, K kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, EB, 0) != hipSuccess || nb < 1) nb = 2;
+    return nb * e->cus;
+}
+
 template <bool RC>
 void launch_march(nemo2d_ensemble* e, int step_idx) {
     const int k = e->cur;
+    int& grid = e->mgrid[e->fp32 ? 1 : 0][RC ? 1 : 0];
     if (!e->fp32) {
         double** x = (double**)e->X[k];
         double** y = (double**)e->X[1 - k];
         StateG<double, double, false> n{x[0], x[1], x[2], x[3], x[4], nullptr, nullptr, nullptr};
         OutG<double, double> a{y[0], y[1], y[2], y[3], y[4]};
+        if (!grid)
+            grid = RC ? resident_grid(e, k_ens_march_rc<double, double>)
+                      : resident_grid(e, k_ens_march<double, double>);
         if constexpr (RC)
-            k_ens_march_rc<double, double><<<e->grid, EB, 0, e->stream>>>(
+            k_ens_march_rc<double, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
                 step_idx, e->partials);
         else
-            k_ens_march<double, double><<<e->grid, EB, 0, e->stream>>>(
+            k_ens_march<double, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
                 step_idx, e->partials);
     } else {
@@ -374,12 +395,15 @@ void launch_march(nemo2d_ensemble* e, int step_idx) {
         OutG<float, double> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
+        if (!grid)
+            grid = RC ? resident_grid(e, k_ens_march_rc<float, double>)
+                      : resident_grid(e, k_ens_march<float, double>);
         if constexpr (RC)
-            k_ens_march_rc<float, double><<<e->grid, EB, 0, e->stream>>>(
+            k_ens_march_rc<float, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
                 step_idx, e->partials);
         else
-            k_ens_march<float, double><<<e->grid, EB, 0, e->stream>>>(
+            k_ens_march<float, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
                 step_idx, e->partials);
     }
@@ -564,6 +588,7 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     e->grid = 4 * cus;
+    e->cus = cus;
     int rc;
     e->hfl = std::move(hfl);
     if ((rc = build_tiles(e, MARCH_RC_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
