@@ -158,7 +158,7 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
         CarryG<R, SS> k;
         k.ct = k.cu = k.cv = 0.0;
         march_wave_g<R, SS, false, EMJ>(mt, p, n, a, flp + off, flvp + off, M.L, tide, it.y, it.z, lane, k);
-        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)w);
+        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)(it.w >> 1));
     }
 }
 
@@ -210,9 +210,9 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
         const int je = jb + EMJ - 1 < L.jpj ? jb + EMJ - 1 : L.jpj;
         CarryGR<R> k;
         k.ct = k.cu = k.cv = 0.0;
-        if (it.w) rcg_march<R, SS, false, ERC_LA, true>(x, k, jb, je);
+        if (it.w & 1) rcg_march<R, SS, false, ERC_LA, true>(x, k, jb, je);
         else rcg_march<R, SS, false, ERC_LA, false>(x, k, jb, je);
-        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)w);
+        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)(it.w >> 1));
     }
 }
 
@@ -288,11 +288,56 @@ struct nemo2d_ensemble {
 
 namespace {
 
+// Blocks of a persistent marching kernel: every block resident at once (a
+// block that waits for a slot would start its fixed share of the items late).
+template <class K>
+int resident_grid(const nemo2d_ensemble* e, K kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, EB, 0) != hipSuccess || nb < 1) nb = 2;
+    return nb * e->cus;
+}
+
+// Waves of the marching kernel a tiling runs (its resident grid).
+int march_waves(nemo2d_ensemble* e, int tiling) {
+    const bool rc = tiling == MARCH_RC_TILING;
+    int& grid = e->mgrid[e->fp32 ? 1 : 0][rc ? 1 : 0];
+    if (!grid) {
+        if (e->fp32)
+            grid = rc ? resident_grid(e, k_ens_march_rc<float, double>) : resident_grid(e, k_ens_march<float, double>);
+        else
+            grid = rc ? resident_grid(e, k_ens_march_rc<double, double>)
+                      : resident_grid(e, k_ens_march<double, double>);
+    }
+    return grid * (EB / 64);
+}
+
+// Order of the marching work items: wave w of W runs items w, w+W, w+2W, ...
+// of the array.  The items are sorted by an estimate of their cost (rows,
+// and 1.4x for items that run the general code) and dealt to the waves in a
+// snake (round t: waves 0..W-1 for even t, W-1..0 for odd t), which evens
+// out the waves' totals (the kernel ends with its slowest wave).  Each item
+// keeps its own checksum slot (w = plain | slot << 1), so a member's
+// partials stay contiguous.
+void order_items(std::vector<int4>& tl, const std::vector<float>& cost, int W) {
+    const int n = (int)tl.size();
+    std::vector<int> idx(n);
+    for (int q = 0; q < n; ++q) idx[q] = q;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int4> out(n);
+    for (int t0 = 0; t0 < n; t0 += W) {
+        const int m = std::min(W, n - t0);
+        const bool rev = (t0 / W) & 1;
+        for (int q = 0; q < m; ++q) out[t0 + (rev ? m - 1 - q : q)] = tl[idx[t0 + q]];
+    }
+    tl.swap(out);
+}
+
 int build_tiles(nemo2d_ensemble* e, int tiling) {
     const Tiling T = TILINGS[tiling];
     // rows per work item (a marching item is one wave's strip)
     const int th = is_march(tiling) ? T.rows : (EB / T.tx) * T.rows;
     std::vector<int4> tl;
+    std::vector<float> cost;
     for (int k = 0; k < e->nmem; ++k) {
         EnsMember& M = e->hmem[k];
         const int nbx = (M.L.jpi + 1 + T.tx - 1) / T.tx, nby = (M.L.jpj + 1 + th - 1) / th;
@@ -314,9 +359,14 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
                                 plain = e->hfl[(size_t)(M.off + M.L.at(i, j))] == G_PLAIN;
                     }
                 }
-                tl.push_back(make_int4(k, bx, by, plain));
+                tl.push_back(make_int4(k, bx, by, plain | ((int)tl.size() << 1)));
+                if (is_march(tiling)) {
+                    const int rows = std::min(by * EMJ + EMJ - 1, M.L.jpj) - by * EMJ + 1;
+                    cost.push_back((plain ? 1.0f : 1.4f) * ((float)rows + 1.5f));
+                }
             }
     }
+    if (is_march(tiling)) order_items(tl, cost, march_waves(e, tiling));
     e->ntiles = (int)tl.size();
     if (e->ntiles > e->ntiles_cap) {
         if (e->tiles) (void)hipFree(e->tiles);
@@ -358,27 +408,15 @@ void launch_step_t(nemo2d_ensemble* e, int step_idx) {
     }
 }
 
-// Blocks of a persistent marching kernel: every block resident at once (a
-// block that waits for a slot would start its fixed share of the items late).
-template <class K>
-int resident_grid(const nemo2d_ensemble* e, K kern) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, EB, 0) != hipSuccess || nb < 1) nb = 2;
-    return nb * e->cus;
-}
-
 template <bool RC>
 void launch_march(nemo2d_ensemble* e, int step_idx) {
     const int k = e->cur;
-    int& grid = e->mgrid[e->fp32 ? 1 : 0][RC ? 1 : 0];
+    const int grid = march_waves(e, RC ? MARCH_RC_TILING : MARCH_TILING) / (EB / 64);
     if (!e->fp32) {
         double** x = (double**)e->X[k];
         double** y = (double**)e->X[1 - k];
         StateG<double, double, false> n{x[0], x[1], x[2], x[3], x[4], nullptr, nullptr, nullptr};
         OutG<double, double> a{y[0], y[1], y[2], y[3], y[4]};
-        if (!grid)
-            grid = RC ? resident_grid(e, k_ens_march_rc<double, double>)
-                      : resident_grid(e, k_ens_march<double, double>);
         if constexpr (RC)
             k_ens_march_rc<double, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
@@ -395,9 +433,6 @@ void launch_march(nemo2d_ensemble* e, int step_idx) {
         OutG<float, double> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
-        if (!grid)
-            grid = RC ? resident_grid(e, k_ens_march_rc<float, double>)
-                      : resident_grid(e, k_ens_march<float, double>);
         if constexpr (RC)
             k_ens_march_rc<float, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,

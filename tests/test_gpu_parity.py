@@ -533,14 +533,20 @@ def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
             assert_fields_equal(m, o, "%s step %d" % (variant, istp))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_dpf"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_dpf", "depths"])
 def test_tiny_values_take_the_ieee_divisions(variant):
     """Numerators below 2^-960 (as in the subnormal band that trails the wave
     front of a run from rest) make the kernels that divide by reciprocals
     fall back to IEEE division (per division, or by redoing the row): a state
     full of such values, and of zeros, must still match the oracle bit for
-    bit."""
-    o = Oracle(300, 70)
+    bit.  'depths': the same on a grid with per-point depths (its own
+    default kernels)."""
+    if variant == "depths":
+        o = random_grid_oracle(300, 70, 44, islands=False, open_edges="", vary_metrics=False,
+                               vary_depths=True)
+        variant = "auto"
+    else:
+        o = Oracle(300, 70)
     rng = np.random.default_rng(9)
     for k, scale in (("sshn", 1e-300), ("un", 1e-305), ("vn", 1e-310)):
         a = o.arr(k)
