@@ -82,6 +82,7 @@ struct nemo2d_model {
     double* st[NS] = {};
     bool uniform = false;
     bool uni_metrics = false;  // metrics and Coriolis uniform (depths may vary)
+    bool dep_derived = false;  // interior hu, hv are the means of adjacent ht
     Scal scal{};
     bool fused_ok = false;
 
@@ -2206,6 +2207,24 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     }
     m->uniform = uni;
     m->uni_metrics = unim;
+    {
+        // hu(i,j) = 0.5*(ht(i,j)+ht(i+1,j)) for i = 1..jpi-1 and hv(i,j) =
+        // 0.5*(ht(i,j)+ht(i,j+1)) for j = 1..jpj-1, bit for bit (arrays as
+        // the caller passes them: ht(1:jpi,1:jpj), hu(0:jpi,1:jpj),
+        // hv(1:jpi,0:jpj), column-major)
+        bool d = true;
+        for (int j = 1; j <= jpj && d; ++j)
+            for (int i = 1; i <= jpi - 1 && d; ++i) {
+                const double w = 0.5 * (ht[(i - 1) + (long)(j - 1) * jpi] + ht[i + (long)(j - 1) * jpi]);
+                d = memcmp(&w, &hu[i + (long)(j - 1) * (jpi + 1)], sizeof(double)) == 0;
+            }
+        for (int j = 1; j <= jpj - 1 && d; ++j)
+            for (int i = 1; i <= jpi && d; ++i) {
+                const double w = 0.5 * (ht[(i - 1) + (long)(j - 1) * jpi] + ht[(i - 1) + (long)j * jpi]);
+                d = memcmp(&w, &hv[(i - 1) + (long)j * jpi], sizeof(double)) == 0;
+            }
+        m->dep_derived = d;
+    }
     m->scal = Scal{e1t[0], e2t[0], e1u[0], e2u[0], e1v[0], e2v[0], e12t[0], e12u[0], e12v[0],
                    ht[0], hu[0], hv[0], fcu[0], fcv[0], {}, 0};
     set_reciprocals(m->scal, m->prm.fric);
@@ -2273,13 +2292,32 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
                                             "uniform metrics and latitude (depth may vary)");
         float** hf[3] = {&m->hf_t, &m->hf_u, &m->hf_v};
         const int sidx[3] = {S_HT, S_HU, S_HV};
+        // when hu, hv are the means of adjacent T depths, the FP32 ones are
+        // the FP32 means of the FP32 T depths (so the marching kernel can
+        // form them from ht on plain tiles); edge values as given
+        std::vector<double> dhu, dhv;
+        if (m->dep_derived) {
+            dhu.assign(hu, hu + nu);
+            dhv.assign(hv, hv + nv);
+            for (int j = 1; j <= jpj; ++j)
+                for (int i = 1; i <= jpi - 1; ++i)
+                    dhu[i + (long)(j - 1) * (jpi + 1)] = (double)(0.5f * ((float)ht[(i - 1) + (long)(j - 1) * jpi] +
+                                                                          (float)ht[i + (long)(j - 1) * jpi]));
+            for (int j = 1; j <= jpj - 1; ++j)
+                for (int i = 1; i <= jpi; ++i)
+                    dhv[(i - 1) + (long)j * jpi] = (double)(0.5f * ((float)ht[(i - 1) + (long)(j - 1) * jpi] +
+                                                                    (float)ht[(i - 1) + (long)j * jpi]));
+        }
         for (int q = 0; q < 3; ++q) {
             if (!*hf[q]) {
                 HIPCHK(hipMalloc((void**)hf[q], n * sizeof(float)));
                 HIPCHK(hipMemset(*hf[q], 0, n * sizeof(float)));
             }
             const auto& d = sd[sidx[q]];
-            int rc = upload_as<float>(m, *hf[q], d.a, d.lb1, d.ub1, d.lb2, d.ub2);
+            const double* src = d.a;
+            if (m->dep_derived && q == 1) src = dhu.data();
+            if (m->dep_derived && q == 2) src = dhv.data();
+            int rc = upload_as<float>(m, *hf[q], src, d.lb1, d.ub1, d.lb2, d.ub2);
             if (rc) return rc;
         }
     }
@@ -2731,7 +2769,7 @@ static MetG<double> metg(const Scal& d) {
 // The generic step that reads sshn, un, vn (and depths) only; the
 // plain-tile table of k_step_march_rc serves it (same strips, segments and
 // rows ahead).
-template <class R, class SS, bool DARR, int NJ, int LA>
+template <class R, class SS, bool DARR, int NJ, int LA, bool DV = false>
 static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& p,
                            const StateG<R, SS, DARR>& n, const OutG<R, SS>& a, double tide,
                            int& nparts) {
@@ -2742,7 +2780,7 @@ static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>&
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_gr<R, SS, DARR, NJ, LA><<<nblocks, 64 * GW, 0, m->stream>>>(
+    k_step_march_gr<R, SS, DARR, NJ, LA, DV><<<nblocks, 64 * GW, 0, m->stream>>>(
         mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, m->rc_plain);
     LAUNCHCHK("step_march_gr");
     nparts = nblocks;
@@ -2754,6 +2792,9 @@ static int launch_march_depths_rc(nemo2d_model* m, const NState& n, const AOut& 
     StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
                                     m->st[S_HV]};
     OutG<double, double> ag{a.sshn, a.sshn_u, a.sshn_v, a.un, a.vn};
+    if (m->dep_derived)
+        return launch_march_gr<double, double, true, 12, 2, true>(m, metg(m->scal), m->prm, ng, ag, tide,
+                                                                  nparts);
     return launch_march_gr<double, double, true, 12, 2>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
 }
 
@@ -2809,16 +2850,18 @@ static int step_mixed(nemo2d_model* m, int istp) {
                                           (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
             OutG<float, double> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                   (float*)y[F_UN], (float*)y[F_VN]};
-            rc = gr ? launch_march_gr<float, double, true, 12, 2>(m, mt, pf, n, a, tide, nparts)
-                    : launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts);
+            rc = !gr ? launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts)
+                 : m->dep_derived ? launch_march_gr<float, double, true, 12, 2, true>(m, mt, pf, n, a, tide, nparts)
+                                  : launch_march_gr<float, double, true, 12, 2>(m, mt, pf, n, a, tide, nparts);
         } else {
             StateG<float, float, true> n{(const float*)x[F_SSHN], (const float*)x[F_SSHN_U],
                                          (const float*)x[F_SSHN_V], (const float*)x[F_UN],
                                          (const float*)x[F_VN], m->hf_t, m->hf_u, m->hf_v};
             OutG<float, float> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                  (float*)y[F_UN], (float*)y[F_VN]};
-            rc = gr ? launch_march_gr<float, float, true, 12, 2>(m, mt, pf, n, a, tide, nparts)
-                    : launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts);
+            rc = !gr ? launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts)
+                 : m->dep_derived ? launch_march_gr<float, float, true, 12, 2, true>(m, mt, pf, n, a, tide, nparts)
+                                  : launch_march_gr<float, float, true, 12, 2>(m, mt, pf, n, a, tide, nparts);
         }
         if (rc) return rc;
         m->uv_stale[1 - k] = gr;

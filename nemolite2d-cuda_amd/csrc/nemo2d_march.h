@@ -402,7 +402,9 @@ struct ColG {
     unsigned f;
 };
 
-template <class R, class SS, bool DARR, bool PLAIN>
+// DV (with DARR): on plain tiles hu and hv are not loaded but formed from
+// ht as the means of the adjacent T depths (derived_depths)
+template <class R, class SS, bool DARR, bool PLAIN, bool DV = false>
 __device__ __forceinline__ void load_colg(ColG<R, SS>& r, const StateG<R, SS, DARR>& n,
                                           const uint16_t* fl, unsigned k) {
     r.un = ldg_(n.un, k);
@@ -410,8 +412,10 @@ __device__ __forceinline__ void load_colg(ColG<R, SS>& r, const StateG<R, SS, DA
     r.ss = ldg_(n.sshn, k);
     if constexpr (DARR) {
         r.ht = ldg_(n.ht, k);
-        r.hu = ldg_(n.hu, k);
-        r.hv = ldg_(n.hv, k);
+        if constexpr (!(PLAIN && DV)) {
+            r.hu = ldg_(n.hu, k);
+            r.hv = ldg_(n.hv, k);
+        }
     }
     if constexpr (PLAIN) r.f = G_PLAIN;
     else r.f = ldg_(fl, k);
@@ -483,9 +487,21 @@ __device__ __forceinline__ NorthG<R> north_gc(const MetG<R>& m, const ColG<R, SS
     return t;
 }
 
+// Depths at the U point (i, j) and V point (i, j) of a plain tile whose hu
+// and hv are the means of adjacent T depths: 0.5 * (ht(i) + ht(i+1)) and
+// 0.5 * (ht(j) + ht(j+1)), the expressions the grid was checked against.
+template <class R, class SS>
+__device__ __forceinline__ R hu_of(const ColG<R, SS>& c) {
+    return R(0.5) * (c.ht + lane_next(c.ht));
+}
+template <class R, class SS>
+__device__ __forceinline__ R hv_of(const ColG<R, SS>& c, const ColG<R, SS>& nr) {
+    return R(0.5) * (c.ht + nr.ht);
+}
+
 // One row j: A, B, C hold rows j-1, j, j+1 (C just arrived: its sshn_u is
 // formed here, and B's sshn_v); D receives row j+1+LA.
-template <class R, class SS, bool DARR, int LA, bool PLAIN>
+template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false>
 __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k,
                                         const ColG<R, SS>& A, ColG<R, SS>& B, ColG<R, SS>& C,
                                         ColG<R, SS>& D, int j) {
@@ -495,7 +511,11 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
     const unsigned kP = (unsigned)L.pitch;
     {
         const int jn = j + 1 + LA <= L.jpj + 2 ? j + 1 + LA : L.jpj + 2;
-        load_colg<R, SS, DARR, PLAIN>(D, x.n, x.fl, x.rowk(jn) + x.lk);
+        load_colg<R, SS, DARR, PLAIN, DV>(D, x.n, x.fl, x.rowk(jn) + x.lk);
+    }
+    if constexpr (DARR && PLAIN && DV) {
+        C.hu = hu_of(C);
+        B.hv = hv_of(B, C);
     }
     const unsigned f = B.f;
     const unsigned ko = x.rowk(j) + x.lk;  // element offset of (i, j)
@@ -616,17 +636,22 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
     k.hu_w = chu_w;
 }
 
-template <class R, class SS, bool DARR, int LA, bool PLAIN>
+template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false>
 __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k, int jb,
                                           int je) {
     const Layout& L = x.L;
     ColG<R, SS> R0, R1, R2, R3, R4;
-    load_colg<R, SS, DARR, PLAIN>(R0, x.n, x.fl, x.rowk(jb - 1) + x.lk);
-    load_colg<R, SS, DARR, PLAIN>(R1, x.n, x.fl, x.rowk(jb) + x.lk);
-    load_colg<R, SS, DARR, PLAIN>(R2, x.n, x.fl, x.rowk(jb + 1) + x.lk);
+    load_colg<R, SS, DARR, PLAIN, DV>(R0, x.n, x.fl, x.rowk(jb - 1) + x.lk);
+    load_colg<R, SS, DARR, PLAIN, DV>(R1, x.n, x.fl, x.rowk(jb) + x.lk);
+    load_colg<R, SS, DARR, PLAIN, DV>(R2, x.n, x.fl, x.rowk(jb + 1) + x.lk);
     if constexpr (LA == 2)
-        load_colg<R, SS, DARR, PLAIN>(R3, x.n, x.fl,
-                                      x.rowk(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lk);
+        load_colg<R, SS, DARR, PLAIN, DV>(R3, x.n, x.fl,
+                                          x.rowk(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lk);
+    if constexpr (DARR && PLAIN && DV) {
+        R0.hu = hu_of(R0);
+        R1.hu = hu_of(R1);
+        R0.hv = hv_of(R0, R1);
+    }
     R0.su = su_at_g<R, SS, DARR, PLAIN>(x, R0, lane_next(R0.ss), jb - 1, x.rowk(jb - 1) + x.lk);
     R1.su = su_at_g<R, SS, DARR, PLAIN>(x, R1, lane_next(R1.ss), jb, x.rowk(jb) + x.lk);
     R0.sv = sv_at_g<R, SS, DARR, PLAIN>(x, R0, R1.ss, jb - 1, x.rowk(jb - 1) + x.lk);
@@ -637,25 +662,25 @@ __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R
     k.hu_w = DARR ? lane_prev(R1.hu) : x.m.hu;
     if constexpr (LA == 1) {
         for (int j = jb; j <= je; j += 4) {
-            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R0, R1, R2, R3, j);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R0, R1, R2, R3, j);
             if (j + 1 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R1, R2, R3, R0, j + 1);
             if (j + 2 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R2, R3, R0, R1, j + 2);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R2, R3, R0, R1, j + 2);
             if (j + 3 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN>(x, k, R3, R0, R1, R2, j + 3);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R3, R0, R1, R2, j + 3);
         }
     } else {
         for (int j = jb; j <= je; j += 5) {
-            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R0, R1, R2, R4, j);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R0, R1, R2, R4, j);
             if (j + 1 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R1, R2, R3, R0, j + 1);
             if (j + 2 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R2, R3, R4, R1, j + 2);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R2, R3, R4, R1, j + 2);
             if (j + 3 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R3, R4, R0, R2, j + 3);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R3, R4, R0, R2, j + 3);
             if (j + 4 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN>(x, k, R4, R0, R1, R3, j + 4);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R4, R0, R1, R3, j + 4);
         }
     }
 }
@@ -663,7 +688,9 @@ __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R
 // plain: per (wave strip of GROUT columns, NJ-row segment) 1 where every
 // class word the wave touches (with LA rows loaded ahead) is G_PLAIN; null
 // for none
-template <class R, class SS, bool DARR, int NJ, int LA>
+// DV: the grid's hu and hv are the means of adjacent T depths (plain tiles
+// form them from ht instead of loading them).
+template <class R, class SS, bool DARR, int NJ, int LA, bool DV = false>
 __global__ void __launch_bounds__(64 * GW)
 k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
                 const uint16_t* __restrict__ fl, const double* __restrict__ flv, Layout L,
@@ -696,7 +723,7 @@ k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
         const int jb = seg * NJ;
         const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
         if (plain && plain[(long)strip * (ntiles / nbx) + seg])
-            rcg_march<R, SS, DARR, LA, true>(x, k, jb, je);
+            rcg_march<R, SS, DARR, LA, true, DV>(x, k, jb, je);
         else
             rcg_march<R, SS, DARR, LA, false>(x, k, jb, je);
     }

@@ -279,6 +279,15 @@ def random_grid_oracle(jpi, jpj, seed, islands=True, open_edges="WEN", vary_metr
             a = o.arr(k)
             a[:] = 30.0 + 20.0 * rng.random(a.shape)
         o.grid_areas()
+    elif vary_depths == "derived":
+        # hu, hv the means of adjacent T depths (as the seamount basins
+        # define them): the default kernel forms them from ht on plain tiles
+        ht, hu, hv = o.arr("ht"), o.arr("hu"), o.arr("hv")
+        ht[:] = 40.0 + 60.0 * rng.random(ht.shape)
+        hu[:, 1:jpi] = 0.5 * (ht[:, :-1] + ht[:, 1:])
+        hu[:, 0], hu[:, jpi] = ht[:, 0], ht[:, -1]
+        hv[1:jpj, :] = 0.5 * (ht[:-1, :] + ht[1:, :])
+        hv[0, :], hv[jpj, :] = ht[0, :], ht[-1, :]
     elif vary_depths:
         for k in ("ht", "hu", "hv"):
             a = o.arr(k)
@@ -476,12 +485,14 @@ def test_snapshots_behind_steps_equal_synchronous_state(tmp_path):
 
 @pytest.mark.parametrize("variant", ["auto", "march12_shd", "cell"])
 @pytest.mark.parametrize("islands", [False, True])
-def test_per_point_depths_bitwise_vs_oracle(variant, islands):
+@pytest.mark.parametrize("depths", [True, "derived"])
+def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
     """Uniform metrics, varying depths, the reference basin's open south edge
     (and inner walls): the marching kernel with per-point depths (the
-    default for such grids) against the oracle."""
+    default for such grids) against the oracle; 'derived': hu and hv the
+    means of adjacent T depths, which the default kernel forms from ht."""
     o = random_grid_oracle(203, 61, 21 + islands, islands=islands, open_edges="", vary_metrics=False,
-                           vary_depths=True)
+                           vary_depths=depths)
     with model_like(o) as m:
         m.set_variant(variant)
         m.record_checksums(25)
@@ -543,7 +554,7 @@ def test_tiny_values_take_the_ieee_divisions(variant):
     default kernels)."""
     if variant == "depths":
         o = random_grid_oracle(300, 70, 44, islands=False, open_edges="", vary_metrics=False,
-                               vary_depths=True)
+                               vary_depths="derived")
         variant = "auto"
     else:
         o = Oracle(300, 70)
