@@ -121,7 +121,10 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default (26 on grids with uniform metrics and depths, else 1),
+ * choice): 0 default (by grid size: 6 up to 300,000 T cells with uniform
+ * statics or 100,000 with per-point depths, 13 up to 2,000,000 with uniform
+ * statics, then 26 with uniform statics, 27 with uniform metrics and
+ * per-point depths where it applies, else 1),
  * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
  * 5 marching and 6 per-cell with Flather points recomputed inline instead
  * of in the epilogue, 7/8 marching (16/32 rows) dividing by precomputed
@@ -145,8 +148,7 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
  * tiles class-free; 17 runs where 27 cannot, as for 22).
  * Marching needs uniform metrics, 2-16 and 18-26 also uniform depths, and
  * 17 and 27 Flather points only on row 0 with the inner point north; other
- * grids use 1 (or 6).  The default for grids with uniform metrics but
- * varying depths is 27 where it applies.  The single-precision form marches
+ * grids use 1 (or 6).  The single-precision form marches
  * where it applies (reading sshn, un, vn only, as 27, once a step has made
  * sshn_u/sshn_v sshn's interpolations; 17 chooses its five-array kernel)
  * unless 1 is chosen.  No counterpart in the reference: a tuning hook. */

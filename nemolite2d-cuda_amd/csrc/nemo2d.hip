@@ -2923,6 +2923,13 @@ int nemo2d_step(nemo2d_model* m, int istp) {
 
 // One fused step of the FP64 model with the given tide; logs the checksums
 // into slot `slot` when recording covers it.
+// Grid sizes (T cells) up to which the default kernel is one thread per cell
+// (uniform statics / per-point depths), and up to which a uniform grid takes
+// the five-array marching kernel.
+constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
+constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
+constexpr long AUTO_SHN_MAX = 2000000;
+
 static int fused_step(nemo2d_model* m, double tide, int slot) {
     const Layout& L = m->L;
     const int k = m->cur;
@@ -2931,8 +2938,22 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
            m->S[1 - k][F_VN]};
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
-    if (var == VAR_AUTO)
-        var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
+    if (var == VAR_AUTO) {
+        // A marching wave walks its 12 rows one after another, so on a small
+        // grid (few waves) the step is the latency of one wave's march; there
+        // one thread per cell is faster.  Crossovers measured on MI355X
+        // (device us per step, default / one thread per cell / five-array
+        // marching): uniform 512^2 31.2 / 16.0 / 20.4, 1024^2 38.9 / 42.5 /
+        // 29.1, 2048^2 64.9 / 166.7 / 94.4; per-point depths 256^2 22.3 /
+        // 12.5, 512^2 23.3 / 26.2.
+        const long cells = (long)L.jpi * L.jpj;
+        if (cells <= (m->uniform ? AUTO_CELL_MAX_UNIFORM : AUTO_CELL_MAX_DEPTHS))
+            var = VAR_CELL_FLIN;
+        else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
+            var = VAR_MARCH12_SHN;
+        else
+            var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
+    }
     const bool depth_var = var == VAR_MARCH12_SHD || var == VAR_MARCH12_GRD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics

@@ -483,14 +483,15 @@ def test_snapshots_behind_steps_equal_synchronous_state(tmp_path):
         assert_fields_equal(m, o, "after snapshots")
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_shd", "cell"])
+@pytest.mark.parametrize("variant", ["auto", "march12_grd", "march12_shd", "cell"])
 @pytest.mark.parametrize("islands", [False, True])
 @pytest.mark.parametrize("depths", [True, "derived"])
 def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
     """Uniform metrics, varying depths, the reference basin's open south edge
     (and inner walls): the marching kernel with per-point depths (the
-    default for such grids) against the oracle; 'derived': hu and hv the
-    means of adjacent T depths, which the default kernel forms from ht."""
+    default for such large grids, named here since this one is small)
+    against the oracle; 'derived': hu and hv the means of adjacent T depths,
+    which that kernel forms from ht."""
     o = random_grid_oracle(203, 61, 21 + islands, islands=islands, open_edges="", vary_metrics=False,
                            vary_depths=depths)
     with model_like(o) as m:
@@ -504,7 +505,8 @@ def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
         assert np.all(np.isfinite(m.checksum_log(25)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2n", "march12_rc2np"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2n", "march12_rc2np",
+                                     "march12_rcg2np"])
 @pytest.mark.parametrize("islands", [False, True])
 def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
     """The fused kernel that reads sshn, un and vn only (sshn_u and sshn_v
@@ -526,7 +528,7 @@ def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
         assert np.all(np.isfinite(m.checksum_log(40)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2np"])
 def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
     """A state whose sshn_u/sshn_v are not sshn's interpolations (set by the
     caller) must be stepped from the stored values: the kernel that recomputes
@@ -544,18 +546,19 @@ def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
             assert_fields_equal(m, o, "%s step %d" % (variant, istp))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_dpf", "depths"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_rcg2np", "march12_dpf",
+                                     "depths"])
 def test_tiny_values_take_the_ieee_divisions(variant):
     """Numerators below 2^-960 (as in the subnormal band that trails the wave
     front of a run from rest) make the kernels that divide by reciprocals
     fall back to IEEE division (per division, or by redoing the row): a state
     full of such values, and of zeros, must still match the oracle bit for
-    bit.  'depths': the same on a grid with per-point depths (its own
-    default kernels)."""
+    bit.  'depths': the same on a grid with per-point depths (the marching
+    kernel for such grids)."""
     if variant == "depths":
         o = random_grid_oracle(300, 70, 44, islands=False, open_edges="", vary_metrics=False,
                                vary_depths="derived")
-        variant = "auto"
+        variant = "march12_grd"
     else:
         o = Oracle(300, 70)
     rng = np.random.default_rng(9)
