@@ -84,6 +84,7 @@ def lib():
         L.nemo2d_bc_ssh.argtypes = [vp, d]
         L.nemo2d_step.argtypes = [vp, i]
         L.nemo2d_set_variant.argtypes = [vp, i]
+        L.nemo2d_set_stage_fusion.argtypes = [vp, i]
         L.nemo2d_set_tide.argtypes = [vp, d, d]
         L.nemo2d_ensemble_create.argtypes = [ctypes.POINTER(vp), i, ip, ip, dp, dp, dp, d, d, d, d, d, i]
         L.nemo2d_ensemble_reset.argtypes = [vp]

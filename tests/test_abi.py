@@ -66,9 +66,13 @@ def test_null_arguments_return_status():
     assert L.nemo2d_create(ctypes.byref(h), 0, 4, 1.0, 1.0, 1.0, 1, 1, 1, 1.0, 0.0, 0.0) == nemo2d.ERR_ARG
     assert b"grid size" in L.nemo2d_last_error()
     assert L.nemo2d_create(ctypes.byref(h), 4, 4, 1.0, 1.0, 1.0, 1, 1, 1, 0.0, 0.0, 0.0) == nemo2d.ERR_ARG
-    for n in ("nemo2d_continuity", "nemo2d_momentum_u", "nemo2d_next"):
+    for n in ("nemo2d_continuity", "nemo2d_momentum_u", "nemo2d_momentum_v", "nemo2d_bc_solid_u",
+              "nemo2d_bc_solid_v", "nemo2d_bc_flather_u", "nemo2d_bc_flather_v", "nemo2d_next"):
         assert getattr(L, n)(None) == nemo2d.ERR_ARG
+    assert L.nemo2d_bc_ssh(None, 0.0) == nemo2d.ERR_ARG
     assert L.nemo2d_step(None, 1) == nemo2d.ERR_ARG
+    assert L.nemo2d_set_stage_fusion(None, 1) == nemo2d.ERR_ARG
+    assert L.nemo2d_set_variant(None, 0) == nemo2d.ERR_ARG
     assert L.nemo2d_destroy(None) == nemo2d.OK
 
 
