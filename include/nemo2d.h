@@ -148,7 +148,8 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
  * tiles class-free; 17 runs where 27 cannot, as for 22).
  * Marching needs uniform metrics, 2-16 and 18-26 also uniform depths, and
  * 17 and 27 Flather points only on row 0 with the inner point north; other
- * grids use 1 (or 6).  The single-precision form marches
+ * grids use 1 (or 6).  The single-precision form (above 300,000 T cells
+ * by default, or with any variant but 1) marches
  * where it applies (reading sshn, un, vn only, as 27, once a step has made
  * sshn_u/sshn_v sshn's interpolations; 17 chooses its five-array kernel)
  * unless 1 is chosen.  No counterpart in the reference: a tuning hook. */

@@ -2816,6 +2816,16 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
     return NEMO2D_OK;
 }
 
+// Grid sizes (T cells) up to which the default kernel is one thread per cell
+// (FP64 with uniform statics / with per-point depths / the single-precision
+// form), and up to which a uniform FP64 grid takes the five-array marching
+// kernel.  On smaller grids a marching step lasts as long as one wave's
+// march of its segment, whatever the grid size.
+constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
+constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
+constexpr long AUTO_CELL_MAX_MIXED = 300000;
+constexpr long AUTO_SHN_MAX = 2000000;
+
 extern "C" {
 
 static int step_mixed(nemo2d_model* m, int istp) {
@@ -2832,7 +2842,11 @@ static int step_mixed(nemo2d_model* m, int istp) {
     dim3 g = grid2d(L.jpi + 1, L.jpj + 1);
     void** x = m->X[k];
     void** y = m->X[1 - k];
-    if (m->variant != VAR_CELL && flather_row0_only(m) && sh_fits(L)) {
+    // one thread per cell when chosen, or by default on small grids (FP32
+    // seamount basins, device us per step, marching / per cell: 512^2 20.3 /
+    // 14.3, 1024^2 29.3 / 36.7)
+    const bool small = m->variant == VAR_AUTO && (long)L.jpi * L.jpj <= AUTO_CELL_MAX_MIXED;
+    if (m->variant != VAR_CELL && !small && flather_row0_only(m) && sh_fits(L)) {
         // the kernel that reads sshn, un, vn only once sshn_u/sshn_v are sshn's
         // interpolations (after any step)
         const bool gr = m->uv_derived && m->variant != VAR_MARCH12_SHD;
@@ -2923,13 +2937,6 @@ int nemo2d_step(nemo2d_model* m, int istp) {
 
 // One fused step of the FP64 model with the given tide; logs the checksums
 // into slot `slot` when recording covers it.
-// Grid sizes (T cells) up to which the default kernel is one thread per cell
-// (uniform statics / per-point depths), and up to which a uniform grid takes
-// the five-array marching kernel.
-constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
-constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
-constexpr long AUTO_SHN_MAX = 2000000;
-
 static int fused_step(nemo2d_model* m, double tide, int slot) {
     const Layout& L = m->L;
     const int k = m->cur;

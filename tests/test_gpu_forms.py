@@ -43,20 +43,25 @@ def test_fp32_error_against_fp64_and_compensation_ablation():
 
 @pytest.mark.parametrize("compensated", [True, False])
 def test_fp32_marching_kernel_equals_per_cell_kernel(compensated):
-    """The single-precision form's marching kernel (shared face terms,
-    per-point depths) gives the per-cell kernel's fields bit for bit."""
+    """The single-precision form's marching kernels (shared face terms,
+    per-point depths: the one reading sshn, un, vn and the depths, which
+    forms hu, hv from ht on this basin, and the five-array one) give the
+    per-cell kernel's fields bit for bit; 'auto' (per cell at this size)
+    too."""
     ni, nj, nsteps = 211, 97, 60
     grid, _ = nemo2d.seamount_grid(ni, nj, seed=5, radius=(5e3, 30e3))
     out = {}
-    for variant in ("auto", "cell"):
+    for variant in ("march12_grd", "march12_shd", "auto", "cell"):
         with nemo2d.Model(ni, nj, grid=grid, precision="fp32", compensated=compensated,
                           cbfr=0.0015) as m:
             m.set_variant(variant)
             m.run(1, nsteps)
             out[variant] = m.retrieve()
-    for k in ("sshn", "un", "vn", "sshn_u", "sshn_v"):
-        a, b = out["auto"][k], out["cell"][k]
-        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (k, np.abs(a - b).max())
+    for variant in ("march12_grd", "march12_shd", "auto"):
+        for k in ("sshn", "un", "vn", "sshn_u", "sshn_v"):
+            a, b = out[variant][k], out["cell"][k]
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (variant, k,
+                                                                          np.abs(a - b).max())
 
 
 def test_fp32_rest_stays_at_rest_and_tide_row():
