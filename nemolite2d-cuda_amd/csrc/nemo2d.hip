@@ -2418,10 +2418,11 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     } while (0)
 
 // ---- the nine stage calls -------------------------------------------------------
-// Each stage's kernel (the eager path).  original/nemolite2d.f90:85-93 calls
-// them in this order every step; made in that order on a grid the fused step
-// handles, they are recorded instead (stage_defer) and nemo2d_next runs the
-// fused step, which gives the nine stages' and next's results bit for bit.
+// Each stage's kernel (the eager path).  original/nemolite2d.f90:375-386
+// calls them in this order every step; made in that order on a grid the
+// fused step handles, they are recorded instead (stage_call, stage_defers)
+// and nemo2d_next runs the fused step, which gives the nine stages' and
+// next's results bit for bit.
 // A call out of order, or any other entry point, first runs the recorded
 // stages one by one (lz_flush).
 enum { LZ_CONT = 0, LZ_MOMU, LZ_MOMV, LZ_SSH, LZ_SOLU, LZ_SOLV, LZ_FLU, LZ_FLV, LZ_N };
