@@ -415,6 +415,7 @@ NAMELIST = """&namctl
 /
 &namhip
    fused = {fused}
+   stage_fusion = {sf}
    write_output = .true.
    async_output = {asy}
    csfile = 'cs.txt'
@@ -422,14 +423,22 @@ NAMELIST = """&namctl
 """
 
 
-@pytest.mark.parametrize("fused,asy", [(".true.", ".false."), (".false.", ".false."),
-                                      (".true.", ".true."), (".false.", ".true.")])
-def test_fortran_driver(tmp_path, fused, asy):
+@pytest.mark.parametrize("fused,asy,sf", [(".true.", ".false.", ".true."),
+                                         (".false.", ".false.", ".true."),
+                                         (".false.", ".false.", ".false."),
+                                         (".true.", ".true.", ".true."),
+                                         (".false.", ".true.", ".true."),
+                                         (".false.", ".true.", ".false.")])
+def test_fortran_driver(tmp_path, fused, asy, sf):
+    """The driver's three step paths: the fused step, the nine calls run as
+    one fused step (stage_fusion, the default) and the nine calls one kernel
+    each; with snapshots written in line or behind the steps."""
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "nemolite2d-cuda_amd", "bin", "nemolite2d_hip")
     g, n = 66, 300
-    (tmp_path / "namelist").write_text(NAMELIST.format(g=g, n=n, irec=150, fused=fused, asy=asy))
+    (tmp_path / "namelist").write_text(NAMELIST.format(g=g, n=n, irec=150, fused=fused, asy=asy,
+                                                       sf=sf))
     p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     o = Oracle(g - 2, g - 2)
