@@ -194,14 +194,25 @@ def load_traffic(variant, n):
     return d.get("bytes_per_launch") if d.get("variant") == v else None
 
 
-def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True):
+def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_going=False):
+    """The forms' figures; with keep_going a form that raises is reported as
+    {"error": ...} (printed to stderr too) and the others still run."""
+    runs = {"fp32_mixed": lambda: form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
+            "ensemble": lambda: form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc)),
+                                              graph=graph),
+            "closure": lambda: form_closure(max(32, int(256 * sc)), 4, 16, 300, 20)}
     out = {}
-    if "fp32_mixed" in which:
-        out["fp32_mixed"] = form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc)))
-    if "ensemble" in which:
-        out["ensemble"] = form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc)), graph=graph)
-    if "closure" in which:
-        out["closure"] = form_closure(max(32, int(256 * sc)), 4, 16, 300, 20)
+    for name in ("fp32_mixed", "ensemble", "closure"):
+        if name not in which:
+            continue
+        if not keep_going:
+            out[name] = runs[name]()
+            continue
+        try:
+            out[name] = runs[name]()
+        except Exception as e:  # the headline figure is still reported
+            print("bench.py: form %s failed: %r" % (name, e), file=sys.stderr, flush=True)
+            out[name] = {"error": repr(e)}
     return out
 
 
@@ -337,7 +348,7 @@ def main():
     if variants:
         out["variants_ms_per_step"] = variants
     if not args.no_forms:
-        out["forms"] = run_forms(args.forms_scale)
+        out["forms"] = run_forms(args.forms_scale, keep_going=True)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     else:
