@@ -15,8 +15,10 @@ Prints one JSON line (metric/unit as the benchmark definition names them):
   roofline       the fused step kernel against MI355X HBM (8 TB/s), with
                  80 algorithmic bytes per cell-update (5 FP64 state arrays
                  read once and written once) over its average launch time
-  cpu_baseline   the C oracle (a restatement of the serial Fortran) on one
-                 host core over a bounded sample of the same workload
+  cpu_baseline   the reference's serial Fortran model itself (built from its
+                 sources into oracle/_ref) on one host core over a bounded
+                 sample of the same workload (steps 1..8); the C oracle (a
+                 restatement of it) where that binary is absent
 """
 import argparse
 import json
@@ -32,6 +34,62 @@ sys.path.insert(0, os.path.join(ROOT, "nemolite2d-cuda_amd"))
 METRIC = "cell-updates/s at 4096² FP64 vs FP32/ensemble/closure; % HBM peak; SSH RMS err"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 80.0
+
+
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "nemolite2d_ref")
+REF_NAMELIST = """&namctl
+   jpiglo = {g}
+   jpjglo = {g}
+   jphgr_msh = 1
+   dx = 1000.0
+   dy = 1000.0
+   dep_const = 100.0
+   nit000 = 1
+   nitend = {n}
+   irecord = {r}
+   rdt = 20.0
+   cbfr = 0.0015
+   visc = 0.1
+/
+"""
+
+
+def cpu_baseline_reference(n, steps):
+    """The reference's own serial model (the binary oracle/_ref builds from
+    original/nemolite2d.f90) on one core: steps 1..steps of the same run from
+    rest, timed by its own dl_timer 'Time-stepping' region
+    (original/nemolite2d.f90:70-79).  Its unconditional istp=0 snapshot
+    (:67-68) goes to /dev/null.  None if the binary is absent or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    if not os.access(REF_EXE, os.X_OK):
+        return None
+    d = tempfile.mkdtemp(prefix="nemo2d_ref_")
+    try:
+        with open(os.path.join(d, "namelist"), "w") as f:
+            f.write(REF_NAMELIST.format(g=n + 2, n=steps, r=steps + 1))
+        os.symlink(os.devnull, os.path.join(d, "go2d_00000.dat"))
+        core = sorted(os.sched_getaffinity(0))[0]
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        p = subprocess.run([REF_EXE], cwd=d, env=env, capture_output=True, text=True, timeout=600,
+                           preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+        if p.returncode != 0:
+            print("bench.py: reference binary failed (%d): %s" % (p.returncode, p.stderr[-500:]),
+                  file=sys.stderr)
+            return None
+        rows = [l.split() for l in p.stdout.splitlines() if l.startswith("Time-stepping")]
+        t = float(rows[-1][2])
+    except (OSError, subprocess.SubprocessError, IndexError, ValueError) as e:
+        print("bench.py: reference binary not timed: %r" % (e,), file=sys.stderr)
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"value": n * n * steps / t, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": "the reference's serial model (original/nemolite2d.f90 and the dl_esm_inf/"
+                      "dl_timer sources it links, built by flang -O2 -ffp-contract=off into "
+                      "oracle/_ref), steps 1..%d of the %dx%d run from rest, its dl_timer "
+                      "'Time-stepping' region: %.2f s, one core" % (steps, n, n, t)}
 
 
 def cpu_baseline(n, steps):
@@ -350,7 +408,9 @@ def main():
     if not args.no_forms:
         out["forms"] = run_forms(args.forms_scale, keep_going=True)
     if args.cpu_steps > 0:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+        # the reference binary when it is there and runs, else the oracle port
+        out["cpu_baseline"] = (cpu_baseline_reference(n, args.cpu_steps)
+                               or cpu_baseline(n, args.cpu_steps))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
