@@ -17,7 +17,7 @@ Prints one JSON line (metric/unit as the benchmark definition names them):
                  read once and written once) over its average launch time
   cpu_baseline   the reference's serial Fortran model itself (built from its
                  sources into oracle/_ref) on one host core over a bounded
-                 sample of the same workload (steps 1..8); the C oracle (a
+                 sample of the same workload (steps 1..16); the C oracle (a
                  restatement of it) where that binary is absent
 """
 import argparse
@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096, help="interior grid size (jpi = jpj)")
-    ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=16, help="CPU baseline sample (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also reset the state 3 more times: each reset issues device-to-"
                          "device copies of the five padded state arrays (plain copy "
