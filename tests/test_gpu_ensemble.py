@@ -127,6 +127,25 @@ def test_switching_tilings_mid_run(precision):
                 assert np.array_equal(r[f], ref[k][f]), (k, f)
 
 
+@pytest.mark.parametrize("tiling", [8, 9])
+def test_many_work_items_in_cost_order(tiling):
+    """More marching work items than the kernel has waves (the items are
+    dealt to the waves in cost order, several per wave, each keeping its own
+    checksum slot): members bitwise equal to their single runs, per-member
+    checksum logs equal to theirs."""
+    m = nemo2d.ensemble_members(40, seed=13, size=(180, 300))
+    nsteps = 25
+    with nemo2d.Ensemble(m, rdt=RDT) as e:
+        e.set_tiling(tiling)
+        e.run(1, nsteps, graph=True)
+        for k in (0, 17, 39):
+            r = e.retrieve(k)
+            ref, log_ref = single(k, m, nsteps)
+            for f in ("sshn", "un", "vn"):
+                assert np.array_equal(r[f], ref[f]), (k, f)
+            assert np.allclose(e.checksum_log(k, nsteps), log_ref, rtol=1e-13, atol=0), k
+
+
 def test_ensemble_rejects_bad_members():
     m = small_members()
     m["jpi"] = m["jpi"].copy()
