@@ -321,13 +321,16 @@ def main():
     m.set_variant(args.variant)
     rest = nemo2d.initial_state(nemo2d.default_grid(n, n))
 
-    # warm-up (steps, the recorded log's read and the on-demand checksums: the
-    # first launch of each kernel loads its code), then reset to rest
+    # warm-up: steps, the recorded log's read, the on-demand checksums and a
+    # read-back of the state (the first launch of each kernel loads its code,
+    # and the process's first large device-to-host copy sets up its copy path,
+    # ~6.5 ms), then reset to rest
     m.record_checksums(max(K, W))
     for istp in range(1, W + 1):
         m.step(istp)
     m.checksum_log(W)
     m.checksums()
+    m.retrieve()
     m.set_state(rest)
     m.record_checksums(K)
 
