@@ -2818,10 +2818,11 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
 }
 
 // Grid sizes (T cells) up to which the default kernel is one thread per cell
-// (FP64 with uniform statics / with per-point depths / the single-precision
-// form), and up to which a uniform FP64 grid takes the five-array marching
-// kernel.  On smaller grids a marching step lasts as long as one wave's
-// march of its segment, whatever the grid size.
+// (FP64 with uniform statics: Flather points inline / with per-point statics:
+// Flather points by the epilogue / the single-precision form), and up to which
+// a uniform FP64 grid takes the five-array marching kernel.  On smaller grids
+// a marching step lasts as long as one wave's march of its segment, whatever
+// the grid size.
 constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
 constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
 constexpr long AUTO_CELL_MAX_MIXED = 300000;
@@ -2956,7 +2957,7 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
         // 12.5, 512^2 23.3 / 26.2.
         const long cells = (long)L.jpi * L.jpj;
         if (cells <= (m->uniform ? AUTO_CELL_MAX_UNIFORM : AUTO_CELL_MAX_DEPTHS))
-            var = VAR_CELL_FLIN;
+            var = m->uniform ? VAR_CELL_FLIN : VAR_CELL;
         else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
             var = VAR_MARCH12_SHN;
         else
