@@ -122,7 +122,8 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
  * choice): 0 default (by grid size: 6 up to 300,000 T cells with uniform
- * statics, 1 up to 100,000 with per-point statics, 13 up to 2,000,000 with uniform
+ * statics and Flather points on row 0 only (1 with others), 1 up to 100,000
+ * with per-point statics, 13 up to 2,000,000 with uniform
  * statics, then 26 with uniform statics, 27 with uniform metrics and
  * per-point depths where it applies, else 1),
  * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,

@@ -2818,8 +2818,9 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
 }
 
 // Grid sizes (T cells) up to which the default kernel is one thread per cell
-// (FP64 with uniform statics: Flather points inline / with per-point statics:
-// Flather points by the epilogue / the single-precision form), and up to which
+// (FP64 with uniform statics: Flather points inline where they are all on
+// row 0, else by the epilogue / with per-point statics: Flather points by the
+// epilogue / the single-precision form), and up to which
 // a uniform FP64 grid takes the five-array marching kernel.  On smaller grids
 // a marching step lasts as long as one wave's march of its segment, whatever
 // the grid size.
@@ -2957,7 +2958,7 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
         // 12.5, 512^2 23.3 / 26.2.
         const long cells = (long)L.jpi * L.jpj;
         if (cells <= (m->uniform ? AUTO_CELL_MAX_UNIFORM : AUTO_CELL_MAX_DEPTHS))
-            var = m->uniform ? VAR_CELL_FLIN : VAR_CELL;
+            var = (m->uniform && flather_row0_only(m)) ? VAR_CELL_FLIN : VAR_CELL;
         else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
             var = VAR_MARCH12_SHN;
         else
