@@ -185,6 +185,27 @@ int nemo2d_retrieve_results(nemo2d_model *m, double *sshn, double *un, double *v
 /* Copy sshn_u (U), sshn_v (V) and ssha (T) back; any may be NULL. */
 int nemo2d_retrieve_aux(nemo2d_model *m, double *sshn_u, double *sshn_v, double *ssha);
 
+/* Copy one grid array back as the device holds it, in the host layout of
+ * nemo2d_set_grid.  which: 0..11 = e1t e2t e1u e2u e1v e2v e12t e12u e12v
+ * ht hu hv; 12, 13 = the Coriolis parameter 2*omega*sin(phi*d2r) at U, V
+ * points; 14, 15 = the Flather coefficient sqrt(g/h) at the Flather U, V
+ * points (0 elsewhere).  Synchronises.  Replaces
+ * cuda_retrieve_grid_constants_ (cuda/cuda_nemolite2d.cu:253-257,
+ * :566-576), which returns xt, yt and ht; xt and yt stay with the driver
+ * here (nemo2d_set_output_grid). */
+int nemo2d_retrieve_grid(nemo2d_model *m, int which, double *out);
+
+/* The per-point class words (bit fields of the U, V and T roles each point
+ * plays in the step's stages) that nemo2d_set_grid derived from the mask, as
+ * the device holds them, in the mask's layout (0:jpi+1, 0:jpj+1).
+ * Synchronises.  nemo2d_classify derives the same words on the host from a
+ * mask and the depths hu (U layout), hv (V layout), without a device.  No
+ * counterpart in the reference, whose kernels test the mask itself
+ * (cuda/cuda_nemolite2d.cu:815-902). */
+int nemo2d_retrieve_classes(nemo2d_model *m, unsigned short *out);
+int nemo2d_classify(int jpi, int jpj, const int *pt, const double *hu, const double *hv,
+                    unsigned short *out);
+
 /* Device time in milliseconds of the work issued on the model's stream
  * between nemo2d_timer_start and nemo2d_timer_stop (HIP events on that
  * stream).  timer_stop synchronises. */

@@ -1831,9 +1831,13 @@ void drop_pending(nemo2d_model* m) {
     m->pend_any = false;
 }
 
-int dalloc(double** p, size_t n) {
+// Allocate n doubles and zero them on stream s.  The zeroing is queued on
+// the stream that later fills or reads the buffer: a fill on the null stream
+// is not ordered against a stream created hipStreamNonBlocking, so a late
+// null-stream zeroing could land after an upload made on the model's stream.
+int dalloc(double** p, size_t n, hipStream_t s) {
     HIPCHK(hipMalloc((void**)p, n * sizeof(double)));
-    HIPCHK(hipMemset(*p, 0, n * sizeof(double)));
+    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(double), s));
     return NEMO2D_OK;
 }
 
@@ -2102,14 +2106,8 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
     m->dx = dx;
     m->dy = dy;
     m->dep_const = dep_const;
-    Layout& L = m->L;
-    L.jpi = jpi;
-    L.jpj = jpj;
-    // columns: guard (i=-1) .. i=jpi+2, with i=0 at COL0; pitch a multiple of
-    // 16 doubles (128 B)
-    L.pitch = ((long)COL0 + jpi + 3 + 15) / 16 * 16;
-    L.rows = (long)jpj + 4;
-    L.base = L.pitch + COL0;
+    m->L = Layout::of(jpi, jpj);
+    const Layout& L = m->L;
     int rc = NEMO2D_OK;
     auto bail = [&](int code) {
         nemo2d_destroy(m);
@@ -2121,12 +2119,13 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
     const size_t n = (size_t)L.size();
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f)
-            if ((rc = dalloc(&m->S[k][f], n))) return bail(rc);
-    if ((rc = dalloc(&m->ssha, n)) || (rc = dalloc(&m->ua, n)) || (rc = dalloc(&m->va, n)) ||
-        (rc = dalloc(&m->flu, n)) || (rc = dalloc(&m->flv, n)))
+            if ((rc = dalloc(&m->S[k][f], n, m->stream))) return bail(rc);
+    if ((rc = dalloc(&m->ssha, n, m->stream)) || (rc = dalloc(&m->ua, n, m->stream)) ||
+        (rc = dalloc(&m->va, n, m->stream)) || (rc = dalloc(&m->flu, n, m->stream)) ||
+        (rc = dalloc(&m->flv, n, m->stream)))
         return bail(rc);
     if (hipMalloc((void**)&m->flags, n * sizeof(uint16_t)) != hipSuccess ||
-        hipMemset(m->flags, 0, n * sizeof(uint16_t)) != hipSuccess)
+        hipMemsetAsync(m->flags, 0, n * sizeof(uint16_t), m->stream) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "flags allocation failed"));
     dim3 g = grid2d(jpi + 1, jpj + 1);
     // partial slots: the most blocks any step kernel launches (the
@@ -2139,8 +2138,11 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         const long mbp = (mb + 7) / 8 * 8;
         m->npart_max = (int)((long)(g.x * g.y) > mbp ? (long)(g.x * g.y) : mbp);
     }
-    if ((rc = dalloc(&m->partials, 6 * ((size_t)m->npart_max + 1))) || (rc = dalloc(&m->cs_now, 3)))
+    if ((rc = dalloc(&m->partials, 6 * ((size_t)m->npart_max + 1), m->stream)) ||
+        (rc = dalloc(&m->cs_now, 3, m->stream)))
         return bail(rc);
+    if (hipStreamSynchronize(m->stream) != hipSuccess)
+        return bail(fail(NEMO2D_ERR_HIP, "nemo2d_create: zeroing the device arrays failed"));
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "pinned allocation failed"));
     if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
@@ -2231,7 +2233,7 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     const size_t n = (size_t)L.size();
     for (int s = 0; s < NS; ++s) {
         if (!m->st[s]) {
-            int rc = dalloc(&m->st[s], n);
+            int rc = dalloc(&m->st[s], n, m->stream);  // zeroed before the upload below
             if (rc) return rc;
         }
         int rc = upload(m, m->st[s], sd[s].a, sd[s].lb1, sd[s].ub1, sd[s].lb2, sd[s].ub2);
@@ -2311,7 +2313,7 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
         for (int q = 0; q < 3; ++q) {
             if (!*hf[q]) {
                 HIPCHK(hipMalloc((void**)hf[q], n * sizeof(float)));
-                HIPCHK(hipMemset(*hf[q], 0, n * sizeof(float)));
+                HIPCHK(hipMemsetAsync(*hf[q], 0, n * sizeof(float), m->stream));
             }
             const auto& d = sd[sidx[q]];
             const double* src = d.a;
@@ -3056,7 +3058,7 @@ int nemo2d_stream_bandwidth(long n, int repeats, double* gbps) {
     if (n < 1 || repeats < 1 || !gbps) return fail(NEMO2D_ERR_ARG, "nemo2d_stream_bandwidth: bad argument");
     double* a[10] = {};
     int rc = NEMO2D_OK;
-    for (int k = 0; k < 10 && rc == NEMO2D_OK; ++k) rc = dalloc(&a[k], (size_t)n);
+    for (int k = 0; k < 10 && rc == NEMO2D_OK; ++k) rc = dalloc(&a[k], (size_t)n, nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == NEMO2D_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = fail(NEMO2D_ERR_HIP, "nemo2d_stream_bandwidth: event");
@@ -3166,7 +3168,7 @@ int nemo2d_record_checksums(nemo2d_model* m, int capacity) {
     m->cs_n = 0;
     m->cs_next = 0;
     if (capacity > 0) {
-        int rc = dalloc(&m->cs_log, 3 * (size_t)capacity);
+        int rc = dalloc(&m->cs_log, 3 * (size_t)capacity, m->stream);
         if (rc) return rc;
         m->cs_cap = capacity;
         m->cs_slot = 6 * ((long)m->npart_max + 1);
@@ -3315,6 +3317,50 @@ int nemo2d_retrieve_aux(nemo2d_model* m, double* sshn_u, double* sshn_v, double*
     return NEMO2D_OK;
 }
 
+int nemo2d_retrieve_grid(nemo2d_model* m, int which, double* out) {
+    if (!m || !out) return fail(NEMO2D_ERR_ARG, "nemo2d_retrieve_grid: bad argument");
+    if (!m->grid_set) return fail(NEMO2D_ERR_STATE, "nemo2d_retrieve_grid: no grid set");
+    if (which < 0 || which >= NS + 2)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_retrieve_grid: which must be 0.." + std::to_string(NS + 1));
+    const int jpi = m->L.jpi, jpj = m->L.jpj;
+    // host layouts of the statics in S_* order, then the Flather coefficients
+    static const char kind[NS + 2] = {'T', 'T', 'U', 'U', 'V', 'V', 'T', 'U', 'V',
+                                      'T', 'U', 'V', 'U', 'V', 'U', 'V'};
+    const double* src = which < NS ? m->st[which] : which == NS ? m->flu : m->flv;
+    const char k = kind[which];
+    int rc = k == 'T' ? download(m, out, src, 1, jpi, 1, jpj)
+             : k == 'U' ? download(m, out, src, 0, jpi, 1, jpj)
+                        : download(m, out, src, 1, jpi, 0, jpj);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_retrieve_classes(nemo2d_model* m, unsigned short* out) {
+    if (!m || !out) return fail(NEMO2D_ERR_ARG, "nemo2d_retrieve_classes: bad argument");
+    if (!m->grid_set) return fail(NEMO2D_ERR_STATE, "nemo2d_retrieve_classes: no grid set");
+    const int jpi = m->L.jpi, jpj = m->L.jpj;
+    const size_t w = (size_t)(jpi + 2) * sizeof(uint16_t);
+    HIPCHK(hipMemcpy2DAsync(out, w, m->flags + m->L.at(0, 0), m->L.pitch * sizeof(uint16_t), w, jpj + 2,
+                            hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return NEMO2D_OK;
+}
+
+int nemo2d_classify(int jpi, int jpj, const int* pt, const double* hu, const double* hv,
+                    unsigned short* out) {
+    if (jpi < 1 || jpj < 1 || !pt || !hu || !hv || !out)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_classify: bad argument");
+    const Layout L = Layout::of(jpi, jpj);
+    Classes cls;
+    std::string err;
+    const int rc = classify(L, pt, hu, hv, cls, err);
+    if (rc) return fail(rc, "nemo2d_classify: " + err);
+    for (int j = 0; j <= jpj + 1; ++j)
+        for (int i = 0; i <= jpi + 1; ++i) out[i + (long)j * (jpi + 2)] = cls.fl[L.at(i, j)];
+    return NEMO2D_OK;
+}
+
 int nemo2d_timer_start(nemo2d_model* m) {
     if (!m) return fail(NEMO2D_ERR_ARG, "nemo2d_timer_start: null model");
     HIPCHK(hipEventRecord(m->ev0, m->stream));
@@ -3343,7 +3389,9 @@ int nemo2d_create_mixed(nemo2d_model** out, int jpi, int jpj, double dx, double 
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f) {
             const size_t es = (f == F_SSHN && m->compensated) ? sizeof(double) : sizeof(float);
-            if (hipMalloc(&m->X[k][f], n * es) != hipSuccess || hipMemset(m->X[k][f], 0, n * es) != hipSuccess) {
+            if (hipMalloc(&m->X[k][f], n * es) != hipSuccess ||
+                hipMemsetAsync(m->X[k][f], 0, n * es, m->stream) != hipSuccess ||
+                hipStreamSynchronize(m->stream) != hipSuccess) {
                 nemo2d_destroy(m);
                 *out = nullptr;
                 return fail(NEMO2D_ERR_HIP, "nemo2d_create_mixed: allocation failed");
@@ -3357,7 +3405,7 @@ int nemo2d_diagnostics(nemo2d_model* m, double out[3]) {
     if (!out) return fail(NEMO2D_ERR_ARG, "nemo2d_diagnostics: null output");
     const Layout& L = m->L;
     if (!m->diag) {
-        int rc = dalloc(&m->diag, 3);
+        int rc = dalloc(&m->diag, 3, m->stream);
         if (rc) return rc;
     }
     dim3 g = grid2d(L.jpi + 1, L.jpj + 1);

@@ -31,6 +31,17 @@ struct Layout {
     long rows;   // storage rows (jpj + 4)
     __host__ __device__ long at(int i, int j) const { return base + (long)j * pitch + i; }
     __host__ __device__ long size() const { return pitch * rows; }
+    // columns: guard (i = -1) .. i = jpi+2, with i = 0 at COL0; pitch a
+    // multiple of 16 doubles (128 B); rows: guard (j = -1) .. j = jpj+2
+    static Layout of(int jpi, int jpj) {
+        Layout L{};
+        L.jpi = jpi;
+        L.jpj = jpj;
+        L.pitch = ((long)COL0 + jpi + 3 + 15) / 16 * 16;
+        L.rows = (long)jpj + 4;
+        L.base = L.pitch + COL0;
+        return L;
+    }
 };
 
 // Per-point classification, computed once on the host from the mask pt.

@@ -377,8 +377,12 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
         EHIP(hipMalloc((void**)&e->partials, 6 * (size_t)e->ntiles * sizeof(double)));
         e->ntiles_cap = e->ntiles;
     }
-    EHIP(hipMemcpy(e->tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice));
-    EHIP(hipMemcpy(e->dmem, e->hmem.data(), e->hmem.size() * sizeof(EnsMember), hipMemcpyHostToDevice));
+    // uploads on the ensemble's (non-blocking) stream, finished before the
+    // host buffers go out of scope
+    EHIP(hipMemcpyAsync(e->tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+    EHIP(hipMemcpyAsync(e->dmem, e->hmem.data(), e->hmem.size() * sizeof(EnsMember), hipMemcpyHostToDevice,
+                        e->stream));
+    EHIP(hipStreamSynchronize(e->stream));
     e->tiling = tiling;
     return NEMO2D_OK;
 }
@@ -495,7 +499,9 @@ int upload_tides(nemo2d_ensemble* e, int nit000, int nsteps) {
     if (e->tide_tab) (void)hipFree(e->tide_tab);
     e->tide_tab = nullptr;
     EHIP(hipMalloc((void**)&e->tide_tab, tab.size() * sizeof(double)));
-    EHIP(hipMemcpy(e->tide_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    EHIP(hipMemcpyAsync(e->tide_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                        e->stream));
+    EHIP(hipStreamSynchronize(e->stream));
     e->tide_stride = nsteps;
     return NEMO2D_OK;
 }
@@ -507,7 +513,7 @@ int ensure_log(nemo2d_ensemble* e, int nsteps) {
         EHIP(hipMalloc((void**)&e->log, 3 * (size_t)e->nmem * nsteps * sizeof(double)));
         e->log_cap = nsteps;
     }
-    EHIP(hipMemset(e->log, 0, 3 * (size_t)e->nmem * e->log_cap * sizeof(double)));
+    EHIP(hipMemsetAsync(e->log, 0, 3 * (size_t)e->nmem * e->log_cap * sizeof(double), e->stream));
     e->log_n = 0;
     return NEMO2D_OK;
 }
@@ -559,11 +565,7 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         if (jpi[k] < 1 || jpj[k] < 1 || !(dep[k] > 0.0) || !(period_h[k] > 0.0))
             return bail(set_error(NEMO2D_ERR_ARG, "nemo2d_ensemble_create: bad member " + std::to_string(k)));
         EnsMember M{};
-        M.L.jpi = jpi[k];
-        M.L.jpj = jpj[k];
-        M.L.pitch = ((long)COL0 + jpi[k] + 3 + 15) / 16 * 16;
-        M.L.rows = (long)jpj[k] + 4;
-        M.L.base = M.L.pitch + COL0;
+        M.L = Layout::of(jpi[k], jpj[k]);
         M.off = off;
         off += (M.L.size() + 63) / 64 * 64;
         const double e12 = dx * dy;
@@ -613,9 +615,13 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         std::copy(c.flu.begin(), c.flu.end(), hflu.begin() + M.off);
         std::copy(c.flv.begin(), c.flv.end(), hflv.begin() + M.off);
     }
-    if (hipMemcpy(e->fl, hfl.data(), hfl.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->flu, hflu.data(), hflu.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->flv, hflv.data(), hflv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(e->fl, hfl.data(), hfl.size() * sizeof(uint16_t), hipMemcpyHostToDevice, e->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(e->flu, hflu.data(), hflu.size() * sizeof(double), hipMemcpyHostToDevice, e->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(e->flv, hflv.data(), hflv.size() * sizeof(double), hipMemcpyHostToDevice, e->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess)
         return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: upload failed"));
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {

@@ -39,7 +39,8 @@ SYMBOLS = (
     "nemo2d_dev_zero", "nemo2d_closure_features", "nemo2d_closure_apply",
     "nemo2d_stream_bandwidth", "nemo2d_format_go2d", "nemo2d_set_output_grid",
     "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
-    "nemo2d_set_stage_fusion",
+    "nemo2d_set_stage_fusion", "nemo2d_retrieve_grid", "nemo2d_retrieve_classes",
+    "nemo2d_classify",
 )
 
 # nemo2d_set_variant choices
@@ -117,6 +118,10 @@ def lib():
         L.nemo2d_retrieve_results.argtypes = [vp] + [dp] * 5
         L.nemo2d_retrieve_aux.argtypes = [vp] + [dp] * 3
         L.nemo2d_timer_stop.argtypes = [vp, dp]
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        L.nemo2d_retrieve_grid.argtypes = [vp, i, dp]
+        L.nemo2d_retrieve_classes.argtypes = [vp, u16p]
+        L.nemo2d_classify.argtypes = [i, i, ip, dp, dp, u16p]
         for s in SYMBOLS:
             if s not in ("nemo2d_last_error", "nemo2d_version") and not s.startswith("nemo2d_nn_"):
                 getattr(L, s).restype = i
@@ -181,6 +186,19 @@ def initial_state(grid, sshn=None):
         sshn_v[jj, :] = 0.5 * (e12t[j1, :] * sshn[j1, :] + e12t[j2, :] * sshn[j2, :]) / grid["e12v"][jj, :]
     return {"sshn": sshn, "sshn_u": sshn_u, "sshn_v": sshn_v,
             "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
+
+
+def classify(grid):
+    """Class words the library derives from grid's mask and depths (host
+    code, no device), in the mask's layout."""
+    pt = np.ascontiguousarray(grid["pt"], dtype=np.int32)
+    jpi, jpj = pt.shape[1] - 2, pt.shape[0] - 2
+    hu = np.ascontiguousarray(grid["hu"], dtype=np.float64)
+    hv = np.ascontiguousarray(grid["hv"], dtype=np.float64)
+    out = np.zeros(pt.shape, dtype=np.uint16)
+    _check(lib().nemo2d_classify(jpi, jpj, pt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(hu),
+                                 _dp(hv), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+    return out
 
 
 def tide(rtime, amp=0.2, period_hours=12.42):
@@ -330,6 +348,29 @@ class Model:
                                              _dp(r["ua"]), _dp(r["va"])))
         _check(lib().nemo2d_retrieve_aux(self._h, _dp(r["sshn_u"]), _dp(r["sshn_v"]), _dp(r["ssha"])))
         return r
+
+    # nemo2d_retrieve_grid's array order, then the Coriolis parameters and
+    # the Flather coefficients
+    GRID_READBACK = ("e1t", "e2t", "e1u", "e2u", "e1v", "e2v", "e12t", "e12u", "e12v",
+                     "ht", "hu", "hv", "fcu", "fcv", "flu", "flv")
+
+    def retrieve_grid(self):
+        """{name: array} of every grid array as the device holds it."""
+        s = shapes(self.jpi, self.jpj)
+        out = {}
+        for k, name in enumerate(self.GRID_READBACK):
+            kind = "T" if name in ("e1t", "e2t", "e12t", "ht") else \
+                "U" if name in ("e1u", "e2u", "e12u", "hu", "fcu", "flu") else "V"
+            a = np.zeros(s[kind])
+            _check(lib().nemo2d_retrieve_grid(self._h, k, _dp(a)))
+            out[name] = a
+        return out
+
+    def retrieve_classes(self):
+        """The per-point class words on the device, in the mask's layout."""
+        out = np.zeros(shapes(self.jpi, self.jpj)["pt"], dtype=np.uint16)
+        _check(lib().nemo2d_retrieve_classes(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+        return out
 
     def diagnostics(self):
         """(volume, energy, wet area) of the current state, FP64 reductions."""

@@ -31,7 +31,16 @@ TIM_COBJ = $(addprefix $(OUT)/timc_,$(addsuffix .o,$(basename $(TIM_C))))
 
 EXE = $(OUT)/nemolite2d_ref
 
-all: $(EXE)
+# The same model with one added statement: after each step of the time loop
+# (original/nemolite2d.f90:74-77) it prints the step's checksums
+# SUM(ABS(.)) of sshn, un and vn over (1:jpi,1:jpj) -- the reference's own
+# field_checksum (shared/dl_esm_inf/finite_difference/src/field_mod.f90:
+# 935-956) -- to 17 significant digits (ES24.16).  The scratch copy the
+# statement is added to lives in $(OUT) only.
+TRACE_SRC = $(OUT)/nemolite2d_trace.f90
+TRACE_EXE = $(OUT)/nemolite2d_ref_trace
+
+all: $(EXE) $(TRACE_EXE)
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -47,6 +56,16 @@ $(EXE): | $(OUT)
 	  $(GCC) $(CFLAGS) -c $(TIM)/$$f -o $(OUT)/timc_$${f%.*}.o; done
 	$(FLANG) $(FFLAGS) -module-dir $(OUT) -I$(OUT) -c $(ORIG)/nemolite2d.f90 -o $(OUT)/nemolite2d.o
 	$(FLANG) $(FFLAGS) -o $@ $(OUT)/nemolite2d.o $(INF_OBJ) $(TIM_FOBJ) $(TIM_COBJ)
+
+$(TRACE_EXE): $(EXE) $(ORIG)/nemolite2d.f90
+	sed -e '/^ *CALL step *$$/a\
+        WRITE (*, "(A,I8,3ES24.16)") "step checksums ", istp, SUM(ABS(sshn(1:jpi, 1:jpj))), \&\
+            SUM(ABS(un(1:jpi, 1:jpj))), SUM(ABS(vn(1:jpi, 1:jpj)))' \
+	    $(ORIG)/nemolite2d.f90 > $(TRACE_SRC)
+	test "$$(grep -c '"step checksums "' $(TRACE_SRC))" = 1
+	mkdir -p $(OUT)/trace_mod
+	$(FLANG) $(FFLAGS) -module-dir $(OUT)/trace_mod -I$(OUT) -c $(TRACE_SRC) -o $(OUT)/nemolite2d_trace.o
+	$(FLANG) $(FFLAGS) -o $@ $(OUT)/nemolite2d_trace.o $(INF_OBJ) $(TIM_FOBJ) $(TIM_COBJ)
 
 clean:
 	rm -rf $(OUT)

@@ -18,11 +18,19 @@ records only numbers:
 * ``ref_go2d_10x10_00012.dat`` -- one such snapshot file byte for byte (a
   10x10 basin after 12 steps), the expected output of the library's
   snapshot writer on the same fields.
+* ``ref_step_checksums.npz`` -- per-step checksums at full precision from
+  ``oracle/_ref/nemolite2d_ref_trace``, the same model with one added
+  statement that prints SUM(ABS(.)) of sshn, un and vn over (1:jpi,1:jpj)
+  after every step (ES24.16, 17 significant digits, which a double round-
+  trips exactly).  Fortran's SUM adds the section in storage order, so a
+  sequential sum of the same fields in that order equals these values bit
+  for bit.
 
 Usage (from the repository root)::
 
     python tests/golden/make_golden.py            # 128^2 and tiny grids
     python tests/golden/make_golden.py --big      # adds 4096^2 (about 30 min)
+    python tests/golden/make_golden.py --trace    # per-step checksums (add --big: 4096^2)
 """
 import argparse
 import json
@@ -57,6 +65,8 @@ NAMELIST = """&namctl
 """
 
 LINE = re.compile(r"^\s*(ua|va) checksum =\s*(\S+)\s*$")
+TRACE_EXE = os.path.join(ROOT, "oracle", "_ref", "nemolite2d_ref_trace")
+TRACE_LINE = re.compile(r"^\s*step checksums\s+(-?\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s*$")
 
 
 def case_name(c):
@@ -87,6 +97,68 @@ def run_case(c, keep_dat=False):
         for x in dats:
             os.remove(os.path.join(d, x))
     return out, d, dats
+
+
+def run_trace(c):
+    """Per-step (istp, sum|sshn|, sum|un|, sum|vn|) of the trace build."""
+    d = os.path.join(RUNS, "trace_" + case_name(c))
+    if os.path.isdir(d):
+        shutil.rmtree(d)
+    os.makedirs(d)
+    with open(os.path.join(d, "namelist"), "w") as f:
+        f.write(NAMELIST.format(**c))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    p = subprocess.run([TRACE_EXE], cwd=d, env=env, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("trace run failed for %s:\n%s\n%s" % (d, p.stdout, p.stderr))
+    rows = []
+    for line in p.stdout.splitlines():
+        m = TRACE_LINE.match(line)
+        if m:
+            rows.append([float(m.group(1))] + [float(m.group(k)) for k in (2, 3, 4)])
+    shutil.rmtree(d)
+    if len(rows) != c["nitend"] - c["nit000"] + 1:
+        raise RuntimeError("trace of %s has %d steps" % (case_name(c), len(rows)))
+    return np.array(rows)
+
+
+def trace_cases(big):
+    cases = [base(cbfr=0.0015), base(cbfr=0.00015)]
+    for (gi, gj) in [(3, 3), (7, 5), (12, 12), (36, 20)]:
+        cases.append(base(jpiglo=gi, jpjglo=gj, nitend=400))
+    cases.append(base(jpiglo=40, jpjglo=30, dx=750.0, dy=1300.0, dep=55.0, rdt=15.0, cbfr=0.004,
+                      visc=30.0, nitend=300))
+    cases.append(base(jpiglo=40, jpjglo=30, nit000=200, nitend=499))
+    if big:
+        cases.append(base(jpiglo=4098, jpjglo=4098, nitend=1000, irecord=1000000))
+    return cases
+
+
+def make_traces(big, jobs):
+    path = os.path.join(HERE, "ref_step_checksums.npz")
+    known = {}
+    if os.path.exists(path):
+        z = np.load(path)
+        for k, nml in enumerate(json.loads(str(z["namelists"]))):
+            known[case_name(nml)] = (nml, z["t%d" % k])
+    todo = [c for c in trace_cases(big) if case_name(c) not in known]
+    todo.sort(key=lambda c: c["jpiglo"] * c["jpjglo"] * (c["nitend"] - c["nit000"] + 1))
+
+    def work(c):
+        t = run_trace(c)
+        print("trace", case_name(c), "%d steps" % len(t), flush=True)
+        return c, t
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for c, t in ex.map(work, todo):
+            known[case_name(c)] = (c, t)
+            items = sorted(known.values(), key=lambda r: case_name(r[0]))
+            arrays = {"t%d" % k: t for k, (_, t) in enumerate(items)}
+            np.savez_compressed(path + ".tmp.npz", namelists=json.dumps([n for n, _ in items]),
+                                source="oracle/_ref/nemolite2d_ref_trace built by oracle/ref.mk "
+                                       "(flang -O2 -ffp-contract=off); columns istp, sum|sshn|, "
+                                       "sum|un|, sum|vn| printed ES24.16", **arrays)
+            os.replace(path + ".tmp.npz", path)
 
 
 def base(**kw):
@@ -136,10 +208,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also run the 4096^2 cases")
     ap.add_argument("--jobs", type=int, default=6)
+    ap.add_argument("--trace", action="store_true", help="only the per-step checksum traces")
     args = ap.parse_args()
 
     subprocess.run(["make", "-f", os.path.join(ROOT, "oracle", "ref.mk")], cwd=ROOT, check=True)
     os.makedirs(RUNS, exist_ok=True)
+    if args.trace:
+        return make_traces(args.big, args.jobs)
 
     jpath = os.path.join(HERE, "ref_checksums.json")
     known = {}

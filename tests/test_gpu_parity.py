@@ -32,11 +32,33 @@ def state_from_oracle(o):
     return {k: np.array(o.arr(k)) for k in FIELDS}
 
 
+GRID_ARRAYS = ("e1t", "e2t", "e1u", "e2u", "e1v", "e2v", "e12t", "e12u", "e12v", "ht", "hu", "hv")
+
+
+def assert_grid_on_device(m, g):
+    """The device holds the grid as it was uploaded: every metric and depth
+    array bit for bit, and the class words the host derives from the mask."""
+    r = m.retrieve_grid()
+    for k in GRID_ARRAYS:
+        a, b = r[k], np.ascontiguousarray(g[k], dtype=np.float64)
+        if not np.array_equal(a.view(np.uint64), b.view(np.uint64)):
+            raise AssertionError("device %s differs from the uploaded array at %d points" % (
+                k, int(np.count_nonzero(a.view(np.uint64) != b.view(np.uint64)))))
+    cls = m.retrieve_classes()
+    host = nemo2d.classify(g)
+    if not np.array_equal(cls, host):
+        raise AssertionError("device class words differ from the host's at %d points" %
+                             int(np.count_nonzero(cls != host)))
+
+
 def model_like(o, **kw):
     p = o.params
-    return nemo2d.Model(o.jpi, o.jpj, dx=p["dx"], dy=p["dy"], dep_const=p["dep_const"],
-                        rdt=p["rdt"], cbfr=p["cbfr"], visc=p["visc"],
-                        grid=copy_grid_from_oracle(o), state=state_from_oracle(o), **kw)
+    g = copy_grid_from_oracle(o)
+    m = nemo2d.Model(o.jpi, o.jpj, dx=p["dx"], dy=p["dy"], dep_const=p["dep_const"],
+                     rdt=p["rdt"], cbfr=p["cbfr"], visc=p["visc"],
+                     grid=g, state=state_from_oracle(o), **kw)
+    assert_grid_on_device(m, g)
+    return m
 
 
 def assert_fields_equal(m, o, what=""):
