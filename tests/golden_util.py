@@ -64,3 +64,31 @@ def rel_close(a, b, rtol=1e-12):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return np.abs(a - b) <= rtol * np.maximum(np.abs(a), np.abs(b))
+
+
+def load_traces(max_cells=None):
+    """Per-step checksum traces of the reference (ref_step_checksums.npz):
+    list of (namelist, array (nsteps, 4) of istp, sum|sshn|, sum|un|, sum|vn|)."""
+    path = os.path.join(GOLDEN, "ref_step_checksums.npz")
+    z = np.load(path)
+    out = []
+    for k, nml in enumerate(json.loads(str(z["namelists"]))):
+        if max_cells and (nml["jpiglo"] - 2) * (nml["jpjglo"] - 2) > max_cells:
+            continue
+        out.append((nml, z["t%d" % k]))
+    return out
+
+
+def trace_id(t):
+    nml = t[0]
+    return "g%dx%d_n%d-%d_cb%s" % (nml["jpiglo"], nml["jpjglo"], nml["nit000"], nml["nitend"],
+                                   nml["cbfr"])
+
+
+def seq_sums(sshn, un, vn):
+    """The reference's SUM(ABS(.)) of sshn, un, vn over (1:jpi,1:jpj) in its
+    storage order (first index fastest), added one term at a time as its
+    SUM does: arrays in this package's layout (T (jpj,jpi); U (jpj,jpi+1)
+    from ji=0; V (jpj+1,jpi) from jj=0)."""
+    return np.array([np.cumsum(np.abs(a.ravel()))[-1] if a.size else 0.0
+                     for a in (sshn, un[:, 1:], vn[1:, :])])

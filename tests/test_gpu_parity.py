@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 import nemo2d
-from golden_util import e16_8, e16_8_close, load_golden, rel_close, trajectories
+from golden_util import (e16_8, e16_8_close, load_golden, load_traces, rel_close, seq_sums,
+                         trace_id, trajectories)
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -270,6 +271,45 @@ def test_reference_golden_checksums(traj):
         assert e16_8_close(log[k, 2], va), (nitend, log[k, 2], va)
 
 
+def check_against_trace(nml, t, path="run"):
+    """Per-step GPU checksum log within 1e-12 relative of the reference's
+    printed 17-digit sums, at every step; the fields at the middle and last
+    step, summed in the reference's order, equal to those digits bit for bit."""
+    n0, n1 = nml["nit000"], nml["nitend"]
+    n = n1 - n0 + 1
+    mid = n0 + n // 2
+    with nemo2d.Model.from_namelist(nml) as m:
+        m.record_checksums(n)
+        for a, b in ((n0, mid), (mid + 1, n1)):
+            if path == "run":
+                m.run(a, b)
+            else:
+                for istp in range(a, b + 1):
+                    m.step_stages(istp)
+            r = m.retrieve()
+            k = b - n0
+            seq = seq_sums(r["sshn"], r["un"], r["vn"])
+            if not np.array_equal(seq, t[k, 1:]):
+                raise AssertionError("step %d: gpu %r reference %r" % (b, seq, t[k, 1:]))
+        log = m.checksum_log(n)
+    assert log.shape == (n, 3)
+    ok = rel_close(log, t[:, 1:], 1e-12)
+    if not ok.all():
+        k, c = np.argwhere(~ok)[0]
+        raise AssertionError("step %d column %d: gpu %r reference %r" % (n0 + k, c, log[k, c], t[k, 1 + c]))
+
+
+@pytest.mark.parametrize("trace", load_traces(max_cells=130 * 130), ids=trace_id)
+def test_reference_trace_every_step(trace):
+    """The fused step against the reference's own per-step checksums."""
+    check_against_trace(*trace)
+
+
+def test_reference_trace_stage_path():
+    nml, t = [x for x in load_traces(max_cells=130 * 130) if x[0]["jpiglo"] == 130][0]
+    check_against_trace(nml, t, path="stages")
+
+
 # ---- general masks and metrics ------------------------------------------------
 def random_grid_oracle(jpi, jpj, seed, islands=True, open_edges="WEN", vary_metrics=True,
                        chained=False, vary_depths=False):
@@ -391,6 +431,13 @@ def test_4096_first_steps_bitwise_vs_oracle():
             m.step(istp)
             o.step(istp)
         assert_fields_equal(m, o, "4096 step 3")
+
+
+def test_4096_reference_trace_every_step():
+    """4096^2 x 1000 steps against the reference's own per-step sums."""
+    big = [x for x in load_traces() if x[0]["jpiglo"] == 4098]
+    assert big, "no 4096^2 trace in tests/golden"
+    check_against_trace(*big[0])
 
 
 def test_4096_reference_checksums_and_stage_path():

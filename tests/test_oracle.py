@@ -15,7 +15,8 @@ import os
 import numpy as np
 import pytest
 
-from golden_util import GOLDEN, e16_8, e16_8_close, load_golden, trajectories
+from golden_util import (GOLDEN, e16_8, e16_8_close, load_golden, load_traces, rel_close,
+                         seq_sums, trace_id, trajectories)
 from oracle import Oracle, fortran_e16_8, tide
 
 RUNS = load_golden()
@@ -180,3 +181,48 @@ def test_reflection_symmetry_without_coriolis():
     o2 = Oracle(10, 8)
     o2.run(1, 60)
     assert not np.allclose(o2.arr("sshn"), o2.arr("sshn")[:, ::-1], rtol=1e-9, atol=0)
+
+
+# ---- per-step checksums of the reference at full precision --------------------
+TRACES = load_traces(max_cells=130 * 130)
+
+
+def test_traces_present():
+    assert len(TRACES) >= 8
+    assert {t[0]["cbfr"] for t in TRACES if t[0]["jpiglo"] == 130} == {0.0015, 0.00015}
+
+
+@pytest.mark.parametrize("trace", TRACES, ids=trace_id)
+def test_oracle_per_step_checksums_equal_reference_trace(trace):
+    """Every step: the oracle's fields, summed as the reference's SUM adds
+    them (in storage order), give the reference's printed 17 digits bit for
+    bit; its compensated checksums are within 1e-12 relative."""
+    nml, t = trace
+    o = Oracle.from_namelist(nml)
+    n0, n1 = nml["nit000"], nml["nitend"]
+    assert np.array_equal(t[:, 0], np.arange(n0, n1 + 1))
+    for k, istp in enumerate(range(n0, n1 + 1)):
+        o.step(istp)
+        seq = seq_sums(o.arr("sshn"), o.arr("un"), o.arr("vn"))
+        if not np.array_equal(seq, t[k, 1:]):
+            raise AssertionError("step %d: oracle %r reference %r" % (istp, seq, t[k, 1:]))
+        assert np.all(rel_close(o.checksums(), t[k, 1:])), istp
+
+
+def test_sequential_sums_sensitivity():
+    """What the bitwise trace comparison can see: any one value off by one
+    ulp of the sum, or every value off by 1e-14 relative.  (Every value one
+    ulp off is below a 16384-term sum's own last bit: the fields themselves
+    are pinned bit for bit to the oracle by the GPU parity tests; the trace
+    pins the oracle's per-step sums to the reference's 17 digits.)"""
+    nml, t = [x for x in TRACES if x[0]["jpiglo"] == 130][0]
+    o = Oracle.from_namelist(nml)
+    for istp in range(1, 301):
+        o.step(istp)
+    s, u, v = np.array(o.arr("sshn")), np.array(o.arr("un")), np.array(o.arr("vn"))
+    assert np.array_equal(seq_sums(s, u, v), t[299, 1:])
+    assert seq_sums(s * (1 + 1e-14), u, v)[0] != t[299, 1]
+    assert seq_sums(s, u, v * (1 + 1e-14))[2] != t[299, 3]
+    s2 = s.copy()
+    s2[40, 50] += np.spacing(t[299, 1])
+    assert seq_sums(s2, u, v)[0] != t[299, 1]
