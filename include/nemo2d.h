@@ -273,15 +273,26 @@ int nemo2d_ensemble_destroy(nemo2d_ensemble *e);
  * 8 (BF16 stored as 16-bit integers), 1 <= channels <= 32, stream a
  * hipStream_t (NULL = default stream).  Convolutions are 'same'-padded 3x3,
  * implicit-GEMM on BF16 matrix cores with FP32 accumulation.
+ * conv_fwd writes the pre-activation b + conv(x) as FP32 into z and/or as
+ *   BF16 into zb, and BF16 gelu(it) (use_gelu) or it into act; any of the
+ *   three may be NULL (not all).
  *   wf: forward weights [9][32][pad8(cin)], wb: data-gradient weights
  *   [9][32][pad8(cout)] (both from nemo2d_nn_pack_weights of FP32
  *   W[9][cin][cout]). */
 int nemo2d_nn_conv_fwd(const void *x, int n, int h, int w, int cin, const void *wf,
-                       const float *bias, int cout, float *z, void *act, int use_gelu,
+                       const float *bias, int cout, float *z, void *zb, void *act, int use_gelu,
                        void *stream);
 int nemo2d_nn_conv_bwd_data(const void *dz, int n, int h, int w, int cout, const void *wb,
                             int cin, float *dx, void *stream);
-/* Work buffer size (floats) for nemo2d_nn_conv_bwd_weight over `pixels`. */
+/* The data gradient with the layer below's GELU backward fused in: dzp
+ * (BF16, [P][pad8(cin)]) = BF16(dX * GELU'(zp)), zp the layer below's
+ * pre-activation [P][pad8(cin)], BF16 bits if zp_bf16 else FP32; padded
+ * channels get 0. */
+int nemo2d_nn_conv_bwd_data_gelu(const void *dz, int n, int h, int w, int cout, const void *wb,
+                                 int cin, const void *zp, int zp_bf16, void *dzp, void *stream);
+/* Work buffer size (floats) for nemo2d_nn_conv_bwd_weight over `pixels`
+ * (per-block partial sums of the weight and bias gradients; the block count
+ * is capped, so the size does not grow with the batch). */
 long nemo2d_nn_wgrad_work_floats(long pixels);
 int nemo2d_nn_conv_bwd_weight(const void *x, const void *dz, int n, int h, int w, int cin,
                               int cout, float *dw, float *db, float *work, void *stream);
@@ -309,6 +320,19 @@ int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
  *   default stream first (where z is produced); synchronous. */
 int nemo2d_closure_features(nemo2d_model *m, const double *mean, const double *scale, void *x);
 int nemo2d_closure_apply(nemo2d_model *m, const float *z, int zstride, const double *scale);
+/* restrict: the coarse model's now-state becomes the restriction of the fine
+ *   model's (fine grid exactly 2x the coarse one per side; coarse cell (I, J)
+ *   covers fine cells 2I-1..2I x 2J-1..2J): sshn the mean of the 4 fine
+ *   cells ((a+b)+c)+d times 0.25, un on a coarse U face the fine-depth-
+ *   weighted mean (h0*u0 + h1*u1)/(h0 + h1) of the 2 fine U faces on it, vn
+ *   likewise; sshn_u, sshn_v from the new sshn by the next-stage rules
+ *   (original/nemolite2d.f90:735-762), 0 where next leaves a point alone.
+ *   As nemo2d_set_state, ua/va restart at zero.  Synchronous.
+ * target: y (device FP32 [jpj*jpi][8]) gets, per pixel, FP32((truth - pred)
+ *   / scale[c]) for the cell's sshn (c = 0), east-face un (1) and north-face
+ *   vn (2), and 0 in channels 3..7 (two models of one size).  Synchronous. */
+int nemo2d_closure_restrict(nemo2d_model *fine, nemo2d_model *coarse);
+int nemo2d_closure_target(nemo2d_model *truth, nemo2d_model *pred, const double *scale, float *y);
 
 /* Device memory for the closure network's tensors, on the current HIP
  * device (no counterpart in the reference).  Copies and zeroing are

@@ -1622,6 +1622,60 @@ k_closure_add(const float* __restrict__ z, int zs, Scale3 sc, Layout L, double* 
     if (j < L.jpj) vn[o] = vn[o] + (double)zp[2] * sc.s[2];
 }
 
+// Restriction of a fine state onto the 2x coarser grid (coarse cell (I, J)
+// covers fine cells 2I-1..2I x 2J-1..2J): ssh the mean of the 4 fine cells,
+// u on a coarse U face the depth-weighted mean of the 2 fine U faces on it,
+// v likewise.  One thread per coarse point (I, J), I in 0..jpi, J in 0..jpj.
+template <class H>
+__global__ void __launch_bounds__(TBX * TBY)
+k_restrict(const double* __restrict__ fs, const double* __restrict__ fu, const double* __restrict__ fv,
+           H hu, H hv, Layout F, Layout C, double* __restrict__ cs, double* __restrict__ cu,
+           double* __restrict__ cv) {
+    const int I = blockIdx.x * TBX + threadIdx.x, J = blockIdx.y * TBY + threadIdx.y;
+    if (I > C.jpi || J > C.jpj) return;
+    const long o = C.at(I, J);
+    if (I >= 1 && J >= 1) {
+        const long a = F.at(2 * I - 1, 2 * J - 1), b = a + 1, c = a + F.pitch, d = c + 1;
+        cs[o] = 0.25 * (((fs[a] + fs[b]) + fs[c]) + fs[d]);
+    }
+    if (J >= 1) {  // U face (I, J): fine faces (2I, 2J-1) and (2I, 2J)
+        const long a = F.at(2 * I, 2 * J - 1), b = a + F.pitch;
+        const double h0 = hu(a), h1 = hu(b);
+        cu[o] = (h0 * fu[a] + h1 * fu[b]) / (h0 + h1);
+    }
+    if (I >= 1) {  // V face (I, J): fine faces (2I-1, 2J) and (2I, 2J)
+        const long a = F.at(2 * I - 1, 2 * J), b = a + 1;
+        const double h0 = hv(a), h1 = hv(b);
+        cv[o] = (h0 * fv[a] + h1 * fv[b]) / (h0 + h1);
+    }
+}
+
+struct DepthArr {
+    const double* p;
+    __device__ double operator()(long o) const { return p[o]; }
+};
+struct DepthConst {
+    double v;
+    __device__ double operator()(long) const { return v; }
+};
+
+// Closure training target: y[p][c] = FP32((truth_c - pred_c) / scale[c]) for
+// the cell's ssh (c = 0), east-face u (1) and north-face v (2); 0 for c >= 3.
+__global__ void __launch_bounds__(TBX * TBY)
+k_closure_target(const double* __restrict__ ts, const double* __restrict__ tu, const double* __restrict__ tv,
+                 const double* __restrict__ ps, const double* __restrict__ pu, const double* __restrict__ pv,
+                 Layout L, Scale3 sc, float* __restrict__ y) {
+    const int i = blockIdx.x * TBX + threadIdx.x + 1, j = blockIdx.y * TBY + threadIdx.y + 1;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j);
+    float* yp = y + 8 * ((long)(j - 1) * L.jpi + (i - 1));
+    yp[0] = (float)((ts[o] - ps[o]) / sc.s[0]);
+    yp[1] = (float)((tu[o] - pu[o]) / sc.s[1]);
+    yp[2] = (float)((tv[o] - pv[o]) / sc.s[2]);
+#pragma unroll
+    for (int c = 3; c < 8; ++c) yp[c] = 0.f;
+}
+
 // sshn_u, sshn_v from sshn by the next-stage rules (original/nemolite2d.f90:
 // 735-762); the points next leaves alone keep their values.
 template <class S>
@@ -3254,6 +3308,75 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
     HIPCHK(hipStreamSynchronize(m->stream));
     m->a_current = false;
     m->uv_derived = true;
+    return NEMO2D_OK;
+}
+
+int nemo2d_closure_restrict(nemo2d_model* f, nemo2d_model* c) {
+    READY_STAGE(f, "nemo2d_closure_restrict");
+    if (!c || c == f) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_restrict: bad coarse model");
+    if (!c->grid_set || c->mixed)
+        return fail(NEMO2D_ERR_STATE, "nemo2d_closure_restrict: the coarse model needs a grid (FP64)");
+    if (f->L.jpi != 2 * c->L.jpi || f->L.jpj != 2 * c->L.jpj)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_closure_restrict: the fine grid must be 2x the coarse grid");
+    if (c->lz_n > 0) c->lz_n = 0;  // recorded stage calls act on the state this replaces
+    int rc = lz_flush(f);  // recorded stage calls of the fine model run first
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(f->stream));  // the fine state is final
+    if ((rc = snap_guard(c, -1))) return rc;
+    c->cur = 0;
+    const size_t bytes = (size_t)c->L.size() * sizeof(double);
+    double** fs = f->S[f->cur];
+    double** cs = c->S[0];
+    for (int k = 0; k < NF; ++k) HIPCHK(hipMemsetAsync(cs[k], 0, bytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->ssha, 0, bytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->ua, 0, bytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->va, 0, bytes, c->stream));
+    const dim3 g = grid2d(c->L.jpi + 1, c->L.jpj + 1);
+    if (f->uniform)
+        k_restrict<<<g, dim3(TBX, TBY), 0, c->stream>>>(fs[F_SSHN], fs[F_UN], fs[F_VN],
+                                                         DepthConst{f->scal.hu}, DepthConst{f->scal.hv},
+                                                         f->L, c->L, cs[F_SSHN], cs[F_UN], cs[F_VN]);
+    else
+        k_restrict<<<g, dim3(TBX, TBY), 0, c->stream>>>(fs[F_SSHN], fs[F_UN], fs[F_VN],
+                                                         DepthArr{f->st[S_HU]}, DepthArr{f->st[S_HV]},
+                                                         f->L, c->L, cs[F_SSHN], cs[F_UN], cs[F_VN]);
+    LAUNCHCHK("restrict");
+    // sshn_u/sshn_v by the next-stage rules; the points next leaves alone stay 0
+    WITH_STATICS(c, st, (k_interp<<<g, dim3(TBX, TBY), 0, c->stream>>>(st, c->flags, cs[F_SSHN], c->L,
+                                                                       cs[F_SSHN_U], cs[F_SSHN_V])));
+    LAUNCHCHK("restrict_interp");
+    for (int k = 0; k < NF; ++k)
+        HIPCHK(hipMemcpyAsync(c->S[1][k], cs[k], bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->uv_stale[0] = c->uv_stale[1] = false;
+    c->uv_derived = false;
+    c->a_current = true;
+    c->state_set = true;
+    c->cs_next = 0;
+    c->cs_n = 0;
+    drop_pending(c);
+    return NEMO2D_OK;
+}
+
+int nemo2d_closure_target(nemo2d_model* truth, nemo2d_model* pred, const double* scale, float* y) {
+    READY_STAGE(truth, "nemo2d_closure_target");
+    READY_STAGE(pred, "nemo2d_closure_target");
+    if (!scale || !y) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_target: null argument");
+    if (truth->L.jpi != pred->L.jpi || truth->L.jpj != pred->L.jpj)
+        return fail(NEMO2D_ERR_ARG, "nemo2d_closure_target: models of different sizes");
+    for (int k = 0; k < 3; ++k)
+        if (!(scale[k] != 0.0)) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_target: zero scale");
+    int rc = lz_flush(truth);
+    if (rc || (rc = lz_flush(pred))) return rc;
+    HIPCHK(hipStreamSynchronize(truth->stream));
+    const Layout& L = pred->L;
+    double** t = truth->S[truth->cur];
+    double** q = pred->S[pred->cur];
+    k_closure_target<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, pred->stream>>>(
+        t[F_SSHN], t[F_UN], t[F_VN], q[F_SSHN], q[F_UN], q[F_VN], L, Scale3{{scale[0], scale[1], scale[2]}},
+        y);
+    LAUNCHCHK("closure_target");
+    HIPCHK(hipStreamSynchronize(pred->stream));
     return NEMO2D_OK;
 }
 

@@ -67,153 +67,347 @@ __device__ __forceinline__ long shifted(const Img& im, long p, int dy, int dx) {
     return p + (long)dy * im.w + dx;
 }
 
-// Forward: tile of 32 pixels x 32 output channels per wave.
-// wt: [9][32][cinp] (cout-major, zero-padded), x: [P][cinp], out z: [P][coutp] FP32,
-// act: [P][coutp] BF16 (gelu(z) or z).
+// ---- forward and data-gradient convolutions -------------------------------
+// A block owns a tile of CT_H rows x CT_W = 32 pixels of one image.  The
+// input tile with its 1-pixel halo is staged once in LDS (pixel-major, the
+// channels of a pixel contiguous, padded to the MFMA K step); each of the 4
+// waves computes CT_H/4 row segments, one 32-pixel x 32-channel MFMA tile
+// (v_mfma_f32_32x32x16_bf16) each, over the 9 taps.  The weights of every
+// tap stay in registers.  Operand layout (32x32x16): lane l holds A row l%32
+// and B column l%32 at 8 consecutive k of half l/32; the K order is the same
+// for both operands, so any assignment of k to (channel) values works.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int CT_W = 32, CT_H = 8;
+constexpr int CT_PIX = (CT_H + 2) * (CT_W + 2);
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
+    const uint4 v = *(const uint4*)p;
+    bf16x8 r;
+    r[0] = (short)(v.x & 0xffff); r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xffff); r[3] = (short)(v.y >> 16);
+    r[4] = (short)(v.z & 0xffff); r[5] = (short)(v.z >> 16);
+    r[6] = (short)(v.w & 0xffff); r[7] = (short)(v.w >> 16);
+    return r;
+}
+
+// Stage src[n][y0-1 .. y0+CT_H][x0-1 .. x0+CT_W][0..cp) into LDS tile[pixel][KT + 8]
+// (channels cp..KT-1 and pixels outside the image zero).
+template <int KT>
+__device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ src, const Img& im, int cp,
+                                           int img, int y0, int x0, uint16_t* tile) {
+    constexpr int PS = KT + 8;  // LDS pixel stride (bf16): +16 B against bank conflicts
+    constexpr int CH = KT / 8;  // 16-byte chunks per pixel
+    for (int e = threadIdx.x; e < CT_PIX * CH; e += NB) {
+        const int q = e / CH, c8 = (e % CH) * 8;
+        const int yy = y0 - 1 + q / (CT_W + 2), xx = x0 - 1 + q % (CT_W + 2);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
+            v = *(const uint4*)(src + (((long)img * im.h + yy) * im.w + xx) * cp + c8);
+        *(uint4*)(tile + q * PS + c8) = v;
+    }
+}
+
+// Forward: z = b + sum_tap sum_ci X[p + off(tap)][ci] W[tap][ci][co];
+// act = gelu(z) (or z).  wt: [9][32][cinp] (cout-major, from pack).
+template <int KT>
 __global__ void __launch_bounds__(NB)
 k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __restrict__ wt,
            const float* __restrict__ bias, int cout, int coutp, float* __restrict__ z,
-           uint16_t* __restrict__ act, int use_gelu) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long tile0 = ((long)blockIdx.x * (NB / 64) + wave) * 32;
-    const long P = im.pixels();
-    if (tile0 >= P) return;  // whole wave leaves together
-    const long p = tile0 + (lane & 31);
-    const int half = lane >> 5;
-    f32x16 acc = {0};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const long ps = p < P ? shifted(im, p, dy, dx) : -1;
-        for (int c0 = 0; c0 < cinp; c0 += 8) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (ps >= 0) a = load4(x + ps * cinp + c0 + 4 * half);
-            const bf16x4 b = load4(wt + ((long)tap * 32 + (lane & 31)) * cinp + c0 + 4 * half);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-        }
-    }
-    const int co = lane & 31;
-    if (co >= cout) return;
-    const float bc = bias[co];
+           uint16_t* __restrict__ zb, uint16_t* __restrict__ act, int use_gelu) {
+    constexpr int PS = KT + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[CT_PIX * PS];
+    const int tiles_x = (im.w + CT_W - 1) / CT_W, tiles_y = (im.h + CT_H - 1) / CT_H;
+    const int b = blockIdx.x;
+    const int img = b / (tiles_x * tiles_y);
+    const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
+    // weights of this lane's column (output channel c), every tap and K step
+    bf16x8 wreg[9][KT / 16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const long q = tile0 + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (q >= P) continue;
-        const float v = acc[r] + bc;
-        z[q * coutp + co] = v;
-        act[q * coutp + co] = f2bf(use_gelu ? gelu(v) : v);
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int kk = 0; kk < KT / 16; ++kk) {
+            const int ci = kk * 16 + 8 * half;
+            bf16x8 w = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (ci < cinp) w = lds8(wt + ((long)tap * 32 + c) * cinp + ci);
+            wreg[tap][kk] = w;
+        }
+    stage_tile<KT>(x, im, cinp, img, y0, x0, tile);
+    __syncthreads();
+    const float bc = c < cout ? bias[c] : 0.f;
+    for (int r = wave; r < CT_H; r += NB / 64) {
+        const int y = y0 + r;
+        if (y >= im.h) break;
+        f32x16 acc = {0};
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const uint16_t* a = tile + ((r + 1 + dy) * (CT_W + 2) + c + 1 + dx) * PS + 8 * half;
+#pragma unroll
+            for (int kk = 0; kk < KT / 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds8(a + kk * 16), wreg[tap][kk], acc, 0, 0, 0);
+        }
+        // accumulator register k: pixel 8*(k/4) + 4*half + k%4 of the row
+        // segment, output channel c
+        if (c >= coutp) continue;
+        const long row = ((long)img * im.h + y) * im.w;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
+            if (xx >= im.w) continue;
+            const long q = (row + xx) * coutp + c;
+            if (c < cout) {
+                const float v = acc[k] + bc;
+                if (z) z[q] = v;
+                if (zb) zb[q] = f2bf(v);
+                if (act) act[q] = f2bf(use_gelu ? gelu(v) : v);
+            } else {  // padded channels: zero for the next layer
+                if (zb) zb[q] = 0;
+                if (act) act[q] = 0;
+            }
+        }
     }
 }
 
-// Data gradient: tile of 32 pixels x 32 input channels per wave.
-// wt: [9][32][coutp] (cin-major), dz: [P][coutp] BF16, dx: [P][cinp] FP32.
+// Data gradient: dX[p][ci] = sum_tap sum_co dZ[p - off(tap)][co] W[tap][ci][co];
+// with zp (the layer below's pre-activation) the output is the layer below's
+// dz = BF16(dX * GELU'(zp)) (GELU backward fused), else FP32 dX.
+// wt: [9][32][coutp] (cin-major, from pack).
+template <int KT>
 __global__ void __launch_bounds__(NB)
 k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16_t* __restrict__ wt,
-                int cin, int cinp, float* __restrict__ dx) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long tile0 = ((long)blockIdx.x * (NB / 64) + wave) * 32;
-    const long P = im.pixels();
-    if (tile0 >= P) return;
-    const long p = tile0 + (lane & 31);
-    const int half = lane >> 5;
-    f32x16 acc = {0};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
-        const long ps = p < P ? shifted(im, p, -dy, -dxo) : -1;
-        for (int c0 = 0; c0 < coutp; c0 += 8) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (ps >= 0) a = load4(dz + ps * coutp + c0 + 4 * half);
-            const bf16x4 b = load4(wt + ((long)tap * 32 + (lane & 31)) * coutp + c0 + 4 * half);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-        }
-    }
-    const int ci = lane & 31;
-    if (ci >= cin) return;
+                int cin, int cinp, float* __restrict__ dx, const float* __restrict__ zp,
+                const uint16_t* __restrict__ zpb, uint16_t* __restrict__ dzp) {
+    constexpr int PS = KT + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[CT_PIX * PS];
+    const int tiles_x = (im.w + CT_W - 1) / CT_W, tiles_y = (im.h + CT_H - 1) / CT_H;
+    const int b = blockIdx.x;
+    const int img = b / (tiles_x * tiles_y);
+    const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
+    bf16x8 wreg[9][KT / 16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const long q = tile0 + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (q < P) dx[q * cinp + ci] = acc[r];
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int kk = 0; kk < KT / 16; ++kk) {
+            const int co = kk * 16 + 8 * half;
+            bf16x8 w = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (co < coutp) w = lds8(wt + ((long)tap * 32 + c) * coutp + co);
+            wreg[tap][kk] = w;
+        }
+    stage_tile<KT>(dz, im, coutp, img, y0, x0, tile);
+    __syncthreads();
+    const bool gl = zp || zpb;
+    for (int r = wave; r < CT_H; r += NB / 64) {
+        const int y = y0 + r;
+        if (y >= im.h) break;
+        const long row = ((long)img * im.h + y) * im.w;
+        // the layer below's pre-activations of this lane's 16 outputs, loaded
+        // before the MFMAs so that their latency hides behind them
+        float zv[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
+            const long q = (row + xx) * cinp + c;
+            zv[k] = 0.f;
+            if (gl && c < cin && xx < im.w) zv[k] = zp ? zp[q] : bf2f(zpb[q]);
+        }
+        f32x16 acc = {0};
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
+            const uint16_t* a = tile + ((r + 1 - dy) * (CT_W + 2) + c + 1 - dxo) * PS + 8 * half;
+#pragma unroll
+            for (int kk = 0; kk < KT / 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds8(a + kk * 16), wreg[tap][kk], acc, 0, 0, 0);
+        }
+        if (c >= cinp) continue;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
+            if (xx >= im.w) continue;
+            const long q = (row + xx) * cinp + c;
+            if (gl) {
+                dzp[q] = c < cin ? f2bf(acc[k] * gelu_grad(zv[k])) : (uint16_t)0;
+            } else if (c < cin) {
+                dx[q] = acc[k];
+            }
+        }
     }
 }
 
-// Weight gradient, split over pixel chunks: block (tap, chunk) writes the
-// partial dW[tap] (32x32) of its chunk to work[chunk][tap][ci][co].
-constexpr int WCHUNK = 4096;  // pixels per chunk (per wave: WCHUNK/4)
-__global__ void __launch_bounds__(NB)
+// ---- weight gradient ---------------------------------------------------------
+// dW[tap][ci][co] = sum_p X[p + off(tap)][ci] dZ[p][co] and db[co] = sum_p dZ[p][co].
+// A block of 9 waves (wave = tap) walks tiles of WG_H rows x WG_W pixels of
+// 4 images at once.  Each tile is staged in LDS transposed to
+// [channel][y][x][image], so that a lane's 8 K values (2 neighbouring pixels
+// x 4 images) of one channel are 16 contiguous bytes for every tap shift;
+// the next tile's global loads are issued before the current tile's MFMAs.
+// The centre-tap wave also multiplies dZ by a ones operand: every row of
+// that product is the bias gradient.  Each block writes its partial sums
+// (9 x 32 x 32 + 32 floats); k_wgrad_sum adds the blocks in fixed order.
+constexpr int WG_H = 2, WG_W = 32, WG_NT = 9 * 64;
+constexpr int WG_XP = (WG_H + 2) * (WG_W + 2);        // halo tile pixels
+constexpr int WG_CSX = 290;                           // X channel stride, dwords (= 2 mod 32)
+constexpr int WG_CSZ = 132;                           // dZ channel stride, dwords (= 4 mod 32)
+constexpr int WG_PART = 9 * 1024 + 32;                // floats per block partial
+constexpr int WG_MAXBLK = 512;
+static_assert(2 * WG_XP <= WG_CSX && 2 * WG_H * WG_W <= WG_CSZ, "LDS channel planes");
+
+struct WgItems {
+    uint2 xv[2][4];  // up to 2 X items per thread, 4 images each
+    uint2 zv[4];     // 1 dZ item, 4 images
+};
+
+__device__ __forceinline__ uint2 ld_px(const uint16_t* __restrict__ src, const Img& im, int cp, int img,
+                                      int y, int x, int c4) {
+    if (img >= im.n || y < 0 || y >= im.h || x < 0 || x >= im.w || c4 * 4 >= cp) return make_uint2(0, 0);
+    return *(const uint2*)(src + (((long)img * im.h + y) * im.w + x) * cp + c4 * 4);
+}
+
+__device__ __forceinline__ void wg_load(const uint16_t* __restrict__ x, int cinp, const uint16_t* __restrict__ dz,
+                                        int coutp, const Img& im, int g, int y0, int x0, WgItems& it) {
+    const int cq = cinp / 4, zq = coutp / 4;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int e = threadIdx.x + s * WG_NT;
+        const int q = e / cq, c4 = e % cq;
+        const bool ok = q < WG_XP;
+        const int yy = y0 - 1 + q / (WG_W + 2), xx = x0 - 1 + q % (WG_W + 2);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) it.xv[s][n] = ok ? ld_px(x, im, cinp, 4 * g + n, yy, xx, c4) : make_uint2(0, 0);
+    }
+    {
+        const int e = threadIdx.x;
+        const int q = e / zq, c4 = e % zq;
+        const bool ok = q < WG_H * WG_W;
+        const int yy = y0 + q / WG_W, xx = x0 + q % WG_W;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) it.zv[n] = ok ? ld_px(dz, im, coutp, 4 * g + n, yy, xx, c4) : make_uint2(0, 0);
+    }
+}
+
+// 4 images x 4 channels of one pixel -> for each channel the 4 images' bf16
+__device__ __forceinline__ void put4(uint32_t* lds, int stride, const uint2* v) {
+    const uint32_t c0a = (v[0].x & 0xffffu) | (v[1].x << 16), c0b = (v[2].x & 0xffffu) | (v[3].x << 16);
+    const uint32_t c1a = (v[0].x >> 16) | (v[1].x & 0xffff0000u), c1b = (v[2].x >> 16) | (v[3].x & 0xffff0000u);
+    const uint32_t c2a = (v[0].y & 0xffffu) | (v[1].y << 16), c2b = (v[2].y & 0xffffu) | (v[3].y << 16);
+    const uint32_t c3a = (v[0].y >> 16) | (v[1].y & 0xffff0000u), c3b = (v[2].y >> 16) | (v[3].y & 0xffff0000u);
+    *(uint2*)(lds) = make_uint2(c0a, c0b);
+    *(uint2*)(lds + stride) = make_uint2(c1a, c1b);
+    *(uint2*)(lds + 2 * stride) = make_uint2(c2a, c2b);
+    *(uint2*)(lds + 3 * stride) = make_uint2(c3a, c3b);
+}
+
+__device__ __forceinline__ void wg_store(int cinp, int coutp, const WgItems& it, uint32_t* xs, uint32_t* zs) {
+    const int cq = cinp / 4, zq = coutp / 4;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int e = threadIdx.x + s * WG_NT;
+        const int q = e / cq, c4 = e % cq;
+        if (q < WG_XP) put4(xs + 4 * c4 * WG_CSX + 2 * q, WG_CSX, it.xv[s]);
+    }
+    const int q = threadIdx.x / zq, c4 = threadIdx.x % zq;
+    if (q < WG_H * WG_W) put4(zs + 4 * c4 * WG_CSZ + 2 * q, WG_CSZ, it.zv);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint2 a, uint2 b) {
+    bf16x8 r;
+    r[0] = (short)(a.x & 0xffff); r[1] = (short)(a.x >> 16);
+    r[2] = (short)(a.y & 0xffff); r[3] = (short)(a.y >> 16);
+    r[4] = (short)(b.x & 0xffff); r[5] = (short)(b.x >> 16);
+    r[6] = (short)(b.y & 0xffff); r[7] = (short)(b.y >> 16);
+    return r;
+}
+
+__global__ void __launch_bounds__(WG_NT)
 k_conv_bwd_weight(const uint16_t* __restrict__ x, int cinp, const uint16_t* __restrict__ dz,
                   int coutp, Img im, float* __restrict__ work) {
-    const int tap = blockIdx.y, chunk = blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int half = lane >> 5, rc = lane & 31;
+    __shared__ __attribute__((aligned(16))) uint32_t xs[32 * WG_CSX];
+    __shared__ __attribute__((aligned(16))) uint32_t zs[32 * WG_CSZ];
+    const int lane = threadIdx.x & 63, tap = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
     const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
-    const long P = im.pixels();
-    const long p0 = (long)chunk * WCHUNK + (long)wave * (WCHUNK / 4);
-    f32x16 acc = {0};
-    for (long k0 = 0; k0 < WCHUNK / 4; k0 += 8) {
-        bf16x4 a, b;
+    const int tx = (im.w + WG_W - 1) / WG_W, ty = (im.h + WG_H - 1) / WG_H, ng = (im.n + 3) / 4;
+    const long ntiles = (long)tx * ty * ng;
+    f32x16 acc = {0}, accb = {0};
+    bf16x8 ones;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const long p = p0 + k0 + 4 * half + v;
-            short av = 0, bv = 0;
-            if (p < P) {
-                const long ps = shifted(im, p, dy, dxo);
-                if (ps >= 0 && rc < cinp) av = (short)x[ps * cinp + rc];
-                if (rc < coutp) bv = (short)dz[p * coutp + rc];
-            }
-            a[v] = av;
-            b[v] = bv;
+    for (int k = 0; k < 8; ++k) ones[k] = (short)0x3f80;  // BF16 1.0
+    WgItems it;
+    long t = blockIdx.x;
+    if (t < ntiles) {
+        const int g = (int)(t / ((long)tx * ty)), r = (int)(t % ((long)tx * ty));
+        wg_load(x, cinp, dz, coutp, im, g, r / tx * WG_H, r % tx * WG_W, it);
+    }
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();  // the previous tile's MFMAs have read the LDS
+        wg_store(cinp, coutp, it, xs, zs);
+        __syncthreads();
+        const long tn = t + gridDim.x;
+        if (tn < ntiles) {
+            const int g = (int)(tn / ((long)tx * ty)), r = (int)(tn % ((long)tx * ty));
+            wg_load(x, cinp, dz, coutp, im, g, r / tx * WG_H, r % tx * WG_W, it);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    // reduce the 4 waves' tiles in LDS, fixed order
-    __shared__ float red[4][32 * 32];
+        // K step: pixels xq, xq+1 (xq = x4 + 2*half) of row y, images 0..3
 #pragma unroll
-    for (int r = 0; r < 16; ++r) red[wave][(8 * (r >> 2) + 4 * half + (r & 3)) * 32 + rc] = acc[r];
-    __syncthreads();
-    for (int e = threadIdx.x; e < 1024; e += NB)
-        work[((long)chunk * 9 + tap) * 1024 + e] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-}
-
-// dW[tap][ci][co] = sum over chunks (fixed order); db[co] = sum_p dz[p][co].
-__global__ void k_conv_wsum(const float* __restrict__ work, int nchunks, int cin, int cout,
-                            float* __restrict__ dw) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;  // over 9*1024
-    if (e >= 9 * 1024) return;
-    const int tap = e / 1024, ci = (e % 1024) / 32, co = e % 32;
-    if (ci >= cin || co >= cout) return;
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += work[(long)c * 9 * 1024 + e];
-    dw[((long)tap * cin + ci) * cout + co] = s;
-}
-
-// Bias gradient, pass 1: block b sums each channel (< 32) over pixels
-// [b*WCHUNK, (b+1)*WCHUNK); thread (g, c) takes pixels g, g+8, ... of the
-// chunk, so a row of the block reads 8 consecutive pixels' channels at once.
-__global__ void __launch_bounds__(NB)
-k_bias_grad_part(const uint16_t* __restrict__ dz, long P, int coutp, int cout,
-                 float* __restrict__ part) {
-    __shared__ float red[NB];
-    const int co = threadIdx.x & 31, g = threadIdx.x >> 5;  // NB/32 pixel groups
-    const long p0 = (long)blockIdx.x * WCHUNK;
-    const long p1 = p0 + WCHUNK < P ? p0 + WCHUNK : P;
-    float s = 0.f;
-    if (co < cout)
-        for (long p = p0 + g; p < p1; p += NB / 32) s += bf2f(dz[p * coutp + co]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        float t = 0.f;
-        for (int q = 0; q < NB / 32; ++q) t += red[q * 32 + co];
-        part[(long)blockIdx.x * 32 + co] = t;
+        for (int y = 0; y < WG_H; ++y)
+#pragma unroll
+            for (int x4 = 0; x4 < WG_W; x4 += 4) {
+                const int xq = x4 + 2 * half;
+                bf16x8 a = {0, 0, 0, 0, 0, 0, 0, 0}, bz = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (c < cinp) {
+                    const uint32_t* pa = xs + c * WG_CSX + 2 * ((y + 1 + dy) * (WG_W + 2) + xq + 1 + dxo);
+                    a = as_bf16x8(*(const uint2*)pa, *(const uint2*)(pa + 2));
+                }
+                if (c < coutp) {
+                    const uint4 v = *(const uint4*)(zs + c * WG_CSZ + 2 * (y * WG_W + xq));
+                    bz = as_bf16x8(make_uint2(v.x, v.y), make_uint2(v.z, v.w));
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bz, acc, 0, 0, 0);
+                if (tap == 4) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ones, bz, accb, 0, 0, 0);
+            }
     }
+    // register k of lane l: row ci = 8*(k/4) + 4*half + k%4, column co = c
+    float* out = work + (long)blockIdx.x * WG_PART;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[tap * 1024 + (8 * (k >> 2) + 4 * half + (k & 3)) * 32 + c] = acc[k];
+    if (tap == 4 && half == 0) out[9 * 1024 + c] = accb[0];
 }
-// pass 2: the chunks' sums in chunk order
-__global__ void k_bias_grad_sum(const float* __restrict__ part, int nchunks, int cout,
-                                float* __restrict__ db) {
-    const int co = threadIdx.x;
-    if (co >= cout) return;
-    float t = 0.f;
-    for (int b = 0; b < nchunks; ++b) t += part[(long)b * 32 + co];
-    db[co] = t;
+
+// dW[tap][ci][co] and db[co] from the blocks' partials.  A block takes 32
+// consecutive entries; its 8 groups of 32 threads add every 8th block partial
+// (4 running sums each), then the groups' sums are added in group order: a
+// fixed order, so the result does not depend on timing.
+constexpr int WS_E = 32, WS_G = 8;
+__global__ void __launch_bounds__(WS_E * WS_G)
+k_wgrad_sum(const float* __restrict__ work, int nblk, int cin, int cout, float* __restrict__ dw,
+            float* __restrict__ db) {
+    __shared__ float red[WS_G][WS_E];
+    const int el = threadIdx.x % WS_E, grp = threadIdx.x / WS_E;
+    const int e = blockIdx.x * WS_E + el;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (e < WG_PART) {
+        int b = grp;
+        for (; b + 3 * WS_G < nblk; b += 4 * WS_G) {
+            s0 += work[(long)b * WG_PART + e];
+            s1 += work[(long)(b + WS_G) * WG_PART + e];
+            s2 += work[(long)(b + 2 * WS_G) * WG_PART + e];
+            s3 += work[(long)(b + 3 * WS_G) * WG_PART + e];
+        }
+        for (; b < nblk; b += WS_G) s0 += work[(long)b * WG_PART + e];
+    }
+    red[grp][el] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (grp != 0 || e >= WG_PART) return;
+    float s = 0.f;
+#pragma unroll
+    for (int g = 0; g < WS_G; ++g) s += red[g][el];
+    if (e < 9 * 1024) {
+        const int tap = e / 1024, ci = (e % 1024) / 32, co = e % 32;
+        if (ci < cin && co < cout) dw[((long)tap * cin + ci) * cout + co] = s;
+    } else if (e - 9 * 1024 < cout) {
+        db[e - 9 * 1024] = s;
+    }
 }
 
 __global__ void k_gelu_bwd(const float* __restrict__ dact, const float* __restrict__ z, long count,
@@ -297,54 +491,77 @@ int pad8(int c) { return (c + 7) / 8 * 8; }
 extern "C" {
 
 int nemo2d_nn_conv_fwd(const void* x, int n, int h, int w, int cin, const void* wf,
-                       const float* bias, int cout, float* z, void* act, int use_gelu,
+                       const float* bias, int cout, float* z, void* zb, void* act, int use_gelu,
                        void* stream) {
-    if (!x || !wf || !bias || !z || !act || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
+    if (!x || !wf || !bias || !(z || zb || act) || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: bad argument");
     const Img im{n, h, w};
-    const long P = (long)n * h * w;
-    const long tiles = (P + 31) / 32;
-    k_conv_fwd<<<(unsigned)((tiles + 3) / 4), NB, 0, (hipStream_t)stream>>>(
-        (const uint16_t*)x, im, pad8(cin), (const uint16_t*)wf, bias, cout, pad8(cout), z,
-        (uint16_t*)act, use_gelu);
+    const long nb = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
+    if (nb > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: too many pixels");
+    auto* k = pad8(cin) <= 16 ? k_conv_fwd<16> : k_conv_fwd<32>;
+    k<<<(unsigned)nb, NB, 0, (hipStream_t)stream>>>((const uint16_t*)x, im, pad8(cin), (const uint16_t*)wf,
+                                                  bias, cout, pad8(cout), z, (uint16_t*)zb, (uint16_t*)act,
+                                                  use_gelu);
     return launched("conv_fwd");
+}
+
+static int bwd_data(const void* dz, int n, int h, int w, int cout, const void* wb, int cin, float* dx,
+                    const float* zp, const void* zpb, void* dzp, void* stream, const char* who) {
+    const Img im{n, h, w};
+    const long nb = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
+    if (nb > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
+    auto* k = pad8(cout) <= 16 ? k_conv_bwd_data<16> : k_conv_bwd_data<32>;
+    k<<<(unsigned)nb, NB, 0, (hipStream_t)stream>>>((const uint16_t*)dz, im, pad8(cout), (const uint16_t*)wb,
+                                                  cin, pad8(cin), dx, zp, (const uint16_t*)zpb,
+                                                  (uint16_t*)dzp);
+    return launched(who);
 }
 
 int nemo2d_nn_conv_bwd_data(const void* dz, int n, int h, int w, int cout, const void* wb, int cin,
                             float* dx, void* stream) {
     if (!dz || !wb || !dx || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_data: bad argument");
-    const Img im{n, h, w};
-    const long P = (long)n * h * w;
-    const long tiles = (P + 31) / 32;
-    k_conv_bwd_data<<<(unsigned)((tiles + 3) / 4), NB, 0, (hipStream_t)stream>>>(
-        (const uint16_t*)dz, im, pad8(cout), (const uint16_t*)wb, cin, pad8(cin), dx);
-    return launched("conv_bwd_data");
+    return bwd_data(dz, n, h, w, cout, wb, cin, dx, nullptr, nullptr, nullptr, stream, "conv_bwd_data");
+}
+
+int nemo2d_nn_conv_bwd_data_gelu(const void* dz, int n, int h, int w, int cout, const void* wb, int cin,
+                                 const void* zp, int zp_bf16, void* dzp, void* stream) {
+    if (!dz || !wb || !zp || !dzp || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_data_gelu: bad argument");
+    return bwd_data(dz, n, h, w, cout, wb, cin, nullptr, zp_bf16 ? nullptr : (const float*)zp,
+                    zp_bf16 ? zp : nullptr, dzp, stream, "conv_bwd_data_gelu");
+}
+
+// Blocks of the weight gradient for a batch: one per tile up to WG_MAXBLK.
+static int wgrad_blocks(long n, long h, long w) {
+    const long t = (n + 3) / 4 * ((h + WG_H - 1) / WG_H) * ((w + WG_W - 1) / WG_W);
+    return (int)(t < WG_MAXBLK ? t : WG_MAXBLK);
 }
 
 // work: at least nemo2d_nn_wgrad_work_floats(n*h*w) floats.
-long nemo2d_nn_wgrad_work_floats(long pixels) { return ((pixels + WCHUNK - 1) / WCHUNK) * 9 * 1024; }
+long nemo2d_nn_wgrad_work_floats(long pixels) {
+    (void)pixels;
+    return (long)WG_MAXBLK * WG_PART;
+}
+
+static int bwd_weight(const void* x, const void* dz, int n, int h, int w, int cin, int cout, float* dw,
+                      float* db, float* work, void* stream, const char* who) {
+    const Img im{n, h, w};
+    const int nblk = wgrad_blocks(n, h, w);
+    hipStream_t s = (hipStream_t)stream;
+    k_conv_bwd_weight<<<nblk, WG_NT, 0, s>>>((const uint16_t*)x, pad8(cin), (const uint16_t*)dz,
+                                             pad8(cout), im, work);
+    int rc = launched(who);
+    if (rc) return rc;
+    k_wgrad_sum<<<(WG_PART + WS_E - 1) / WS_E, WS_E * WS_G, 0, s>>>(work, nblk, cin, cout, dw, db);
+    return launched(who);
+}
 
 int nemo2d_nn_conv_bwd_weight(const void* x, const void* dz, int n, int h, int w, int cin, int cout,
                               float* dw, float* db, float* work, void* stream) {
     if (!x || !dz || !dw || !db || !work || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_weight: bad argument");
-    const Img im{n, h, w};
-    const long P = (long)n * h * w;
-    const int nchunks = (int)((P + WCHUNK - 1) / WCHUNK);
-    hipStream_t s = (hipStream_t)stream;
-    k_conv_bwd_weight<<<dim3(nchunks, 9), NB, 0, s>>>((const uint16_t*)x, pad8(cin),
-                                                        (const uint16_t*)dz, pad8(cout), im, work);
-    int rc = launched("conv_bwd_weight");
-    if (rc) return rc;
-    k_conv_wsum<<<(9 * 1024 + 255) / 256, 256, 0, s>>>(work, nchunks, cin, cout, dw);
-    if ((rc = launched("conv_wsum"))) return rc;
-    // the weight partials are consumed; the work buffer takes the bias
-    // partials (nchunks x 32 floats, within its nchunks x 9 x 1024)
-    k_bias_grad_part<<<nchunks, NB, 0, s>>>((const uint16_t*)dz, P, pad8(cout), cout, work);
-    if ((rc = launched("bias_grad_part"))) return rc;
-    k_bias_grad_sum<<<1, 32, 0, s>>>(work, nchunks, cout, db);
-    return launched("bias_grad_sum");
+    return bwd_weight(x, dz, n, h, w, cin, cout, dw, db, work, stream, "conv_bwd_weight");
 }
 
 int nemo2d_nn_gelu_bwd(const float* dact, const float* z, long count, void* dz, void* stream) {

@@ -40,7 +40,8 @@ SYMBOLS = (
     "nemo2d_stream_bandwidth", "nemo2d_format_go2d", "nemo2d_set_output_grid",
     "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
     "nemo2d_set_stage_fusion", "nemo2d_retrieve_grid", "nemo2d_retrieve_classes",
-    "nemo2d_classify",
+    "nemo2d_classify", "nemo2d_nn_conv_bwd_data_gelu", "nemo2d_closure_restrict",
+    "nemo2d_closure_target",
 )
 
 # nemo2d_set_variant choices
@@ -106,6 +107,8 @@ def lib():
                                          ctypes.POINTER(ctypes.c_long)]
         L.nemo2d_closure_features.argtypes = [vp, dp, dp, vp]
         L.nemo2d_closure_apply.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i, dp]
+        L.nemo2d_closure_restrict.argtypes = [vp, vp]
+        L.nemo2d_closure_target.argtypes = [vp, vp, dp, vp]
         L.nemo2d_dev_alloc.argtypes = [ctypes.c_long, ctypes.POINTER(vp)]
         L.nemo2d_dev_free.argtypes = [vp]
         L.nemo2d_dev_upload.argtypes = [vp, vp, ctypes.c_long]
@@ -372,6 +375,12 @@ class Model:
         _check(lib().nemo2d_retrieve_classes(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
         return out
 
+    def retrieve_ssh(self):
+        """sshn alone (T layout)."""
+        out = np.zeros(shapes(self.jpi, self.jpj)["T"])
+        _check(lib().nemo2d_retrieve_results(self._h, _dp(out), None, None, None, None))
+        return out
+
     def diagnostics(self):
         """(volume, energy, wet area) of the current state, FP64 reductions."""
         out = np.zeros(3)
@@ -402,12 +411,26 @@ class Model:
     def snapshot_wait(self):
         _check(lib().nemo2d_snapshot_wait(self._h))
 
-    def closure_features(self, mean, scale, x):
+    def closure_features(self, mean, scale, x, offset=0):
         """Closure input features of the now-state into the device array x
-        (BF16 bits [jpj*jpi, 8]); mean, scale: 4 values each."""
+        (BF16 bits [jpj*jpi, 8], starting `offset` bytes in); mean, scale: 4
+        values each."""
         m = np.ascontiguousarray(mean, dtype=np.float64)
         sc = np.ascontiguousarray(scale, dtype=np.float64)
-        _check(lib().nemo2d_closure_features(self._h, _dp(m), _dp(sc), x.ptr))
+        _check(lib().nemo2d_closure_features(self._h, _dp(m), _dp(sc),
+                                             ctypes.c_void_p(x.ptr.value + int(offset))))
+
+    def restrict_from(self, fine):
+        """This (coarse) model's now-state becomes the restriction of the fine
+        model's (on the device)."""
+        _check(lib().nemo2d_closure_restrict(fine._h, self._h))
+
+    def closure_target(self, pred, scale, y, offset=0):
+        """FP32 (self - pred) / scale of the per-cell ssh, east-face u and
+        north-face v into the device array y ([jpj*jpi, 8], `offset` bytes in)."""
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        _check(lib().nemo2d_closure_target(self._h, pred._h, _dp(sc),
+                                           ctypes.c_void_p(y.ptr.value + int(offset))))
 
     def closure_apply(self, z, scale):
         """Adds the device FP32 increments z [jpj*jpi, C] times scale (3

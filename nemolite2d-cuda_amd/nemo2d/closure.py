@@ -85,8 +85,9 @@ def _setup_nn(L):
         return
     vp, i, l, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float
     fp = ctypes.POINTER(ctypes.c_float)
-    L.nemo2d_nn_conv_fwd.argtypes = [vp, i, i, i, i, vp, fp, i, fp, vp, i, vp]
+    L.nemo2d_nn_conv_fwd.argtypes = [vp, i, i, i, i, vp, fp, i, fp, vp, vp, i, vp]
     L.nemo2d_nn_conv_bwd_data.argtypes = [vp, i, i, i, i, vp, i, fp, vp]
+    L.nemo2d_nn_conv_bwd_data_gelu.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, vp]
     L.nemo2d_nn_conv_bwd_weight.argtypes = [vp, vp, i, i, i, i, i, fp, fp, fp, vp]
     L.nemo2d_nn_wgrad_work_floats.argtypes = [l]
     L.nemo2d_nn_wgrad_work_floats.restype = l
@@ -95,8 +96,8 @@ def _setup_nn(L):
     L.nemo2d_nn_adam.argtypes = [fp, fp, fp, fp, l, f, f, f, f, i, vp]
     L.nemo2d_nn_pack_weights.argtypes = [fp, i, i, vp, vp, vp]
     L.nemo2d_nn_to_bf16.argtypes = [fp, l, vp, vp]
-    for n in ("conv_fwd", "conv_bwd_data", "conv_bwd_weight", "gelu_bwd", "mse", "adam",
-              "pack_weights", "to_bf16"):
+    for n in ("conv_fwd", "conv_bwd_data", "conv_bwd_data_gelu", "conv_bwd_weight", "gelu_bwd",
+              "mse", "adam", "pack_weights", "to_bf16"):
         getattr(L, "nemo2d_nn_" + n).restype = i
     L._nn_ready = True
 
@@ -111,7 +112,11 @@ class ConvNet:
     INIT_SCALE = float(np.sqrt(1.0 / 6.0))
     STREAM = None  # the default stream
 
-    def __init__(self, seed=0, layers=LAYERS, init_scale=None):
+    def __init__(self, seed=0, layers=LAYERS, init_scale=None, fp32_preact=False):
+        """fp32_preact: keep the hidden layers' pre-activations (read by the
+        GELU backward) in FP32 instead of BF16; the output layer's is FP32
+        either way."""
+        self.fp32_preact = fp32_preact
         self.L = lib()
         _setup_nn(self.L)
         self.layers = layers
@@ -164,10 +169,11 @@ class ConvNet:
         if self._shape == (n, h, w):
             return
         P = n * h * w
-        self.z = [DevArray((P, pad8(co)), np.float32) for _, co, _ in self.layers]
+        nl = len(self.layers)
+        self.z = [DevArray((P, pad8(co)), np.float32 if (k == nl - 1 or self.fp32_preact) else np.int16)
+                  for k, (_, co, _) in enumerate(self.layers)]
         self.act = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
         self.dz = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
-        self.dact = DevArray((P, 32), np.float32)
         self.gw = [DevArray((9, ci, co), np.float32) for ci, co, _ in self.layers]
         self.gb = [DevArray((co,), np.float32) for _, co, _ in self.layers]
         self.work = DevArray((int(self.L.nemo2d_nn_wgrad_work_floats(P)),), np.float32)
@@ -181,9 +187,14 @@ class ConvNet:
         self._buffers(n, h, w)
         self.x = x_bf16
         a = x_bf16
+        nl = len(self.layers)
         for k, (cin, cout, g) in enumerate(self.layers):
+            last = k == nl - 1
+            f32 = self.z[k].dtype == np.float32
             _check(self.L.nemo2d_nn_conv_fwd(a.ptr, n, h, w, cin, self.wf[k].ptr, self.b[k].fptr(),
-                                             cout, self.z[k].fptr(), self.act[k].ptr, 1 if g else 0,
+                                             cout, self.z[k].fptr() if f32 else None,
+                                             None if f32 else self.z[k].ptr,
+                                             None if last else self.act[k].ptr, 1 if g else 0,
                                              self.STREAM))
             a = self.act[k]
         return self.z[-1]
@@ -205,11 +216,12 @@ class ConvNet:
                                                     self.gw[k].fptr(), self.gb[k].fptr(),
                                                     self.work.fptr(), self.STREAM))
             if k > 0:
-                # dact as a contiguous [P, pad8(cin)] array at the start of its buffer
-                _check(self.L.nemo2d_nn_conv_bwd_data(self.dz[k].ptr, n, h, w, cout, self.wb[k].ptr,
-                                                      cin, self.dact.fptr(), self.STREAM))
-                _check(self.L.nemo2d_nn_gelu_bwd(self.dact.fptr(), self.z[k - 1].fptr(),
-                                                 P * pad8(cin), self.dz[k - 1].ptr, self.STREAM))
+                # the layer below's dz = dX * GELU'(z below), in one pass
+                zb = self.z[k - 1].dtype != np.float32
+                _check(self.L.nemo2d_nn_conv_bwd_data_gelu(self.dz[k].ptr, n, h, w, cout,
+                                                           self.wb[k].ptr, cin, self.z[k - 1].ptr,
+                                                           1 if zb else 0, self.dz[k - 1].ptr,
+                                                           self.STREAM))
         if not want_loss:
             return None
         return float(self.part.download().astype(np.float64).sum())

@@ -163,11 +163,28 @@ class Closure:
     """Normalisation + ConvNet; corrects a coarse state every K steps."""
 
     def __init__(self, X, Y, seed=0):
+        self._init(X.reshape(-1, 4).mean(0), X.reshape(-1, 4).std(0) + 1e-12,
+                   Y.reshape(-1, 3).std(0) + 1e-30, seed)
+
+    def _init(self, xm, xs, ys, seed):
         from .closure import ConvNet
         self.net = ConvNet(seed=seed)
-        self.xm = X.reshape(-1, 4).mean(0)
-        self.xs = X.reshape(-1, 4).std(0) + 1e-12
-        self.ys = Y.reshape(-1, 3).std(0) + 1e-30
+        self.xm, self.xs, self.ys = np.asarray(xm), np.asarray(xs), np.asarray(ys)
+
+    @classmethod
+    def from_stats(cls, xm, xs, ys, seed=0):
+        """Input means/scales (4) and target scales (3) given directly."""
+        c = cls.__new__(cls)
+        c._init(xm, xs, ys, seed)
+        return c
+
+    def train_batch(self, xb, yb, n, h, w, wch, lr, want_loss=False):
+        """One Adam step on a batch already on the device (xb BF16 features,
+        yb FP32 scaled targets, n images of h x w)."""
+        self.net.forward(xb, n, h, w)
+        loss = self.net.backward(yb, wch, want_loss=want_loss)
+        self.net.step(lr=lr)
+        return loss
 
     def _x(self, X):
         n, h, w, _ = X.shape
@@ -253,5 +270,156 @@ def evaluate(setup, closure, n_windows, tol=2.0, judge_every=5):
             fell_back = True
             cc.set_state(sp)
     for m in (fm, cp, cc):
+        m.close()
+    return np.array(err_plain), np.array(err_clos), fell_back
+
+
+# ---- the closure at scale: pairs made and used on the device ------------------------
+class Trio:
+    """One training basin: the fine model, a coarse model holding the
+    restricted fine state (the truth) and a coarse model stepping from the
+    previous window's restricted state (the prediction)."""
+
+    def __init__(self, setup):
+        self.setup = setup
+        self.fine = setup.fine_model()
+        self.truth = setup.coarse_model()
+        self.pred = setup.coarse_model()
+        self.w = 0  # windows of K coarse steps done
+
+    def advance(self):
+        """Fine model on by one window; truth = R(fine)."""
+        w = self.w
+        self.fine.run(2 * K_STEPS * w + 1, 2 * K_STEPS * (w + 1))
+        self.truth.restrict_from(self.fine)
+        self.w = w + 1
+
+    def restart_pred(self):
+        """Prediction: from R(fine) now, K coarse steps on."""
+        self.pred.restrict_from(self.fine)
+        t0 = K_STEPS * self.w
+        self.pred.run(t0 + 1, t0 + K_STEPS)
+
+    def close(self):
+        for m in (self.fine, self.truth, self.pred):
+            m.close()
+
+
+def pilot_stats(setup, windows=(40, 120, 200)):
+    """Normalisation statistics from a few pairs of one basin, made by the
+    host path: input mean/scale (4) and target scale (3)."""
+    fm, cm = setup.fine_model(), setup.coarse_model()
+    X, Y = [], []
+    done = 0
+    for w in windows:
+        fm.run(2 * K_STEPS * done + 1, 2 * K_STEPS * w)
+        f0 = fm.retrieve()
+        c0 = restrict_state(f0, setup.fine_grid, setup.coarse_grid)
+        cm.set_state(c0)
+        cm.run(K_STEPS * w + 1, K_STEPS * (w + 1))
+        cp_ = cm.retrieve()
+        fm.run(2 * K_STEPS * w + 1, 2 * K_STEPS * (w + 1))
+        done = w + 1
+        truth = t_fields(restrict_state(fm.retrieve(), setup.fine_grid, setup.coarse_grid))
+        X.append(features(cp_, setup.coarse_grid["ht"]))
+        Y.append(truth - t_fields(cp_))
+    fm.close()
+    cm.close()
+    X, Y = np.array(X), np.array(Y)
+    return X.reshape(-1, 4).mean(0), X.reshape(-1, 4).std(0) + 1e-12, Y.reshape(-1, 3).std(0) + 1e-30
+
+
+def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, iters_per_batch=1,
+                 stats=None, log=None, net_seed=0, lr_min_frac=0.05):
+    """Closure trained on pairs made on the device: `group` basins advance
+    side by side, one window of K coarse steps at a time, and each window's
+    `group` pairs (one per basin, written straight into the device batch)
+    are one training batch.  len(seeds) * windows pairs in all, each used
+    for iters_per_batch Adam steps (cosine learning rate).  Returns the
+    closure and a record of what was done."""
+    import time as _time
+    from .closure import DevArray
+    bathy = bathy or {}
+    seeds = list(seeds)
+    t0 = _time.perf_counter()
+    if stats is None:
+        stats = pilot_stats(Setup(nc, seeds[0], **bathy))
+    clos = Closure.from_stats(*stats, seed=net_seed)
+    t_stats = _time.perf_counter() - t0
+    P = nc * nc
+    xb = DevArray((group * P, 8), np.int16)
+    yb = DevArray((group * P, 8), np.float32)
+    wch = clos.net.upload(np.ones(3, dtype=np.float32))
+    groups = [seeds[k:k + group] for k in range(0, len(seeds), group)]
+    total_iters = sum(windows * iters_per_batch for _ in groups)
+    it = 0
+    losses = []
+    pairs = 0
+    t1 = _time.perf_counter()
+    for gs in groups:
+        trios = [Trio(Setup(nc, sd, **bathy)) for sd in gs]
+        for _ in range(spinup):
+            for tr in trios:
+                tr.advance()
+        for tr in trios:
+            tr.restart_pred()
+        for w in range(windows):
+            for tr in trios:
+                tr.advance()
+            for k, tr in enumerate(trios):
+                tr.pred.closure_features(clos.xm, clos.xs, xb, offset=k * P * 8 * 2)
+                tr.truth.closure_target(tr.pred, clos.ys, yb, offset=k * P * 8 * 4)
+                tr.restart_pred()
+            pairs += len(trios)
+            for _ in range(iters_per_batch):
+                lr_it = lr * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + np.cos(np.pi * it / total_iters)))
+                record = it % 100 == 0 or it == total_iters - 1
+                loss = clos.train_batch(xb, yb, len(trios), nc, nc, wch, lr_it, want_loss=record)
+                if record:
+                    losses.append((it, loss))
+                    if log:
+                        log("closure online it %d/%d pairs %d loss %.4g" % (it, total_iters, pairs, loss))
+                it += 1
+        for tr in trios:
+            tr.close()
+    t_train = _time.perf_counter() - t1
+    xb.close()
+    yb.close()
+    return clos, {"pairs": pairs, "iterations": it, "batch": group, "losses": losses,
+                  "seconds": {"stats": t_stats, "generate_and_train": t_train},
+                  "basins": len(seeds), "windows_per_basin": windows, "spinup_windows": spinup}
+
+
+def evaluate_device(setup, closure, n_windows, tol=2.0, judge_every=5, spinup=2):
+    """evaluate() with the restriction on the device: the fine run (the
+    judge) beside a plain coarse and a coarse+closure run from the same
+    restricted start; every judge_every windows the closure run's ssh RMS
+    error against the restricted fine state is compared with the plain
+    run's, and above tol times it the closure is switched off (the plain
+    run's state taken over).  Returns per-window ssh RMS errors and whether
+    the fallback fired."""
+    fm, cp_, cc, tm = setup.fine_model(), setup.coarse_model(), setup.coarse_model(), setup.coarse_model()
+    fm.run(1, 2 * K_STEPS * spinup)
+    cp_.restrict_from(fm)
+    cc.restrict_from(fm)
+    err_plain, err_clos = [], []
+    use, fell_back = True, False
+    for wdx in range(spinup, spinup + n_windows):
+        t0 = K_STEPS * wdx
+        fm.run(2 * t0 + 1, 2 * (t0 + K_STEPS))
+        cp_.run(t0 + 1, t0 + K_STEPS)
+        cc.run(t0 + 1, t0 + K_STEPS)
+        if use:
+            closure.correct_model(cc)
+        tm.restrict_from(fm)
+        truth = tm.retrieve_ssh()
+        sp, sc = cp_.retrieve_ssh(), cc.retrieve_ssh()
+        err_plain.append(rms(sp, truth))
+        err_clos.append(rms(sc, truth))
+        if use and (wdx - spinup + 1) % judge_every == 0 and err_clos[-1] > tol * err_plain[-1]:
+            use = False
+            fell_back = True
+            cc.set_state(cp_.retrieve())
+    for m in (fm, cp_, cc, tm):
         m.close()
     return np.array(err_plain), np.array(err_clos), fell_back

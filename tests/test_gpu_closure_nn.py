@@ -53,7 +53,7 @@ def test_first_layer_exact_accumulation():
     from nemo2d import closure as C
     rng = np.random.default_rng(3)
     n, h, w, cin, cout = 1, 7, 9, 4, 32
-    net = C.ConvNet(seed=3)
+    net = C.ConvNet(seed=3, fp32_preact=True)
     wgt = bf16(rng.standard_normal((9, cin, cout)))
     bias = rng.standard_normal(cout).astype(np.float32).astype(np.float64)
     net.set_weights([wgt] + [np.zeros((9, a, b)) for a, b, _ in C.LAYERS[1:]],
@@ -81,13 +81,34 @@ def test_gradients_match_reference():
 
 
 def test_gradients_match_reference_over_several_pixel_chunks():
-    """P = 3*41*43 = 5289 pixels: the weight gradient is summed over two
-    4096-pixel chunks and the loss over many blocks."""
+    """3 images of 41 x 43 pixels: a batch that is not a multiple of the
+    weight gradient's 4-image groups, tiles cut by the image edges, the
+    weight gradient summed over many blocks and the loss over many blocks."""
     net, params, x, t, xg, tg, (n, h, w) = setup(4, n=3, h=41, w=43)
     wch = net.upload(np.array([1.0, 1.0, 1.0]))
     net.forward(xg, n, h, w)
     loss = float(net.backward(tg, wch))
     gw, gb = net.grads()
+    ref_loss, ref_grads = cr.loss_and_grads(params, x, t, np.ones(3))
+    assert abs(loss - ref_loss) <= 3e-2 * ref_loss
+    for k in range(4):
+        assert relerr(gw[k], ref_grads[k][0]) < 5e-2, (k, relerr(gw[k], ref_grads[k][0]))
+        assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
+
+
+@pytest.mark.parametrize("fp32_preact", [False, True])
+def test_gradients_with_either_preactivation_storage(fp32_preact):
+    """The GELU backward reads the layer below's pre-activation stored in
+    BF16 (the default) or FP32: both give the reference's gradients."""
+    from nemo2d.closure import ConvNet
+    net, params, x, t, xg, tg, (n, h, w) = setup(6, n=5, h=37, w=70)
+    ws, bs = net.weights()
+    net2 = ConvNet(seed=6, fp32_preact=fp32_preact)
+    net2.set_weights(ws, bs)
+    wch = net2.upload(np.ones(3))
+    net2.forward(xg, n, h, w)
+    loss = float(net2.backward(tg, wch))
+    gw, gb = net2.grads()
     ref_loss, ref_grads = cr.loss_and_grads(params, x, t, np.ones(3))
     assert abs(loss - ref_loss) <= 3e-2 * ref_loss
     for k in range(4):

@@ -58,3 +58,38 @@ def test_device_correction_matches_host_correction_bitwise():
     m.run(31, 40)
     assert np.all(np.isfinite(m.retrieve()["sshn"]))
     m.close()
+
+
+def test_device_restriction_and_target_equal_host_path_bitwise():
+    """The on-device restriction of a fine state and the training target
+    equal the numpy restriction (restrict_state) and target bit for bit."""
+    from nemo2d.closure import DevArray
+    setup = cp.Setup(24, 7, **KW)
+    fm, cm, tm = setup.fine_model(), setup.coarse_model(), setup.coarse_model()
+    fm.run(1, 60)
+    f = fm.retrieve()
+    host = cp.restrict_state(f, setup.fine_grid, setup.coarse_grid)
+    cm.restrict_from(fm)
+    dev = cm.retrieve()
+    for k in ("sshn", "un", "vn", "sshn_u", "sshn_v"):
+        assert np.array_equal(dev[k], host[k]), (k, np.abs(dev[k] - host[k]).max())
+    # the restricted model steps on like one given the same state by set_state
+    ref = setup.coarse_model()
+    ref.set_state(host)
+    cm.run(31, 40)
+    ref.run(31, 40)
+    a, b = cm.retrieve(), ref.retrieve()
+    for k in ("sshn", "un", "vn"):
+        assert np.array_equal(a[k], b[k]), k
+    # target: (truth - pred) / scale per cell, FP32
+    fm.run(61, 80)
+    tm.restrict_from(fm)
+    truth = cp.restrict_state(fm.retrieve(), setup.fine_grid, setup.coarse_grid)
+    scale = np.array([1e-3, 2e-4, 3e-4])
+    y = DevArray((24 * 24, 8), np.float32)
+    tm.closure_target(cm, scale, y)
+    want = ((cp.t_fields(truth) - cp.t_fields(a)) / scale).astype(np.float32).reshape(-1, 3)
+    got = y.download()
+    assert np.array_equal(got[:, :3], want) and not got[:, 3:].any()
+    for m in (fm, cm, tm, ref):
+        m.close()
