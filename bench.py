@@ -164,66 +164,93 @@ def form_ensemble(nmem, steps, seed=7, graph=True):
             "fp32_rms_error_relative_to_fp64_rms (16 members)": rel}
 
 
-# Seamounts scaled to the 512 km fine basin of the closure benchmark (32
-# mounts of 20-200 km would cover it and drive the depth to the 10 m floor,
-# where the shallow-water run blows up).
-CLOSURE_BATHY = dict(n_mounts=8, radius=(10e3, 60e3), height=(10.0, 60.0))
+# The closure workload: a 1024^2 coarse basin (dx = 2 km, rdt = 40 s) and the
+# 4-layer corrector, trained on pairs from 64 seeded 2048^2 seamount fine runs
+# (dx = 1 km, rdt = 20 s; the seamounts of the 8192^2 FP32 workload: 32
+# mounts, heights U(10,60) m, radii U(20,200) km) restricted to 1024^2, 313
+# windows of 10 coarse steps each (20,032 pairs).
+CLOSURE_BATHY = dict(n_mounts=32, radius=(20e3, 200e3), height=(10.0, 60.0))
+CLOSURE_LR = 1e-3
+CLOSURE_ROLLOUT = 2   # the prediction model restarts from the fine state every 2nd window
+CONV_FLOP_PER_PIXEL = 2 * 9 * 32 * 32   # one 32 -> 32 layer, forward or weight gradient
+MFMA_BF16_PEAK = 2.5e15
 
 
-def form_closure(nc, runs, snaps, iters, windows):
-    """Coarse (nc^2, dx=2 km) + learned closure against the fine solver
-    ((2nc)^2, dx=1 km)."""
+def form_closure(sc, eval_windows=60):
+    """Coarse (1024^2) + learned closure against the fine (2048^2) solver:
+    training on the device, the held-out basin judged by the fine run,
+    throughput of the three runs, and the convolution kernels' rates at the
+    training batch (4 x 1024^2)."""
     import nemo2d
     from nemo2d import closure_pipeline as cp
+    nc = 1024 if sc >= 1 else max(64, int(1024 * sc) // 32 * 32)
+    basins = max(4, int(round(64 * min(sc, 1.0))))
+    windows = max(10, int(round(313 * min(sc, 1.0))))
     t0 = time.perf_counter()
-    Xs, Ys = [], []
-    for seed in range(runs):
-        X, Y = cp.generate_pairs(cp.Setup(nc, seed, **CLOSURE_BATHY), n_snap=snaps)
-        Xs.append(X)
-        Ys.append(Y)
-    X, Y = np.concatenate(Xs), np.concatenate(Ys)
-    t_data = time.perf_counter() - t0
-    clos = cp.Closure(X, Y, seed=0)
-    t0 = time.perf_counter()
-    losses = clos.train(X, Y, iters=iters, batch=4, lr=2e-3)
+    clos, info = cp.train_online(nc, range(basins), windows, group=4, bathy=CLOSURE_BATHY, lr=CLOSURE_LR,
+                                 rollout=CLOSURE_ROLLOUT)
     t_train = time.perf_counter() - t0
     held = cp.Setup(nc, 1000, **CLOSURE_BATHY)
     t0 = time.perf_counter()
-    ep, ec, fell = cp.evaluate(held, clos, n_windows=windows)
+    ep, ec, fell = cp.evaluate_device(held, clos, eval_windows, tol=1.0, judge_every=1)
+    _, ec_free, _ = cp.evaluate_device(held, clos, eval_windows, tol=float("inf"))
     t_eval = time.perf_counter() - t0
-    # throughput: plain coarse, coarse with the on-device correction every
-    # 10 steps, fine
-    csteps = 10 * windows
+    first = [k for k in range(len(ec_free)) if not ec_free[k] < ep[k]]
+    # throughput: plain coarse, coarse with the on-device correction every 10
+    # steps, fine; each over the evaluation's windows
+    K = cp.K_STEPS
     rates = {}
     for name in ("coarse", "coarse+closure", "fine"):
         m = held.fine_model() if name == "fine" else held.coarse_model()
-        nst = 2 * csteps if name == "fine" else csteps
+        nst = (2 if name == "fine" else 1) * K * eval_windows
         m.run(1, 2)
+        m.checksums()
         t1 = time.perf_counter()
         if name == "coarse+closure":
-            for w in range(windows):
-                m.run(10 * w + 1, 10 * w + 10)
+            for w in range(eval_windows):
+                m.run(K * w + 1, K * w + K)
                 clos.correct_model(m)
-            m.checksums()
         else:
             m.run(1, nst)
-            m.checksums()
+        m.checksums()
         dt = time.perf_counter() - t1
         n = held.nf if name == "fine" else held.nc
         rates[name] = n * n * nst / dt
         m.close()
-    return {"workload": "coarse %dx%d (dx=2 km, rdt=40 s) + 4-layer 3x3-conv closure (32 channels, "
-                        "GELU, BF16 MFMA, FP32 accumulation) every 10 steps; trained on %d pairs "
-                        "from %d seeded seamount fine runs %dx%d (dx=1 km, rdt=20 s, 8 mounts, "
-                        "radii 10-60 km, heights 10-60 m); judged on a "
-                        "held-out basin for %d coarse steps" % (nc, nc, len(X), runs, 2 * nc, 2 * nc,
-                                                               csteps),
-            "ssh_rms_vs_fine": {"coarse": float(ep.mean()), "coarse+closure": float(ec.mean())},
+    # the convolution kernels at the training batch
+    rng = np.random.default_rng(0)
+    P = 4 * nc * nc
+    xp = np.zeros((P, 8), np.float32)
+    xp[:, :4] = rng.standard_normal((P, 4))
+    xg = clos.net.to_bf16(xp)
+    kt = clos.net.time_kernels(xg, 4, nc, nc)
+    flop = CONV_FLOP_PER_PIXEL * P
+    kernels = {}
+    for name, t in kt.items():
+        # HBM bytes per pixel: forward reads X (64) and writes z, act (64 + 64);
+        # data gradient reads dZ, z below (64 + 64) and writes dz below (64);
+        # weight gradient reads X and dZ (64 + 64)
+        bpp = {"conv_fwd": 192, "conv_bwd_data_gelu": 192, "conv_bwd_weight": 128}[name]
+        kernels[name] = {"us": t * 1e6, "tflops": flop / t / 1e12, "frac_of_bf16_mfma_peak": flop / t / MFMA_BF16_PEAK,
+                         "hbm_gbs": bpp * P / t / 1e9, "frac_of_hbm": bpp * P / t / 1e9 / HBM_PEAK_GBS}
+    return {"workload": "coarse %dx%d (dx=2 km, rdt=40 s) + 4-layer 3x3-conv closure (4->32->32->32->3, "
+                        "GELU, BF16 MFMA 32x32x16, FP32 accumulation) applied every %d steps; trained on "
+                        "the device on %d pairs from %d seeded seamount fine runs %dx%d (dx=1 km, rdt=20 s, "
+                        "32 mounts, heights 10-60 m, radii 20-200 km) restricted to %dx%d, %d windows "
+                        "each, batch 4, one Adam step per batch, the prediction restarted from the fine "
+                        "state every %d windows and corrected by the network in between; judged on a "
+                        "held-out basin (seed 1000) "
+                        "by the fine run for %d windows" % (nc, nc, K, info["pairs"], basins, 2 * nc, 2 * nc,
+                                                            nc, nc, windows, CLOSURE_ROLLOUT, eval_windows),
+            "ssh_rms_vs_fine": {"coarse": float(ep.mean()), "coarse+closure (judged every window)": float(ec.mean()),
+                                "coarse+closure (no judge)": float(np.nanmean(ec_free)) if np.isfinite(ec_free).any() else None},
+            "closure_better_than_coarse_until_window": first[0] if first else eval_windows,
             "fallback_fired": bool(fell),
-            "train_loss_first_last": [losses[0], losses[-1]],
+            "train_loss_first_last": [info["losses"][0][1], info["losses"][-1][1]],
             "cell_updates_per_s": rates,
             "fine_equivalent_rate_of_coarse+closure": rates["coarse+closure"] * 8.0,
-            "seconds": {"data": t_data, "train": t_train, "eval": t_eval}}
+            "conv_kernels_batch_4x%dx%d" % (nc, nc): kernels,
+            "seconds": {"train_total": t_train, "eval": t_eval, **info["seconds"]}}
 
 
 # the fused-step kernel each variant runs on the uniform benchmark grid
@@ -258,7 +285,7 @@ def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_
     runs = {"fp32_mixed": lambda: form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
             "ensemble": lambda: form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc)),
                                               graph=graph),
-            "closure": lambda: form_closure(max(32, int(256 * sc)), 4, 16, 300, 20)}
+            "closure": lambda: form_closure(sc)}
     out = {}
     for name in ("fp32_mixed", "ensemble", "closure"):
         if name not in which:
@@ -292,7 +319,7 @@ def main():
                     help="skip the single-precision, ensemble and closure forms")
     ap.add_argument("--forms-scale", type=float, default=1.0,
                     help="scale of the forms' workloads (1.0: 8192^2 x 5000 FP32, 256-basin "
-                         "ensemble x 3000, closure at 256^2)")
+                         "ensemble x 3000, closure 1024^2/2048^2 with 64 basins x 313 windows)")
     ap.add_argument("--form", choices=("fp32_mixed", "ensemble", "closure"),
                     help="run only this form (at --forms-scale) and print its JSON line; "
                          "for per-form kernel traces")

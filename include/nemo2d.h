@@ -298,8 +298,9 @@ int nemo2d_nn_conv_bwd_weight(const void *x, const void *dz, int n, int h, int w
                               int cout, float *dw, float *db, float *work, void *stream);
 /* dz = dact * GELU'(z), BF16. */
 int nemo2d_nn_gelu_bwd(const float *dact, const float *z, long count, void *dz, void *stream);
-/* Weighted MSE over channels < ch: gradient dy (BF16) and per-block loss
- * parts (nblocks floats, to be summed). */
+/* Weighted MSE over channels < ch (1 <= ch <= 8) of [pixels][8] FP32
+ * tensors: gradient dy (BF16 [pixels][8], 0 in channels >= ch) and per-block
+ * loss parts (nblocks floats, to be summed). */
 int nemo2d_nn_mse(const float *y, const float *t, long pixels, int ch, const float *wch,
                   void *dy, float *part, int nblocks, void *stream);
 int nemo2d_nn_adam(float *p, const float *g, float *m, float *v, long n, float lr, float b1,
@@ -342,6 +343,8 @@ int nemo2d_dev_free(void *p);
 int nemo2d_dev_upload(void *dst, const void *src, long bytes);
 int nemo2d_dev_download(void *dst, const void *src, long bytes);
 int nemo2d_dev_zero(void *p, long bytes);
+/* Wait for all work on the current device (a timing aid). */
+int nemo2d_dev_synchronize(void);
 
 /* Device timers of the serial model's timed regions (dl_timer's
  * 'Continuity', 'Momentum', 'BCs', 'Next', original/nemolite2d.f90:391-769,

@@ -27,7 +27,6 @@ using namespace nemo2d;
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int NB = 256;  // threads per block (4 waves)
@@ -41,41 +40,25 @@ __device__ __forceinline__ float gelu_grad(float z) {
            z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-__device__ __forceinline__ bf16x4 load4(const uint16_t* p) {
-    // 4 consecutive bf16 (8-byte aligned)
-    const uint2 v = *(const uint2*)p;
-    bf16x4 r;
-    r[0] = (short)(v.x & 0xffff);
-    r[1] = (short)(v.x >> 16);
-    r[2] = (short)(v.y & 0xffff);
-    r[3] = (short)(v.y >> 16);
-    return r;
-}
-
 struct Img {
     int n, h, w;
     __device__ long pixels() const { return (long)n * h * w; }
 };
 
-// Pixel p shifted by (dy, dx) within its image, or -1 outside.
-__device__ __forceinline__ long shifted(const Img& im, long p, int dy, int dx) {
-    const int x = (int)(p % im.w);
-    const long t = p / im.w;
-    const int y = (int)(t % im.h);
-    const int yy = y + dy, xx = x + dx;
-    if (yy < 0 || yy >= im.h || xx < 0 || xx >= im.w) return -1;
-    return p + (long)dy * im.w + dx;
-}
-
 // ---- forward and data-gradient convolutions -------------------------------
 // A block owns a tile of CT_H rows x CT_W = 32 pixels of one image.  The
 // input tile with its 1-pixel halo is staged once in LDS (pixel-major, the
 // channels of a pixel contiguous, padded to the MFMA K step); each of the 4
-// waves computes CT_H/4 row segments, one 32-pixel x 32-channel MFMA tile
-// (v_mfma_f32_32x32x16_bf16) each, over the 9 taps.  The weights of every
-// tap stay in registers.  Operand layout (32x32x16): lane l holds A row l%32
-// and B column l%32 at 8 consecutive k of half l/32; the K order is the same
-// for both operands, so any assignment of k to (channel) values works.
+// waves computes CT_H/4 row segments, one 32-channel x 32-pixel MFMA tile
+// (v_mfma_f32_32x32x16_bf16) each, over the 9 taps (taps outer, the next
+// tap's weights loaded during this tap's MFMAs).  Operand layout
+// (32x32x16): lane l holds A row l%32 and B column l%32 at 8 consecutive k
+// of half l/32; the K order is the same for both operands, so any
+// assignment of k to (channel) values works.  The weights are the A operand
+// (rows = output channels) and the pixels the B operand (columns), so that
+// accumulator register r of lane l holds channel 8*(r/4) + 4*(l/32) + r%4 of
+// pixel l%32: each lane owns runs of 4 consecutive channels of one pixel,
+// stored 8 (BF16) or 16 (FP32) bytes at a time.
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int CT_W = 32, CT_H = 8;
 constexpr int CT_PIX = (CT_H + 2) * (CT_W + 2);
@@ -97,15 +80,68 @@ __device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ src, con
                                            int img, int y0, int x0, uint16_t* tile) {
     constexpr int PS = KT + 8;  // LDS pixel stride (bf16): +16 B against bank conflicts
     constexpr int CH = KT / 8;  // 16-byte chunks per pixel
-    for (int e = threadIdx.x; e < CT_PIX * CH; e += NB) {
+    constexpr int IT = (CT_PIX * CH + NB - 1) / NB;
+    // every load of the thread issued before the first LDS store
+    uint4 v[IT];
+#pragma unroll
+    for (int s = 0; s < IT; ++s) {
+        const int e = threadIdx.x + s * NB;
         const int q = e / CH, c8 = (e % CH) * 8;
         const int yy = y0 - 1 + q / (CT_W + 2), xx = x0 - 1 + q % (CT_W + 2);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
-            v = *(const uint4*)(src + (((long)img * im.h + yy) * im.w + xx) * cp + c8);
-        *(uint4*)(tile + q * PS + c8) = v;
+        v[s] = make_uint4(0, 0, 0, 0);
+        if (e < CT_PIX * CH && c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
+            v[s] = *(const uint4*)(src + (((long)img * im.h + yy) * im.w + xx) * cp + c8);
+    }
+#pragma unroll
+    for (int s = 0; s < IT; ++s) {
+        const int e = threadIdx.x + s * NB;
+        if (e < CT_PIX * CH) *(uint4*)(tile + (e / CH) * PS + (e % CH) * 8) = v[s];
     }
 }
+
+// Lane's weights of one tap: column c (output channel of the forward, input
+// channel of the data gradient), K values kk*16 + 8*half .. +7 (0 past kp).
+template <int KT>
+__device__ __forceinline__ void load_tap(const uint16_t* __restrict__ wt, int tap, int c, int half, int kp,
+                                         bf16x8 (&w)[KT / 16]) {
+#pragma unroll
+    for (int kk = 0; kk < KT / 16; ++kk) {
+        const int k0 = kk * 16 + 8 * half;
+        bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (k0 < kp) v = lds8(wt + ((long)tap * 32 + c) * kp + k0);
+        w[kk] = v;
+    }
+}
+
+// One 3x3 convolution pass over a block's tile: each wave owns rows wave and
+// wave + 4 of the tile (two 32 x 32 accumulators); the taps are the outer
+// loop, with the next tap's weights loaded while this tap's MFMAs run.
+// SIGN = +1: forward (input at p + off(tap)); -1: data gradient (dZ at
+// p - off(tap)).
+template <int KT, int SIGN>
+__device__ __forceinline__ void conv_rows(const uint16_t* tile, const uint16_t* __restrict__ wt, int kp,
+                                          int wave, int c, int half, f32x16& acc0, f32x16& acc1) {
+    constexpr int PS = KT + 8;
+    bf16x8 w[KT / 16], wn[KT / 16];
+    load_tap<KT>(wt, 0, c, half, kp, w);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        if (tap + 1 < 9) load_tap<KT>(wt, tap + 1, c, half, kp, wn);
+        const int dy = SIGN * (tap / 3 - 1), dx = SIGN * (tap % 3 - 1);
+        const uint16_t* a0 = tile + ((wave + 1 + dy) * (CT_W + 2) + c + 1 + dx) * PS + 8 * half;
+        const uint16_t* a1 = a0 + 4 * (CT_W + 2) * PS;
+#pragma unroll
+        for (int kk = 0; kk < KT / 16; ++kk) {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[kk], lds8(a0 + kk * 16), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[kk], lds8(a1 + kk * 16), acc1, 0, 0, 0);
+        }
+        if (tap + 1 < 9)
+#pragma unroll
+            for (int kk = 0; kk < KT / 16; ++kk) w[kk] = wn[kk];
+    }
+}
+
+static_assert(CT_H == 8 && NB == 256, "each of the 4 waves owns 2 rows of the tile");
 
 // Forward: z = b + sum_tap sum_ci X[p + off(tap)][ci] W[tap][ci][co];
 // act = gelu(z) (or z).  wt: [9][32][cinp] (cout-major, from pack).
@@ -121,58 +157,48 @@ k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __r
     const int img = b / (tiles_x * tiles_y);
     const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
-    // weights of this lane's column (output channel c), every tap and K step
-    bf16x8 wreg[9][KT / 16];
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-        for (int kk = 0; kk < KT / 16; ++kk) {
-            const int ci = kk * 16 + 8 * half;
-            bf16x8 w = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (ci < cinp) w = lds8(wt + ((long)tap * 32 + c) * cinp + ci);
-            wreg[tap][kk] = w;
-        }
     stage_tile<KT>(x, im, cinp, img, y0, x0, tile);
     __syncthreads();
-    const float bc = c < cout ? bias[c] : 0.f;
-    for (int r = wave; r < CT_H; r += NB / 64) {
-        const int y = y0 + r;
+    f32x16 acc[2] = {{0}, {0}};
+    conv_rows<KT, 1>(tile, wt, cinp, wave, c, half, acc[0], acc[1]);
+    const int xx = x0 + c;  // this lane's pixel column
+    if (xx >= im.w) return;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + wave + 4 * h;
         if (y >= im.h) break;
-        f32x16 acc = {0};
+        const long q0 = (((long)img * im.h + y) * im.w + xx) * coutp;
+        // channels co0 .. co0+3 in registers 4g .. 4g+3
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-            const uint16_t* a = tile + ((r + 1 + dy) * (CT_W + 2) + c + 1 + dx) * PS + 8 * half;
+        for (int g = 0; g < 4; ++g) {
+            const int co0 = 8 * g + 4 * half;
+            if (co0 >= coutp) break;
+            float v[4];
+            uint16_t vb[4], va[4];
 #pragma unroll
-            for (int kk = 0; kk < KT / 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds8(a + kk * 16), wreg[tap][kk], acc, 0, 0, 0);
-        }
-        // accumulator register k: pixel 8*(k/4) + 4*half + k%4 of the row
-        // segment, output channel c
-        if (c >= coutp) continue;
-        const long row = ((long)img * im.h + y) * im.w;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
-            if (xx >= im.w) continue;
-            const long q = (row + xx) * coutp + c;
-            if (c < cout) {
-                const float v = acc[k] + bc;
-                if (z) z[q] = v;
-                if (zb) zb[q] = f2bf(v);
-                if (act) act[q] = f2bf(use_gelu ? gelu(v) : v);
-            } else {  // padded channels: zero for the next layer
-                if (zb) zb[q] = 0;
-                if (act) act[q] = 0;
+            for (int r = 0; r < 4; ++r) {
+                const bool live = co0 + r < cout;
+                v[r] = live ? acc[h][4 * g + r] + bias[co0 + r] : 0.f;
+                vb[r] = live ? f2bf(v[r]) : (uint16_t)0;  // padded channels: zero
+                va[r] = live ? f2bf(use_gelu ? gelu(v[r]) : v[r]) : (uint16_t)0;
             }
+            if (z) {
+                if (co0 + 4 <= cout) {
+                    *(float4*)(z + q0 + co0) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+                    for (int r = 0; r < 4 && co0 + r < cout; ++r) z[q0 + co0 + r] = v[r];
+                }
+            }
+            if (zb) *(uint2*)(zb + q0 + co0) = make_uint2(vb[0] | (uint32_t)vb[1] << 16, vb[2] | (uint32_t)vb[3] << 16);
+            if (act) *(uint2*)(act + q0 + co0) = make_uint2(va[0] | (uint32_t)va[1] << 16, va[2] | (uint32_t)va[3] << 16);
         }
     }
 }
 
 // Data gradient: dX[p][ci] = sum_tap sum_co dZ[p - off(tap)][co] W[tap][ci][co];
-// with zp (the layer below's pre-activation) the output is the layer below's
-// dz = BF16(dX * GELU'(zp)) (GELU backward fused), else FP32 dX.
-// wt: [9][32][coutp] (cin-major, from pack).
+// with a pre-activation of the layer below (zp FP32 or zpb BF16) the output
+// is that layer's dz = BF16(dX * GELU'(z)) (GELU backward fused), else FP32
+// dX.  wt: [9][32][coutp] (cin-major, from pack).
 template <int KT>
 __global__ void __launch_bounds__(NB)
 k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16_t* __restrict__ wt,
@@ -185,52 +211,39 @@ k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16
     const int img = b / (tiles_x * tiles_y);
     const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
-    bf16x8 wreg[9][KT / 16];
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-        for (int kk = 0; kk < KT / 16; ++kk) {
-            const int co = kk * 16 + 8 * half;
-            bf16x8 w = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (co < coutp) w = lds8(wt + ((long)tap * 32 + c) * coutp + co);
-            wreg[tap][kk] = w;
-        }
     stage_tile<KT>(dz, im, coutp, img, y0, x0, tile);
     __syncthreads();
+    f32x16 acc[2] = {{0}, {0}};
+    conv_rows<KT, -1>(tile, wt, coutp, wave, c, half, acc[0], acc[1]);
+    const int xx = x0 + c;  // this lane's pixel column
+    if (xx >= im.w) return;
     const bool gl = zp || zpb;
-    for (int r = wave; r < CT_H; r += NB / 64) {
-        const int y = y0 + r;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + wave + 4 * h;
         if (y >= im.h) break;
-        const long row = ((long)img * im.h + y) * im.w;
-        // the layer below's pre-activations of this lane's 16 outputs, loaded
-        // before the MFMAs so that their latency hides behind them
-        float zv[16];
+        const long q0 = (((long)img * im.h + y) * im.w + xx) * cinp;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
-            const long q = (row + xx) * cinp + c;
-            zv[k] = 0.f;
-            if (gl && c < cin && xx < im.w) zv[k] = zp ? zp[q] : bf2f(zpb[q]);
-        }
-        f32x16 acc = {0};
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
-            const uint16_t* a = tile + ((r + 1 - dy) * (CT_W + 2) + c + 1 - dxo) * PS + 8 * half;
-#pragma unroll
-            for (int kk = 0; kk < KT / 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds8(a + kk * 16), wreg[tap][kk], acc, 0, 0, 0);
-        }
-        if (c >= cinp) continue;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int xx = x0 + 8 * (k >> 2) + 4 * half + (k & 3);
-            if (xx >= im.w) continue;
-            const long q = (row + xx) * cinp + c;
+        for (int g = 0; g < 4; ++g) {
+            const int ci0 = 8 * g + 4 * half;
+            if (ci0 >= cinp) break;
             if (gl) {
-                dzp[q] = c < cin ? f2bf(acc[k] * gelu_grad(zv[k])) : (uint16_t)0;
-            } else if (c < cin) {
-                dx[q] = acc[k];
+                float zv[4];
+                if (zpb) {
+                    const uint2 u = *(const uint2*)(zpb + q0 + ci0);
+                    zv[0] = bf2f(u.x & 0xffff); zv[1] = bf2f(u.x >> 16);
+                    zv[2] = bf2f(u.y & 0xffff); zv[3] = bf2f(u.y >> 16);
+                } else {
+                    const float4 u = *(const float4*)(zp + q0 + ci0);
+                    zv[0] = u.x; zv[1] = u.y; zv[2] = u.z; zv[3] = u.w;
+                }
+                uint16_t o[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    o[r] = ci0 + r < cin ? f2bf(acc[h][4 * g + r] * gelu_grad(zv[r])) : (uint16_t)0;
+                *(uint2*)(dzp + q0 + ci0) = make_uint2(o[0] | (uint32_t)o[1] << 16, o[2] | (uint32_t)o[3] << 16);
+            } else {
+                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[q0 + ci0 + r] = acc[h][4 * g + r];
             }
         }
     }
@@ -238,15 +251,15 @@ k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16
 
 // ---- weight gradient ---------------------------------------------------------
 // dW[tap][ci][co] = sum_p X[p + off(tap)][ci] dZ[p][co] and db[co] = sum_p dZ[p][co].
-// A block of 9 waves (wave = tap) walks tiles of WG_H rows x WG_W pixels of
+// A block of 10 waves (waves 0..8: one tap each) walks tiles of WG_H rows x WG_W pixels of
 // 4 images at once.  Each tile is staged in LDS transposed to
 // [channel][y][x][image], so that a lane's 8 K values (2 neighbouring pixels
 // x 4 images) of one channel are 16 contiguous bytes for every tap shift;
 // the next tile's global loads are issued before the current tile's MFMAs.
-// The centre-tap wave also multiplies dZ by a ones operand: every row of
-// that product is the bias gradient.  Each block writes its partial sums
+// Wave 9 multiplies dZ by a ones operand: every row of that product is the
+// bias gradient.  Each block writes its partial sums
 // (9 x 32 x 32 + 32 floats); k_wgrad_sum adds the blocks in fixed order.
-constexpr int WG_H = 2, WG_W = 32, WG_NT = 9 * 64;
+constexpr int WG_H = 2, WG_W = 32, WG_NT = 10 * 64;
 constexpr int WG_XP = (WG_H + 2) * (WG_W + 2);        // halo tile pixels
 constexpr int WG_CSX = 290;                           // X channel stride, dwords (= 2 mod 32)
 constexpr int WG_CSZ = 132;                           // dZ channel stride, dwords (= 4 mod 32)
@@ -320,7 +333,7 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint2 a, uint2 b) {
     return r;
 }
 
-__global__ void __launch_bounds__(WG_NT)
+__global__ void __launch_bounds__(WG_NT, 5)  // 5 waves per SIMD: 2 blocks per CU
 k_conv_bwd_weight(const uint16_t* __restrict__ x, int cinp, const uint16_t* __restrict__ dz,
                   int coutp, Img im, float* __restrict__ work) {
     __shared__ __attribute__((aligned(16))) uint32_t xs[32 * WG_CSX];
@@ -329,10 +342,11 @@ k_conv_bwd_weight(const uint16_t* __restrict__ x, int cinp, const uint16_t* __re
     const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
     const int tx = (im.w + WG_W - 1) / WG_W, ty = (im.h + WG_H - 1) / WG_H, ng = (im.n + 3) / 4;
     const long ntiles = (long)tx * ty * ng;
-    f32x16 acc = {0}, accb = {0};
+    f32x16 acc = {0};
     bf16x8 ones;
 #pragma unroll
     for (int k = 0; k < 8; ++k) ones[k] = (short)0x3f80;  // BF16 1.0
+    const bool bias_wave = tap == 9;
     WgItems it;
     long t = blockIdx.x;
     if (t < ntiles) {
@@ -354,24 +368,29 @@ k_conv_bwd_weight(const uint16_t* __restrict__ x, int cinp, const uint16_t* __re
 #pragma unroll
             for (int x4 = 0; x4 < WG_W; x4 += 4) {
                 const int xq = x4 + 2 * half;
-                bf16x8 a = {0, 0, 0, 0, 0, 0, 0, 0}, bz = {0, 0, 0, 0, 0, 0, 0, 0};
-                if (c < cinp) {
-                    const uint32_t* pa = xs + c * WG_CSX + 2 * ((y + 1 + dy) * (WG_W + 2) + xq + 1 + dxo);
-                    a = as_bf16x8(*(const uint2*)pa, *(const uint2*)(pa + 2));
+                bf16x8 a = ones, bz = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (!bias_wave) {
+                    a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                    if (c < cinp) {
+                        const uint32_t* pa = xs + c * WG_CSX + 2 * ((y + 1 + dy) * (WG_W + 2) + xq + 1 + dxo);
+                        a = as_bf16x8(*(const uint2*)pa, *(const uint2*)(pa + 2));
+                    }
                 }
                 if (c < coutp) {
                     const uint4 v = *(const uint4*)(zs + c * WG_CSZ + 2 * (y * WG_W + xq));
                     bz = as_bf16x8(make_uint2(v.x, v.y), make_uint2(v.z, v.w));
                 }
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bz, acc, 0, 0, 0);
-                if (tap == 4) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ones, bz, accb, 0, 0, 0);
             }
     }
     // register k of lane l: row ci = 8*(k/4) + 4*half + k%4, column co = c
     float* out = work + (long)blockIdx.x * WG_PART;
+    if (bias_wave) {
+        if (half == 0) out[9 * 1024 + c] = acc[0];
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) out[tap * 1024 + (8 * (k >> 2) + 4 * half + (k & 3)) * 32 + c] = acc[k];
-    if (tap == 4 && half == 0) out[9 * 1024 + c] = accb[0];
 }
 
 // dW[tap][ci][co] and db[co] from the blocks' partials.  A block takes 32
@@ -416,23 +435,34 @@ __global__ void k_gelu_bwd(const float* __restrict__ dact, const float* __restri
     if (k < count) dz[k] = f2bf(dact[k] * gelu_grad(z[k]));
 }
 
-// Weighted MSE over channels < ch of [P][cp] tensors: loss (one float per
-// block into part), gradient dy (BF16).
+// Weighted MSE over channels < ch (<= 8) of [P][cp] tensors, cp = 8: loss
+// (one float per block into part), gradient dy (BF16).  A thread takes whole
+// pixels: 32 bytes of y and t in, 16 bytes of dy out.
 __global__ void __launch_bounds__(NB)
-k_mse(const float* __restrict__ y, const float* __restrict__ t, long P, int cp, int ch,
+k_mse(const float* __restrict__ y, const float* __restrict__ t, long P, int ch,
       const float* __restrict__ wch, float inv_count, uint16_t* __restrict__ dy,
       float* __restrict__ part) {
     __shared__ float red[NB];
+    float w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = c < ch ? wch[c] : 0.f;
     float s = 0.f;
-    for (long k = (long)blockIdx.x * NB + threadIdx.x; k < P * cp; k += (long)gridDim.x * NB) {
-        const int c = (int)(k % cp);
-        float g = 0.f;
-        if (c < ch) {
-            const float d = y[k] - t[k];
-            s += wch[c] * d * d;
-            g = 2.f * wch[c] * d * inv_count;
+    for (long p = (long)blockIdx.x * NB + threadIdx.x; p < P; p += (long)gridDim.x * NB) {
+        const float4* yp = (const float4*)(y + 8 * p);
+        const float4* tp = (const float4*)(t + 8 * p);
+        const float4 y0 = yp[0], y1 = yp[1], t0 = tp[0], t1 = tp[1];
+        const float d[8] = {y0.x - t0.x, y0.y - t0.y, y0.z - t0.z, y0.w - t0.w,
+                            y1.x - t1.x, y1.y - t1.y, y1.z - t1.z, y1.w - t1.w};
+        uint32_t g[4];
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) {
+            const float d0 = c < ch ? d[c] : 0.f, d1 = c + 1 < ch ? d[c + 1] : 0.f;
+            s += w[c] * d0 * d0;
+            s += w[c + 1] * d1 * d1;
+            g[c / 2] = (uint32_t)f2bf(2.f * w[c] * d0 * inv_count) |
+                       (uint32_t)f2bf(2.f * w[c + 1] * d1 * inv_count) << 16;
         }
-        dy[k] = f2bf(g);
+        *(uint4*)(dy + 8 * p) = make_uint4(g[0], g[1], g[2], g[3]);
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -575,10 +605,10 @@ int nemo2d_nn_gelu_bwd(const float* dact, const float* z, long count, void* dz, 
 // part: at least nblocks floats; the host sums them (nblocks returned).
 int nemo2d_nn_mse(const float* y, const float* t, long pixels, int ch, const float* wch, void* dy,
                   float* part, int nblocks, void* stream) {
-    if (!y || !t || !wch || !dy || !part || pixels < 1 || bad_ch(ch) || nblocks < 1)
+    if (!y || !t || !wch || !dy || !part || pixels < 1 || ch < 1 || ch > 8 || nblocks < 1)
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_mse: bad argument");
-    k_mse<<<nblocks, NB, 0, (hipStream_t)stream>>>(y, t, pixels, pad8(ch), ch, wch,
-                                                   1.0f / (float)(pixels * ch), (uint16_t*)dy, part);
+    k_mse<<<nblocks, NB, 0, (hipStream_t)stream>>>(y, t, pixels, ch, wch, 1.0f / (float)(pixels * ch),
+                                                   (uint16_t*)dy, part);
     return launched("mse");
 }
 
@@ -639,6 +669,12 @@ int nemo2d_dev_upload(void* dst, const void* src, long bytes) {
 
 int nemo2d_dev_download(void* dst, const void* src, long bytes) {
     return dev_copy(dst, src, bytes, hipMemcpyDeviceToHost, "nemo2d_dev_download");
+}
+
+int nemo2d_dev_synchronize(void) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("nemo2d_dev_synchronize: ") + hipGetErrorString(e));
+    return NEMO2D_OK;
 }
 
 int nemo2d_dev_zero(void* p, long bytes) {

@@ -11,8 +11,9 @@
 // tiling) one wave's column strip marched with the single-model path's
 // shared-face-term body; either way an FP64 member is bitwise the same as
 // that member run alone.  Per-member checksums are reduced in a fixed order by a second
-// kernel.  A run of many steps is captured once into a HIP graph and
-// replayed.  The tile shape is chosen by timing the candidates
+// kernel.  The step index and the now-state buffer live in a device control
+// word that a third (one-thread) kernel moves on, so one step is captured
+// into a HIP graph once and replayed for every step.  The tile shape is chosen by timing the candidates
 // (nemo2d_ensemble_tune).
 #include <hip/hip_runtime.h>
 
@@ -42,6 +43,23 @@ namespace {
 
 constexpr int EB = 256;  // threads per block of the ensemble step
 
+// Step control in device memory: the step index (relative to the run's
+// first step: the tide table's column and the checksum log's slot) and which
+// of the two state buffers holds the now-state.  The step kernels read it and
+// k_ens_advance moves it on, so one captured step can be replayed.
+struct EnsCtl {
+    int step, cur;
+};
+// The two now-state buffers (sshn, sshn_u, sshn_v, un, vn each).
+struct EnsBufs {
+    void* x[2][5];
+};
+
+template <class T>
+__device__ __forceinline__ T* pick(const EnsBufs& b, int set, int f) {
+    return (T*)(set ? b.x[1][f] : b.x[0][f]);
+}
+
 struct EnsMember {
     Layout L;      // member layout; offsets relative to the member's base
     long off;      // element offset of the member in the pooled arrays
@@ -70,11 +88,17 @@ constexpr bool is_march(int t) { return t == MARCH_TILING || t == MARCH_RC_TILIN
 template <int TX, int ROWS>
 __global__ void __launch_bounds__(EB)
 k_ens_step64(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, int ntiles,
-             NState n0, AOut a0, const uint16_t* __restrict__ flp, const double* __restrict__ flup,
-             const double* __restrict__ flvp, Params p, const double* __restrict__ tide_tab,
-             int tide_stride, int step_idx, double* __restrict__ partials) {
+             EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
+             const double* __restrict__ flup, const double* __restrict__ flvp, Params p,
+             const double* __restrict__ tide_tab, int tide_stride, double* __restrict__ partials) {
     constexpr int TY = EB / TX;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int cur = ctl->cur, step_idx = ctl->step;
+    const NState n0{pick<const double>(bufs, cur, 0), pick<const double>(bufs, cur, 1),
+                    pick<const double>(bufs, cur, 2), pick<const double>(bufs, cur, 3),
+                    pick<const double>(bufs, cur, 4)};
+    const AOut a0{pick<double>(bufs, 1 - cur, 0), pick<double>(bufs, 1 - cur, 1), pick<double>(bufs, 1 - cur, 2),
+                  pick<double>(bufs, 1 - cur, 3), pick<double>(bufs, 1 - cur, 4)};
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
         const EnsMember& M = mem[tl.x];
@@ -97,12 +121,18 @@ k_ens_step64(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, 
 template <int TX, int ROWS>
 __global__ void __launch_bounds__(EB)
 k_ens_step32(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, int ntiles,
-             MixedState<true> n0, MixedOut<true> a0, const uint16_t* __restrict__ flp,
+             EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
              const double* __restrict__ flup, const double* __restrict__ flvp, ParamsT<float> p,
-             const double* __restrict__ tide_tab, int tide_stride, int step_idx,
-             double* __restrict__ partials) {
+             const double* __restrict__ tide_tab, int tide_stride, double* __restrict__ partials) {
     constexpr int TY = EB / TX;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int cur = ctl->cur, step_idx = ctl->step;
+    const MixedState<true> n0{pick<const double>(bufs, cur, 0), pick<const float>(bufs, cur, 1),
+                              pick<const float>(bufs, cur, 2), pick<const float>(bufs, cur, 3),
+                              pick<const float>(bufs, cur, 4)};
+    const MixedOut<true> a0{pick<double>(bufs, 1 - cur, 0), pick<float>(bufs, 1 - cur, 1),
+                            pick<float>(bufs, 1 - cur, 2), pick<float>(bufs, 1 - cur, 3),
+                            pick<float>(bufs, 1 - cur, 4)};
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
         const EnsMember& M = mem[tl.x];
@@ -130,10 +160,16 @@ k_ens_step32(const EnsMember* __restrict__ mem, const int4* __restrict__ tiles, 
 template <class R, class SS>
 __global__ void __launch_bounds__(EB)
 k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
-            StateG<R, SS, false> n0, OutG<R, SS> a0, const uint16_t* __restrict__ flp,
+            EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
             const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
-            int tide_stride, int step_idx, double* __restrict__ partials) {
+            int tide_stride, double* __restrict__ partials) {
     const int lane = threadIdx.x & 63;
+    const int cur = ctl->cur, step_idx = ctl->step;
+    const StateG<R, SS, false> n0{pick<const SS>(bufs, cur, 0), pick<const R>(bufs, cur, 1),
+                                  pick<const R>(bufs, cur, 2), pick<const R>(bufs, cur, 3),
+                                  pick<const R>(bufs, cur, 4), nullptr, nullptr, nullptr};
+    const OutG<R, SS> a0{pick<SS>(bufs, 1 - cur, 0), pick<R>(bufs, 1 - cur, 1), pick<R>(bufs, 1 - cur, 2),
+                         pick<R>(bufs, 1 - cur, 3), pick<R>(bufs, 1 - cur, 4)};
     // the wave index made wave-uniform (an SGPR): the work item, the member's
     // record and its metrics are then scalar loads held in SGPRs
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -169,10 +205,16 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
 template <class R, class SS>
 __global__ void __launch_bounds__(EB)
 k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
-               StateG<R, SS, false> n0, OutG<R, SS> a0, const uint16_t* __restrict__ flp,
+               EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
                const double* __restrict__ flvp, ParamsT<R> p, const double* __restrict__ tide_tab,
-               int tide_stride, int step_idx, double* __restrict__ partials) {
+               int tide_stride, double* __restrict__ partials) {
     const int lane = threadIdx.x & 63;
+    const int cur = ctl->cur, step_idx = ctl->step;
+    const StateG<R, SS, false> n0{pick<const SS>(bufs, cur, 0), pick<const R>(bufs, cur, 1),
+                                  pick<const R>(bufs, cur, 2), pick<const R>(bufs, cur, 3),
+                                  pick<const R>(bufs, cur, 4), nullptr, nullptr, nullptr};
+    const OutG<R, SS> a0{pick<SS>(bufs, 1 - cur, 0), pick<R>(bufs, 1 - cur, 1), pick<R>(bufs, 1 - cur, 2),
+                         pick<R>(bufs, 1 - cur, 3), pick<R>(bufs, 1 - cur, 4)};
     // the wave index made wave-uniform (an SGPR): the work item, the member's
     // record and its metrics are then scalar loads held in SGPRs
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -247,11 +289,21 @@ k_ens_interp(const EnsMember* __restrict__ mem, const SS* __restrict__ sshn, R* 
 }
 
 // Per-member checksums: block m reduces its members' tile partials in order.
+// (log slot = the step index of the control word)
 __global__ void __launch_bounds__(256)
 k_ens_finish(const EnsMember* __restrict__ mem, const double* __restrict__ partials,
-             double* __restrict__ log, int cap, int slot) {
+             double* __restrict__ log, int cap, const EnsCtl* __restrict__ ctl) {
     const EnsMember& M = mem[blockIdx.x];
+    const int slot = ctl->step;
     finish_partials(partials + 6 * (long)M.tile0, M.ntiles, log + 3 * ((long)blockIdx.x * cap + slot));
+}
+
+// The step is done: the next one has the next index and the other buffer.
+__global__ void k_ens_advance(EnsCtl* __restrict__ ctl) {
+    if (threadIdx.x == 0) {
+        const EnsCtl c = *ctl;
+        *ctl = EnsCtl{c.step + 1, 1 - c.cur};
+    }
 }
 
 }  // namespace
@@ -279,11 +331,20 @@ struct nemo2d_ensemble {
     int grid = 0;     // blocks of the per-cell tilings
     int cus = 256;
     int mgrid[2][2] = {};  // blocks of the marching kernels [fp32][rc]: as many as stay resident
-    double* tide_tab = nullptr;
+    double* tide_tab = nullptr;  // [member][tide_stride]: tide of the run's steps
     int tide_stride = 0;
     double* log = nullptr;
     int log_cap = 0, log_n = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EnsCtl* ctl = nullptr;       // device step control
+    EnsCtl h_ctl{};              // its value at the start of a run
+    // one captured step (step kernel, checksum finish, advance), replayed
+    // for every step of a run; rebuilt when the tiling or a buffer it names
+    // changes (gen counts reallocations)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_tiling = -1;
+    long gen = 0, g_gen = -1;
 };
 
 namespace {
@@ -376,6 +437,7 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
         EHIP(hipMalloc((void**)&e->tiles, e->ntiles * sizeof(int4)));
         EHIP(hipMalloc((void**)&e->partials, 6 * (size_t)e->ntiles * sizeof(double)));
         e->ntiles_cap = e->ntiles;
+        ++e->gen;
     }
     // uploads on the ensemble's (non-blocking) stream, finished before the
     // host buffers go out of scope
@@ -387,64 +449,51 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
     return NEMO2D_OK;
 }
 
+EnsBufs bufs_of(const nemo2d_ensemble* e) {
+    EnsBufs b;
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 5; ++f) b.x[k][f] = e->X[k][f];
+    return b;
+}
+
 template <int TX, int ROWS>
-void launch_step_t(nemo2d_ensemble* e, int step_idx) {
-    const int k = e->cur;
+void launch_step_t(nemo2d_ensemble* e) {
     if (!e->fp32) {
-        double** x = (double**)e->X[k];
-        double** y = (double**)e->X[1 - k];
-        NState n{x[0], x[1], x[2], x[3], x[4]};
-        AOut a{y[0], y[1], y[2], y[3], y[4]};
-        k_ens_step64<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, n, a, e->fl,
-                                                              e->flu, e->flv, e->prm, e->tide_tab,
-                                                              e->tide_stride, step_idx, e->partials);
+        k_ens_step64<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
+                                                              e->fl, e->flu, e->flv, e->prm, e->tide_tab,
+                                                              e->tide_stride, e->partials);
     } else {
-        void** x = e->X[k];
-        void** y = e->X[1 - k];
-        MixedState<true> n{(const double*)x[0], (const float*)x[1], (const float*)x[2],
-                           (const float*)x[3], (const float*)x[4]};
-        MixedOut<true> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
-        k_ens_step32<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, n, a, e->fl,
-                                                              e->flu, e->flv, pf, e->tide_tab,
-                                                              e->tide_stride, step_idx, e->partials);
+        k_ens_step32<TX, ROWS><<<e->grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
+                                                              e->fl, e->flu, e->flv, pf, e->tide_tab,
+                                                              e->tide_stride, e->partials);
     }
 }
 
 template <bool RC>
-void launch_march(nemo2d_ensemble* e, int step_idx) {
-    const int k = e->cur;
+void launch_march(nemo2d_ensemble* e) {
     const int grid = march_waves(e, RC ? MARCH_RC_TILING : MARCH_TILING) / (EB / 64);
     if (!e->fp32) {
-        double** x = (double**)e->X[k];
-        double** y = (double**)e->X[1 - k];
-        StateG<double, double, false> n{x[0], x[1], x[2], x[3], x[4], nullptr, nullptr, nullptr};
-        OutG<double, double> a{y[0], y[1], y[2], y[3], y[4]};
         if constexpr (RC)
             k_ens_march_rc<double, double><<<grid, EB, 0, e->stream>>>(
-                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
-                step_idx, e->partials);
+                e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
+                e->tide_stride, e->partials);
         else
             k_ens_march<double, double><<<grid, EB, 0, e->stream>>>(
-                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, e->prm, e->tide_tab, e->tide_stride,
-                step_idx, e->partials);
+                e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
+                e->tide_stride, e->partials);
     } else {
-        void** x = e->X[k];
-        void** y = e->X[1 - k];
-        StateG<float, double, false> n{(const double*)x[0], (const float*)x[1], (const float*)x[2],
-                                       (const float*)x[3], (const float*)x[4], nullptr, nullptr, nullptr};
-        OutG<float, double> a{(double*)y[0], (float*)y[1], (float*)y[2], (float*)y[3], (float*)y[4]};
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
         if constexpr (RC)
             k_ens_march_rc<float, double><<<grid, EB, 0, e->stream>>>(
-                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
-                step_idx, e->partials);
+                e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, pf, e->tide_tab,
+                e->tide_stride, e->partials);
         else
             k_ens_march<float, double><<<grid, EB, 0, e->stream>>>(
-                e->dmem, e->tiles, e->ntiles, n, a, e->fl, e->flv, pf, e->tide_tab, e->tide_stride,
-                step_idx, e->partials);
+                e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, pf, e->tide_tab,
+                e->tide_stride, e->partials);
     }
 }
 
@@ -462,56 +511,85 @@ void launch_interp(nemo2d_ensemble* e) {
     }
 }
 
-int launch_step(nemo2d_ensemble* e, int step_idx, int slot) {
-    if (e->uv_stale && e->tiling != MARCH_RC_TILING) {
-        launch_interp(e);
-        e->uv_stale = false;
-    }
+// One step as the device control word names it: the step kernel of the
+// current tiling, the per-member checksum finish, and the advance of the
+// control word.  Host state (e->cur) is not touched: the caller accounts
+// for the steps it issued.
+int launch_step(nemo2d_ensemble* e) {
     switch (e->tiling) {
-        case MARCH_TILING: launch_march<false>(e, step_idx); break;
-        case MARCH_RC_TILING: launch_march<true>(e, step_idx); e->uv_stale = true; break;
-        case 0: launch_step_t<64, 1>(e, step_idx); break;
-        case 1: launch_step_t<64, 2>(e, step_idx); break;
-        case 2: launch_step_t<64, 4>(e, step_idx); break;
-        case 3: launch_step_t<128, 1>(e, step_idx); break;
-        case 4: launch_step_t<128, 2>(e, step_idx); break;
-        case 5: launch_step_t<256, 1>(e, step_idx); break;
-        case 6: launch_step_t<32, 1>(e, step_idx); break;
-        default: launch_step_t<32, 2>(e, step_idx); break;
+        case MARCH_TILING: launch_march<false>(e); break;
+        case MARCH_RC_TILING: launch_march<true>(e); break;
+        case 0: launch_step_t<64, 1>(e); break;
+        case 1: launch_step_t<64, 2>(e); break;
+        case 2: launch_step_t<64, 4>(e); break;
+        case 3: launch_step_t<128, 1>(e); break;
+        case 4: launch_step_t<128, 2>(e); break;
+        case 5: launch_step_t<256, 1>(e); break;
+        case 6: launch_step_t<32, 1>(e); break;
+        default: launch_step_t<32, 2>(e); break;
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble step: ") + hipGetErrorString(err));
-    if (slot >= 0) {
-        k_ens_finish<<<e->nmem, 256, 0, e->stream>>>(e->dmem, e->partials, e->log, e->log_cap, slot);
-        err = hipGetLastError();
-        if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble finish: ") + hipGetErrorString(err));
-    }
-    e->cur = 1 - e->cur;
+    k_ens_finish<<<e->nmem, 256, 0, e->stream>>>(e->dmem, e->partials, e->log, e->log_cap, e->ctl);
+    k_ens_advance<<<1, 64, 0, e->stream>>>(e->ctl);
+    err = hipGetLastError();
+    if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble finish: ") + hipGetErrorString(err));
     return NEMO2D_OK;
 }
 
-// Tide values of every member for steps nit000..nitend (host C library).
+// The captured step for the current tiling and buffers (captured once,
+// replayed for every step of every run until one of them changes).
+int ensure_graph(nemo2d_ensemble* e) {
+    if (e->gexec && e->g_tiling == e->tiling && e->g_gen == e->gen) return NEMO2D_OK;
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_step(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : set_error(NEMO2D_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    }
+    e->graph = g;
+    EHIP(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    e->g_tiling = e->tiling;
+    e->g_gen = e->gen;
+    return NEMO2D_OK;
+}
+
+// Tide values of every member for steps nit000..nitend (host C library),
+// in a table whose row length only grows (the captured step names it).
 int upload_tides(nemo2d_ensemble* e, int nit000, int nsteps) {
-    std::vector<double> tab((size_t)e->nmem * nsteps);
+    if (nsteps > e->tide_stride) {
+        EHIP(hipStreamSynchronize(e->stream));
+        if (e->tide_tab) (void)hipFree(e->tide_tab);
+        e->tide_tab = nullptr;
+        EHIP(hipMalloc((void**)&e->tide_tab, (size_t)e->nmem * nsteps * sizeof(double)));
+        e->tide_stride = nsteps;
+        ++e->gen;
+    }
+    const int ts = e->tide_stride;
+    std::vector<double> tab((size_t)e->nmem * ts, 0.0);
     for (int k = 0; k < e->nmem; ++k)
         for (int s = 0; s < nsteps; ++s)
-            tab[(size_t)k * nsteps + s] = tide_eval(e->amp[k], e->omega[k], (double)(nit000 + s) * e->prm.rdt);
-    if (e->tide_tab) (void)hipFree(e->tide_tab);
-    e->tide_tab = nullptr;
-    EHIP(hipMalloc((void**)&e->tide_tab, tab.size() * sizeof(double)));
+            tab[(size_t)k * ts + s] = tide_eval(e->amp[k], e->omega[k], (double)(nit000 + s) * e->prm.rdt);
     EHIP(hipMemcpyAsync(e->tide_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
                         e->stream));
     EHIP(hipStreamSynchronize(e->stream));
-    e->tide_stride = nsteps;
     return NEMO2D_OK;
 }
 
 int ensure_log(nemo2d_ensemble* e, int nsteps) {
     if (nsteps > e->log_cap) {
+        EHIP(hipStreamSynchronize(e->stream));
         if (e->log) (void)hipFree(e->log);
         e->log = nullptr;
         EHIP(hipMalloc((void**)&e->log, 3 * (size_t)e->nmem * nsteps * sizeof(double)));
         e->log_cap = nsteps;
+        ++e->gen;
     }
     EHIP(hipMemsetAsync(e->log, 0, 3 * (size_t)e->nmem * e->log_cap * sizeof(double), e->stream));
     e->log_n = 0;
@@ -581,7 +659,8 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         e->omega.push_back(tide_omega_of(period_h[k]));
     }
     e->total = off;
-    if (hipMalloc((void**)&e->dmem, nmem * sizeof(EnsMember)) != hipSuccess)
+    if (hipMalloc((void**)&e->dmem, nmem * sizeof(EnsMember)) != hipSuccess ||
+        hipMalloc((void**)&e->ctl, sizeof(EnsCtl)) != hipSuccess)
         return bail(set_error(NEMO2D_ERR_HIP, "nemo2d_ensemble_create: allocation failed"));
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < 5; ++f) {
@@ -657,38 +736,28 @@ int nemo2d_ensemble_run(nemo2d_ensemble* e, int nit000, int nitend, int use_grap
     const int nsteps = nitend - nit000 + 1;
     int rc;
     if ((rc = upload_tides(e, nit000, nsteps)) || (rc = ensure_log(e, nsteps))) return rc;
-    EHIP(hipDeviceSynchronize());
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    if (use_graph) {
-        const int cur0 = e->cur;
-        const bool stale0 = e->uv_stale;
-        EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < nsteps && !rc; ++s) rc = launch_step(e, s, s);
-        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
-        e->cur = cur0;  // capture did not run anything
-        // the replay leaves the interpolations as the capture's last step did
-        if (rc || ce != hipSuccess) e->uv_stale = stale0;
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        if (ce != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-        EHIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    if (e->uv_stale && e->tiling != MARCH_RC_TILING) {  // this tiling reads sshn_u/sshn_v
+        launch_interp(e);
+        EHIP(hipGetLastError());
+        e->uv_stale = false;
     }
+    e->h_ctl = EnsCtl{0, e->cur};
+    EHIP(hipMemcpyAsync(e->ctl, &e->h_ctl, sizeof(EnsCtl), hipMemcpyHostToDevice, e->stream));
+    if (use_graph && (rc = ensure_graph(e))) return rc;
     EHIP(hipEventRecord(e->ev0, e->stream));
-    if (use_graph) {
-        hipError_t le = hipGraphLaunch(exec, e->stream);
-        if (le == hipSuccess && (nsteps & 1)) e->cur = 1 - e->cur;
-        if (le != hipSuccess) rc = set_error(NEMO2D_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(le));
-    } else {
-        for (int s = 0; s < nsteps && !rc; ++s) rc = launch_step(e, s, s);
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        if (use_graph) {
+            const hipError_t le = hipGraphLaunch(e->gexec, e->stream);
+            if (le != hipSuccess) rc = set_error(NEMO2D_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(le));
+        } else {
+            rc = launch_step(e);
+        }
     }
     EHIP(hipEventRecord(e->ev1, e->stream));
     EHIP(hipEventSynchronize(e->ev1));
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (rc) return rc;
+    if (nsteps & 1) e->cur = 1 - e->cur;
+    if (e->tiling == MARCH_RC_TILING) e->uv_stale = true;
     float t = 0.f;
     EHIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
     if (ms) *ms = (double)t;
@@ -763,7 +832,9 @@ int nemo2d_ensemble_destroy(nemo2d_ensemble* e) {
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < 5; ++f)
             if (e->X[k][f]) (void)hipFree(e->X[k][f]);
-    void* ptrs[] = {e->dmem, e->fl, e->flu, e->flv, e->tiles, e->partials, e->tide_tab, e->log};
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    void* ptrs[] = {e->dmem, e->fl, e->flu, e->flv, e->tiles, e->partials, e->tide_tab, e->log, e->ctl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (e->ev0) (void)hipEventDestroy(e->ev0);

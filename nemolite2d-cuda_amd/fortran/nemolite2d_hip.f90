@@ -31,7 +31,7 @@ program nemolite2d_hip
     integer :: jpiglo, jpjglo, jpi, jpj, jphgr_msh, nit000, nitend, irecord
     real(wp) :: dx, dy, dep_const, rdt, cbfr, visc
     logical :: fused, write_output, async_output, stage_fusion
-    character(len=256) :: csfile
+    character(len=256) :: csfile, grid_file
 
     type(c_ptr) :: model = c_null_ptr
     integer :: istp, nsteps, k
@@ -128,7 +128,7 @@ contains
         integer :: ios
         namelist /namctl/ jpiglo, jpjglo, jphgr_msh, dx, dy, dep_const, &
             nit000, nitend, irecord, rdt, cbfr, visc
-        namelist /namhip/ fused, write_output, async_output, stage_fusion, csfile
+        namelist /namhip/ fused, write_output, async_output, stage_fusion, csfile, grid_file
 
         jpiglo = 258; jpjglo = 258; jphgr_msh = 1
         dx = 1000._wp; dy = 1000._wp; dep_const = 100._wp
@@ -136,6 +136,7 @@ contains
         rdt = 10._wp; cbfr = 0.001_wp; visc = 100._wp
         fused = .false.; write_output = .true.; async_output = .false.; stage_fusion = .true.
         csfile = 'nemo2d_checksums.txt'
+        grid_file = 'grid.txt'
 
         open (1, file='namelist', status='OLD')
         rewind (1)
@@ -145,8 +146,9 @@ contains
         read (1, nml=namhip, iostat=ios)   ! optional group
         close (1)
         write (*, nml=namctl)
-        if (jphgr_msh == 0) stop "It is not ready to Read in grid from a file"
-        if (jphgr_msh /= 1) stop "Wrong grid defination type, check your setup !!!!"
+        ! jphgr_msh = 0 reads the grid from grid_file (the serial model stops
+        ! there, original/nemolite2d.f90:155-158); 1 builds the uniform basin
+        if (jphgr_msh /= 0 .and. jphgr_msh /= 1) stop "Wrong grid defination type, check your setup !!!!"
         jpiglo = jpiglo - 2
         jpjglo = jpjglo - 2
         jpi = jpiglo
@@ -165,31 +167,54 @@ contains
         allocate (sshn(jpi, jpj), sshn_u(0:jpi, jpj), sshn_v(jpi, 0:jpj))
         allocate (un(0:jpi, jpj), vn(jpi, 0:jpj), ua(0:jpi, jpj), va(jpi, 0:jpj))
 
-        pt = 1
-        pt(0, :) = 0
-        pt(jpi + 1, :) = 0
-        pt(:, jpj + 1) = 0
-        pt(1:jpi, 0) = -1
-        pt(0, 0) = 0
-        pt(jpi + 1, 0) = 0
+        if (jphgr_msh == 0) then
+            call read_grid_file
+        else
+            pt = 1
+            pt(0, :) = 0
+            pt(jpi + 1, :) = 0
+            pt(:, jpj + 1) = 0
+            pt(1:jpi, 0) = -1
+            pt(0, 0) = 0
+            pt(jpi + 1, 0) = 0
 
-        e1t = dx; e2t = dy; e1u = dx; e2u = dy; e1v = dx; e2v = dy
-        gphiu = 50._wp; gphiv = 50._wp
-        ht = dep_const; hu = dep_const; hv = dep_const
+            e1t = dx; e2t = dy; e1u = dx; e2u = dy; e1v = dx; e2v = dy
+            gphiu = 50._wp; gphiv = 50._wp
+            ht = dep_const; hu = dep_const; hv = dep_const
+        end if
+        ! T-point coordinates: cell centres, each half a cell width on from
+        ! the last (for uniform widths, the serial model's xt(ji-1) + dx)
         do jj = 1, jpj
             xt(1, jj) = 0.0_wp + 0.5_wp*e1t(1, jj)
             do ji = 2, jpi
-                xt(ji, jj) = xt(ji - 1, jj) + dx
+                xt(ji, jj) = xt(ji - 1, jj) + 0.5_wp*(e1t(ji - 1, jj) + e1t(ji, jj))
             end do
         end do
         yt(:, 1) = 0.0_wp + 0.5_wp*e2t(:, 1)
         do jj = 2, jpj
-            yt(:, jj) = yt(:, jj - 1) + dy
+            yt(:, jj) = yt(:, jj - 1) + 0.5_wp*(e2t(:, jj - 1) + e2t(:, jj))
         end do
         e12t = e1t*e2t
         e12u = e1u*e2u
         e12v = e1v*e2v
     end subroutine grid
+
+    !> Grid from a text file (jphgr_msh = 0): list-directed, the interior
+    !! sizes jpi jpj, then pt(0:jpi+1,0:jpj+1), e1t, e2t (1:jpi,1:jpj), e1u,
+    !! e2u (0:jpi,1:jpj), e1v, e2v (1:jpi,0:jpj), gphiu (U), gphiv (V), ht (T),
+    !! hu (U), hv (V), each in storage order (first index fastest).  The areas
+    !! e12* are formed as the products, as for the built-in grid.
+    subroutine read_grid_file
+        integer :: ios, fi, fj
+        open (3, file=trim(grid_file), status='OLD', action='READ', iostat=ios)
+        if (ios /= 0) stop "grid file (grid_file in &namhip) not found"
+        read (3, *, iostat=ios) fi, fj
+        if (ios /= 0) stop "grid file: cannot read its sizes"
+        if (fi /= jpi .or. fj /= jpj) stop "grid file: sizes differ from jpiglo-2, jpjglo-2"
+        read (3, *, iostat=ios) pt, e1t, e2t, e1u, e2u, e1v, e2v, gphiu, gphiv, ht, hu, hv
+        if (ios /= 0) stop "grid file: too few or unreadable values"
+        close (3)
+    end subroutine read_grid_file
 
     !> Rest state (original/nemolite2d.f90:324-371).
     subroutine initialisation

@@ -41,7 +41,7 @@ SYMBOLS = (
     "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
     "nemo2d_set_stage_fusion", "nemo2d_retrieve_grid", "nemo2d_retrieve_classes",
     "nemo2d_classify", "nemo2d_nn_conv_bwd_data_gelu", "nemo2d_closure_restrict",
-    "nemo2d_closure_target",
+    "nemo2d_closure_target", "nemo2d_dev_synchronize",
 )
 
 # nemo2d_set_variant choices
@@ -172,22 +172,42 @@ def default_grid(jpi, jpj, dx=1000.0, dy=1000.0, dep_const=100.0):
     return g
 
 
+GRID_FILE_ARRAYS = ("e1t", "e2t", "e1u", "e2u", "e1v", "e2v", "gphiu", "gphiv", "ht", "hu", "hv")
+
+
+def write_grid_file(path, grid):
+    """The grid as the Fortran driver reads it with jphgr_msh = 0: the
+    interior sizes, then pt and the metrics, latitudes and depths, each in
+    Fortran storage order (first index fastest), one value a line, 17
+    significant digits (a double read back is the same double)."""
+    pt = np.asarray(grid["pt"])
+    jpj, jpi = pt.shape[0] - 2, pt.shape[1] - 2
+    with open(path, "w") as f:
+        f.write("%d %d\n" % (jpi, jpj))
+        f.write("\n".join("%d" % v for v in pt.ravel()) + "\n")
+        for k in GRID_FILE_ARRAYS:
+            f.write("\n".join("%.17e" % v for v in np.asarray(grid[k], dtype=np.float64).ravel()) + "\n")
+
+
 def initial_state(grid, sshn=None):
     """Initial now-state of original/nemolite2d.f90:324-371 (sshn given or 0;
     sshn_u, sshn_v interpolated with clamped indices; un = vn = 0)."""
     jpj, jpi = grid["e1t"].shape
     s = shapes(jpi, jpj)
-    sshn = np.zeros(s["T"]) if sshn is None else np.ascontiguousarray(sshn, dtype=np.float64)
+    if sshn is None:  # at rest every interpolation is 0
+        return {"sshn": np.zeros(s["T"]), "sshn_u": np.zeros(s["U"]), "sshn_v": np.zeros(s["V"]),
+                "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
+    sshn = np.ascontiguousarray(sshn, dtype=np.float64)
     e12t = grid["e12t"]
-    sshn_u = np.zeros(s["U"])
-    for ji in range(0, jpi + 1):
-        i1, i2 = min(ji + 1, jpi) - 1, max(ji, 1) - 1
-        sshn_u[:, ji] = 0.5 * (e12t[:, i1] * sshn[:, i1] + e12t[:, i2] * sshn[:, i2]) / grid["e12u"][:, ji]
-    sshn_v = np.zeros(s["V"])
-    for jj in range(0, jpj + 1):
-        j1, j2 = min(jj + 1, jpj) - 1, max(jj, 1) - 1
-        sshn_v[jj, :] = 0.5 * (e12t[j1, :] * sshn[j1, :] + e12t[j2, :] * sshn[j2, :]) / grid["e12v"][jj, :]
-    return {"sshn": sshn, "sshn_u": sshn_u, "sshn_v": sshn_v,
+    es = e12t * sshn
+    # column ji takes T columns min(ji+1, jpi) and max(ji, 1) (1-based)
+    i1 = np.minimum(np.arange(jpi + 1) + 1, jpi) - 1
+    i2 = np.maximum(np.arange(jpi + 1), 1) - 1
+    sshn_u = 0.5 * (es[:, i1] + es[:, i2]) / grid["e12u"]
+    j1 = np.minimum(np.arange(jpj + 1) + 1, jpj) - 1
+    j2 = np.maximum(np.arange(jpj + 1), 1) - 1
+    sshn_v = 0.5 * (es[j1, :] + es[j2, :]) / grid["e12v"]
+    return {"sshn": sshn, "sshn_u": np.ascontiguousarray(sshn_u), "sshn_v": np.ascontiguousarray(sshn_v),
             "un": np.zeros(s["U"]), "vn": np.zeros(s["V"])}
 
 
@@ -492,7 +512,9 @@ def seamount_grid(jpi, jpj, seed, n_mounts=32, dx=1000.0, dy=1000.0, dep_const=1
             continue
         x = (np.arange(i0, i1) + 0.5) * dx - x0
         y = (np.arange(j0, j1) + 0.5) * dy - y0
-        bump[j0:j1, i0:i1] += h * np.exp(-(y[:, None] ** 2 + x[None, :] ** 2) / (r * r))
+        # exp(-(x^2 + y^2)/r^2) as the product of its two factors (one
+        # exponential per row and column instead of per cell)
+        bump[j0:j1, i0:i1] += np.outer(h * np.exp(-y * y / (r * r)), np.exp(-x * x / (r * r)))
     ht = np.maximum(dep_const - bump, min_depth)
     hu = np.empty((jpj, jpi + 1))
     hu[:, 1:jpi] = 0.5 * (ht[:, :-1] + ht[:, 1:])

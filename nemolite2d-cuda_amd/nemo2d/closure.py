@@ -90,6 +90,7 @@ def _setup_nn(L):
     L.nemo2d_nn_conv_bwd_data_gelu.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, vp]
     L.nemo2d_nn_conv_bwd_weight.argtypes = [vp, vp, i, i, i, i, i, fp, fp, fp, vp]
     L.nemo2d_nn_wgrad_work_floats.argtypes = [l]
+    L.nemo2d_dev_synchronize.argtypes = []
     L.nemo2d_nn_wgrad_work_floats.restype = l
     L.nemo2d_nn_gelu_bwd.argtypes = [fp, fp, l, vp, vp]
     L.nemo2d_nn_mse.argtypes = [fp, fp, l, i, fp, vp, fp, i, vp]
@@ -137,6 +138,7 @@ class ConvNet:
         self.step_count = 0
         self.pack()
         self._shape = None
+        self._sets = {}
 
     def pack(self):
         for k, (cin, cout, _) in enumerate(self.layers):
@@ -166,7 +168,15 @@ class ConvNet:
         return out
 
     def _buffers(self, n, h, w):
+        """Activation and gradient buffers for batches of n images of h x w
+        (kept per shape, so that alternating shapes do not reallocate)."""
         if self._shape == (n, h, w):
+            return
+        if self._shape is not None:
+            self._sets[self._shape] = self._bufset()
+        if (n, h, w) in self._sets:
+            self._use(self._sets.pop((n, h, w)))
+            self._shape = (n, h, w)
             return
         P = n * h * w
         nl = len(self.layers)
@@ -174,12 +184,22 @@ class ConvNet:
                   for k, (_, co, _) in enumerate(self.layers)]
         self.act = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
         self.dz = [DevArray((P, pad8(co)), np.int16) for _, co, _ in self.layers]
-        self.gw = [DevArray((9, ci, co), np.float32) for ci, co, _ in self.layers]
-        self.gb = [DevArray((co,), np.float32) for _, co, _ in self.layers]
-        self.work = DevArray((int(self.L.nemo2d_nn_wgrad_work_floats(P)),), np.float32)
-        self.nparts = 512
-        self.part = DevArray((self.nparts,), np.float32)
+        if not hasattr(self, "gw"):  # gradients and reduction space: one set for all shapes
+            self.gw = [DevArray((9, ci, co), np.float32) for ci, co, _ in self.layers]
+            self.gb = [DevArray((co,), np.float32) for _, co, _ in self.layers]
+            self.work = DevArray((int(self.L.nemo2d_nn_wgrad_work_floats(P)),), np.float32)
+            self.nparts = 512
+            self.part = DevArray((self.nparts,), np.float32)
         self._shape = (n, h, w)
+
+    _BUF_KEYS = ("z", "act", "dz")
+
+    def _bufset(self):
+        return {k: getattr(self, k) for k in self._BUF_KEYS}
+
+    def _use(self, d):
+        for k, v in d.items():
+            setattr(self, k, v)
 
     def forward(self, x_bf16, n, h, w):
         """x_bf16: device [n*h*w, 8] BF16 bits (channels 0..3 used).  Returns
@@ -234,6 +254,36 @@ class ConvNet:
                 _check(self.L.nemo2d_nn_adam(p.fptr(), g.fptr(), m.fptr(), v.fptr(), p.size, lr, b1,
                                              b2, eps, self.step_count, self.STREAM))
         self.pack()
+
+    def time_kernels(self, x_bf16, n, h, w, reps=20, layer=1):
+        """Seconds per launch of the convolution kernels of one 32 -> 32
+        layer (forward, data gradient with GELU backward, weight gradient
+        with the partial sum) on a batch of n images h x w, each launched reps
+        times back to back after a forward/backward pass (host clock around
+        the launches, device synchronised on both sides)."""
+        import time
+        self.forward(x_bf16, n, h, w)
+        cin, cout, _ = self.layers[layer]
+        k = layer
+        fw = lambda: self.L.nemo2d_nn_conv_fwd(self.act[k - 1].ptr, n, h, w, cin, self.wf[k].ptr,  # noqa: E731
+                                               self.b[k].fptr(), cout, None, self.z[k].ptr,
+                                               self.act[k].ptr, 1, self.STREAM)
+        bd = lambda: self.L.nemo2d_nn_conv_bwd_data_gelu(self.dz[k].ptr, n, h, w, cout,  # noqa: E731
+                                                         self.wb[k].ptr, cin, self.z[k - 1].ptr, 1,
+                                                         self.dz[k - 1].ptr, self.STREAM)
+        bw = lambda: self.L.nemo2d_nn_conv_bwd_weight(self.act[k - 1].ptr, self.dz[k].ptr, n, h, w,  # noqa: E731
+                                                      cin, cout, self.gw[k].fptr(), self.gb[k].fptr(),
+                                                      self.work.fptr(), self.STREAM)
+        out = {}
+        for name, f in (("conv_fwd", fw), ("conv_bwd_data_gelu", bd), ("conv_bwd_weight", bw)):
+            _check(f())
+            _check(self.L.nemo2d_dev_synchronize())
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _check(f())
+            _check(self.L.nemo2d_dev_synchronize())
+            out[name] = (time.perf_counter() - t0) / reps
+        return out
 
     def grads(self):
         return ([g.download().astype(np.float64) for g in self.gw],

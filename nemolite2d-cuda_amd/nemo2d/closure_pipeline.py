@@ -170,6 +170,9 @@ class Closure:
         from .closure import ConvNet
         self.net = ConvNet(seed=seed)
         self.xm, self.xs, self.ys = np.asarray(xm), np.asarray(xs), np.asarray(ys)
+        # factor on each predicted increment (ssh, u, v) when a model is
+        # corrected: 1 applies the network's increment as trained
+        self.gain = np.ones(3)
 
     @classmethod
     def from_stats(cls, xm, xs, ys, seed=0):
@@ -230,7 +233,7 @@ class Closure:
             self._feat = DevArray((P, 8), np.int16)
         model.closure_features(self.xm, self.xs, self._feat)
         z = self.net.forward(self._feat, 1, model.jpj, model.jpi)
-        model.closure_apply(z, self.ys)
+        model.closure_apply(z, self.ys * self.gain)
 
     def correct(self, state, grid):
         d = self.predict(features(state, grid["ht"])[None])[0]
@@ -276,9 +279,10 @@ def evaluate(setup, closure, n_windows, tol=2.0, judge_every=5):
 
 # ---- the closure at scale: pairs made and used on the device ------------------------
 class Trio:
-    """One training basin: the fine model, a coarse model holding the
-    restricted fine state (the truth) and a coarse model stepping from the
-    previous window's restricted state (the prediction)."""
+    """One training basin at a time: the fine model, a coarse model holding
+    the restricted fine state (the truth) and a coarse model stepping from an
+    earlier restricted state (the prediction).  The models are kept and
+    given the next basin's grids (load), so device memory is allocated once."""
 
     def __init__(self, setup):
         self.setup = setup
@@ -286,6 +290,15 @@ class Trio:
         self.truth = setup.coarse_model()
         self.pred = setup.coarse_model()
         self.w = 0  # windows of K coarse steps done
+
+    def load(self, setup):
+        """The next basin (same sizes): its grids, the fine model at rest."""
+        self.setup = setup
+        self.fine.set_grid(setup.fine_grid)
+        self.fine.set_state(initial_state(setup.fine_grid))
+        self.truth.set_grid(setup.coarse_grid)
+        self.pred.set_grid(setup.coarse_grid)
+        self.w = 0
 
     def advance(self):
         """Fine model on by one window; truth = R(fine)."""
@@ -297,6 +310,9 @@ class Trio:
     def restart_pred(self):
         """Prediction: from R(fine) now, K coarse steps on."""
         self.pred.restrict_from(self.fine)
+        self.step_pred()
+
+    def step_pred(self):
         t0 = K_STEPS * self.w
         self.pred.run(t0 + 1, t0 + K_STEPS)
 
@@ -330,14 +346,20 @@ def pilot_stats(setup, windows=(40, 120, 200)):
 
 
 def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, iters_per_batch=1,
-                 stats=None, log=None, net_seed=0, lr_min_frac=0.05):
+                 stats=None, log=None, net_seed=0, lr_min_frac=0.05, rollout=1):
     """Closure trained on pairs made on the device: `group` basins advance
     side by side, one window of K coarse steps at a time, and each window's
     `group` pairs (one per basin, written straight into the device batch)
-    are one training batch.  len(seeds) * windows pairs in all, each used
-    for iters_per_batch Adam steps (cosine learning rate).  Returns the
-    closure and a record of what was done."""
+    are one training batch: len(seeds) * windows pairs in all, each used for
+    iters_per_batch Adam steps (cosine learning rate).  With rollout > 1 the
+    prediction model is restarted from the restricted fine state only every
+    `rollout` windows and in between carries on from its own state corrected
+    by the network as it is being trained, so that the pairs include states
+    the closure itself leads to.  The next group's basins are generated on
+    the host while the current group runs.  Returns the closure and a record
+    of what was done."""
     import time as _time
+    from concurrent.futures import ThreadPoolExecutor
     from .closure import DevArray
     bathy = bathy or {}
     seeds = list(seeds)
@@ -351,42 +373,67 @@ def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, ite
     yb = DevArray((group * P, 8), np.float32)
     wch = clos.net.upload(np.ones(3, dtype=np.float32))
     groups = [seeds[k:k + group] for k in range(0, len(seeds), group)]
-    total_iters = sum(windows * iters_per_batch for _ in groups)
+    total_iters = len(groups) * windows * iters_per_batch
     it = 0
     losses = []
     pairs = 0
     t1 = _time.perf_counter()
-    for gs in groups:
-        trios = [Trio(Setup(nc, sd, **bathy)) for sd in gs]
+    t_wait = 0.0
+    trios = []
+    pool = ThreadPoolExecutor(max_workers=group)
+
+    def make(gs):
+        return [pool.submit(Setup, nc, sd, **bathy) for sd in gs]
+
+    pending = make(groups[0])
+    for gi in range(len(groups)):
+        tw = _time.perf_counter()
+        setups = [f.result() for f in pending]
+        t_wait += _time.perf_counter() - tw
+        if gi + 1 < len(groups):
+            pending = make(groups[gi + 1])
+        while len(trios) < len(setups):
+            trios.append(Trio(setups[len(trios)]))
+        active = trios[:len(setups)]
+        for tr, su in zip(active, setups):
+            tr.load(su)
         for _ in range(spinup):
-            for tr in trios:
+            for tr in active:
                 tr.advance()
-        for tr in trios:
+        for tr in active:
             tr.restart_pred()
         for w in range(windows):
-            for tr in trios:
+            for tr in active:
                 tr.advance()
-            for k, tr in enumerate(trios):
+            for k, tr in enumerate(active):
                 tr.pred.closure_features(clos.xm, clos.xs, xb, offset=k * P * 8 * 2)
                 tr.truth.closure_target(tr.pred, clos.ys, yb, offset=k * P * 8 * 4)
-                tr.restart_pred()
-            pairs += len(trios)
+            pairs += len(active)
             for _ in range(iters_per_batch):
                 lr_it = lr * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + np.cos(np.pi * it / total_iters)))
                 record = it % 100 == 0 or it == total_iters - 1
-                loss = clos.train_batch(xb, yb, len(trios), nc, nc, wch, lr_it, want_loss=record)
+                loss = clos.train_batch(xb, yb, len(active), nc, nc, wch, lr_it, want_loss=record)
                 if record:
                     losses.append((it, loss))
                     if log:
                         log("closure online it %d/%d pairs %d loss %.4g" % (it, total_iters, pairs, loss))
                 it += 1
-        for tr in trios:
-            tr.close()
+            for tr in active:
+                if (w + 1) % rollout == 0:
+                    tr.restart_pred()
+                else:
+                    clos.correct_model(tr.pred)
+                    tr.step_pred()
+    pool.shutdown()
+    for tr in trios:
+        tr.close()
     t_train = _time.perf_counter() - t1
     xb.close()
     yb.close()
     return clos, {"pairs": pairs, "iterations": it, "batch": group, "losses": losses,
-                  "seconds": {"stats": t_stats, "generate_and_train": t_train},
+                  "rollout_windows": rollout, "iters_per_batch": iters_per_batch, "lr": lr,
+                  "seconds": {"stats": t_stats, "generate_and_train": t_train,
+                              "waiting_for_basin_setup": t_wait},
                   "basins": len(seeds), "windows_per_basin": windows, "spinup_windows": spinup}
 
 

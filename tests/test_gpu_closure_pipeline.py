@@ -93,3 +93,20 @@ def test_device_restriction_and_target_equal_host_path_bitwise():
     assert np.array_equal(got[:, :3], want) and not got[:, 3:].any()
     for m in (fm, cm, tm, ref):
         m.close()
+
+
+def test_closure_at_1024_beats_plain_coarse_and_fallback_fires():
+    """The closure at the benchmark's size: 1024^2 coarse basins, pairs from
+    2048^2 fine runs made and used on the device (16 basins x 40 windows),
+    judged on a held-out basin by the fine run: over 20 windows (200 coarse
+    steps) the coarse+closure run's ssh is closer to the fine solution than
+    the plain coarse run's; an impossible tolerance switches the closure off."""
+    bathy = dict(n_mounts=32, radius=(20e3, 200e3), height=(10.0, 60.0))
+    clos, info = cp.train_online(1024, range(16), 40, group=4, bathy=bathy, lr=1e-3, rollout=2)
+    assert info["pairs"] == 640 and all(np.isfinite(l) for _, l in info["losses"])
+    held = cp.Setup(1024, 1000, **bathy)
+    ep, ec, fell = cp.evaluate_device(held, clos, 20, tol=1.0, judge_every=1)
+    print("plain ssh rms", ep.mean(), "closure ssh rms", ec.mean(), "fell back", fell)
+    assert ec.mean() < 0.9 * ep.mean(), (ec.mean(), ep.mean())
+    _, _, fell2 = cp.evaluate_device(held, clos, 5, tol=1e-9, judge_every=1)
+    assert fell2

@@ -539,6 +539,75 @@ def test_fortran_driver(tmp_path, fused, asy, sf):
     assert (tmp_path / "go2d_00300.dat").read_bytes() == txt
 
 
+GRID_NAMELIST = """&namctl
+   jpiglo = {gi}
+   jpjglo = {gj}
+   jphgr_msh = {msh}
+   dx = {dx}
+   dy = {dy}
+   dep_const = {dep}
+   nit000 = 1
+   nitend = {n}
+   irecord = 100000
+   rdt = {rdt}
+   cbfr = {cbfr}
+   visc = {visc}
+/
+&namhip
+   fused = {fused}
+   write_output = .false.
+   csfile = 'cs.txt'
+   grid_file = 'grid.txt'
+/
+"""
+
+
+def run_driver(d, **kw):
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "nemolite2d-cuda_amd", "bin", "nemolite2d_hip")
+    d.mkdir(exist_ok=True)
+    (d / "namelist").write_text(GRID_NAMELIST.format(**kw))
+    p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return p.stdout, np.loadtxt(d / "cs.txt")
+
+
+def test_fortran_driver_grid_from_file(tmp_path):
+    """jphgr_msh = 0: the driver reads the grid from a file.  The reference
+    basin written to a file gives the built-in grid's checksums bit for bit;
+    a grid with islands, open edges on three sides and varying metrics,
+    depths and latitudes gives the oracle's checksums on that grid, through
+    the fused step and through the nine stage calls."""
+    base = dict(gi=42, gj=34, dx=1000.0, dy=1000.0, dep=100.0, n=120, rdt=20.0, cbfr=0.0015, visc=0.1,
+                fused=".true.")
+    (tmp_path / "a").mkdir()
+    nemo2d.write_grid_file(str(tmp_path / "a" / "grid.txt"), nemo2d.default_grid(40, 32))
+    out_a, cs_a = run_driver(tmp_path / "a", msh=0, **base)
+    out_b, cs_b = run_driver(tmp_path / "b", msh=1, **base)
+    assert np.array_equal(cs_a, cs_b)
+    assert [l for l in out_a.splitlines() if "checksum =" in l] == \
+        [l for l in out_b.splitlines() if "checksum =" in l]
+    o = random_grid_oracle(37, 29, 3, vary_metrics=True)
+    p = o.params
+    (tmp_path / "c").mkdir()
+    nemo2d.write_grid_file(str(tmp_path / "c" / "grid.txt"), copy_grid_from_oracle(o))
+    # the driver starts from rest on this grid
+    for k in ("sshn", "sshn_u", "sshn_v", "un", "vn"):
+        o.arr(k)[:] = 0.0
+    n = 80
+    ref, _ = o.run(1, n)
+    for fused in (".true.", ".false."):
+        out, cs = run_driver(tmp_path / "c", msh=0, gi=39, gj=31, dx=p["dx"], dy=p["dy"],
+                             dep=p["dep_const"], n=n, rdt=p["rdt"], cbfr=p["cbfr"], visc=p["visc"],
+                             fused=fused)
+        assert cs.shape == (n, 4)
+        assert np.all(rel_close(cs[:, 1:], ref, 1e-12)), fused
+        lines = [l for l in out.splitlines() if "checksum =" in l]
+        assert lines[0].split("=")[1].strip() == e16_8(ref[-1, 1])
+        assert lines[1].split("=")[1].strip() == e16_8(ref[-1, 2])
+
+
 def test_snapshots_behind_steps_equal_synchronous_state(tmp_path):
     """snapshot_begin queues the copy and the file; the steps queued after
     it do not disturb it (the buffer it copies is next written two steps

@@ -226,3 +226,19 @@ def test_sequential_sums_sensitivity():
     s2 = s.copy()
     s2[40, 50] += np.spacing(t[299, 1])
     assert seq_sums(s2, u, v)[0] != t[299, 1]
+
+
+def test_python_initial_state_equals_oracle_initialise():
+    """nemo2d.initial_state (the host side of the driver's initialisation,
+    original/nemolite2d.f90:324-371) against the oracle's, on varying areas
+    and a random ssh: bit for bit."""
+    import nemo2d
+    o = Oracle(23, 17)
+    rng = np.random.default_rng(1)
+    o.arr("e12t")[:] = 1e6 * (1 + 0.3 * rng.random(o.arr("e12t").shape))
+    o.arr("sshn")[:] = rng.standard_normal((17, 23))
+    o.initialise()
+    g = {k: np.array(o.arr(k)) for k in ("e12t", "e12u", "e12v", "e1t")}
+    st = nemo2d.initial_state(g, np.array(o.arr("sshn")))
+    for k in ("sshn_u", "sshn_v"):
+        assert np.array_equal(st[k], o.arr(k)), k
