@@ -54,6 +54,11 @@ REF_NAMELIST = """&namctl
 """
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench.py: " + msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline_reference(n, steps):
     """The reference's own serial model (the binary oracle/_ref builds from
     original/nemolite2d.f90) on one core: steps 1..steps of the same run from
@@ -188,7 +193,7 @@ def form_closure(sc, eval_windows=60):
     windows = max(10, int(round(313 * min(sc, 1.0))))
     t0 = time.perf_counter()
     clos, info = cp.train_online(nc, range(basins), windows, group=4, bathy=CLOSURE_BATHY, lr=CLOSURE_LR,
-                                 rollout=CLOSURE_ROLLOUT)
+                                 rollout=CLOSURE_ROLLOUT, log=progress)
     t_train = time.perf_counter() - t0
     held = cp.Setup(nc, 1000, **CLOSURE_BATHY)
     t0 = time.perf_counter()
@@ -290,6 +295,7 @@ def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_
     for name in ("fp32_mixed", "ensemble", "closure"):
         if name not in which:
             continue
+        progress("form %s" % name)
         if not keep_going:
             out[name] = runs[name]()
             continue
@@ -440,6 +446,7 @@ def main():
     if not args.no_forms:
         out["forms"] = run_forms(args.forms_scale, keep_going=True)
     if args.cpu_steps > 0:
+        progress("CPU baseline")
         # the reference binary when it is there and runs, else the oracle port
         out["cpu_baseline"] = (cpu_baseline_reference(n, args.cpu_steps)
                                or cpu_baseline(n, args.cpu_steps))

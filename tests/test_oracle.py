@@ -242,3 +242,17 @@ def test_python_initial_state_equals_oracle_initialise():
     st = nemo2d.initial_state(g, np.array(o.arr("sshn")))
     for k in ("sshn_u", "sshn_v"):
         assert np.array_equal(st[k], o.arr(k)), k
+
+
+def test_oracle_4096_first_steps_equal_reference_trace():
+    """The 4096^2 trace's first steps (the full 1000 are checked on the GPU
+    path, which equals the oracle bit for bit)."""
+    big = [t for t in load_traces() if t[0]["jpiglo"] == 4098]
+    assert big, "no 4096^2 trace in tests/golden"
+    nml, t = big[0]
+    assert t.shape == (1000, 4)
+    o = Oracle.from_namelist(nml)
+    for istp in (1, 2, 3):
+        o.step(istp)
+        seq = seq_sums(o.arr("sshn"), o.arr("un"), o.arr("vn"))
+        assert np.array_equal(seq, t[istp - 1, 1:]), (istp, seq, t[istp - 1, 1:])
