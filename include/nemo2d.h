@@ -296,6 +296,17 @@ int nemo2d_nn_conv_bwd_data_gelu(const void *dz, int n, int h, int w, int cout, 
 long nemo2d_nn_wgrad_work_floats(long pixels);
 int nemo2d_nn_conv_bwd_weight(const void *x, const void *dz, int n, int h, int w, int cin,
                               int cout, float *dw, float *db, float *work, void *stream);
+/* nemo2d_nn_conv_bwd_weight without a work buffer and without allocating:
+ * the block partials go to a fixed 18.9 MB device array of the library
+ * (one per GPU, held until the process ends), so the call makes no
+ * hipMalloc/hipFree and needs no workspace from the caller.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid it).  Any batch up to
+ * 2147483647 pixels (larger: NEMO2D_ERR_ARG); the results are bitwise those
+ * of nemo2d_nn_conv_bwd_weight.  Calls that overlap on one GPU share the
+ * array and race. */
+int nemo2d_nn_conv_bwd_weight_noalloc(const void *x, const void *dz, int n, int h, int w, int cin,
+                                      int cout, float *dw, float *db, void *stream);
 /* dz = dact * GELU'(z), BF16. */
 int nemo2d_nn_gelu_bwd(const float *dact, const float *z, long count, void *dz, void *stream);
 /* Weighted MSE over channels < ch (1 <= ch <= 8) of [pixels][8] FP32

@@ -17,6 +17,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 
 #include "nemo2d.h"
@@ -592,6 +593,57 @@ int nemo2d_nn_conv_bwd_weight(const void* x, const void* dz, int n, int h, int w
     if (!x || !dz || !dw || !db || !work || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_weight: bad argument");
     return bwd_weight(x, dz, n, h, w, cin, cout, dw, db, work, stream, "conv_bwd_weight");
+}
+
+}  // extern "C"
+
+namespace {
+
+// Block partials of the allocation-free weight gradient: every call on a
+// device shares that device's copy (its address comes from
+// hipGetSymbolAddress on the first call there).  Each call writes every
+// entry it later reads (no zeroing needed); calls that overlap on one device
+// race on it.
+constexpr long WG_SCRATCH_FLOATS = (long)WG_MAXBLK * WG_PART;  // 18.9 MB
+__device__ __align__(256) float scratch[WG_SCRATCH_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[MAX_DEVICES] = {};
+
+int scratch_of_current_device(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_error(NEMO2D_ERR_HIP, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(NEMO2D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = (float*)p;
+    }
+    *out = g_scratch[dev];
+    return NEMO2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nemo2d_nn_conv_bwd_weight_noalloc(const void* x, const void* dz, int n, int h, int w, int cin, int cout,
+                                      float* dw, float* db, void* stream) {
+    if (!x || !dz || !dw || !db || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_weight_noalloc: bad argument");
+    if ((long)n * h * w > 0x7fffffffL)
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_bwd_weight_noalloc: " + std::to_string((long)n * h * w) +
+                                             " pixels exceed the maximum of 2147483647");
+    static_assert((long)WG_MAXBLK * WG_PART <= WG_SCRATCH_FLOATS, "scratch holds every block's partial");
+    float* scr = nullptr;
+    const int rc = scratch_of_current_device(&scr);
+    if (rc) return rc;
+    return bwd_weight(x, dz, n, h, w, cin, cout, dw, db, scr, stream, "conv_bwd_weight_noalloc");
 }
 
 int nemo2d_nn_gelu_bwd(const float* dact, const float* z, long count, void* dz, void* stream) {

@@ -41,7 +41,7 @@ SYMBOLS = (
     "nemo2d_snapshot_begin", "nemo2d_snapshot_wait", "nemo2d_stage_timers", "nemo2d_stage_times",
     "nemo2d_set_stage_fusion", "nemo2d_retrieve_grid", "nemo2d_retrieve_classes",
     "nemo2d_classify", "nemo2d_nn_conv_bwd_data_gelu", "nemo2d_closure_restrict",
-    "nemo2d_closure_target", "nemo2d_dev_synchronize",
+    "nemo2d_closure_target", "nemo2d_dev_synchronize", "nemo2d_nn_conv_bwd_weight_noalloc",
 )
 
 # nemo2d_set_variant choices

@@ -89,6 +89,7 @@ def _setup_nn(L):
     L.nemo2d_nn_conv_bwd_data.argtypes = [vp, i, i, i, i, vp, i, fp, vp]
     L.nemo2d_nn_conv_bwd_data_gelu.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, vp]
     L.nemo2d_nn_conv_bwd_weight.argtypes = [vp, vp, i, i, i, i, i, fp, fp, fp, vp]
+    L.nemo2d_nn_conv_bwd_weight_noalloc.argtypes = [vp, vp, i, i, i, i, i, fp, fp, vp]
     L.nemo2d_nn_wgrad_work_floats.argtypes = [l]
     L.nemo2d_dev_synchronize.argtypes = []
     L.nemo2d_nn_wgrad_work_floats.restype = l
@@ -97,8 +98,8 @@ def _setup_nn(L):
     L.nemo2d_nn_adam.argtypes = [fp, fp, fp, fp, l, f, f, f, f, i, vp]
     L.nemo2d_nn_pack_weights.argtypes = [fp, i, i, vp, vp, vp]
     L.nemo2d_nn_to_bf16.argtypes = [fp, l, vp, vp]
-    for n in ("conv_fwd", "conv_bwd_data", "conv_bwd_data_gelu", "conv_bwd_weight", "gelu_bwd",
-              "mse", "adam", "pack_weights", "to_bf16"):
+    for n in ("conv_fwd", "conv_bwd_data", "conv_bwd_data_gelu", "conv_bwd_weight",
+              "conv_bwd_weight_noalloc", "gelu_bwd", "mse", "adam", "pack_weights", "to_bf16"):
         getattr(L, "nemo2d_nn_" + n).restype = i
     L._nn_ready = True
 
