@@ -59,6 +59,26 @@ def progress(msg):
     print("bench.py: " + msg, file=sys.stderr, flush=True)
 
 
+def host_facts():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, the CPUs
+    this process may use, and the floating-point mode of the timed code."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "fp_mode": "IEEE double, no fast-math: subnormals kept (no FTZ/DAZ), no FMA contraction"}
+
+
 def cpu_baseline_reference(n, steps):
     """The reference's own serial model (the binary oracle/_ref builds from
     original/nemolite2d.f90) on one core: steps 1..steps of the same run from
@@ -450,6 +470,7 @@ def main():
         # the reference binary when it is there and runs, else the oracle port
         out["cpu_baseline"] = (cpu_baseline_reference(n, args.cpu_steps)
                                or cpu_baseline(n, args.cpu_steps))
+        out["cpu_baseline"]["host"] = host_facts()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
