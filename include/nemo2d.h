@@ -121,40 +121,32 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default (by grid size: 6 up to 300,000 T cells with uniform
- * statics and Flather points on row 0 only (1 with others), 1 up to 100,000
- * with per-point statics, 13 up to 2,000,000 with uniform
- * statics, then 26 with uniform statics, 27 with uniform metrics and
- * per-point depths where it applies, else 1),
- * 1 one thread per cell, 2/3/4 column-marching with 16/32/64-row segments,
- * 5 marching and 6 per-cell with Flather points recomputed inline instead
- * of in the epilogue, 7/8 marching (16/32 rows) dividing by precomputed
- * reciprocals with two exact FMA corrections, 9/10 marching (16/32 rows)
- * with the next row's loads issued one iteration ahead, 11/12 marching
- * (16/32 rows) with face terms shared between neighbouring points and the
- * next row's loads issued ahead, 13/14/15/16 shared face terms at
- * 12/16/24/32 rows, 17 shared face terms at 12 rows reading per-point
- * depths, 18 shared face terms at 12 rows with each lane loading only its
- * own column (neighbours by lane shifts, next row's loads ahead), 19 the
- * same dividing by precomputed reciprocals with two exact FMA corrections,
- * 20/21 as 18/19 with the block's waves kept in step row by row, 22 the
- * same reading sshn, un and vn only (sshn_u and sshn_v recomputed from sshn
- * as next forms them; used once a step has made them so and only with
- * Flather points on row 0 alone, else 21 runs), 23 as 22 with loads two rows
- * ahead and no step-keeping, 24 as 23 dividing by reciprocals under one
- * guard per row (the row is done again with IEEE divisions when any
- * numerator is outside the exact range), 25/26 as 23/24 running class-free
- * code on the tiles whose points are all plain interior points, 27 as 17
- * reading sshn, un, vn and the depths only (sshn_u/sshn_v recomputed, plain
- * tiles class-free; 17 runs where 27 cannot, as for 22).
- * Marching needs uniform metrics, 2-16 and 18-26 also uniform depths, and
- * 17 and 27 Flather points only on row 0 with the inner point north; other
- * grids use 1 (or 6).  The single-precision form (above 300,000 T cells
- * by default, or with any variant but 1) marches
- * where it applies (reading sshn, un, vn only, as 27, once a step has made
- * sshn_u/sshn_v sshn's interpolations; 17 chooses its five-array kernel)
- * unless 1 is chosen.  No counterpart in the reference: a tuning hook. */
+ * choice): 0 default, by grid size (uniform statics: 2 up to 300,000 T cells
+ * when the Flather points are all on row 0, else 1; 4 up to 2,000,000; then
+ * 10.  Per-point depths with uniform metrics: 1 up to 100,000, then 11.
+ * Per-point metrics: 1).  1 one thread per cell (Flather points by an
+ * epilogue), 2 the same with Flather points recomputed inline, 3 five-array
+ * column-marching with 16-row segments (the kernel for arrays beyond 4 GiB,
+ * where the others' 32-bit offsets do not reach), 4 column-marching with
+ * face terms shared between neighbouring points (12-row segments), 5 the
+ * same reading per-point depths, 6 the same with each lane loading only its
+ * own column (neighbours by DPP lane shifts), 7 as 6 dividing by
+ * precomputed reciprocals with two exact FMA corrections, 8 and 9 as 6 and
+ * 7 with the block's waves kept in step row by row, 10 reading sshn, un and
+ * vn only (sshn_u/sshn_v recomputed from sshn as next forms them), loads two
+ * rows ahead, reciprocal divisions under one guard per row (the row is done
+ * again with IEEE divisions when any numerator is out of range), class-free
+ * code on tiles of plain interior points, 11 as 5 reading sshn, un, vn and
+ * the depths only.  Marching (3..11) needs uniform metrics, 3, 4 and 6..10
+ * also uniform depths, 5 and 11 Flather points only on row 0 with the inner
+ * point north; 10 and 11 need sshn_u/sshn_v to be sshn's interpolations
+ * (true after any step; the first step after nemo2d_set_state runs 9 or 5).
+ * Other grids use 1 (or 2).  The single-precision form marches where it
+ * applies unless 1 is chosen.  No counterpart in the reference: a tuning
+ * hook.  nemo2d_last_variant gives the kernel the last nemo2d_step (or fused
+ * nemo2d_next) ran. */
 int nemo2d_set_variant(nemo2d_model *m, int variant);
+int nemo2d_last_variant(nemo2d_model *m, int *variant);
 
 /* Steps nit000..nitend back to back (nemo2d_step each), recording the
  * per-step checksums on the device when checksum recording is on. */
@@ -258,9 +250,12 @@ int nemo2d_ensemble_set_tiling(nemo2d_ensemble *e, int choice);
  * un, vn only (sshn_u/sshn_v formed from sshn as next forms them, and filled
  * in before a step of another choice), class-free on plain work items. */
 int nemo2d_ensemble_tiling(int choice, int *tx, int *ty);
-/* Steps nit000..nitend; use_graph != 0 captures the whole loop into a HIP
- * graph first (capture not timed) and replays it.  *ms = device time of the
- * steps.  Per-member, per-step checksums are logged. */
+/* Steps nit000..nitend; use_graph != 0 replays one captured step (a HIP
+ * graph of the step kernel, the checksum finish and the advance of a device
+ * control word holding the step index and the now-state buffer) once per
+ * step; the graph is captured on first use and kept until the tiling or a
+ * buffer it names changes.  *ms = device time of the steps.  Per-member,
+ * per-step checksums are logged. */
 int nemo2d_ensemble_run(nemo2d_ensemble *e, int nit000, int nitend, int use_graph, double *ms);
 int nemo2d_ensemble_checksum_log(nemo2d_ensemble *e, int member, double *out, int max_steps,
                                  int *nsteps);

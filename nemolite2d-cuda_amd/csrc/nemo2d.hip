@@ -134,6 +134,7 @@ struct nemo2d_model {
     // float depths, scalar float metrics
     bool mixed = false, compensated = true;
     int variant = 0;              // fused-step kernel variant (0 = default)
+    int last_var = 0;             // the variant the last fused step ran (0: none yet)
     double tide_amp = 0.2;                      // original/nemolite2d.f90:629
     double tide_omega = tide_omega_of(12.42);   // :630
     void* X[2][NF] = {};          // float (or double sshn) now-state ping-pong
@@ -206,7 +207,6 @@ k_step_mixed(ScalF s, MixedState<COMP> n, MixedOut<COMP> a, const float* __restr
 constexpr int MW = 4;     // waves per block (side by side in i)
 constexpr int MOUT = 63;  // output columns per wave
 
-template <bool FAST>
 struct RegWin {
     using real = double;
     Scal s;
@@ -222,22 +222,8 @@ struct RegWin {
     NEMO2D_REGS(e2v) NEMO2D_REGS(e12t) NEMO2D_REGS(e12u) NEMO2D_REGS(e12v) NEMO2D_REGS(ht)
     NEMO2D_REGS(hu) NEMO2D_REGS(hv) NEMO2D_REGS(fcu) NEMO2D_REGS(fcv)
 #undef NEMO2D_REGS
-    template <int TAG> __device__ double qd(double x, double d) const {
-        if constexpr (FAST) return div_rcp(x, d, s.rcp[TAG]);
-        else return x / d;
-    }
+    template <int TAG> __device__ double qd(double x, double d) const { return x / d; }
 
-    // rows move down one; row 2 is then reloaded
-    __device__ __forceinline__ void shift() {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            un_[0][c] = un_[1][c]; un_[1][c] = un_[2][c];
-            vn_[0][c] = vn_[1][c]; vn_[1][c] = vn_[2][c];
-            sshn_[0][c] = sshn_[1][c]; sshn_[1][c] = sshn_[2][c];
-            su_[0][c] = su_[1][c]; su_[1][c] = su_[2][c];
-            sv_[0][c] = sv_[1][c]; sv_[1][c] = sv_[2][c];
-        }
-    }
     // prefetched next row (software pipelining: loaded one iteration ahead)
     double nun_[3], nvn_[3], nsshn_[3], nsu_[3], nsv_[3];
     __device__ __forceinline__ void shift_in_next() {
@@ -281,10 +267,12 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <int NJ, bool FLIN, bool FAST, bool PF = false>
+// The five-array marching step for grids whose arrays exceed the shared-term
+// kernels' 32-bit byte offsets: 16-row segments, the next row's loads issued
+// one iteration ahead, IEEE divisions, Flather points left to the epilogue.
+constexpr int M16 = 16;
 __global__ void __launch_bounds__(64 * MW)
-k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
-             const double* __restrict__ flu, const double* __restrict__ flv, Layout L, Params p,
+k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl, Layout L, Params p,
              double tide, double* __restrict__ partials, int nbx, int ntiles) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -298,38 +286,28 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         const int il = i < L.jpi + 1 ? i : L.jpi + 1;  // clamped column for loads
         const bool own = lane < MOUT && i <= L.jpi;
         const bool tcol = i >= 1 && i <= L.jpi;        // column holds T points
-        const int jb = by * NJ;
-        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        const int jb = by * M16;
+        const int je = jb + M16 - 1 < L.jpj ? jb + M16 - 1 : L.jpj;
         StUniform st{sc_};
-        RegWin<FAST> w;
+        RegWin w;
         w.s = sc_;
         w.load(1, n, L.at(il - 1, jb - 1));
         w.load(2, n, L.at(il - 1, jb));
-        unsigned f_nxt = 0;
-        if constexpr (PF) {
-            w.load_next(n, L.at(il - 1, jb + 1));
-            f_nxt = fl[L.at(il, jb)];
-        }
+        w.load_next(n, L.at(il - 1, jb + 1));
+        unsigned f_nxt = fl[L.at(il, jb)];
         double s_prev = 0.0;   // new ssh at (i, j-1)
         double sv_prev = 0.0;  // old sshn_v at (i, j-1)
         unsigned f_prev = 0;
         for (int j = jb; j <= je; ++j) {
             const long o = L.at(il, j);
-            unsigned f;
-            if constexpr (PF) {
-                // rows j-1..j+1 from the window and the prefetched row; the
-                // loads of row j+2 (and the flags of row j+1) have a whole
-                // iteration to arrive
-                w.shift_in_next();
-                f = f_nxt;
-                const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
-                w.load_next(n, L.at(il - 1, jn));
-                f_nxt = fl[L.at(il, j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1)];
-            } else {
-                w.shift();
-                w.load(2, n, L.at(il - 1, j + 1));
-                f = fl[o];
-            }
+            // rows j-1..j+1 from the window and the prefetched row; the loads
+            // of row j+2 (and the flags of row j+1) have a whole iteration to
+            // arrive
+            w.shift_in_next();
+            const unsigned f = f_nxt;
+            const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
+            w.load_next(n, L.at(il - 1, jn));
+            f_nxt = fl[L.at(il, j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1)];
             double s_c = 0.0;
             if (j >= 1 && tcol) s_c = (f & T_CLAMP) ? tide : continuity_w(w, p.rdt);
             const double s_e = lane_next(s_c);
@@ -340,17 +318,8 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                         ct_ += fabs(s_c);
                     }
                     const unsigned c = ucls(f);
-                    if (c != CLS_FLATHER || FLIN) {
-                        double u = 0.0;
-                        if (c == CLS_COMP) {
-                            u = momentum_u_w(w, f, p);
-                        } else if (c == CLS_FLATHER) {
-                            const long oi = (f & U_FL_EAST) ? o + 1 : o - 1;
-                            const unsigned fi = fl[oi];
-                            const double ui =
-                                ucls(fi) == CLS_COMP ? momentum_u_at(st, n, oi, P, fi, p) : 0.0;
-                            u = ui + flu[o] * (n.sshn_u[o] - n.sshn_u[oi]);
-                        }
+                    if (c != CLS_FLATHER) {  // Flather points: the epilogue
+                        const double u = c == CLS_COMP ? momentum_u_w(w, f, p) : 0.0;
                         a.un[o] = u;
                         if (i >= 1) cu_ += fabs(u);
                     }
@@ -358,17 +327,8 @@ k_step_march(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 }
                 if (i >= 1) {
                     const unsigned c = vcls(f);
-                    if (c != CLS_FLATHER || FLIN) {
-                        double v = 0.0;
-                        if (c == CLS_COMP) {
-                            v = momentum_v_w(w, f, p);
-                        } else if (c == CLS_FLATHER) {
-                            const long oi = (f & V_FL_NORTH) ? o + P : o - P;
-                            const unsigned fi = fl[oi];
-                            const double vi =
-                                vcls(fi) == CLS_COMP ? momentum_v_at(st, n, oi, P, fi, p) : 0.0;
-                            v = vi + flv[o] * (n.sshn_v[o] - n.sshn_v[oi]);
-                        }
+                    if (c != CLS_FLATHER) {  // Flather points: the epilogue
+                        const double v = c == CLS_COMP ? momentum_v_w(w, f, p) : 0.0;
                         a.vn[o] = v;
                         if (j >= 1) cv_ += fabs(v);
                     }
@@ -604,23 +564,16 @@ struct ShCtx {
     __device__ unsigned rowb(int j) const { return (unsigned)(8 * L.at(0, j)); }
 };
 
-// One row j: A, B, C hold rows j-1, j, j+1; with PF, D receives row j+2
-// (C was loaded one row earlier), else C is loaded here.
-template <bool PF>
+// One row j: A, B, C hold rows j-1, j, j+1; C is loaded here.
 __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A, const Row5& B,
-                                       Row5& C, Row5& D, int j) {
+                                       Row5& C, int j) {
     const Layout& L = x.L;
     const Scal& s = x.s;
     const long P = L.pitch;
     const double h = 0.5;
     const long o = L.at(x.il, j);
     const unsigned f = k.f_nxt;
-    if constexpr (PF) {
-        const int jn = j + 2 <= L.jpj + 2 ? j + 2 : L.jpj + 2;
-        load_row5(D, x.n, x.rowb(jn) + x.lb8);
-    } else {
-        load_row5(C, x.n, x.rowb(j + 1) + x.lb8);
-    }
+    load_row5(C, x.n, x.rowb(j + 1) + x.lb8);
     {
         const int jf = j + 1 <= L.jpj + 1 ? j + 1 : L.jpj + 1;
         k.f_nxt = ldb(x.fl, (x.rowb(jf) + x.lb8 + 8) >> 2);
@@ -738,7 +691,7 @@ __device__ __forceinline__ void sh_row(const ShCtx& x, ShCarry& k, const Row5& A
     k.f_prev = f;
 }
 
-template <int NJ, bool PF>
+template <int NJ>
 __global__ void __launch_bounds__(64 * MW)
 k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 const double* __restrict__ flv, Layout L, Params p, double tide,
@@ -765,33 +718,20 @@ k_step_march_sh(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         x.tcol = x.i >= 1 && x.i <= L.jpi;
         const int jb = by * NJ;
         const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
-        Row5 R0, R1, R2, R3;
+        Row5 R0, R1, R2;
         load_row5(R0, n, x.rowb(jb - 1) + x.lb8);
         load_row5(R1, n, x.rowb(jb) + x.lb8);
-        if constexpr (PF) load_row5(R2, n, x.rowb(jb + 1) + x.lb8);
         k.f_nxt = fl[L.at(x.il, jb)];
         k.f_prev = 0xffffffffu;  // no row below inside this segment
         k.s_prev = 0.0;
         k.sv_prev = 0.0;
         k.south = north_terms(sc_, R0, R1);
-        if constexpr (PF) {
-            for (int j = jb; j <= je; j += 4) {
-                sh_row<true>(x, k, R0, R1, R2, R3, j);
-                if (j + 1 > je) break;
-                sh_row<true>(x, k, R1, R2, R3, R0, j + 1);
-                if (j + 2 > je) break;
-                sh_row<true>(x, k, R2, R3, R0, R1, j + 2);
-                if (j + 3 > je) break;
-                sh_row<true>(x, k, R3, R0, R1, R2, j + 3);
-            }
-        } else {
-            for (int j = jb; j <= je; j += 3) {
-                sh_row<false>(x, k, R0, R1, R2, R3, j);
-                if (j + 1 > je) break;
-                sh_row<false>(x, k, R1, R2, R0, R3, j + 1);
-                if (j + 2 > je) break;
-                sh_row<false>(x, k, R2, R0, R1, R3, j + 2);
-            }
+        for (int j = jb; j <= je; j += 3) {
+            sh_row(x, k, R0, R1, R2, j);
+            if (j + 1 > je) break;
+            sh_row(x, k, R1, R2, R0, j + 1);
+            if (j + 2 > je) break;
+            sh_row(x, k, R2, R0, R1, j + 2);
         }
         // the last row's sshn_v needs the new ssh one row up
         if (x.own && x.i >= 1) {
@@ -1301,16 +1241,15 @@ __device__ __forceinline__ void rc_commit(const RcCtx& x, RcCarry& k, const Col5
     B.sv = o.bsv;
 }
 
-// One row j: A, B, C hold rows j-1, j, j+1; D receives row j+1+LA (the
-// rows between are in flight).  With DM 2 the row's divisions go by
-// reciprocal with one guard for the whole row; if it fails on any lane, the
-// wave does the row again with IEEE divisions.
-template <int DM, bool BAR, int LA, bool PLAIN = false>
+// One row j: A, B, C hold rows j-1, j, j+1; D receives row j+3 (row j+2
+// is in flight).  With DM 2 the row's divisions go by reciprocal with one
+// guard for the whole row; if it fails on any lane, the wave does the row
+// again with IEEE divisions.
+template <int DM, bool PLAIN = false>
 __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
                                        Col5F& C, Col5F& D, int j) {
-    if constexpr (BAR) __syncthreads();  // the block's waves start the row together
     {
-        const int jn = j + 1 + LA <= x.L.jpj + 2 ? j + 1 + LA : x.L.jpj + 2;
+        const int jn = j + 3 <= x.L.jpj + 2 ? j + 3 : x.L.jpj + 2;
         load_col3<PLAIN>(D, x.n, x.fl, x.rowb(jn) + x.lb8);
     }
     if constexpr (DM == 2) {
@@ -1326,8 +1265,8 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
     }
 }
 
-// One segment rows jb..je of a wave's strip.
-template <int DM, bool BAR, int LA, bool PLAIN>
+// One segment rows jb..je of a wave's strip, loads two rows ahead.
+template <int DM, bool PLAIN>
 __device__ __forceinline__ void rc_march(const RcCtx& x, RcCarry& k, int jb, int je) {
     const Layout& L = x.L;
     const NState& n = x.n;
@@ -1337,8 +1276,7 @@ __device__ __forceinline__ void rc_march(const RcCtx& x, RcCarry& k, int jb, int
     load_col3<PLAIN>(R0, n, fl, x.rowb(jb - 1) + x.lb8);
     load_col3<PLAIN>(R1, n, fl, x.rowb(jb) + x.lb8);
     load_col3<PLAIN>(R2, n, fl, x.rowb(jb + 1) + x.lb8);
-    if constexpr (LA == 2)
-        load_col3<PLAIN>(R3, n, fl, x.rowb(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lb8);
+    load_col3<PLAIN>(R3, n, fl, x.rowb(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lb8);
     // the segment's first interpolations and south terms, IEEE divisions
     R0.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R0.f, jb - 1), R0.ss, lane_next(R0.ss), n.sshn_u,
                             x.rowb(jb - 1) + x.lb8, nullptr);
@@ -1350,35 +1288,23 @@ __device__ __forceinline__ void rc_march(const RcCtx& x, RcCarry& k, int jb, int
     k.un_w = lane_prev(R1.un);
     k.su_w = lane_prev(R1.su);
     // slot roles rotate, nothing is copied: rows j-1, j, j+1 held, the next
-    // LA rows in flight or loading
-    if constexpr (LA == 1) {
-        for (int j = jb; j <= je; j += 4) {
-            rc_row<DM, BAR, 1, PLAIN>(x, k, R0, R1, R2, R3, j);
-            if (j + 1 > je) break;
-            rc_row<DM, BAR, 1, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
-            if (j + 2 > je) break;
-            rc_row<DM, BAR, 1, PLAIN>(x, k, R2, R3, R0, R1, j + 2);
-            if (j + 3 > je) break;
-            rc_row<DM, BAR, 1, PLAIN>(x, k, R3, R0, R1, R2, j + 3);
-        }
-    } else {
-        for (int j = jb; j <= je; j += 5) {
-            rc_row<DM, BAR, 2, PLAIN>(x, k, R0, R1, R2, R4, j);
-            if (j + 1 > je) break;
-            rc_row<DM, BAR, 2, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
-            if (j + 2 > je) break;
-            rc_row<DM, BAR, 2, PLAIN>(x, k, R2, R3, R4, R1, j + 2);
-            if (j + 3 > je) break;
-            rc_row<DM, BAR, 2, PLAIN>(x, k, R3, R4, R0, R2, j + 3);
-            if (j + 4 > je) break;
-            rc_row<DM, BAR, 2, PLAIN>(x, k, R4, R0, R1, R3, j + 4);
-        }
+    // two rows in flight or loading
+    for (int j = jb; j <= je; j += 5) {
+        rc_row<DM, PLAIN>(x, k, R0, R1, R2, R4, j);
+        if (j + 1 > je) break;
+        rc_row<DM, PLAIN>(x, k, R1, R2, R3, R0, j + 1);
+        if (j + 2 > je) break;
+        rc_row<DM, PLAIN>(x, k, R2, R3, R4, R1, j + 2);
+        if (j + 3 > je) break;
+        rc_row<DM, PLAIN>(x, k, R3, R4, R0, R2, j + 3);
+        if (j + 4 > je) break;
+        rc_row<DM, PLAIN>(x, k, R4, R0, R1, R3, j + 4);
     }
 }
 
-// With PL, a wave whose tile the table marks plain (every point it touches,
-// halos included, has the class word RC_PLAIN) runs the class-free code.
-template <int NJ, int DM, bool BAR, int LA = 1, bool PL = false>
+// A wave whose tile the table marks plain (every point it touches, halos
+// included, has the class word RC_PLAIN) runs the class-free code.
+template <int DM>
 __global__ void __launch_bounds__(64 * MW)
 k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 const double* __restrict__ flv, Layout L, Params p, double tide,
@@ -1407,13 +1333,13 @@ k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         x.tcol = x.i >= 1 && x.i <= L.jpi;
         x.ucol = x.i >= 0 && x.i <= L.jpi;
         x.vcol = x.i >= 1 && x.i <= L.jpi;
-        const int jb = by * NJ;
-        const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
+        const int jb = by * RC_NJ;
+        const int je = jb + RC_NJ - 1 < L.jpj ? jb + RC_NJ - 1 : L.jpj;
         const int nseg = ntiles / nbx;
-        if (PL && plain[(long)(bx * MW + wave) * nseg + by])
-            rc_march<DM, BAR, LA, true>(x, k, jb, je);
+        if (plain[(long)(bx * MW + wave) * nseg + by])
+            rc_march<DM, true>(x, k, jb, je);
         else
-            rc_march<DM, BAR, LA, false>(x, k, jb, je);
+            rc_march<DM, false>(x, k, jb, je);
     }
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
@@ -2695,46 +2621,32 @@ static int run_stages(nemo2d_model* m, int istp) {
 // Fused-step kernel variants (nemo2d_set_variant); 0 picks the default.
 enum {
     VAR_AUTO = 0,
-    VAR_CELL = 1,          // one thread per cell, Flather in the epilogue
-    VAR_MARCH16 = 2,       // column-marching, 16-row segments
-    VAR_MARCH32 = 3,       //                  32-row segments
-    VAR_MARCH64 = 4,       //                  64-row segments
-    VAR_MARCH32_FLIN = 5,  // 32-row segments, Flather points recomputed inline
-    VAR_CELL_FLIN = 6,     // one thread per cell, Flather inline
-    VAR_MARCH16_FD = 7,    // marching, 16-row segments, division by reciprocals
-    VAR_MARCH32_FD = 8,    //                32-row segments
-    VAR_MARCH16_PF = 9,    // marching, 16-row segments, next row prefetched
-    VAR_MARCH32_PF = 10,   //                32-row segments
-    VAR_MARCH16_SH = 11,   // shared face terms, next row prefetched, 16 rows
-    VAR_MARCH32_SH = 12,   //                                         32 rows
-    VAR_MARCH12_SHN = 13,  // shared face terms, 12/16/24/32-row segments
-    VAR_MARCH16_SHN = 14,
-    VAR_MARCH24_SHN = 15,
-    VAR_MARCH32_SHN = 16,
-    VAR_MARCH12_SHD = 17,  // shared face terms, 12 rows, per-point depths
-    VAR_MARCH12_DP = 18,    // shared face terms, own-column rows, DPP neighbours
-    VAR_MARCH12_DPF = 19,   //   the same dividing by precomputed reciprocals
-    VAR_MARCH12_DPB = 20,   // 18 and 19 with the block's waves in step per row
-    VAR_MARCH12_DPFB = 21,
-    VAR_MARCH12_RC = 22,     // reads sshn, un, vn only (sshn_u/v recomputed), in step per row
-    VAR_MARCH12_RC2N = 23,   //   loads two rows ahead, no per-row step-keeping
-    VAR_MARCH12_RCG2N = 24,  //   and reciprocal divisions under one guard per row
-    VAR_MARCH12_RC2NP = 25,  // 23 and 24 with class-free code on plain interior tiles
-    VAR_MARCH12_RCG2NP = 26,
-    VAR_MARCH12_GRD = 27,    // 17 reading sshn, un, vn (and depths) only, plain tiles
-    VAR_LAST = 27
+    VAR_CELL = 1,           // one thread per cell, Flather in the epilogue
+    VAR_CELL_FLIN = 2,      // one thread per cell, Flather inline
+    VAR_MARCH16 = 3,        // five-array column-marching, 16-row segments (the fallback
+                            // where the shared-term kernels' 32-bit offsets do not fit)
+    VAR_MARCH12_SHN = 4,    // shared face terms, 12-row segments
+    VAR_MARCH12_SHD = 5,    // shared face terms, 12 rows, per-point depths
+    VAR_MARCH12_DP = 6,     // shared face terms, own-column rows, DPP neighbours
+    VAR_MARCH12_DPF = 7,    //   the same dividing by precomputed reciprocals
+    VAR_MARCH12_DPB = 8,    // 6 and 7 with the block's waves in step per row
+    VAR_MARCH12_DPFB = 9,
+    VAR_MARCH12_RC = 10,    // reads sshn, un, vn only (sshn_u/v recomputed), loads two rows
+                            // ahead, reciprocal divisions under one guard per row, class-free
+                            // code on plain interior tiles
+    VAR_MARCH12_GRD = 11,   // 5 reading sshn, un, vn (and depths) only, plain tiles
+    VAR_LAST = 11
 };
 
-template <int NJ, bool FLIN, bool FAST = false, bool PF = false>
-static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
+static int launch_march16(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
-    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int nby = (L.jpj + 1 + M16 - 1) / M16;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march<NJ, FLIN, FAST, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
-        m->scal, n, a, m->flags, m->flu, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
+    k_step_march<<<nblocks, 64 * MW, 0, m->stream>>>(m->scal, n, a, m->flags, L, m->prm, tide,
+                                                      m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march");
     nparts = nblocks;
     return NEMO2D_OK;
@@ -2743,16 +2655,16 @@ static int launch_march(nemo2d_model* m, const NState& n, const AOut& a, double 
 // the shared-term kernels address the arrays by 32-bit byte offsets
 static bool sh_fits(const Layout& L) { return (unsigned long)L.size() * sizeof(double) < (1ul << 32); }
 
-template <int NJ, bool PF>
+template <int NJ>
 static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (!sh_fits(L)) return launch_march<16, false, false, true>(m, n, a, tide, nparts);
+    if (!sh_fits(L)) return launch_march16(m, n, a, tide, nparts);
     const int nbx = (L.jpi + 1 + MW * MOUT - 1) / (MW * MOUT);
     const int nby = (L.jpj + 1 + NJ - 1) / NJ;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_sh<NJ, PF><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march_sh<NJ><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles);
     LAUNCHCHK("step_march_sh");
     nparts = nblocks;
@@ -2762,7 +2674,7 @@ static int launch_march_sh(nemo2d_model* m, const NState& n, const AOut& a, doub
 template <int NJ, bool FD, bool BAR>
 static int launch_march_dp(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (!sh_fits(L)) return launch_march<16, false, false, true>(m, n, a, tide, nparts);
+    if (!sh_fits(L)) return launch_march16(m, n, a, tide, nparts);
     const int nbx = (L.jpi + 1 + MW * DOUT - 1) / (MW * DOUT);
     const int nby = (L.jpj + 1 + NJ - 1) / NJ;
     const int ntiles = nbx * nby;
@@ -2777,17 +2689,17 @@ static int launch_march_dp(nemo2d_model* m, const NState& n, const AOut& a, doub
     return NEMO2D_OK;
 }
 
-template <int NJ, int DM, bool BAR, int LA = 1, bool PL = false>
+template <int DM>
 static int launch_march_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (DM != 0 && !m->scal.fast_div) return launch_march_rc<NJ, 0, BAR, LA, PL>(m, n, a, tide, nparts);
-    static_assert(!PL || (NJ == RC_NJ && LA <= RC_LA), "the plain-tile table's tiling");
+    if (DM != 0 && !m->scal.fast_div) return launch_march_rc<0>(m, n, a, tide, nparts);
+    static_assert(RC_LA >= 2, "the plain-tile table covers the rows loaded ahead");
     const int nbx = (L.jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
-    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
+    const int nby = (L.jpj + 1 + RC_NJ - 1) / RC_NJ;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_rc<NJ, DM, BAR, LA, PL><<<nblocks, 64 * MW, 0, m->stream>>>(
+    k_step_march_rc<DM><<<nblocks, 64 * MW, 0, m->stream>>>(
         m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles, m->rc_plain);
     LAUNCHCHK("step_march_rc");
     nparts = nblocks;
@@ -3018,7 +2930,7 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
         else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
             var = VAR_MARCH12_SHN;
         else
-            var = m->uniform ? VAR_MARCH12_RCG2NP : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
+            var = m->uniform ? VAR_MARCH12_RC : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
     }
     const bool depth_var = var == VAR_MARCH12_SHD || var == VAR_MARCH12_GRD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
@@ -3026,10 +2938,9 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
     // the kernels that recompute sshn_u/sshn_v need them to be sshn's
     // interpolations (true after any step) and no Flather points off row 0
-    const bool rc_var = var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP;
-    if (rc_var && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
+    if (var == VAR_MARCH12_RC && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
     if (var == VAR_MARCH12_GRD && !m->uv_derived) var = VAR_MARCH12_SHD;
-    const bool rc_kernel = (var >= VAR_MARCH12_RC && var <= VAR_MARCH12_RCG2NP) || var == VAR_MARCH12_GRD;
+    const bool rc_kernel = var == VAR_MARCH12_RC || var == VAR_MARCH12_GRD;
     if (!rc_kernel && m->uv_stale[k]) {
         const int urc = uv_sync(m);
         if (urc) return urc;
@@ -3043,45 +2954,24 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
     switch (var) {
         case VAR_CELL: rc = launch_cell<false>(m, n, a, tide, nparts); break;
         case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
-        case VAR_MARCH16: rc = launch_march<16, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH64: rc = launch_march<64, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_FLIN: rc = launch_march<32, true>(m, n, a, tide, nparts); flin = true; break;
-        case VAR_MARCH16_FD:
-            rc = m->scal.fast_div ? launch_march<16, false, true>(m, n, a, tide, nparts)
-                                  : launch_march<16, false>(m, n, a, tide, nparts);
-            break;
-        case VAR_MARCH32_FD:
-            rc = m->scal.fast_div ? launch_march<32, false, true>(m, n, a, tide, nparts)
-                                  : launch_march<32, false>(m, n, a, tide, nparts);
-            break;
-        case VAR_MARCH16_PF: rc = launch_march<16, false, false, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_PF: rc = launch_march<32, false, false, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SH: rc = launch_march_sh<16, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_SH: rc = launch_march_sh<32, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_SHN: rc = launch_march_sh<12, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH16_SHN: rc = launch_march_sh<16, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH24_SHN: rc = launch_march_sh<24, false>(m, n, a, tide, nparts); break;
-        case VAR_MARCH32_SHN: rc = launch_march_sh<32, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_SHN: rc = launch_march_sh<12>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_SHD: rc = launch_march_depths(m, n, a, tide, nparts); break;
         case VAR_MARCH12_GRD: rc = launch_march_depths_rc(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DP: rc = launch_march_dp<12, false, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPF: rc = launch_march_dp<12, true, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPFB: rc = launch_march_dp<12, true, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RC: rc = launch_march_rc<12, 0, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RC2N: rc = launch_march_rc<12, 0, false, 2>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RCG2N: rc = launch_march_rc<12, 2, false, 2>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RC2NP: rc = launch_march_rc<12, 0, false, 2, true>(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_RCG2NP: rc = launch_march_rc<12, 2, false, 2, true>(m, n, a, tide, nparts); break;
-        default: rc = launch_march<32, false>(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_RC: rc = launch_march_rc<2>(m, n, a, tide, nparts); break;
+        default: rc = launch_march16(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;
+    m->last_var = var;
     m->uv_stale[1 - k] = rc_kernel;
     m->uv_derived = true;
     m->cur = 1 - k;
     m->a_current = false;
     // the shared-term kernels set the row-0 Flather v points themselves
-    const bool sh = var >= VAR_MARCH16_SH && var <= VAR_LAST && sh_fits(L) &&
+    const bool sh = var >= VAR_MARCH12_SHN && var <= VAR_LAST && sh_fits(L) &&
                     ((var != VAR_MARCH12_SHD && var != VAR_MARCH12_GRD) || depth_march_ok);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && has_fl) {
@@ -3161,6 +3051,12 @@ int nemo2d_set_stage_fusion(nemo2d_model* m, int on) {
     }
     m->lz_n = 0;
     m->lz_on = on != 0;
+    return NEMO2D_OK;
+}
+
+int nemo2d_last_variant(nemo2d_model* m, int* variant) {
+    if (!m || !variant) return fail(NEMO2D_ERR_ARG, "nemo2d_last_variant: bad argument");
+    *variant = m->last_var;
     return NEMO2D_OK;
 }
 

@@ -42,16 +42,13 @@ SYMBOLS = (
     "nemo2d_set_stage_fusion", "nemo2d_retrieve_grid", "nemo2d_retrieve_classes",
     "nemo2d_classify", "nemo2d_nn_conv_bwd_data_gelu", "nemo2d_closure_restrict",
     "nemo2d_closure_target", "nemo2d_dev_synchronize", "nemo2d_nn_conv_bwd_weight_noalloc",
+    "nemo2d_last_variant",
 )
 
 # nemo2d_set_variant choices
-VARIANTS = {"auto": 0, "cell": 1, "march16": 2, "march32": 3, "march64": 4, "march32_flin": 5,
-            "cell_flin": 6, "march16_fd": 7, "march32_fd": 8, "march16_pf": 9, "march32_pf": 10,
-            "march16_sh": 11, "march32_sh": 12, "march12_shn": 13, "march16_shn": 14,
-            "march24_shn": 15, "march32_shn": 16, "march12_shd": 17,
-            "march12_dp": 18, "march12_dpf": 19, "march12_dpb": 20, "march12_dpfb": 21,
-            "march12_rc": 22, "march12_rc2n": 23, "march12_rcg2n": 24,
-            "march12_rc2np": 25, "march12_rcg2np": 26, "march12_grd": 27}
+VARIANTS = {"auto": 0, "cell": 1, "cell_flin": 2, "march16": 3, "march12_shn": 4, "march12_shd": 5,
+            "march12_dp": 6, "march12_dpf": 7, "march12_dpb": 8, "march12_dpfb": 9, "march12_rc": 10,
+            "march12_grd": 11}
 
 
 class Nemo2dError(RuntimeError):
@@ -86,6 +83,7 @@ def lib():
         L.nemo2d_bc_ssh.argtypes = [vp, d]
         L.nemo2d_step.argtypes = [vp, i]
         L.nemo2d_set_variant.argtypes = [vp, i]
+        L.nemo2d_last_variant.argtypes = [vp, ip]
         L.nemo2d_set_stage_fusion.argtypes = [vp, i]
         L.nemo2d_set_tide.argtypes = [vp, d, d]
         L.nemo2d_ensemble_create.argtypes = [ctypes.POINTER(vp), i, ip, ip, dp, dp, dp, d, d, d, d, d, i]
@@ -342,6 +340,12 @@ class Model:
     def set_variant(self, name):
         """Kernel of the fused step (see VARIANTS); results do not depend on it."""
         _check(lib().nemo2d_set_variant(self._h, VARIANTS[name] if isinstance(name, str) else int(name)))
+
+    def last_variant(self):
+        """Name of the kernel the last fused step ran (None before any)."""
+        v = ctypes.c_int(0)
+        _check(lib().nemo2d_last_variant(self._h, ctypes.byref(v)))
+        return {n: k for k, n in VARIANTS.items()}.get(v.value) if v.value else None
 
     def run(self, nit000, nitend):
         _check(lib().nemo2d_run(self._h, int(nit000), int(nitend)))

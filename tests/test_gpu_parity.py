@@ -225,7 +225,7 @@ def test_fused_and_stages_mix():
         assert np.array_equal(r["ua"], r["un"]) or np.array_equal(r["ua"], o.arr("ua"))
 
 
-@pytest.mark.parametrize("variant", ["auto", "cell", "march16_pf"])
+@pytest.mark.parametrize("variant", ["auto", "cell", "march16"])
 def test_checksum_log_with_mixed_paths_and_restarts(variant):
     """The per-step log (fused steps leave block partials that the log read
     reduces; stage steps log at once) matches exact sums of the oracle's
@@ -271,29 +271,44 @@ def test_reference_golden_checksums(traj):
         assert e16_8_close(log[k, 2], va), (nitend, log[k, 2], va)
 
 
-def check_against_trace(nml, t, path="run"):
-    """Per-step GPU checksum log within 1e-12 relative of the reference's
-    printed 17-digit sums, at every step; the fields at the middle and last
-    step, summed in the reference's order, equal to those digits bit for bit."""
+def check_against_trace(nml, t, path="run", samples=None, rtol=1e-12):
+    """The fields at the sample steps (default: the middle and last step),
+    summed in the reference's order, equal to its printed 17 digits bit for
+    bit, and the per-step GPU checksum log within rtol of those digits at
+    every step, and within 1e-13 of the exact sums of its own fields at the
+    sample steps."""
     n0, n1 = nml["nit000"], nml["nitend"]
     n = n1 - n0 + 1
-    mid = n0 + n // 2
+    if samples is None:
+        samples = [n0 + n // 2, n1]
+    exact = {}
     with nemo2d.Model.from_namelist(nml) as m:
         m.record_checksums(n)
-        for a, b in ((n0, mid), (mid + 1, n1)):
+        a = n0
+        for b in samples:
             if path == "run":
                 m.run(a, b)
             else:
                 for istp in range(a, b + 1):
                     m.step_stages(istp)
+            a = b + 1
             r = m.retrieve()
             k = b - n0
             seq = seq_sums(r["sshn"], r["un"], r["vn"])
             if not np.array_equal(seq, t[k, 1:]):
                 raise AssertionError("step %d: gpu %r reference %r" % (b, seq, t[k, 1:]))
+            exact[k] = [float(np.sum(np.abs(x).astype(np.longdouble)))
+                        for x in (r["sshn"], r["un"][:, 1:], r["vn"][1:, :])]
+        if a <= n1:
+            m.run(a, n1) if path == "run" else [m.step_stages(istp) for istp in range(a, n1 + 1)]
         log = m.checksum_log(n)
     assert log.shape == (n, 3)
-    ok = rel_close(log, t[:, 1:], 1e-12)
+    for k, e in exact.items():
+        assert np.all(rel_close(log[k], e, 1e-13)), (n0 + k, log[k], e)
+    rel = np.abs(log - t[:, 1:]) / np.maximum(np.maximum(np.abs(log), np.abs(t[:, 1:])), 1e-300)
+    print("trace %s: largest relative difference of the log from the reference's digits %.3g at "
+          "step %d" % (trace_id((nml, t)), rel.max(), n0 + int(np.argmax(rel.max(axis=1)))))
+    ok = rel_close(log, t[:, 1:], rtol)
     if not ok.all():
         k, c = np.argwhere(~ok)[0]
         raise AssertionError("step %d column %d: gpu %r reference %r" % (n0 + k, c, log[k, c], t[k, 1 + c]))
@@ -434,10 +449,17 @@ def test_4096_first_steps_bitwise_vs_oracle():
 
 
 def test_4096_reference_trace_every_step():
-    """4096^2 x 1000 steps against the reference's own per-step sums."""
+    """4096^2 x 1000 steps against the reference's own per-step sums: the
+    fields at 10 steps from the first to the last, summed in the reference's
+    order, bit for bit; the log within 1e-11 of the reference's digits at
+    every step.  (The reference's SUM adds up to 16.7M terms one at a time
+    and rounds at each: its digits stray from the exact sums of the same
+    fields by up to 2.7e-12 relative (measured, step 992), while the log
+    matches the exact sums to 1e-13; hence 1e-11 here and 1e-12 on the
+    smaller grids, where the largest difference measured is 8.7e-14.)"""
     big = [x for x in load_traces() if x[0]["jpiglo"] == 4098]
     assert big, "no 4096^2 trace in tests/golden"
-    check_against_trace(*big[0])
+    check_against_trace(*big[0], samples=[1, 2, 7, 10, 50, 100, 250, 500, 750, 1000], rtol=1e-11)
 
 
 def test_4096_reference_checksums_and_stage_path():
@@ -652,8 +674,7 @@ def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
         assert np.all(np.isfinite(m.checksum_log(25)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2n", "march12_rc2np",
-                                     "march12_rcg2np"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
 @pytest.mark.parametrize("islands", [False, True])
 def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
     """The fused kernel that reads sshn, un and vn only (sshn_u and sshn_v
@@ -675,7 +696,7 @@ def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
         assert np.all(np.isfinite(m.checksum_log(40)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_rcg2np"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
 def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
     """A state whose sshn_u/sshn_v are not sshn's interpolations (set by the
     caller) must be stepped from the stored values: the kernel that recomputes
@@ -693,8 +714,7 @@ def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
             assert_fields_equal(m, o, "%s step %d" % (variant, istp))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rcg2n", "march12_rcg2np", "march12_dpf",
-                                     "depths"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march12_dpf", "depths"])
 def test_tiny_values_take_the_ieee_divisions(variant):
     """Numerators below 2^-960 (as in the subnormal band that trails the wave
     front of a run from rest) make the kernels that divide by reciprocals
@@ -718,6 +738,27 @@ def test_tiny_values_take_the_ieee_divisions(variant):
             m.step(istp)
             o.step(istp)
             assert_fields_equal(m, o, "%s step %d" % (variant, istp))
+
+
+def test_default_kernel_by_grid_size():
+    """The kernel the default picks, as the library reports it: one thread
+    per cell (Flather inline) on the 128^2 basin, shared face terms at
+    1024^2, and beyond 2M cells the five-array kernel for the first step
+    after set_state then the sshn-only kernel; on a seamount grid the
+    per-point-depth marching kernels."""
+    cases = [(128, 128, None, ["cell_flin", "cell_flin"]),
+             (1024, 1024, None, ["march12_shn", "march12_shn"]),
+             (1500, 1500, None, ["march12_dpfb", "march12_rc"]),
+             (1200, 1200, "seamount", ["march12_shd", "march12_grd"])]
+    for ni, nj, kind, want in cases:
+        grid = nemo2d.seamount_grid(ni, nj, seed=2)[0] if kind else None
+        with nemo2d.Model(ni, nj, grid=grid) as m:
+            assert m.last_variant() is None
+            got = []
+            for istp in (1, 2):
+                m.step(istp)
+                got.append(m.last_variant())
+            assert got == want, (ni, nj, kind, got)
 
 
 @pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
