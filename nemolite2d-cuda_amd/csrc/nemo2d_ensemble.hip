@@ -29,6 +29,7 @@
 #include "nemo2d_internal.h"
 #include "nemo2d_step.h"
 #include "nemo2d_march.h"
+#include "nemo2d_rc.h"
 
 using namespace nemo2d;
 
@@ -175,6 +176,7 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
+        if (it.x < 0) continue;  // an empty slot of the item order
         const EnsMember& M = mem[it.x];
         const long off = M.off;
         MetG<R> mt;
@@ -220,6 +222,7 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
+        if (it.x < 0) continue;  // an empty slot of the item order
         const EnsMember& M = mem[it.x];
         const long off = M.off;
         CtxGR<R, SS, false> x;
@@ -254,6 +257,63 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
         k.ct = k.cu = k.cv = 0.0;
         if (it.w & 1) rcg_march<R, SS, false, ERC_LA, true>(x, k, jb, je);
         else rcg_march<R, SS, false, ERC_LA, false>(x, k, jb, je);
+        wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)(it.w >> 1));
+    }
+}
+
+// The FP64 step of the same tiling on the single-grid path's own segment
+// march (rc_march): with DM 2 the divisions of plain items go by the
+// member's precomputed reciprocals under the per-row guard, which falls
+// back to IEEE divisions on a row where any lane's quotient could differ,
+// so every member stays bitwise the single run.
+static_assert(ROUT == GROUT && RC_NJ == EMJ && RC_LA == ERC_LA,
+              "the work items and their plain flags are made for the marching tiling");
+template <int DM>
+__global__ void __launch_bounds__(EB, 3)  // 3 waves per SIMD, as the single-grid kernel
+k_ens_march_rc64(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
+                 EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
+                 const double* __restrict__ flvp, Params p, const double* __restrict__ tide_tab,
+                 int tide_stride, double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const int cur = ctl->cur, step_idx = ctl->step;
+    const double* const* nb = (const double* const*)bufs.x[cur];
+    double* const* ab = (double* const*)bufs.x[1 - cur];
+    const NState n0{nb[0], nb[1], nb[2], nb[3], nb[4]};
+    const AOut a0{ab[0], ab[1], ab[2], ab[3], ab[4]};
+    // the wave index made wave-uniform (an SGPR): the work item, the member's
+    // record and its statics are then scalar loads held in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
+        const int4 it = items[w];
+        if (it.x < 0) continue;  // an empty slot of the item order
+        const EnsMember& M = mem[it.x];
+        const long off = M.off;
+        RcCtx x;
+        x.s = M.s;
+        x.p = p;
+        x.n = NState{n0.sshn + off, n0.sshn_u + off, n0.sshn_v + off, n0.un + off, n0.vn + off};
+        x.a = AOut{a0.sshn + off, a0.sshn_u + off, a0.sshn_v + off, a0.un + off, a0.vn + off};
+        x.fl = flp + off;
+        x.flv = flvp + off;
+        x.L = M.L;
+        x.tide = tide_tab[(long)it.x * tide_stride + step_idx];
+        const Layout& L = x.L;
+        // lane 0 is column i0 - 1 (>= -1, inside the left padding)
+        x.i = it.y * ROUT + lane - 1;
+        x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;
+        x.lb8 = (unsigned)(8 * x.il);  // wraps for -1; sums stay in range
+        x.own = lane >= 1 && lane <= ROUT && x.i <= L.jpi;
+        x.tcol = x.i >= 1 && x.i <= L.jpi;
+        x.ucol = x.i >= 0 && x.i <= L.jpi;
+        x.vcol = x.i >= 1 && x.i <= L.jpi;
+        const int jb = it.z * RC_NJ;
+        const int je = jb + RC_NJ - 1 < L.jpj ? jb + RC_NJ - 1 : L.jpj;
+        RcCarry k;
+        k.ct = k.cu = k.cv = 0.0;
+        // (items touching a boundary divide by IEEE division: their general
+        // code with the guard's bookkeeping would not fit the register budget)
+        if (it.w & 1) rc_march<DM, true>(x, k, jb, je);
+        else rc_march<0, false>(x, k, jb, je);
         wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)(it.w >> 1));
     }
 }
@@ -324,6 +384,7 @@ struct nemo2d_ensemble {
     double *flu = nullptr, *flv = nullptr;
     int tiling = 0;
     bool uv_stale = false;       // sshn_u/sshn_v of the current state not written
+    bool fast_div = true;        // every member's divisors allow division by reciprocals
     std::vector<uint16_t> hfl;   // class words (pooled), for the plain work items
     int4* tiles = nullptr;
     int ntiles = 0, ntiles_cap = 0;
@@ -366,29 +427,63 @@ int march_waves(nemo2d_ensemble* e, int tiling) {
         if (e->fp32)
             grid = rc ? resident_grid(e, k_ens_march_rc<float, double>) : resident_grid(e, k_ens_march<float, double>);
         else
-            grid = rc ? resident_grid(e, k_ens_march_rc<double, double>)
+            grid = rc ? (e->fast_div ? resident_grid(e, k_ens_march_rc64<2>) : resident_grid(e, k_ens_march_rc64<0>))
                       : resident_grid(e, k_ens_march<double, double>);
     }
     return grid * (EB / 64);
 }
 
 // Order of the marching work items: wave w of W runs items w, w+W, w+2W, ...
-// of the array.  The items are sorted by an estimate of their cost (rows,
-// and 1.4x for items that run the general code) and dealt to the waves in a
-// snake (round t: waves 0..W-1 for even t, W-1..0 for odd t), which evens
-// out the waves' totals (the kernel ends with its slowest wave).  Each item
-// keeps its own checksum slot (w = plain | slot << 1), so a member's
-// partials stay contiguous.
+// of the array.  Blocks go to the eight XCDs in turn (block b to XCD b % 8,
+// wave w in block w / 4), and each XCD reads through its own L2, so the
+// items, made in member, segment, strip order, are cut into eight runs of
+// equal estimated cost (rows, and 1.4x for items that run the general
+// code), one per XCD: neighbouring items, which read each other's halo
+// rows and columns, then go through the same L2.  Within a run the items
+// are sorted by cost (stably: items of one cost keep their order) and dealt
+// to the XCD's waves in a snake (round t: waves in order for even t,
+// reversed for odd t), which evens out the waves' totals (the kernel ends
+// with its slowest wave).  Slots left over in a run's last round hold empty
+// items (member -1).  Each item keeps its own checksum slot (w = plain |
+// slot << 1), so a member's partials stay contiguous.
+constexpr int NXCD = 8;
 void order_items(std::vector<int4>& tl, const std::vector<float>& cost, int W) {
     const int n = (int)tl.size();
-    std::vector<int> idx(n);
-    for (int q = 0; q < n; ++q) idx[q] = q;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<int4> out(n);
-    for (int t0 = 0; t0 < n; t0 += W) {
-        const int m = std::min(W, n - t0);
-        const bool rev = (t0 / W) & 1;
-        for (int q = 0; q < m; ++q) out[t0 + (rev ? m - 1 - q : q)] = tl[idx[t0 + q]];
+    constexpr int WB = EB / 64;  // waves per block
+    std::vector<std::vector<int>> xw(NXCD);
+    for (int w = 0; w < W; ++w) xw[(w / WB) % NXCD].push_back(w);
+    // cuts of equal cost, each XCD's share in proportion to its waves
+    double total = 0.0;
+    for (int q = 0; q < n; ++q) total += cost[q];
+    std::vector<int> cut(NXCD + 1, n);
+    cut[0] = 0;
+    {
+        double acc = 0.0, share = 0.0;
+        int x = 0, q = 0;
+        for (x = 0; x < NXCD - 1; ++x) {
+            share += total * (double)xw[x].size() / (double)W;
+            while (q < n && acc + 0.5 * cost[q] < share) acc += cost[q++];
+            cut[x + 1] = q;
+        }
+    }
+    int rounds = 0;
+    for (int x = 0; x < NXCD; ++x)
+        if (!xw[x].empty()) {
+            const int len = cut[x + 1] - cut[x];
+            rounds = std::max(rounds, (len + (int)xw[x].size() - 1) / (int)xw[x].size());
+        }
+    std::vector<int4> out((size_t)rounds * W, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < NXCD; ++x) {
+        const int wx = (int)xw[x].size();
+        if (!wx) continue;
+        std::vector<int> idx;
+        for (int q = cut[x]; q < cut[x + 1]; ++q) idx.push_back(q);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        for (int t0 = 0, r = 0; t0 < (int)idx.size(); t0 += wx, ++r) {
+            const int m = std::min(wx, (int)idx.size() - t0);
+            const bool rev = r & 1;
+            for (int q = 0; q < m; ++q) out[(size_t)r * W + xw[x][rev ? wx - 1 - q : q]] = tl[idx[t0 + q]];
+        }
     }
     tl.swap(out);
 }
@@ -475,11 +570,16 @@ template <bool RC>
 void launch_march(nemo2d_ensemble* e) {
     const int grid = march_waves(e, RC ? MARCH_RC_TILING : MARCH_TILING) / (EB / 64);
     if (!e->fp32) {
-        if constexpr (RC)
-            k_ens_march_rc<double, double><<<grid, EB, 0, e->stream>>>(
-                e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
-                e->tide_stride, e->partials);
-        else
+        if constexpr (RC) {
+            if (e->fast_div)
+                k_ens_march_rc64<2><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
+                                                                 e->fl, e->flv, e->prm, e->tide_tab,
+                                                                 e->tide_stride, e->partials);
+            else
+                k_ens_march_rc64<0><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
+                                                                 e->fl, e->flv, e->prm, e->tide_tab,
+                                                                 e->tide_stride, e->partials);
+        } else
             k_ens_march<double, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
                 e->tide_stride, e->partials);
@@ -649,6 +749,7 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
         const double e12 = dx * dy;
         M.s = Scal{dx, dy, dx, dy, dx, dy, e12, e12, e12, dep[k], dep[k], dep[k], fc, fc, {}, 0};
         set_reciprocals(M.s, e->prm.fric);
+        e->fast_div = e->fast_div && M.s.fast_div;
         M.sf = ScalF{(float)dx, (float)dy, (float)dx, (float)dy, (float)dx, (float)dy, (float)e12,
                      (float)e12, (float)e12, (float)fc, (float)fc, (float)dep[k], (float)dep[k],
                      (float)dep[k]};
