@@ -113,6 +113,7 @@ struct nemo2d_model {
     // every point the wave touches has the plain interior class word
     uint8_t* rc_plain = nullptr;
     double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
+    double* chk = nullptr;       // one word: nemo2d_set_state's interpolation check
     double* cs_log = nullptr;    // 3 doubles per logged step
     // Deferred checksum finish: the fused step leaves each logged step's
     // block partials in its own slot of cs_part; reading the log reduces
@@ -1239,6 +1240,33 @@ k_interp(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn,
     }
 }
 
+// Whether given sshn_u/sshn_v are next's interpolations of sshn, bit for bit,
+// at every point next sets: any point that differs sets *differs.
+template <class S>
+__global__ void __launch_bounds__(TBX * TBY)
+k_interp_check(S st, const uint16_t* __restrict__ fl, const double* __restrict__ sshn, Layout L,
+               const double* __restrict__ sshn_u, const double* __restrict__ sshn_v,
+               unsigned* __restrict__ differs) {
+    const int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y;
+    if (i > L.jpi || j > L.jpj) return;
+    const long o = L.at(i, j), P = L.pitch;
+    const unsigned f = fl[o];
+    bool d = false;
+    if (j >= 1) {
+        const unsigned c = sucls(f);
+        if (c != NX_SKIP)
+            d = __double_as_longlong(sshn_u[o]) !=
+                __double_as_longlong(next_su(st, c, o, sshn[o], sshn[o + 1], 0.0));
+    }
+    if (i >= 1) {
+        const unsigned c = svcls(f);
+        if (c != NX_SKIP)
+            d = d || __double_as_longlong(sshn_v[o]) !=
+                         __double_as_longlong(next_sv(st, c, o, P, sshn[o], sshn[o + P], 0.0));
+    }
+    if (d) atomicOr(differs, 1u);
+}
+
 // The same for the single-precision form's state (FP32 sshn_u/sshn_v from an
 // FP64 or FP32 sshn, FP32 arithmetic and areas, as its step forms them).
 template <class SS>
@@ -1736,7 +1764,8 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         m->npart_max = (int)((long)(g.x * g.y) > mbp ? (long)(g.x * g.y) : mbp);
     }
     if ((rc = dalloc(&m->partials, 6 * ((size_t)m->npart_max + 1), m->stream)) ||
-        (rc = dalloc(&m->cs_now, 3, m->stream)))
+        (rc = dalloc(&m->cs_now, 3, m->stream)) ||
+        (rc = dalloc(&m->chk, 1, m->stream)))
         return bail(rc);
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "nemo2d_create: zeroing the device arrays failed"));
@@ -1921,6 +1950,7 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
         }
     }
     m->grid_set = true;
+    m->uv_derived = false;  // the state's sshn_u/sshn_v were not formed on this grid
     return NEMO2D_OK;
 }
 
@@ -1978,10 +2008,21 @@ int nemo2d_set_state(nemo2d_model* m, const double* sshn, const double* sshn_u, 
     // give the second buffer the same values
     for (int f = 0; f < NF; ++f)
         HIPCHK(hipMemcpyAsync(m->S[1][f], m->S[0][f], bytes, hipMemcpyDeviceToDevice, m->stream));
+    // sshn_u/sshn_v as given: when they are sshn's interpolations bit for bit
+    // (a state from rest, or one this library wrote), the first step may
+    // already take a kernel that forms them from sshn
+    HIPCHK(hipMemsetAsync(m->chk, 0, sizeof(double), m->stream));
+    {
+        const Layout& L = m->L;
+        double** s0 = m->S[0];
+        WITH_STATICS(m, st, (k_interp_check<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
+                                st, m->flags, s0[F_SSHN], L, s0[F_SSHN_U], s0[F_SSHN_V], (unsigned*)m->chk)));
+        LAUNCHCHK("interp_check");
+    }
+    HIPCHK(hipMemcpyAsync(m->h_pinned, m->chk, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    // sshn_u/sshn_v as given (not known to be sshn's interpolations)
     m->uv_stale[0] = m->uv_stale[1] = false;
-    m->uv_derived = false;
+    m->uv_derived = m->grid_set && *(const unsigned*)m->h_pinned == 0u;
     m->lz_n = 0;  // recorded stage calls act on the state this replaces
     m->a_current = true;
     m->state_set = true;
@@ -3183,7 +3224,7 @@ int nemo2d_destroy(nemo2d_model* m) {
     dfree(m->ssha); dfree(m->ua); dfree(m->va);
     dfree(m->flags); dfree(m->flu); dfree(m->flv); dfree(m->rc_plain);
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
-    dfree(m->partials); dfree(m->cs_now); dfree(m->cs_log); dfree(m->cs_part); dfree(m->d_pend);
+    dfree(m->partials); dfree(m->cs_now); dfree(m->chk); dfree(m->cs_log); dfree(m->cs_part); dfree(m->d_pend);
     for (int k = 0; k < 2; ++k)
         for (int f = 0; f < NF; ++f) dfree(m->X[k][f]);
     dfree(m->hf_t); dfree(m->hf_u); dfree(m->hf_v); dfree(m->diag);

@@ -743,13 +743,13 @@ def test_tiny_values_take_the_ieee_divisions(variant):
 def test_default_kernel_by_grid_size():
     """The kernel the default picks, as the library reports it: one thread
     per cell (Flather inline) on the 128^2 basin, shared face terms at
-    1024^2, and beyond 2M cells the five-array kernel for the first step
-    after set_state then the sshn-only kernel; on a seamount grid the
-    per-point-depth marching kernels."""
+    1024^2, and beyond 2M cells the sshn-only kernel (from the first step:
+    the state from rest passes set_state's interpolation check); on a
+    seamount grid the per-point-depth marching kernel."""
     cases = [(128, 128, None, ["cell_flin", "cell_flin"]),
              (1024, 1024, None, ["march12_shn", "march12_shn"]),
-             (1500, 1500, None, ["march12_dpfb", "march12_rc"]),
-             (1200, 1200, "seamount", ["march12_shd", "march12_grd"])]
+             (1500, 1500, None, ["march12_rc", "march12_rc"]),
+             (1200, 1200, "seamount", ["march12_grd", "march12_grd"])]
     for ni, nj, kind, want in cases:
         grid = nemo2d.seamount_grid(ni, nj, seed=2)[0] if kind else None
         with nemo2d.Model(ni, nj, grid=grid) as m:
@@ -759,6 +759,29 @@ def test_default_kernel_by_grid_size():
                 m.step(istp)
                 got.append(m.last_variant())
             assert got == want, (ni, nj, kind, got)
+
+
+def test_first_step_kernel_follows_the_interpolation_check():
+    """nemo2d_set_state checks whether the given sshn_u/sshn_v are next's
+    interpolations of sshn bit for bit.  An evolved state (written by next)
+    passes: the first step already takes the sshn-only kernel.  The same
+    state with one sshn_u value one ulp off fails: the first step takes the
+    five-array kernel, which reads the stored value.  Both runs bitwise equal
+    to the oracle from the same state."""
+    for perturb in (False, True):
+        o = Oracle(1500, 1500)
+        o.run(1, 3)
+        if perturb:
+            su = o.arr("sshn_u")
+            su[700, 700] = np.nextafter(su[700, 700], np.inf)
+        with model_like(o) as m:
+            kernels = []
+            for istp in (4, 5):
+                m.step(istp)
+                o.step(istp)
+                kernels.append(m.last_variant())
+            assert kernels == (["march12_dpfb", "march12_rc"] if perturb else ["march12_rc"] * 2), kernels
+            assert_fields_equal(m, o, "perturbed" if perturb else "evolved")
 
 
 @pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
