@@ -163,16 +163,22 @@ def form_fp32(n, steps, seed=1):
             "energy": {"fp64": out["fp64"][2][1], "fp32": out["fp32"][2][1]}}
 
 
-def form_ensemble(nmem, steps, seed=7, graph=True):
-    """Batched basins: one persistent kernel per step, graph-captured loop,
-    autotuned tiling; FP64 and FP32 on the same members."""
+def form_ensemble(nmem, steps, seed=7, graph=True, tiling=None):
+    """Batched basins: one kernel per step, graph-captured step replayed,
+    autotuned tiling (or the given one); FP64 and FP32 on the same members."""
     import nemo2d
     mem = nemo2d.ensemble_members(nmem, seed)
     res = {}
     states = {}
     for prec in (64, 32):
         e = nemo2d.Ensemble(mem, rdt=12.0, precision=prec)
-        choice, _ = e.tune(nsteps=10)
+        if tiling is None:
+            progress("ensemble fp%d: tuning" % prec)
+            choice, _ = e.tune(nsteps=10)
+        else:
+            choice = tiling
+            e.set_tiling(choice)
+        progress("ensemble fp%d: tiling %d, %d steps%s" % (prec, choice, steps, " (graph)" if graph else ""))
         ms = e.run(1, steps, graph=graph)
         res[prec] = {"cell_updates_per_s": e.cells * steps / (ms * 1e-3), "ms": ms,
                      "tiling": nemo2d.Ensemble.tiling_shape(choice)}
@@ -201,16 +207,19 @@ CONV_FLOP_PER_PIXEL = 2 * 9 * 32 * 32   # one 32 -> 32 layer, forward or weight 
 MFMA_BF16_PEAK = 2.5e15
 
 
-def form_closure(sc, eval_windows=60):
+def form_closure(sc, eval_windows=60, train=None):
     """Coarse (1024^2) + learned closure against the fine (2048^2) solver:
     training on the device, the held-out basin judged by the fine run,
     throughput of the three runs, and the convolution kernels' rates at the
-    training batch (4 x 1024^2)."""
+    training batch (4 x 1024^2).  train = (basins, windows) overrides the
+    training's size (a short run at the full grid size, for kernel traces)."""
     import nemo2d
     from nemo2d import closure_pipeline as cp
     nc = 1024 if sc >= 1 else max(64, int(1024 * sc) // 32 * 32)
     basins = max(4, int(round(64 * min(sc, 1.0))))
     windows = max(10, int(round(313 * min(sc, 1.0))))
+    if train:
+        basins, windows = train
     t0 = time.perf_counter()
     clos, info = cp.train_online(nc, range(basins), windows, group=4, bathy=CLOSURE_BATHY, lr=CLOSURE_LR,
                                  rollout=CLOSURE_ROLLOUT, log=progress)
@@ -297,13 +306,14 @@ def load_traffic(v, n):
     return d.get("bytes_per_launch") if d.get("variant") == v else None
 
 
-def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_going=False):
+def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_going=False,
+              closure_train=None, ens_tiling=None, ens_steps=None):
     """The forms' figures; with keep_going a form that raises is reported as
     {"error": ...} (printed to stderr too) and the others still run."""
     runs = {"fp32_mixed": lambda: form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
-            "ensemble": lambda: form_ensemble(max(8, int(256 * sc)), max(10, int(3000 * sc)),
-                                              graph=graph),
-            "closure": lambda: form_closure(sc)}
+            "ensemble": lambda: form_ensemble(max(8, int(256 * sc)), ens_steps or max(10, int(3000 * sc)),
+                                              graph=graph, tiling=ens_tiling),
+            "closure": lambda: form_closure(sc, train=closure_train)}
     out = {}
     for name in ("fp32_mixed", "ensemble", "closure"):
         if name not in which:
@@ -342,6 +352,14 @@ def main():
     ap.add_argument("--form", choices=("fp32_mixed", "ensemble", "closure"),
                     help="run only this form (at --forms-scale) and print its JSON line; "
                          "for per-form kernel traces")
+    ap.add_argument("--closure-train", type=lambda v: tuple(int(x) for x in v.split(",")),
+                    help="with --form closure: BASINS,WINDOWS of training instead of the "
+                         "workload's 64,313 (grid sizes unchanged; for kernel traces)")
+    ap.add_argument("--ens-tiling", type=int,
+                    help="with --form ensemble: run this tiling instead of the autotuner's pick")
+    ap.add_argument("--ens-steps", type=int,
+                    help="with --form ensemble: steps per run instead of the workload's 3000 "
+                         "(for counter passes)")
     ap.add_argument("--no-graph", action="store_true",
                     help="with --form ensemble: launch the steps one by one instead of "
                          "replaying the captured graph")
@@ -356,8 +374,11 @@ def main():
         sys.exit(2)
 
     if args.form:
-        print(json.dumps({args.form: run_forms(args.forms_scale, (args.form,),
-                                               graph=not args.no_graph)[args.form]}), flush=True)
+        print(json.dumps({args.form: run_forms(args.forms_scale, (args.form,), graph=not args.no_graph,
+                                               closure_train=args.closure_train,
+                                               ens_tiling=args.ens_tiling,
+                                               ens_steps=args.ens_steps)[args.form]}),
+              flush=True)
         return
 
     import nemo2d

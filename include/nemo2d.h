@@ -140,7 +140,8 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
  * the depths only.  Marching (3..11) needs uniform metrics, 3, 4 and 6..10
  * also uniform depths, 5 and 11 Flather points only on row 0 with the inner
  * point north; 10 and 11 need sshn_u/sshn_v to be sshn's interpolations
- * (true after any step; the first step after nemo2d_set_state runs 9 or 5).
+ * (true after any step; nemo2d_set_state checks the given arrays on the
+ * device, and when they are not, the first step after it runs 9 or 5).
  * Other grids use 1 (or 2).  The single-precision form marches where it
  * applies unless 1 is chosen.  No counterpart in the reference: a tuning
  * hook.  nemo2d_last_variant gives the kernel the last nemo2d_step (or fused
@@ -231,7 +232,8 @@ int nemo2d_diagnostics(nemo2d_model *m, double out[3]);
  * precision 64: each member is bitwise the single-model FP64 path
  * (nemo2d_create + nemo2d_set_tide on the same basin); precision 32: the
  * single-precision form with FP64 ssh accumulation.  All members start at
- * rest.  One persistent kernel advances every member by one step. */
+ * rest.  One kernel over (member, tile) work items advances every member
+ * by one step. */
 typedef struct nemo2d_ensemble nemo2d_ensemble;
 int nemo2d_ensemble_create(nemo2d_ensemble **e, int nmem, const int *jpi, const int *jpj,
                            const double *dep, const double *amp, const double *period_h,
@@ -246,16 +248,20 @@ int nemo2d_ensemble_set_tiling(nemo2d_ensemble *e, int choice);
 /* Work-item shape (columns x rows of cells) of tiling `choice`: choices
  * 0..7 are blocks of 256 cells, one per thread; choice 8 is one wave's
  * strip of 63 columns by 12 rows, column-marching with shared face terms;
- * choice 9 (the default) the same over strips of 61 columns reading sshn,
- * un, vn only (sshn_u/sshn_v formed from sshn as next forms them, and filled
- * in before a step of another choice), class-free on plain work items. */
+ * choices 9 and 10 the same over strips of 61 columns reading sshn, un, vn
+ * only (sshn_u/sshn_v formed from sshn as next forms them, and filled in
+ * before a step of another choice), class-free on plain work items: 9 on a
+ * resident grid of blocks taking the items in a cost-balanced order, 10 (the
+ * default) one wave per item, neighbouring strips of a segment in one block
+ * and each XCD a contiguous run of items. */
 int nemo2d_ensemble_tiling(int choice, int *tx, int *ty);
-/* Steps nit000..nitend; use_graph != 0 replays one captured step (a HIP
- * graph of the step kernel, the checksum finish and the advance of a device
- * control word holding the step index and the now-state buffer) once per
- * step; the graph is captured on first use and kept until the tiling or a
- * buffer it names changes.  *ms = device time of the steps.  Per-member,
- * per-step checksums are logged. */
+/* Steps nit000..nitend; use_graph != 0 replays a HIP graph of 8 captured
+ * steps (each the step kernel and the checksum finish, whose last block
+ * advances a device control word holding the step index and the now-state
+ * buffer) while 8 or more steps remain, and launches the rest one by one;
+ * the graph is captured on first use and kept until the tiling or a buffer
+ * it names changes.  *ms = device time of the steps.  Per-member, per-step
+ * checksums are logged. */
 int nemo2d_ensemble_run(nemo2d_ensemble *e, int nit000, int nitend, int use_graph, double *ms);
 int nemo2d_ensemble_checksum_log(nemo2d_ensemble *e, int member, double *out, int max_steps,
                                  int *nsteps);

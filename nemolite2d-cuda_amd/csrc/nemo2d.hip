@@ -262,13 +262,6 @@ struct RegWin {
     }
 };
 
-// Block b of a launch goes to XCD b % 8; hand each XCD a contiguous range of
-// tiles so that tiles sharing halo rows share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-    const int per = nb >> 3;  // nb is a multiple of 8
-    return (b & 7) * per + (b >> 3);
-}
-
 // The five-array marching step for grids whose arrays exceed the shared-term
 // kernels' 32-bit byte offsets: 16-row segments, the next row's loads issued
 // one iteration ahead, IEEE divisions, Flather points left to the epilogue.

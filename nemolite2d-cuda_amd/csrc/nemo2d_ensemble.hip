@@ -5,15 +5,16 @@
 // f-plane at 50 N, original/nemolite2d.f90:180-279) with its own size,
 // constant depth and tidal amplitude/period.  All members live in pooled
 // device arrays, each in its own padded layout.  One step of the whole
-// ensemble is one persistent kernel: a fixed grid of blocks walks a list of
-// (member, tile) work items.  The work items are blocks of cells computed
-// one per thread with the single-model path's per-cell body, or (the last
-// tiling) one wave's column strip marched with the single-model path's
-// shared-face-term body; either way an FP64 member is bitwise the same as
-// that member run alone.  Per-member checksums are reduced in a fixed order by a second
-// kernel.  The step index and the now-state buffer live in a device control
-// word that a third (one-thread) kernel moves on, so one step is captured
-// into a HIP graph once and replayed for every step.  The tile shape is chosen by timing the candidates
+// ensemble is one kernel over a list of (member, tile) work items: blocks of
+// cells computed one per thread with the single-model path's per-cell body,
+// or one wave's column strip marched with the single-model path's marching
+// body (by a resident grid of blocks walking the items, or one item per wave
+// on a grid the hardware dispatches); either way an FP64 member is bitwise
+// the same as that member run alone.  Per-member checksums are reduced in a
+// fixed order by a second kernel.  The step index and the now-state buffer
+// live in a device control word that the second kernel's last block moves
+// on, so one step is captured into a HIP graph once and replayed for every
+// step.  The tile shape is chosen by timing the candidates
 // (nemo2d_ensemble_tune).
 #include <hip/hip_runtime.h>
 
@@ -47,9 +48,11 @@ constexpr int EB = 256;  // threads per block of the ensemble step
 // Step control in device memory: the step index (relative to the run's
 // first step: the tide table's column and the checksum log's slot) and which
 // of the two state buffers holds the now-state.  The step kernels read it and
-// k_ens_advance moves it on, so one captured step can be replayed.
+// the last block of k_ens_finish to end moves it on (done counts the blocks
+// that have ended), so one captured step can be replayed.
 struct EnsCtl {
     int step, cur;
+    unsigned done, pad_;
 };
 // The two now-state buffers (sshn, sshn_u, sshn_v, un, vn each).
 struct EnsBufs {
@@ -70,21 +73,29 @@ struct EnsMember {
 };
 
 // Candidate tilings: TX columns x (EB/TX)*ROWS rows per work item of a
-// block, one point per thread; the last two are column-marching: a work item
-// is one wave's strip of GOUT columns over EMJ rows (shared face terms), or
-// of GROUT columns read as sshn, un, vn only (sshn_u/sshn_v formed from
-// sshn, plain tiles class-free).
+// block, one point per thread; the last three are column-marching: a work
+// item is one wave's strip of GOUT columns over EMJ rows (shared face
+// terms), or of GROUT columns read as sshn, un, vn only (sshn_u/sshn_v formed
+// from sshn, plain tiles class-free).  The sshn-only items run either on a
+// resident grid of blocks whose waves take the items in a cost-balanced
+// order, or (the last tiling) one item per wave on a grid of one block per
+// four items that the hardware dispatches as slots free up, the items in
+// member, segment, strip order so that a block's four waves march four
+// neighbouring strips of one segment together (their halo columns shared
+// in the CU's L1) and each XCD gets a contiguous run of segments.
 struct Tiling {
     int tx, rows;
 };
 constexpr int EMJ = 12;  // rows per marching work item
 constexpr Tiling TILINGS[] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 1}, {32, 2},
-                              {GOUT, EMJ}, {GROUT, EMJ}};
+                              {GOUT, EMJ}, {GROUT, EMJ}, {GROUT, EMJ}};
 constexpr int NTILINGS = sizeof(TILINGS) / sizeof(TILINGS[0]);
-constexpr int MARCH_TILING = NTILINGS - 2;
-constexpr int MARCH_RC_TILING = NTILINGS - 1;
+constexpr int MARCH_TILING = NTILINGS - 3;
+constexpr int MARCH_RC_TILING = NTILINGS - 2;   // resident grid, cost-balanced items
+constexpr int MARCH_RCD_TILING = NTILINGS - 1;  // dispatched, items in grid order
 constexpr int ERC_LA = 2;  // rows loaded ahead by the sshn-only marching item
-constexpr bool is_march(int t) { return t == MARCH_TILING || t == MARCH_RC_TILING; }
+constexpr bool is_rc(int t) { return t == MARCH_RC_TILING || t == MARCH_RCD_TILING; }
+constexpr bool is_march(int t) { return t == MARCH_TILING || is_rc(t); }
 
 template <int TX, int ROWS>
 __global__ void __launch_bounds__(EB)
@@ -203,8 +214,9 @@ k_ens_march(const EnsMember* __restrict__ mem, const int4* __restrict__ items, i
 // The same step reading sshn, un, vn only: a work item (member, strip,
 // segment, plain) marches one wave's strip of GROUT columns; the new state's
 // sshn_u/sshn_v are not written (k_ens_interp fills them in when a kernel
-// that reads them comes next).
-template <class R, class SS>
+// that reads them comes next).  DISP: one item per wave, block b taking the
+// items of block xcd_tile(b) (the dispatched tiling).
+template <class R, class SS, bool DISP>
 __global__ void __launch_bounds__(EB)
 k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
                EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
@@ -220,7 +232,8 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
     // the wave index made wave-uniform (an SGPR): the work item, the member's
     // record and its metrics are then scalar loads held in SGPRs
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
+    const int b0 = DISP ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (int w = b0 * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
         if (it.x < 0) continue;  // an empty slot of the item order
         const EnsMember& M = mem[it.x];
@@ -268,7 +281,7 @@ k_ens_march_rc(const EnsMember* __restrict__ mem, const int4* __restrict__ items
 // so every member stays bitwise the single run.
 static_assert(ROUT == GROUT && RC_NJ == EMJ && RC_LA == ERC_LA,
               "the work items and their plain flags are made for the marching tiling");
-template <int DM>
+template <int DM, bool DISP>
 __global__ void __launch_bounds__(EB, 3)  // 3 waves per SIMD, as the single-grid kernel
 k_ens_march_rc64(const EnsMember* __restrict__ mem, const int4* __restrict__ items, int nitems,
                  EnsBufs bufs, const EnsCtl* __restrict__ ctl, const uint16_t* __restrict__ flp,
@@ -283,7 +296,8 @@ k_ens_march_rc64(const EnsMember* __restrict__ mem, const int4* __restrict__ ite
     // the wave index made wave-uniform (an SGPR): the work item, the member's
     // record and its statics are then scalar loads held in SGPRs
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int w = blockIdx.x * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
+    const int b0 = DISP ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (int w = b0 * (EB / 64) + wv; w < nitems; w += gridDim.x * (EB / 64)) {
         const int4 it = items[w];
         if (it.x < 0) continue;  // an empty slot of the item order
         const EnsMember& M = mem[it.x];
@@ -348,21 +362,26 @@ k_ens_interp(const EnsMember* __restrict__ mem, const SS* __restrict__ sshn, R* 
     }
 }
 
-// Per-member checksums: block m reduces its members' tile partials in order.
-// (log slot = the step index of the control word)
+// Per-member checksums: block m reduces its members' tile partials in order
+// into the log slot of the control word's step index.  Then the step is
+// done: the block that ends last (every block read the control word before
+// it counted itself in done) gives the next step the next index and the
+// other buffer, and clears done for it.
 __global__ void __launch_bounds__(256)
 k_ens_finish(const EnsMember* __restrict__ mem, const double* __restrict__ partials,
-             double* __restrict__ log, int cap, const EnsCtl* __restrict__ ctl) {
+             double* __restrict__ log, int cap, EnsCtl* __restrict__ ctl) {
     const EnsMember& M = mem[blockIdx.x];
-    const int slot = ctl->step;
-    finish_partials(partials + 6 * (long)M.tile0, M.ntiles, log + 3 * ((long)blockIdx.x * cap + slot));
-}
-
-// The step is done: the next one has the next index and the other buffer.
-__global__ void k_ens_advance(EnsCtl* __restrict__ ctl) {
+    const int step = ctl->step, cur = ctl->cur;
+    finish_partials(partials + 6 * (long)M.tile0, M.ntiles, log + 3 * ((long)blockIdx.x * cap + step));
+    __syncthreads();  // every thread of the block has read the control word
     if (threadIdx.x == 0) {
-        const EnsCtl c = *ctl;
-        *ctl = EnsCtl{c.step + 1, 1 - c.cur};
+        __threadfence();
+        if (atomicAdd(&ctl->done, 1u) == gridDim.x - 1) {
+            ctl->step = step + 1;
+            ctl->cur = 1 - cur;
+            ctl->done = 0u;
+            __threadfence();
+        }
     }
 }
 
@@ -399,9 +418,10 @@ struct nemo2d_ensemble {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     EnsCtl* ctl = nullptr;       // device step control
     EnsCtl h_ctl{};              // its value at the start of a run
-    // one captured step (step kernel, checksum finish, advance), replayed
-    // for every step of a run; rebuilt when the tiling or a buffer it names
-    // changes (gen counts reallocations)
+    // GRAPH_STEPS captured steps (step kernel, checksum finish and advance
+    // each), replayed for every run of that many steps; a run's remaining
+    // steps are launched one by one.  Rebuilt when the tiling or a buffer it
+    // names changes (gen counts reallocations)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_tiling = -1;
@@ -422,12 +442,13 @@ int resident_grid(const nemo2d_ensemble* e, K kern) {
 // Waves of the marching kernel a tiling runs (its resident grid).
 int march_waves(nemo2d_ensemble* e, int tiling) {
     const bool rc = tiling == MARCH_RC_TILING;
+    if (tiling == MARCH_RCD_TILING) return 0;  // dispatched: the grid follows the items
     int& grid = e->mgrid[e->fp32 ? 1 : 0][rc ? 1 : 0];
     if (!grid) {
         if (e->fp32)
-            grid = rc ? resident_grid(e, k_ens_march_rc<float, double>) : resident_grid(e, k_ens_march<float, double>);
+            grid = rc ? resident_grid(e, k_ens_march_rc<float, double, false>) : resident_grid(e, k_ens_march<float, double>);
         else
-            grid = rc ? (e->fast_div ? resident_grid(e, k_ens_march_rc64<2>) : resident_grid(e, k_ens_march_rc64<0>))
+            grid = rc ? (e->fast_div ? resident_grid(e, k_ens_march_rc64<2, false>) : resident_grid(e, k_ens_march_rc64<0, false>))
                       : resident_grid(e, k_ens_march<double, double>);
     }
     return grid * (EB / 64);
@@ -502,7 +523,7 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
         for (int by = 0; by < nby; ++by)
             for (int bx = 0; bx < nbx; ++bx) {
                 int plain = 0;
-                if (tiling == MARCH_RC_TILING) {
+                if (is_rc(tiling)) {
                     // every class word the wave touches (lanes at columns
                     // bx*GROUT-1 .. +63, rows jb-1 .. je+1+ERC_LA) plain
                     const int i0 = bx * GROUT - 1, i1 = bx * GROUT + 62;
@@ -522,7 +543,7 @@ int build_tiles(nemo2d_ensemble* e, int tiling) {
                 }
             }
     }
-    if (is_march(tiling)) order_items(tl, cost, march_waves(e, tiling));
+    if (tiling == MARCH_TILING || tiling == MARCH_RC_TILING) order_items(tl, cost, march_waves(e, tiling));
     e->ntiles = (int)tl.size();
     if (e->ntiles > e->ntiles_cap) {
         if (e->tiles) (void)hipFree(e->tiles);
@@ -566,19 +587,21 @@ void launch_step_t(nemo2d_ensemble* e) {
     }
 }
 
-template <bool RC>
+template <bool RC, bool DISP = false>
 void launch_march(nemo2d_ensemble* e) {
-    const int grid = march_waves(e, RC ? MARCH_RC_TILING : MARCH_TILING) / (EB / 64);
+    // dispatched: one block per four items, a multiple of 8 blocks (xcd_tile)
+    const int grid = DISP ? ((e->ntiles + EB / 64 - 1) / (EB / 64) + 7) / 8 * 8
+                          : march_waves(e, RC ? MARCH_RC_TILING : MARCH_TILING) / (EB / 64);
     if (!e->fp32) {
         if constexpr (RC) {
             if (e->fast_div)
-                k_ens_march_rc64<2><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
-                                                                 e->fl, e->flv, e->prm, e->tide_tab,
-                                                                 e->tide_stride, e->partials);
+                k_ens_march_rc64<2, DISP><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e),
+                                                                       e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
+                                                                       e->tide_stride, e->partials);
             else
-                k_ens_march_rc64<0><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl,
-                                                                 e->fl, e->flv, e->prm, e->tide_tab,
-                                                                 e->tide_stride, e->partials);
+                k_ens_march_rc64<0, DISP><<<grid, EB, 0, e->stream>>>(e->dmem, e->tiles, e->ntiles, bufs_of(e),
+                                                                       e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
+                                                                       e->tide_stride, e->partials);
         } else
             k_ens_march<double, double><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, e->prm, e->tide_tab,
@@ -587,7 +610,7 @@ void launch_march(nemo2d_ensemble* e) {
         ParamsT<float> pf{(float)e->prm.rdt, (float)e->prm.cbfr, (float)e->prm.visc,
                           (float)e->prm.fric};
         if constexpr (RC)
-            k_ens_march_rc<float, double><<<grid, EB, 0, e->stream>>>(
+            k_ens_march_rc<float, double, DISP><<<grid, EB, 0, e->stream>>>(
                 e->dmem, e->tiles, e->ntiles, bufs_of(e), e->ctl, e->fl, e->flv, pf, e->tide_tab,
                 e->tide_stride, e->partials);
         else
@@ -612,13 +635,14 @@ void launch_interp(nemo2d_ensemble* e) {
 }
 
 // One step as the device control word names it: the step kernel of the
-// current tiling, the per-member checksum finish, and the advance of the
+// current tiling and the per-member checksum finish, which also advances the
 // control word.  Host state (e->cur) is not touched: the caller accounts
 // for the steps it issued.
 int launch_step(nemo2d_ensemble* e) {
     switch (e->tiling) {
         case MARCH_TILING: launch_march<false>(e); break;
         case MARCH_RC_TILING: launch_march<true>(e); break;
+        case MARCH_RCD_TILING: launch_march<true, true>(e); break;
         case 0: launch_step_t<64, 1>(e); break;
         case 1: launch_step_t<64, 2>(e); break;
         case 2: launch_step_t<64, 4>(e); break;
@@ -631,14 +655,16 @@ int launch_step(nemo2d_ensemble* e) {
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble step: ") + hipGetErrorString(err));
     k_ens_finish<<<e->nmem, 256, 0, e->stream>>>(e->dmem, e->partials, e->log, e->log_cap, e->ctl);
-    k_ens_advance<<<1, 64, 0, e->stream>>>(e->ctl);
     err = hipGetLastError();
     if (err != hipSuccess) return set_error(NEMO2D_ERR_HIP, std::string("ensemble finish: ") + hipGetErrorString(err));
     return NEMO2D_OK;
 }
 
-// The captured step for the current tiling and buffers (captured once,
-// replayed for every step of every run until one of them changes).
+// The captured steps for the current tiling and buffers (captured once,
+// replayed until one of them changes).  Each replay is one host call, and
+// consecutive graph launches leave a gap on the device: GRAPH_STEPS steps
+// per graph make it one gap per GRAPH_STEPS steps.
+constexpr int GRAPH_STEPS = 8;
 int ensure_graph(nemo2d_ensemble* e) {
     if (e->gexec && e->g_tiling == e->tiling && e->g_gen == e->gen) return NEMO2D_OK;
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
@@ -646,7 +672,8 @@ int ensure_graph(nemo2d_ensemble* e) {
     e->gexec = nullptr;
     e->graph = nullptr;
     EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = launch_step(e);
+    int rc = 0;
+    for (int s = 0; s < GRAPH_STEPS && !rc; ++s) rc = launch_step(e);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e->stream, &g);
     if (rc || ce != hipSuccess) {
@@ -812,7 +839,7 @@ int nemo2d_ensemble_create(nemo2d_ensemble** out, int nmem, const int* jpi, cons
     e->cus = cus;
     int rc;
     e->hfl = std::move(hfl);
-    if ((rc = build_tiles(e, MARCH_RC_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
+    if ((rc = build_tiles(e, MARCH_RCD_TILING)) || (rc = nemo2d_ensemble_reset(e))) return bail(rc);
     *out = e;
     return NEMO2D_OK;
 }
@@ -837,28 +864,32 @@ int nemo2d_ensemble_run(nemo2d_ensemble* e, int nit000, int nitend, int use_grap
     const int nsteps = nitend - nit000 + 1;
     int rc;
     if ((rc = upload_tides(e, nit000, nsteps)) || (rc = ensure_log(e, nsteps))) return rc;
-    if (e->uv_stale && e->tiling != MARCH_RC_TILING) {  // this tiling reads sshn_u/sshn_v
+    if (e->uv_stale && !is_rc(e->tiling)) {  // this tiling reads sshn_u/sshn_v
         launch_interp(e);
         EHIP(hipGetLastError());
         e->uv_stale = false;
     }
-    e->h_ctl = EnsCtl{0, e->cur};
+    e->h_ctl = EnsCtl{0, e->cur, 0u, 0u};
     EHIP(hipMemcpyAsync(e->ctl, &e->h_ctl, sizeof(EnsCtl), hipMemcpyHostToDevice, e->stream));
     if (use_graph && (rc = ensure_graph(e))) return rc;
     EHIP(hipEventRecord(e->ev0, e->stream));
-    for (int s = 0; s < nsteps && !rc; ++s) {
-        if (use_graph) {
+    // (graph replays while GRAPH_STEPS steps remain, then single launches;
+    // each step moves the device control word on, whichever way it came)
+    for (int s = 0; s < nsteps && !rc;) {
+        if (use_graph && nsteps - s >= GRAPH_STEPS) {
             const hipError_t le = hipGraphLaunch(e->gexec, e->stream);
             if (le != hipSuccess) rc = set_error(NEMO2D_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(le));
+            s += GRAPH_STEPS;
         } else {
             rc = launch_step(e);
+            ++s;
         }
     }
     EHIP(hipEventRecord(e->ev1, e->stream));
     EHIP(hipEventSynchronize(e->ev1));
     if (rc) return rc;
     if (nsteps & 1) e->cur = 1 - e->cur;
-    if (e->tiling == MARCH_RC_TILING) e->uv_stale = true;
+    if (is_rc(e->tiling)) e->uv_stale = true;
     float t = 0.f;
     EHIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
     if (ms) *ms = (double)t;

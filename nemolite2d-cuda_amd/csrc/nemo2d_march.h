@@ -730,6 +730,13 @@ k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
     block_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)blockIdx.x);
 }
 
+// Block b of a launch goes to XCD b % 8; hand each XCD a contiguous range of
+// tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int per = nb >> 3;  // nb is a multiple of 8
+    return (b & 7) * per + (b >> 3);
+}
+
 // Sums of the wave's lanes (fixed order), written by lane 0 to out[0..5].
 __device__ __forceinline__ void wave_reduce3(DD a0, DD a1, DD a2, double* out) {
     for (int mk = 32; mk >= 1; mk >>= 1) {

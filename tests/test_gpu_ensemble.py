@@ -30,7 +30,7 @@ def single(k, m, nsteps):
         return s.retrieve(), s.checksum_log(nsteps)
 
 
-@pytest.mark.parametrize("tiling", [0, 8, 9])
+@pytest.mark.parametrize("tiling", [0, 8, 9, 10])
 def test_fp64_members_bitwise_equal_single_model_and_oracle(tiling):
     m = small_members()
     nsteps = 80
@@ -60,13 +60,13 @@ def test_graph_plain_and_tilings_agree():
     results = []
     with nemo2d.Ensemble(m, rdt=RDT) as e:
         for tiling, graph in ((0, True), (0, False), (3, True), (5, True), (7, False), (2, True),
-                              (8, True), (8, False), (9, True), (9, False)):
+                              (8, True), (8, False), (9, True), (9, False), (10, True), (10, False)):
             e.set_tiling(tiling)
             e.reset()
             e.run(1, nsteps, graph=graph)
             results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
         choice, ms = e.tune(nsteps=5)
-        assert 0 <= choice < 10 and ms > 0
+        assert 0 <= choice < 11 and ms > 0
         e.run(1, nsteps)
         results.append([e.retrieve(k) for k in range(len(m["jpi"]))])
     for other in results[1:]:
@@ -95,7 +95,7 @@ def test_fp32_marching_tiling_equals_per_cell_tiling():
     m = nemo2d.ensemble_members(7, seed=9, size=(20, 150))
     out = []
     with nemo2d.Ensemble(m, rdt=RDT, precision=32) as e:
-        for tiling in (0, 8, 9):
+        for tiling in (0, 8, 9, 10):
             e.set_tiling(tiling)
             e.reset()
             e.run(1, 120)
@@ -117,8 +117,9 @@ def test_switching_tilings_mid_run(precision):
         e.run(1, 60)
         ref = [e.retrieve(k) for k in range(5)]
         e.reset()
-        for tiling, (a, b), graph in ((9, (1, 20), True), (0, (21, 30), True), (9, (31, 45), False),
-                                      (8, (46, 50), False), (9, (51, 55), True), (3, (56, 60), True)):
+        for tiling, (a, b), graph in ((9, (1, 20), True), (0, (21, 30), True), (10, (31, 45), False),
+                                      (8, (46, 50), False), (9, (51, 55), True), (10, (56, 58), True),
+                                      (3, (59, 60), True)):
             e.set_tiling(tiling)
             e.run(a, b, graph=graph)
         for k in range(5):
@@ -127,12 +128,13 @@ def test_switching_tilings_mid_run(precision):
                 assert np.array_equal(r[f], ref[k][f]), (k, f)
 
 
-@pytest.mark.parametrize("tiling", [8, 9])
+@pytest.mark.parametrize("tiling", [8, 9, 10])
 def test_many_work_items_in_cost_order(tiling):
-    """More marching work items than the kernel has waves (the items are
-    dealt to the waves in cost order, several per wave, each keeping its own
-    checksum slot): members bitwise equal to their single runs, per-member
-    checksum logs equal to theirs."""
+    """More marching work items than the kernel has waves (tilings 8 and 9
+    deal the items to the waves in cost order, several per wave; tiling 10
+    gives each its own wave), each item keeping its own checksum slot:
+    members bitwise equal to their single runs, per-member checksum logs
+    equal to theirs."""
     m = nemo2d.ensemble_members(40, seed=13, size=(180, 300))
     nsteps = 25
     with nemo2d.Ensemble(m, rdt=RDT) as e:
