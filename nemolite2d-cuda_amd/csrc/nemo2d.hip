@@ -1542,12 +1542,16 @@ int set_error(int code, const std::string& msg) { return fail(code, msg); }
 void set_reciprocals(Scal& s, double fric) {
     const double d[QD_N] = {s.e12t, s.e1t, s.e2t, s.e2u + s.e2u, s.e1v + s.e1v, s.e12u, s.e12v,
                             s.hu + 0.0, s.hv + 0.0, fric};
-    bool ok = true;
+    bool ok = true, one = true;
     for (int k = 0; k < QD_N; ++k) {
         ok = ok && d[k] >= 0x1p-60 && d[k] <= 0x1p60;
         s.rcp[k] = 1.0 / d[k];
+        // |d*RN(1/d) - 1| <= 2^-54 (the FMA evaluates it to within a relative
+        // 2^-53; the margin covers that): RN(x*y) is then a faithful x/d
+        one = one && fabs(fma(d[k], s.rcp[k], -1.0)) <= 0x1p-54 * (1.0 - 0x1p-40);
     }
     s.fast_div = ok ? 1 : 0;
+    s.one_corr = ok && one ? 1 : 0;
 }
 
 
@@ -2344,6 +2348,7 @@ template <int DM>
 static int launch_march_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
     if (DM != 0 && !m->scal.fast_div) return launch_march_rc<0>(m, n, a, tide, nparts);
+    if (DM == 2 && m->scal.one_corr) return launch_march_rc<3>(m, n, a, tide, nparts);
     static_assert(RC_LA >= 2, "the plain-tile table covers the rows loaded ahead");
     const int nbx = (L.jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
     const int nby = (L.jpj + 1 + RC_NJ - 1) / RC_NJ;

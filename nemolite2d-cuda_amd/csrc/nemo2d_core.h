@@ -85,6 +85,7 @@ struct Scal {
     double e1t, e2t, e1u, e2u, e1v, e2v, e12t, e12u, e12v, ht, hu, hv, fcu, fcv;
     double rcp[QD_N];  // RN(1/divisor) for each QD_ tag (uniform statics)
     int fast_div;      // divisors in range: divide by reciprocals
+    int one_corr;      // and every |divisor * rcp - 1| <= 2^-54: one correction suffices
 };
 
 struct StUniform {
@@ -206,6 +207,21 @@ __device__ __forceinline__ double div_rcp_acc(double x, double d, double y, DivC
     double r = fma(d, q, -x);
     q = fma(-r, y, q);
     r = fma(d, q, -x);
+    q = fma(-r, y, q);
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    c.emin = e < c.emin ? e : c.emin;
+    c.emax = e > c.emax ? e : c.emax;
+    c.odd = c.odd || __builtin_amdgcn_class(x, 0x297);  // NaN, infinity, subnormal
+    return q;
+}
+// The one-correction form: RN(x*y) is already a faithful quotient when
+// |d*y - 1| <= 2^-54 (then |x*y - x/d| < ulp(x/d)/2, and rounding adds at
+// most another half ulp), so one remainder correction gives the correctly
+// rounded quotient (Markstein), with the same guard on x.  The host allows
+// it only for grids whose every divisor passes that test (Scal::one_corr).
+__device__ __forceinline__ double div_rcp1_acc(double x, double d, double y, DivCheck& c) {
+    double q = x * y;
+    const double r = fma(d, q, -x);
     q = fma(-r, y, q);
     const int e = __builtin_amdgcn_frexp_exp(x);
     c.emin = e < c.emin ? e : c.emin;

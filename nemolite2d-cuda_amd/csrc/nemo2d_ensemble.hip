@@ -326,8 +326,14 @@ k_ens_march_rc64(const EnsMember* __restrict__ mem, const int4* __restrict__ ite
         k.ct = k.cu = k.cv = 0.0;
         // (items touching a boundary divide by IEEE division: their general
         // code with the guard's bookkeeping would not fit the register budget)
-        if (it.w & 1) rc_march<DM, true>(x, k, jb, je);
-        else rc_march<0, false>(x, k, jb, je);
+        // (a member whose divisors all pass the one-correction test divides
+        // with one correction: the branch is on the item, uniform per wave)
+        if (it.w & 1) {
+            if (DM == 2 && M.s.one_corr) rc_march<3, true>(x, k, jb, je);
+            else rc_march<DM, true>(x, k, jb, je);
+        } else {
+            rc_march<0, false>(x, k, jb, je);
+        }
         wave_reduce3(DD{k.ct, 0.0}, DD{k.cu, 0.0}, DD{k.cv, 0.0}, partials + 6 * (long)(it.w >> 1));
     }
 }

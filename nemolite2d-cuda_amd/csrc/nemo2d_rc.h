@@ -43,11 +43,13 @@ struct Col5 {
 // east, nr: the row above (own column)
 // x / d for the uniform divisor d of tag T (QD_*).  DM 0: the IEEE
 // division; 1: the guarded division by the precomputed reciprocal (div_rcp);
-// 2: the unguarded one, its guard gathered in *ck (div_rcp_acc).  All give
-// x / d's bits (2 when the gathered guard holds).
+// 2: the unguarded one, its guard gathered in *ck (div_rcp_acc); 3: as 2
+// with one correction (div_rcp1_acc; grids with Scal::one_corr only).  All
+// give x / d's bits (2 and 3 when the gathered guard holds).
 template <int DM, int T>
 __device__ __forceinline__ double udiv(const Scal& s, double x, double d, DivCheck* ck = nullptr) {
-    if constexpr (DM == 2) return div_rcp_acc(x, d, s.rcp[T], *ck);
+    if constexpr (DM == 3) return div_rcp1_acc(x, d, s.rcp[T], *ck);
+    else if constexpr (DM == 2) return div_rcp_acc(x, d, s.rcp[T], *ck);
     else if constexpr (DM == 1) return div_rcp(x, d, s.rcp[T]);
     else return x / d;
 }
@@ -341,9 +343,9 @@ __device__ __forceinline__ void rc_commit(const RcCtx& x, RcCarry& k, const Col5
 }
 
 // One row j: A, B, C hold rows j-1, j, j+1; D receives row j+3 (row j+2
-// is in flight).  With DM 2 the row's divisions go by reciprocal with one
-// guard for the whole row; if it fails on any lane, the wave does the row
-// again with IEEE divisions.
+// is in flight).  With DM 2 or 3 the row's divisions go by reciprocal with
+// one guard for the whole row; if it fails on any lane, the wave does the
+// row again with IEEE divisions.
 template <int DM, bool PLAIN = false>
 __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& A, Col5F& B,
                                        Col5F& C, Col5F& D, int j) {
@@ -351,10 +353,10 @@ __device__ __forceinline__ void rc_row(const RcCtx& x, RcCarry& k, const Col5F& 
         const int jn = j + 3 <= x.L.jpj + 2 ? j + 3 : x.L.jpj + 2;
         load_col3<PLAIN>(D, x.n, x.fl, x.rowb(jn) + x.lb8);
     }
-    if constexpr (DM == 2) {
+    if constexpr (DM == 2 || DM == 3) {
         DivCheck ck;
         div_check_reset(ck);
-        RcOut o = rc_compute<2, PLAIN>(x, k, A, B, C, j, &ck);
+        RcOut o = rc_compute<DM, PLAIN>(x, k, A, B, C, j, &ck);
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!div_check_ok(ck)) != 0, 0))
             o = rc_compute<0, PLAIN>(x, k, A, B, C, j, nullptr);
         rc_commit<PLAIN>(x, k, A, B, C, o, j);

@@ -2,7 +2,9 @@
 div_rcp_acc wherever its gathered guard holds, in
 nemolite2d-cuda_amd/csrc/nemo2d_core.h), restated on the host in
 tests/native/div_rcp_check.c, gives the bits of the IEEE quotient x / d on
-random, near-midpoint and special operands.  The reference divides with
+random, near-midpoint and special operands; so does the one-correction form
+(div_rcp1_acc) on divisors that pass the library's test for it
+(|d*RN(1/d) - 1| <= 2^-54, which makes RN(x*y) a faithful quotient).  The reference divides with
 IEEE division throughout (original/nemolite2d.f90:403, :505, :597, :737);
 the device kernels that use the reciprocal form must give the same bits."""
 import os
@@ -21,6 +23,6 @@ def test_division_by_reciprocal_is_exact(tmp_path):
                            os.path.join(HERE, "native", "div_rcp_check.c"), "-lm"])
     for seed in (1, 2, 3):
         out = subprocess.check_output([exe, "400000", str(seed)], text=True).split()
-        tried, bad, guarded = int(out[0]), int(out[1]), int(out[2])
-        assert tried > 2_000_000 and guarded > tried // 2
+        tried, bad, guarded, tried1 = (int(v) for v in out[:4])
+        assert tried > 2_000_000 and guarded > tried // 2 and tried1 > tried // 3
         assert bad == 0, "%d of %d quotients differ from IEEE division" % (bad, tried)

@@ -786,9 +786,13 @@ def test_first_step_kernel_follows_the_interpolation_check():
 
 @pytest.mark.parametrize("variant", [v for v in nemo2d.VARIANTS if v != "auto"])
 def test_every_kernel_variant_is_bitwise_equal(variant):
-    """Default grid (uniform statics: all variants apply) and a general mask
-    with uniform metrics (Flather points on three edges)."""
-    for o in (Oracle(300, 70), random_grid_oracle(137, 45, 11, vary_metrics=False)):
+    """Default grid (uniform statics: all variants apply; every divisor d
+    has |d*RN(1/d) - 1| <= 2^-54, so the reciprocal divisions take one
+    correction), the same basin 150 m deep (1/150 fails that test: two
+    corrections) and a general mask with uniform metrics (Flather points on
+    three edges)."""
+    for o in (Oracle(300, 70), Oracle(300, 70, dep_const=150.0),
+              random_grid_oracle(137, 45, 11, vary_metrics=False)):
         with model_like(o) as m:
             m.set_variant(variant)
             m.record_checksums(30)
