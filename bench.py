@@ -207,6 +207,37 @@ CONV_FLOP_PER_PIXEL = 2 * 9 * 32 * 32   # one 32 -> 32 layer, forward or weight 
 MFMA_BF16_PEAK = 2.5e15
 
 
+def conv_kernel_rates(net, nc, reps=20):
+    """The closure network's convolution kernels of one 32 -> 32 layer at the
+    training batch (4 images nc x nc): time per launch, MFMA flop/s and HBM
+    bytes/s against the MI355X peaks."""
+    rng = np.random.default_rng(0)
+    P = 4 * nc * nc
+    xp = np.zeros((P, 8), np.float32)
+    xp[:, :4] = rng.standard_normal((P, 4))
+    xg = net.to_bf16(xp)
+    kt = net.time_kernels(xg, 4, nc, nc, reps=reps)
+    flop = CONV_FLOP_PER_PIXEL * P
+    kernels = {}
+    for name, t in kt.items():
+        # HBM bytes per pixel: forward reads X (64) and writes z, act (64 + 64);
+        # data gradient reads dZ, z below (64 + 64) and writes dz below (64);
+        # weight gradient reads X and dZ (64 + 64)
+        bpp = {"conv_fwd": 192, "conv_bwd_data_gelu": 192, "conv_bwd_weight": 128}[name]
+        kernels[name] = {"us": t * 1e6, "tflops": flop / t / 1e12, "frac_of_bf16_mfma_peak": flop / t / MFMA_BF16_PEAK,
+                         "hbm_gbs": bpp * P / t / 1e9, "frac_of_hbm": bpp * P / t / 1e9 / HBM_PEAK_GBS}
+    return kernels
+
+
+def form_conv(nc=1024, reps=20):
+    """The convolution kernels alone (an untrained network of the closure's
+    shape), for kernel traces and counters."""
+    from nemo2d.closure import ConvNet
+    return {"workload": "closure network 3x3 convolutions, one 32 -> 32 layer, batch 4 x %dx%d (BF16, "
+                        "FP32 accumulation), %d launches each" % (nc, nc, reps),
+            "conv_kernels_batch_4x%dx%d" % (nc, nc): conv_kernel_rates(ConvNet(seed=0), nc, reps)}
+
+
 def form_closure(sc, eval_windows=60, train=None):
     """Coarse (1024^2) + learned closure against the fine (2048^2) solver:
     training on the device, the held-out basin judged by the fine run,
@@ -251,22 +282,7 @@ def form_closure(sc, eval_windows=60, train=None):
         n = held.nf if name == "fine" else held.nc
         rates[name] = n * n * nst / dt
         m.close()
-    # the convolution kernels at the training batch
-    rng = np.random.default_rng(0)
-    P = 4 * nc * nc
-    xp = np.zeros((P, 8), np.float32)
-    xp[:, :4] = rng.standard_normal((P, 4))
-    xg = clos.net.to_bf16(xp)
-    kt = clos.net.time_kernels(xg, 4, nc, nc)
-    flop = CONV_FLOP_PER_PIXEL * P
-    kernels = {}
-    for name, t in kt.items():
-        # HBM bytes per pixel: forward reads X (64) and writes z, act (64 + 64);
-        # data gradient reads dZ, z below (64 + 64) and writes dz below (64);
-        # weight gradient reads X and dZ (64 + 64)
-        bpp = {"conv_fwd": 192, "conv_bwd_data_gelu": 192, "conv_bwd_weight": 128}[name]
-        kernels[name] = {"us": t * 1e6, "tflops": flop / t / 1e12, "frac_of_bf16_mfma_peak": flop / t / MFMA_BF16_PEAK,
-                         "hbm_gbs": bpp * P / t / 1e9, "frac_of_hbm": bpp * P / t / 1e9 / HBM_PEAK_GBS}
+    kernels = conv_kernel_rates(clos.net, nc)
     return {"workload": "coarse %dx%d (dx=2 km, rdt=40 s) + 4-layer 3x3-conv closure (4->32->32->32->3, "
                         "GELU, BF16 MFMA 32x32x16, FP32 accumulation) applied every %d steps; trained on "
                         "the device on %d pairs from %d seeded seamount fine runs %dx%d (dx=1 km, rdt=20 s, "
@@ -313,11 +329,12 @@ def run_forms(sc, which=("fp32_mixed", "ensemble", "closure"), graph=True, keep_
     """The forms' figures; with keep_going a form that raises is reported as
     {"error": ...} (printed to stderr too) and the others still run."""
     runs = {"fp32_mixed": lambda: form_fp32(int(8192 * sc) // 64 * 64 or 64, max(10, int(5000 * sc))),
+            "conv": lambda: form_conv(),
             "ensemble": lambda: form_ensemble(max(8, int(256 * sc)), ens_steps or max(10, int(3000 * sc)),
                                               graph=graph, tiling=ens_tiling),
             "closure": lambda: form_closure(sc, train=closure_train)}
     out = {}
-    for name in ("fp32_mixed", "ensemble", "closure"):
+    for name in ("fp32_mixed", "ensemble", "closure", "conv"):
         if name not in which:
             continue
         progress("form %s" % name)
@@ -351,9 +368,10 @@ def main():
     ap.add_argument("--forms-scale", type=float, default=1.0,
                     help="scale of the forms' workloads (1.0: 8192^2 x 5000 FP32, 256-basin "
                          "ensemble x 3000, closure 1024^2/2048^2 with 64 basins x 313 windows)")
-    ap.add_argument("--form", choices=("fp32_mixed", "ensemble", "closure"),
+    ap.add_argument("--form", choices=("fp32_mixed", "ensemble", "closure", "conv"),
                     help="run only this form (at --forms-scale) and print its JSON line; "
-                         "for per-form kernel traces")
+                         "for per-form kernel traces (conv: the closure's convolution "
+                         "kernels alone)")
     ap.add_argument("--closure-train", type=lambda v: tuple(int(x) for x in v.split(",")),
                     help="with --form closure: BASINS,WINDOWS of training instead of the "
                          "workload's 64,313 (grid sizes unchanged; for kernel traces)")

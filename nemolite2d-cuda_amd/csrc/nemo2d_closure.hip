@@ -31,14 +31,44 @@ namespace {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int NB = 256;  // threads per block (4 waves)
+constexpr int NBC = 512; // threads per block of the convolutions (8 waves, one tile row each)
 
 __device__ __forceinline__ uint16_t f2bf(float f) { return bf16_rne(f); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// Two FP32 values as packed BF16 (round to nearest even: the gfx950
+// v_cvt_pk_bf16_f32 conversion), a in the low half.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){a, b}, bf16x2_t));
+}
+
+// GELU(z) = z/2 (1 + erf(z/sqrt2)) and its derivative, with erf by
+// Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7; the values are rounded to
+// BF16, whose half ulp is 2e-3 relative): for a = |z|/sqrt2, t = 1/(1+p a),
+// 1 - erf(a) = P(t) exp(-a^2), so 1 + erf(z/sqrt2) is P e for z < 0 (no
+// cancellation) and 2 - P e for z >= 0, and the derivative's Gaussian is the
+// same e = exp(-z^2/2).
+struct GeluParts {
+    float ope;  // 1 + erf(z / sqrt2)
+    float e;    // exp(-z^2 / 2)
+};
+__device__ __forceinline__ GeluParts gelu_parts(float z) {
+    const float a = fabsf(z) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
+    const float e = __expf(-0.5f * z * z);
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    const float h = p * t * e;
+    return GeluParts{z >= 0.0f ? 2.0f - h : h, e};
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * gelu_parts(z).ope; }
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const GeluParts g = gelu_parts(z);
+    return fmaf(z * 0.39894228040143268f, g.e, 0.5f * g.ope);
 }
 
 struct Img {
@@ -47,12 +77,16 @@ struct Img {
 };
 
 // ---- forward and data-gradient convolutions -------------------------------
-// A block owns a tile of CT_H rows x CT_W = 32 pixels of one image.  The
-// input tile with its 1-pixel halo is staged once in LDS (pixel-major, the
-// channels of a pixel contiguous, padded to the MFMA K step); each of the 4
-// waves computes CT_H/4 row segments, one 32-channel x 32-pixel MFMA tile
-// (v_mfma_f32_32x32x16_bf16) each, over the 9 taps (taps outer, the next
-// tap's weights loaded during this tap's MFMAs).  Operand layout
+// A tile is CT_H rows x CT_W = 32 pixels of one image.  The kernels are
+// persistent: block b takes tiles b, b + G, b + 2G, ... (G blocks, as many as
+// stay resident), with the 9 taps' weights staged in LDS once per block and
+// the input tiles double-buffered: the next tile's global loads are issued
+// before this tile's MFMAs and written to the other LDS buffer after its
+// epilogue, so their latency hides behind the arithmetic.  A tile in LDS is
+// pixel-major with its 1-pixel halo (the channels of a pixel contiguous,
+// padded to the MFMA K step, zero outside the image); each of the 4 waves
+// computes CT_H/4 row segments, one 32-channel x 32-pixel MFMA tile
+// (v_mfma_f32_32x32x16_bf16) each, over the 9 taps.  Operand layout
 // (32x32x16): lane l holds A row l%32 and B column l%32 at 8 consecutive k
 // of half l/32; the K order is the same for both operands, so any
 // assignment of k to (channel) values works.  The weights are the A operand
@@ -74,180 +108,328 @@ __device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
     return r;
 }
 
-// Stage src[n][y0-1 .. y0+CT_H][x0-1 .. x0+CT_W][0..cp) into LDS tile[pixel][KT + 8]
-// (channels cp..KT-1 and pixels outside the image zero).
+// Tile geometry of tile index t.
+struct TilePos {
+    int img, y0, x0;
+};
+__device__ __forceinline__ TilePos tile_pos(const Img& im, int t) {
+    const int tiles_x = (im.w + CT_W - 1) / CT_W, tiles_y = (im.h + CT_H - 1) / CT_H;
+    return TilePos{t / (tiles_x * tiles_y), (t / tiles_x) % tiles_y * CT_H, t % tiles_x * CT_W};
+}
+
+// One tile's global data src[img][y0-1 .. y0+CT_H][x0-1 .. x0+CT_W][0..cp)
+// in registers (channels cp..KT-1 and pixels outside the image zero), then
+// into LDS tile[pixel][KT + 8].
 template <int KT>
-__device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ src, const Img& im, int cp,
-                                           int img, int y0, int x0, uint16_t* tile) {
-    constexpr int PS = KT + 8;  // LDS pixel stride (bf16): +16 B against bank conflicts
-    constexpr int CH = KT / 8;  // 16-byte chunks per pixel
-    constexpr int IT = (CT_PIX * CH + NB - 1) / NB;
-    // every load of the thread issued before the first LDS store
+struct TileRegs {
+    static constexpr int CH = KT / 8;  // 16-byte chunks per pixel
+    static constexpr int IT = (CT_PIX * CH + NBC - 1) / NBC;
     uint4 v[IT];
+};
+template <int KT>
+__device__ __forceinline__ void tile_load(const uint16_t* __restrict__ src, const Img& im, int cp,
+                                          const TilePos& tp, TileRegs<KT>& r) {
+    constexpr int CH = TileRegs<KT>::CH;
 #pragma unroll
-    for (int s = 0; s < IT; ++s) {
-        const int e = threadIdx.x + s * NB;
+    for (int s = 0; s < TileRegs<KT>::IT; ++s) {
+        const int e = threadIdx.x + s * NBC;
         const int q = e / CH, c8 = (e % CH) * 8;
-        const int yy = y0 - 1 + q / (CT_W + 2), xx = x0 - 1 + q % (CT_W + 2);
-        v[s] = make_uint4(0, 0, 0, 0);
+        const int yy = tp.y0 - 1 + q / (CT_W + 2), xx = tp.x0 - 1 + q % (CT_W + 2);
+        r.v[s] = make_uint4(0, 0, 0, 0);
         if (e < CT_PIX * CH && c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
-            v[s] = *(const uint4*)(src + (((long)img * im.h + yy) * im.w + xx) * cp + c8);
+            r.v[s] = *(const uint4*)(src + (((long)tp.img * im.h + yy) * im.w + xx) * cp + c8);
     }
+}
+template <int KT>
+__device__ __forceinline__ void tile_store(const TileRegs<KT>& r, uint16_t* tile) {
+    constexpr int PS = KT + 8;  // LDS pixel stride (bf16): +16 B against bank conflicts
+    constexpr int CH = TileRegs<KT>::CH;
 #pragma unroll
-    for (int s = 0; s < IT; ++s) {
-        const int e = threadIdx.x + s * NB;
-        if (e < CT_PIX * CH) *(uint4*)(tile + (e / CH) * PS + (e % CH) * 8) = v[s];
+    for (int s = 0; s < TileRegs<KT>::IT; ++s) {
+        const int e = threadIdx.x + s * NBC;
+        if (e < CT_PIX * CH) *(uint4*)(tile + (e / CH) * PS + (e % CH) * 8) = r.v[s];
     }
 }
 
-// Lane's weights of one tap: column c (output channel of the forward, input
-// channel of the data gradient), K values kk*16 + 8*half .. +7 (0 past kp).
+// The 9 taps' weights wt[tap][32][kp] into LDS wl[tap][32][KT + 8] (K values
+// kp..KT-1 zero).
 template <int KT>
-__device__ __forceinline__ void load_tap(const uint16_t* __restrict__ wt, int tap, int c, int half, int kp,
-                                         bf16x8 (&w)[KT / 16]) {
+__device__ __forceinline__ void weights_store(const uint16_t* __restrict__ wt, int kp, uint16_t* wl) {
+    constexpr int RS = KT + 8, CH = KT / 8;
+    for (int e = threadIdx.x; e < 9 * 32 * CH; e += NBC) {
+        const int row = e / CH, c8 = (e % CH) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (c8 < kp) v = *(const uint4*)(wt + (long)row * kp + c8);
+        *(uint4*)(wl + row * RS + c8) = v;
+    }
+}
+
+// The operands of one tap are read from LDS while the previous tap's MFMAs
+// run (two register sets, alternating), so no MFMA waits for its reads.
+template <int KT>
+struct TapOps {
+    bf16x8 w[KT / 16], b0[KT / 16];
+};
+template <int KT, int SIGN>
+__device__ __forceinline__ void tap_read(const uint16_t* tile, const uint16_t* wl, int wave, int c, int half,
+                                         int tap, TapOps<KT>& o) {
+    constexpr int PS = KT + 8, RS = KT + 8;
+    const int dy = SIGN * (tap / 3 - 1), dx = SIGN * (tap % 3 - 1);
+    const uint16_t* wr = wl + (tap * 32 + c) * RS + 8 * half;
+    const uint16_t* a0 = tile + ((wave + 1 + dy) * (CT_W + 2) + c + 1 + dx) * PS + 8 * half;
 #pragma unroll
     for (int kk = 0; kk < KT / 16; ++kk) {
-        const int k0 = kk * 16 + 8 * half;
-        bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (k0 < kp) v = lds8(wt + ((long)tap * 32 + c) * kp + k0);
-        w[kk] = v;
+        o.w[kk] = lds8(wr + kk * 16);
+        o.b0[kk] = lds8(a0 + kk * 16);
     }
 }
-
-// One 3x3 convolution pass over a block's tile: each wave owns rows wave and
-// wave + 4 of the tile (two 32 x 32 accumulators); the taps are the outer
-// loop, with the next tap's weights loaded while this tap's MFMAs run.
-// SIGN = +1: forward (input at p + off(tap)); -1: data gradient (dZ at
-// p - off(tap)).
+// One 3x3 convolution pass over a tile in LDS: wave w computes row w of the
+// tile (one 32 x 32 accumulator).  SIGN = +1: forward (input at p +
+// off(tap)); -1: data gradient (dZ at p - off(tap)).
 template <int KT, int SIGN>
-__device__ __forceinline__ void conv_rows(const uint16_t* tile, const uint16_t* __restrict__ wt, int kp,
-                                          int wave, int c, int half, f32x16& acc0, f32x16& acc1) {
-    constexpr int PS = KT + 8;
-    bf16x8 w[KT / 16], wn[KT / 16];
-    load_tap<KT>(wt, 0, c, half, kp, w);
+__device__ __forceinline__ void conv_row(const uint16_t* tile, const uint16_t* wl, int wave, int c,
+                                         int half, f32x16& acc0) {
+    TapOps<KT> op[2];
+    tap_read<KT, SIGN>(tile, wl, wave, c, half, 0, op[0]);
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
-        if (tap + 1 < 9) load_tap<KT>(wt, tap + 1, c, half, kp, wn);
-        const int dy = SIGN * (tap / 3 - 1), dx = SIGN * (tap % 3 - 1);
-        const uint16_t* a0 = tile + ((wave + 1 + dy) * (CT_W + 2) + c + 1 + dx) * PS + 8 * half;
-        const uint16_t* a1 = a0 + 4 * (CT_W + 2) * PS;
+        if (tap + 1 < 9) tap_read<KT, SIGN>(tile, wl, wave, c, half, tap + 1, op[(tap + 1) & 1]);
+        // keep the reads ahead of the MFMAs (the scheduler would sink each
+        // read to its use, exposing its latency before every MFMA)
+        __builtin_amdgcn_sched_barrier(0);
+        const TapOps<KT>& o = op[tap & 1];
 #pragma unroll
-        for (int kk = 0; kk < KT / 16; ++kk) {
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[kk], lds8(a0 + kk * 16), acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[kk], lds8(a1 + kk * 16), acc1, 0, 0, 0);
-        }
-        if (tap + 1 < 9)
-#pragma unroll
-            for (int kk = 0; kk < KT / 16; ++kk) w[kk] = wn[kk];
+        for (int kk = 0; kk < KT / 16; ++kk)
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.w[kk], o.b0[kk], acc0, 0, 0, 0);
     }
 }
 
-static_assert(CT_H == 8 && NB == 256, "each of the 4 waves owns 2 rows of the tile");
+static_assert(CT_H * 64 == NBC, "each of the 8 waves owns one row of the tile");
+
+// The persistent tile loop shared by the forward and data-gradient kernels:
+// epi.init() loads what every tile's epilogue needs, epi.prefetch(tile)
+// issues a tile's epilogue loads before its MFMAs, epi.compute(tile,
+// accumulators) forms its outputs in registers and epi.store() writes them.  Vector memory counters count
+// loads and stores together, in issue order, so a wait for a load also
+// waits for every store issued before it: each iteration therefore issues
+// the previous tile's stores first, then the next tile's loads, and only
+// after this tile's MFMAs and epilogue arithmetic moves the next tile from
+// registers to LDS (the one wait of the iteration, a whole compute phase
+// after those stores and loads were issued).
+template <int KT, int SIGN, class Epi>
+__device__ __forceinline__ void conv_tiles(const uint16_t* __restrict__ src, const Img& im, int cp,
+                                           const uint16_t* __restrict__ wt, int ntiles, uint16_t* tiles,
+                                           uint16_t* wl, Epi& epi) {
+    constexpr int TS = CT_PIX * (KT + 8);  // one LDS tile buffer (bf16)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
+    int t = blockIdx.x;
+    if (t >= ntiles) return;  // (uniform per block)
+    TileRegs<KT> r;
+    tile_load<KT>(src, im, cp, tile_pos(im, t), r);
+    epi.init();
+    weights_store<KT>(wt, cp, wl);  // (the weights' K is the input's channels)
+    tile_store<KT>(r, tiles);
+    __syncthreads();
+    int cur = 0;
+    for (bool prev = false; t < ntiles; t += gridDim.x, prev = true) {
+        const int nxt = t + gridDim.x;
+        if (prev) epi.store();                                                // the previous tile's
+        if (nxt < ntiles) tile_load<KT>(src, im, cp, tile_pos(im, nxt), r);  // in flight meanwhile
+        epi.prefetch(tile_pos(im, t));  // this tile's epilogue operands, needed after the MFMAs
+        f32x16 acc = {0};
+        conv_row<KT, SIGN>(tiles + cur * TS, wl, wave, c, half, acc);
+        epi.compute(tile_pos(im, t), acc);
+        // (that buffer's readers passed the last barrier)
+        if (nxt < ntiles) tile_store<KT>(r, tiles + (1 - cur) * TS);
+        __syncthreads();  // the next tile is in LDS, and no wave still reads this one
+        cur ^= 1;
+    }
+    epi.store();
+}
+// LDS of the persistent convolution kernels: two tile buffers and the
+// weights, (2 * CT_PIX + 9 * 32) * (KT + 8) bf16 (77,440 B for KT = 32).
+template <int KT>
+constexpr int conv_lds_elems() { return (2 * CT_PIX + 9 * 32) * (KT + 8); }
 
 // Forward: z = b + sum_tap sum_ci X[p + off(tap)][ci] W[tap][ci][co];
-// act = gelu(z) (or z).  wt: [9][32][cinp] (cout-major, from pack).
-template <int KT>
-__global__ void __launch_bounds__(NB)
-k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __restrict__ wt,
-           const float* __restrict__ bias, int cout, int coutp, float* __restrict__ z,
-           uint16_t* __restrict__ zb, uint16_t* __restrict__ act, int use_gelu) {
-    constexpr int PS = KT + 8;
-    __shared__ __attribute__((aligned(16))) uint16_t tile[CT_PIX * PS];
-    const int tiles_x = (im.w + CT_W - 1) / CT_W, tiles_y = (im.h + CT_H - 1) / CT_H;
-    const int b = blockIdx.x;
-    const int img = b / (tiles_x * tiles_y);
-    const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
-    stage_tile<KT>(x, im, cinp, img, y0, x0, tile);
-    __syncthreads();
-    f32x16 acc[2] = {{0}, {0}};
-    conv_rows<KT, 1>(tile, wt, cinp, wave, c, half, acc[0], acc[1]);
-    const int xx = x0 + c;  // this lane's pixel column
-    if (xx >= im.w) return;
+// act = gelu(z) (or z).  wt: [9][32][cinp] (cout-major, from pack).  The
+// epilogue: lane (c, half) of wave w holds, for row w of the tile, channels
+// co0 .. co0+3 (co0 = 8g + 4 half) of pixel c in accumulator registers
+// 4g .. 4g+3; padded channels co >= cout are zero.
+struct FwdEpi {
+    Img im;
+    const float* bias;
+    int cout, coutp, use_gelu;
+    float* z;
+    uint16_t* zb;
+    uint16_t* act;
+    int wave, c, half;
+    TilePos tp;
+    uint2 ob[4], oa[4];
+    float4 oz[4];
+    float bv[4][4];  // bias of channels 8g + 4 half + r (0 past cout)
+    __device__ __forceinline__ void init() {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int y = y0 + wave + 4 * h;
-        if (y >= im.h) break;
-        const long q0 = (((long)img * im.h + y) * im.w + xx) * coutp;
-        // channels co0 .. co0+3 in registers 4g .. 4g+3
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = 8 * g + 4 * half + r;
+                const float b = bias[co < cout ? co : 0];  // (always in bounds)
+                bv[g][r] = co < cout ? b : 0.f;
+            }
+    }
+    __device__ __forceinline__ void prefetch(const TilePos&) {}
+    __device__ __forceinline__ void compute(const TilePos& t, const f32x16& acc) {
+        tp = t;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int co0 = 8 * g + 4 * half;
+            float v[4], va[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = co0 + r < cout ? acc[4 * g + r] + bv[g][r] : 0.f;
+                va[r] = use_gelu ? gelu(v[r]) : v[r];  // GELU(0) = 0
+            }
+            oz[g] = make_float4(v[0], v[1], v[2], v[3]);
+            ob[g] = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
+            oa[g] = make_uint2(pk_bf16(va[0], va[1]), pk_bf16(va[2], va[3]));
+        }
+    }
+    __device__ __forceinline__ void store() const {
+        const int xx = tp.x0 + c, y = tp.y0 + wave;  // this lane's pixel
+        if (xx >= im.w || y >= im.h) return;
+        const long q0 = (((long)tp.img * im.h + y) * im.w + xx) * coutp;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int co0 = 8 * g + 4 * half;
             if (co0 >= coutp) break;
-            float v[4];
-            uint16_t vb[4], va[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const bool live = co0 + r < cout;
-                v[r] = live ? acc[h][4 * g + r] + bias[co0 + r] : 0.f;
-                vb[r] = live ? f2bf(v[r]) : (uint16_t)0;  // padded channels: zero
-                va[r] = live ? f2bf(use_gelu ? gelu(v[r]) : v[r]) : (uint16_t)0;
-            }
             if (z) {
                 if (co0 + 4 <= cout) {
-                    *(float4*)(z + q0 + co0) = make_float4(v[0], v[1], v[2], v[3]);
+                    *(float4*)(z + q0 + co0) = oz[g];
                 } else {
+                    const float v[4] = {oz[g].x, oz[g].y, oz[g].z, oz[g].w};
                     for (int r = 0; r < 4 && co0 + r < cout; ++r) z[q0 + co0 + r] = v[r];
                 }
             }
-            if (zb) *(uint2*)(zb + q0 + co0) = make_uint2(vb[0] | (uint32_t)vb[1] << 16, vb[2] | (uint32_t)vb[3] << 16);
-            if (act) *(uint2*)(act + q0 + co0) = make_uint2(va[0] | (uint32_t)va[1] << 16, va[2] | (uint32_t)va[3] << 16);
+            if (zb) *(uint2*)(zb + q0 + co0) = ob[g];
+            if (act) *(uint2*)(act + q0 + co0) = oa[g];
         }
     }
+};
+
+template <int KT>
+__global__ void __launch_bounds__(NBC, 4)  // 4 waves per SIMD: two blocks per CU
+k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __restrict__ wt,
+           const float* __restrict__ bias, int cout, int coutp, float* __restrict__ z,
+           uint16_t* __restrict__ zb, uint16_t* __restrict__ act, int use_gelu, int ntiles) {
+    __shared__ __attribute__((aligned(16))) uint16_t lds[conv_lds_elems<KT>()];
+    const int lane = threadIdx.x & 63;
+    FwdEpi epi;
+    epi.im = im;
+    epi.bias = bias;
+    epi.cout = cout;
+    epi.coutp = coutp;
+    epi.use_gelu = use_gelu;
+    epi.z = z;
+    epi.zb = zb;
+    epi.act = act;
+    epi.wave = threadIdx.x >> 6;
+    epi.c = lane & 31;
+    epi.half = lane >> 5;
+    conv_tiles<KT, 1>(x, im, cinp, wt, ntiles, lds, lds + 2 * CT_PIX * (KT + 8), epi);
 }
 
 // Data gradient: dX[p][ci] = sum_tap sum_co dZ[p - off(tap)][co] W[tap][ci][co];
 // with a pre-activation of the layer below (zp FP32 or zpb BF16) the output
 // is that layer's dz = BF16(dX * GELU'(z)) (GELU backward fused), else FP32
 // dX.  wt: [9][32][coutp] (cin-major, from pack).
-template <int KT>
-__global__ void __launch_bounds__(NB)
-k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16_t* __restrict__ wt,
-                int cin, int cinp, float* __restrict__ dx, const float* __restrict__ zp,
-                const uint16_t* __restrict__ zpb, uint16_t* __restrict__ dzp) {
-    constexpr int PS = KT + 8;
-    __shared__ __attribute__((aligned(16))) uint16_t tile[CT_PIX * PS];
-    const int tiles_x = (im.w + CT_W - 1) / CT_W, tiles_y = (im.h + CT_H - 1) / CT_H;
-    const int b = blockIdx.x;
-    const int img = b / (tiles_x * tiles_y);
-    const int y0 = (b / tiles_x) % tiles_y * CT_H, x0 = b % tiles_x * CT_W;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
-    stage_tile<KT>(dz, im, coutp, img, y0, x0, tile);
-    __syncthreads();
-    f32x16 acc[2] = {{0}, {0}};
-    conv_rows<KT, -1>(tile, wt, coutp, wave, c, half, acc[0], acc[1]);
-    const int xx = x0 + c;  // this lane's pixel column
-    if (xx >= im.w) return;
-    const bool gl = zp || zpb;
+struct BwdEpi {
+    Img im;
+    int cin, cinp;
+    float* dx;
+    const float* zp;
+    const uint16_t* zpb;
+    uint16_t* dzp;
+    int wave, c, half;
+    TilePos tp;
+    uint2 od[4];
+    float4 ox[4];
+    float4 zr[4];  // the layer below's pre-activations at this lane's pixel (prefetched)
+    __device__ __forceinline__ long at(const TilePos& p) const {
+        const int xx = p.x0 + c, y = p.y0 + wave;
+        return (xx < im.w && y < im.h) ? (((long)p.img * im.h + y) * im.w + xx) * cinp : -1;
+    }
+    __device__ __forceinline__ void init() {}
+    __device__ __forceinline__ void prefetch(const TilePos& p) {
+        if (!(zp || zpb)) return;
+        const long q0 = at(p);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int y = y0 + wave + 4 * h;
-        if (y >= im.h) break;
-        const long q0 = (((long)img * im.h + y) * im.w + xx) * cinp;
+        for (int g = 0; g < 4; ++g) {
+            const int ci0 = 8 * g + 4 * half;
+            zr[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (q0 >= 0 && ci0 < cinp) {
+                if (zpb) {
+                    const uint2 u = *(const uint2*)(zpb + q0 + ci0);
+                    zr[g] = make_float4(bf2f(u.x & 0xffff), bf2f(u.x >> 16), bf2f(u.y & 0xffff), bf2f(u.y >> 16));
+                } else {
+                    zr[g] = *(const float4*)(zp + q0 + ci0);
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void compute(const TilePos& t, const f32x16& acc) {
+        tp = t;
+        const bool gl = zp || zpb;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int ci0 = 8 * g + 4 * half;
+            ox[g] = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+            if (gl) {
+                const float zv[4] = {zr[g].x, zr[g].y, zr[g].z, zr[g].w};
+                float o[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) o[r] = ci0 + r < cin ? acc[4 * g + r] * gelu_grad(zv[r]) : 0.f;
+                od[g] = make_uint2(pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]));
+            }
+        }
+    }
+    __device__ __forceinline__ void store() const {
+        const long q0 = at(tp);
+        if (q0 < 0) return;
+        const bool gl = zp || zpb;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int ci0 = 8 * g + 4 * half;
             if (ci0 >= cinp) break;
             if (gl) {
-                float zv[4];
-                if (zpb) {
-                    const uint2 u = *(const uint2*)(zpb + q0 + ci0);
-                    zv[0] = bf2f(u.x & 0xffff); zv[1] = bf2f(u.x >> 16);
-                    zv[2] = bf2f(u.y & 0xffff); zv[3] = bf2f(u.y >> 16);
-                } else {
-                    const float4 u = *(const float4*)(zp + q0 + ci0);
-                    zv[0] = u.x; zv[1] = u.y; zv[2] = u.z; zv[3] = u.w;
-                }
-                uint16_t o[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    o[r] = ci0 + r < cin ? f2bf(acc[h][4 * g + r] * gelu_grad(zv[r])) : (uint16_t)0;
-                *(uint2*)(dzp + q0 + ci0) = make_uint2(o[0] | (uint32_t)o[1] << 16, o[2] | (uint32_t)o[3] << 16);
+                *(uint2*)(dzp + q0 + ci0) = od[g];
             } else {
-                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[q0 + ci0 + r] = acc[h][4 * g + r];
+                const float v[4] = {ox[g].x, ox[g].y, ox[g].z, ox[g].w};
+                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[q0 + ci0 + r] = v[r];
             }
         }
     }
+};
+
+template <int KT>
+__global__ void __launch_bounds__(NBC, 4)  // 4 waves per SIMD: two blocks per CU
+k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16_t* __restrict__ wt,
+                int cin, int cinp, float* __restrict__ dx, const float* __restrict__ zp,
+                const uint16_t* __restrict__ zpb, uint16_t* __restrict__ dzp, int ntiles) {
+    __shared__ __attribute__((aligned(16))) uint16_t lds[conv_lds_elems<KT>()];
+    const int lane = threadIdx.x & 63;
+    BwdEpi epi;
+    epi.im = im;
+    epi.cin = cin;
+    epi.cinp = cinp;
+    epi.dx = dx;
+    epi.zp = zp;
+    epi.zpb = zpb;
+    epi.dzp = dzp;
+    epi.wave = threadIdx.x >> 6;
+    epi.c = lane & 31;
+    epi.half = lane >> 5;
+    conv_tiles<KT, -1>(dz, im, coutp, wt, ntiles, lds, lds + 2 * CT_PIX * (KT + 8), epi);
 }
 
 // ---- weight gradient ---------------------------------------------------------
@@ -514,6 +696,23 @@ int launched(const char* what) {
     return NEMO2D_OK;
 }
 
+// Blocks of a persistent convolution kernel: as many as stay resident on the
+// device (the occupancy of each kernel, ID, is asked once), at most one per
+// tile.
+template <int ID, class K>
+int conv_grid(K kern, long ntiles) {
+    static int resident = 0;  // per ID
+    if (!resident) {
+        int dev = 0, per_cu = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NBC, 0) != hipSuccess || per_cu < 1)
+            return 0;
+        resident = per_cu * cus;
+    }
+    return (int)(ntiles < resident ? ntiles : resident);
+}
+
 bool bad_ch(int c) { return c < 1 || c > 32; }
 int pad8(int c) { return (c + 7) / 8 * 8; }
 
@@ -527,24 +726,28 @@ int nemo2d_nn_conv_fwd(const void* x, int n, int h, int w, int cin, const void* 
     if (!x || !wf || !bias || !(z || zb || act) || n < 1 || h < 1 || w < 1 || bad_ch(cin) || bad_ch(cout))
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: bad argument");
     const Img im{n, h, w};
-    const long nb = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
-    if (nb > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: too many pixels");
-    auto* k = pad8(cin) <= 16 ? k_conv_fwd<16> : k_conv_fwd<32>;
-    k<<<(unsigned)nb, NB, 0, (hipStream_t)stream>>>((const uint16_t*)x, im, pad8(cin), (const uint16_t*)wf,
-                                                  bias, cout, pad8(cout), z, (uint16_t*)zb, (uint16_t*)act,
-                                                  use_gelu);
+    const long nt = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
+    if (nt > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: too many pixels");
+    const bool k16 = pad8(cin) <= 16;
+    auto* k = k16 ? k_conv_fwd<16> : k_conv_fwd<32>;
+    const int g = k16 ? conv_grid<0>(k, nt) : conv_grid<1>(k, nt);
+    if (g < 1) return set_error(NEMO2D_ERR_HIP, "nemo2d_nn_conv_fwd: occupancy query failed");
+    k<<<g, NBC, 0, (hipStream_t)stream>>>((const uint16_t*)x, im, pad8(cin), (const uint16_t*)wf, bias, cout,
+                                         pad8(cout), z, (uint16_t*)zb, (uint16_t*)act, use_gelu, (int)nt);
     return launched("conv_fwd");
 }
 
 static int bwd_data(const void* dz, int n, int h, int w, int cout, const void* wb, int cin, float* dx,
                     const float* zp, const void* zpb, void* dzp, void* stream, const char* who) {
     const Img im{n, h, w};
-    const long nb = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
-    if (nb > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
-    auto* k = pad8(cout) <= 16 ? k_conv_bwd_data<16> : k_conv_bwd_data<32>;
-    k<<<(unsigned)nb, NB, 0, (hipStream_t)stream>>>((const uint16_t*)dz, im, pad8(cout), (const uint16_t*)wb,
-                                                  cin, pad8(cin), dx, zp, (const uint16_t*)zpb,
-                                                  (uint16_t*)dzp);
+    const long nt = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
+    if (nt > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
+    const bool k16 = pad8(cout) <= 16;
+    auto* k = k16 ? k_conv_bwd_data<16> : k_conv_bwd_data<32>;
+    const int g = k16 ? conv_grid<2>(k, nt) : conv_grid<3>(k, nt);
+    if (g < 1) return set_error(NEMO2D_ERR_HIP, std::string(who) + ": occupancy query failed");
+    k<<<g, NBC, 0, (hipStream_t)stream>>>((const uint16_t*)dz, im, pad8(cout), (const uint16_t*)wb, cin,
+                                         pad8(cin), dx, zp, (const uint16_t*)zpb, (uint16_t*)dzp, (int)nt);
     return launched(who);
 }
 
