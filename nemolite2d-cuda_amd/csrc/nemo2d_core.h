@@ -160,9 +160,13 @@ struct MemWin {
 #define QD_(TAG, X, D) w.template qd<TAG>((X), (D))
 
 // Division used by the point functions: IEEE for double (bitwise parity);
-// the single-precision form uses the fast reciprocal-based quotient.
+// the single-precision form multiplies by the hardware reciprocal
+// (v_rcp_f32, 1 ulp), a quotient within ~2 ulp of the FP32 one: one
+// multiply per division where the divisor is a uniform metric (the
+// reciprocal is computed once), a reciprocal and a multiply where it is a
+// per-point depth.  (A correctly rounded FP32 division is ~9 instructions.)
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
-__device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Correctly rounded x / d for a divisor d > 0 in [2^-60, 2^60] with y =
 // RN(1/d) precomputed: q0 = RN(x*y) and two FMA remainder corrections

@@ -134,7 +134,7 @@ __device__ __forceinline__ NorthG<R> north_g(const MetG<R>& m, const DepG<R, SS,
 template <class R>
 __device__ __forceinline__ R next_g(unsigned cls, R lo, R hi, R keep, R e12t, R e12x) {
     switch (cls) {
-        case NX_AVG: return R(0.5) * (e12t * lo + e12t * hi) / e12x;
+        case NX_AVG: return dv(R(0.5) * (e12t * lo + e12t * hi), e12x);  // (IEEE for double)
         case NX_HI: return hi;
         case NX_LO: return lo;
         default: return keep;

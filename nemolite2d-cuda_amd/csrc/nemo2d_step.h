@@ -211,7 +211,7 @@ __device__ __forceinline__ void cell_step_mixed(const ScalF& s, const MixedState
                     s_hi = (float)mixed_ssha(we_, fl[oe], p.rdt, tide);
                 }
                 const float s_lo = (float)s_c;
-                r = su == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12u
+                r = su == NX_AVG ? dv(0.5f * (s.e12t * s_lo + s.e12t * s_hi), s.e12u)
                                  : (su == NX_HI ? s_hi : s_lo);
             }
             a.sshn_u[o] = r;
@@ -242,7 +242,7 @@ __device__ __forceinline__ void cell_step_mixed(const ScalF& s, const MixedState
                     s_hi = (float)mixed_ssha(wn_, fl[on], p.rdt, tide);
                 }
                 const float s_lo = (float)s_c;
-                r = sv == NX_AVG ? 0.5f * (s.e12t * s_lo + s.e12t * s_hi) / s.e12v
+                r = sv == NX_AVG ? dv(0.5f * (s.e12t * s_lo + s.e12t * s_hi), s.e12v)
                                  : (sv == NX_HI ? s_hi : s_lo);
             }
             a.sshn_v[o] = r;
