@@ -1542,16 +1542,19 @@ int set_error(int code, const std::string& msg) { return fail(code, msg); }
 void set_reciprocals(Scal& s, double fric) {
     const double d[QD_N] = {s.e12t, s.e1t, s.e2t, s.e2u + s.e2u, s.e1v + s.e1v, s.e12u, s.e12v,
                             s.hu + 0.0, s.hv + 0.0, fric};
-    bool ok = true, one = true;
+    bool ok = true, one = true, one_met = true;
     for (int k = 0; k < QD_N; ++k) {
         ok = ok && d[k] >= 0x1p-60 && d[k] <= 0x1p60;
         s.rcp[k] = 1.0 / d[k];
         // |d*RN(1/d) - 1| <= 2^-54 (the FMA evaluates it to within a relative
         // 2^-53; the margin covers that): RN(x*y) is then a faithful x/d
-        one = one && fabs(fma(d[k], s.rcp[k], -1.0)) <= 0x1p-54 * (1.0 - 0x1p-40);
+        const bool q = fabs(fma(d[k], s.rcp[k], -1.0)) <= 0x1p-54 * (1.0 - 0x1p-40);
+        one = one && q;
+        if (k != QD_HU && k != QD_HV) one_met = one_met && q;
     }
     s.fast_div = ok ? 1 : 0;
     s.one_corr = ok && one ? 1 : 0;
+    s.one_corr_met = ok && one_met ? 1 : 0;
 }
 
 
@@ -2405,8 +2408,17 @@ static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>&
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
-    k_step_march_gr<R, SS, DARR, NJ, LA, DV><<<nblocks, 64 * GW, 0, m->stream>>>(
-        mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, m->rc_plain);
+    RcpG rc;
+    for (int q = 0; q < QD_N; ++q) rc.r[q] = m->scal.rcp[q];
+    // FP64 with every metric divisor passing the one-correction test: plain
+    // tiles divide by the metrics' reciprocals (the per-point depths by IEEE)
+    if (sizeof(R) == sizeof(double) && m->scal.one_corr_met)
+        k_step_march_gr<R, SS, DARR, NJ, LA, DV, sizeof(R) == sizeof(double) ? 3 : 0>
+            <<<nblocks, 64 * GW, 0, m->stream>>>(mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx,
+                                                 ntiles, m->rc_plain, rc);
+    else
+        k_step_march_gr<R, SS, DARR, NJ, LA, DV><<<nblocks, 64 * GW, 0, m->stream>>>(
+            mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, m->rc_plain, rc);
     LAUNCHCHK("step_march_gr");
     nparts = nblocks;
     return NEMO2D_OK;

@@ -86,6 +86,7 @@ struct Scal {
     double rcp[QD_N];  // RN(1/divisor) for each QD_ tag (uniform statics)
     int fast_div;      // divisors in range: divide by reciprocals
     int one_corr;      // and every |divisor * rcp - 1| <= 2^-54: one correction suffices
+    int one_corr_met;  // the same for the metric divisors alone (all but hu, hv)
 };
 
 struct StUniform {

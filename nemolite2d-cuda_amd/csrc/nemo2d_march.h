@@ -141,6 +141,32 @@ __device__ __forceinline__ R next_g(unsigned cls, R lo, R hi, R keep, R e12t, R 
     }
 }
 
+// Reciprocals RN(1/d) of the uniform-metric divisors (QD_ tags) for the
+// one-correction division of the FP64 sshn-only kernel (DM 3: every metric
+// divisor d of the grid has |d*RN(1/d) - 1| <= 2^-54; the per-point depths
+// hu, hv divide by IEEE division).
+struct RcpG {
+    double r[QD_N];
+};
+// x / d for the uniform-metric divisor d of tag T: DM 0 dv (IEEE for
+// double), DM 3 div_rcp1_acc with its guard gathered in *ck (double only).
+template <int DM, int T, class R>
+__device__ __forceinline__ R gq(const RcpG& rc, R x, R d, DivCheck* ck) {
+    if constexpr (DM == 3) return div_rcp1_acc(x, d, rc.r[T], *ck);
+    else return dv(x, d);
+}
+// next_g with the average's division by gq
+template <int DM, int T, class R>
+__device__ __forceinline__ R next_gq(const RcpG& rc, unsigned cls, R lo, R hi, R keep, R e12t, R e12x,
+                                     DivCheck* ck) {
+    switch (cls) {
+        case NX_AVG: return gq<DM, T>(rc, R(0.5) * (e12t * lo + e12t * hi), e12x, ck);
+        case NX_HI: return hi;
+        case NX_LO: return lo;
+        default: return keep;
+    }
+}
+
 template <class R, class SS, bool DARR>
 struct CtxG {
     MetG<R> m;
@@ -434,6 +460,7 @@ __device__ __forceinline__ R keep_g(R v, bool keep, const R* arr, unsigned k) {
 template <class R, class SS, bool DARR>
 struct CtxGR : CtxG<R, SS, DARR> {
     bool ucol, vcol;
+    RcpG rc;  // (DM 3)
     __device__ unsigned ucls_at(unsigned f, int r) const {
         return (ucol && r >= 1 && r <= this->L.jpj) ? sucls(f) : NX_SKIP;
     }
@@ -449,26 +476,29 @@ struct CarryGR {
     double ct, cu, cv;
 };
 
-template <class R, class SS, bool DARR, bool PLAIN>
+template <class R, class SS, bool DARR, bool PLAIN, int DM = 0>
 __device__ __forceinline__ R su_at_g(const CtxGR<R, SS, DARR>& x, const ColG<R, SS>& c, SS ss_e,
-                                     int r, unsigned k) {
+                                     int r, unsigned k, DivCheck* ck = nullptr) {
     const unsigned cls = PLAIN ? NX_AVG : x.ucls_at(c.f, r);
-    const R v = next_g<R>(cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_e), R(0), x.m.e12t, x.m.e12u);
+    const R v = next_gq<DM, QD_E12U, R>(x.rc, cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_e), R(0), x.m.e12t,
+                                        x.m.e12u, ck);
     return PLAIN ? v : keep_g(v, cls == NX_SKIP, x.n.sshn_u, k);
 }
-template <class R, class SS, bool DARR, bool PLAIN>
+template <class R, class SS, bool DARR, bool PLAIN, int DM = 0>
 __device__ __forceinline__ R sv_at_g(const CtxGR<R, SS, DARR>& x, const ColG<R, SS>& c, SS ss_n,
-                                     int r, unsigned k) {
+                                     int r, unsigned k, DivCheck* ck = nullptr) {
     const unsigned cls = PLAIN ? NX_AVG : x.vcls_at(c.f, r);
-    const R v = next_g<R>(cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_n), R(0), x.m.e12t, x.m.e12v);
+    const R v = next_gq<DM, QD_E12V, R>(x.rc, cls == NX_SKIP ? NX_AVG : cls, R(c.ss), R(ss_n), R(0), x.m.e12t,
+                                        x.m.e12v, ck);
     return PLAIN ? v : keep_g(v, cls == NX_SKIP, x.n.sshn_v, k);
 }
 
 // north terms of row c (own column; vn, sshn_v and hv one column east) with
 // row nr above
-template <class R, class SS, bool DARR>
+template <class R, class SS, bool DARR, int DM = 0>
 __device__ __forceinline__ NorthG<R> north_gc(const MetG<R>& m, const ColG<R, SS>& c, R vn_e,
-                                              R sv_e, R hv_e, const ColG<R, SS>& nr) {
+                                              R sv_e, R hv_e, const ColG<R, SS>& nr,
+                                              const RcpG* rc = nullptr, DivCheck* ck = nullptr) {
     const R h = R(0.5);
     const R hv_c = DARR ? c.hv : m.hv, ht_n = DARR ? nr.ht : m.ht;
     NorthG<R> t;
@@ -482,8 +512,13 @@ __device__ __forceinline__ NorthG<R> north_gc(const MetG<R>& m, const ColG<R, SS
     const R depn = ht_n + R(nr.ss);
     const R sv = sgn_half(vn_n);
     t.hn = ((h + sv) * c.vn + (h - sv) * nr.vn) * vn_n * depn;
-    t.dvdyn = dv(nr.vn - c.vn, m.e2t) * (ht_n + R(nr.ss));
-    t.qdudy = dv(nr.un - c.un, m.e2u + m.e2u);
+    if constexpr (DM == 3) {
+        t.dvdyn = gq<DM, QD_E2T>(*rc, nr.vn - c.vn, m.e2t, ck) * (ht_n + R(nr.ss));
+        t.qdudy = gq<DM, QD_E2U2>(*rc, nr.un - c.un, m.e2u + m.e2u, ck);
+    } else {
+        t.dvdyn = dv(nr.vn - c.vn, m.e2t) * (ht_n + R(nr.ss));
+        t.qdudy = dv(nr.un - c.un, m.e2u + m.e2u);
+    }
     return t;
 }
 
@@ -499,40 +534,53 @@ __device__ __forceinline__ R hv_of(const ColG<R, SS>& c, const ColG<R, SS>& nr) 
     return R(0.5) * (c.ht + nr.ht);
 }
 
-// One row j: A, B, C hold rows j-1, j, j+1 (C just arrived: its sshn_u is
-// formed here, and B's sshn_v); D receives row j+1+LA.
-template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false>
-__device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k,
-                                        const ColG<R, SS>& A, ColG<R, SS>& B, ColG<R, SS>& C,
-                                        ColG<R, SS>& D, int j) {
+// What row j computes: C's sshn_u and B's sshn_v (the now-state's
+// interpolations first needed here), the row's north terms, the west values
+// carried to the next row, and the new sshn, u and v.
+template <class R, class SS>
+struct RcgOut {
+    NorthG<R> nt;
+    R csu, bsv, cun_w, csu_w, chu_w;
+    SS s_c;
+    R u, v;
+};
+
+// Row j's arithmetic, no stores: A, B, C hold rows j-1, j, j+1 (C just
+// arrived; with DARR, PLAIN and DV its hu and B's hv are already formed).
+// DM 3: the uniform-metric divisions by reciprocal, their guard gathered in
+// *ck; DM 0: dv.
+template <class R, class SS, bool DARR, bool PLAIN, int DM>
+__device__ __forceinline__ RcgOut<R, SS> rcg_compute(const CtxGR<R, SS, DARR>& x, const CarryGR<R>& k,
+                                                     const ColG<R, SS>& A, const ColG<R, SS>& B,
+                                                     const ColG<R, SS>& C, int j, DivCheck* ck) {
     const Layout& L = x.L;
     const MetG<R>& m = x.m;
     const R h = R(0.5);
     const unsigned kP = (unsigned)L.pitch;
-    {
-        const int jn = j + 1 + LA <= L.jpj + 2 ? j + 1 + LA : L.jpj + 2;
-        load_colg<R, SS, DARR, PLAIN, DV>(D, x.n, x.fl, x.rowk(jn) + x.lk);
-    }
-    if constexpr (DARR && PLAIN && DV) {
-        C.hu = hu_of(C);
-        B.hv = hv_of(B, C);
-    }
     const unsigned f = B.f;
     const unsigned ko = x.rowk(j) + x.lk;  // element offset of (i, j)
-    C.su = su_at_g<R, SS, DARR, PLAIN>(x, C, lane_next(C.ss), j + 1, ko + kP);
-    B.sv = sv_at_g<R, SS, DARR, PLAIN>(x, B, C.ss, j, ko);
+    RcgOut<R, SS> o;
+    o.csu = su_at_g<R, SS, DARR, PLAIN, DM>(x, C, lane_next(C.ss), j + 1, ko + kP, ck);
+    o.bsv = sv_at_g<R, SS, DARR, PLAIN, DM>(x, B, C.ss, j, ko, ck);
+    const R csu = o.csu, bsv = o.bsv;
     // neighbours by lane shifts (every lane active here)
     const R bun_w = k.un_w, bsu_w = k.su_w;
-    const R bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
-    const R bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv);
+    const R bvn_w = lane_prev(B.vn), bsv_w = lane_prev(bsv);
+    const R bvn_e = lane_next(B.vn), bsv_e = lane_next(bsv);
     const SS bss_e = lane_next(B.ss);
-    const R cun_w = lane_prev(C.un), csu_w = lane_prev(C.su);
+    const R cun_w = lane_prev(C.un), csu_w = lane_prev(csu);
     const R hu_b = DARR ? B.hu : m.hu, hv_b = DARR ? B.hv : m.hv, ht_b = DARR ? B.ht : m.ht;
     const R hu_a = DARR ? A.hu : m.hu, hu_c = DARR ? C.hu : m.hu;
     const R bhu_w = DARR ? k.hu_w : m.hu;
     const R chu_w = DARR ? lane_prev(C.hu) : m.hu;
     const R bhv_w = DARR ? lane_prev(B.hv) : m.hv, bhv_e = DARR ? lane_next(B.hv) : m.hv;
-    const NorthG<R> nt = north_gc<R, SS, DARR>(m, B, bvn_e, bsv_e, bhv_e, C);
+    o.cun_w = cun_w;
+    o.csu_w = csu_w;
+    o.chu_w = chu_w;
+    ColG<R, SS> Bn = B;  // B with its interpolation of this row
+    Bn.sv = bsv;
+    o.nt = north_gc<R, SS, DARR, DM>(m, Bn, bvn_e, bsv_e, bhv_e, C, &x.rc, ck);
+    const NorthG<R>& nt = o.nt;
     const NorthG<R>& st = k.south;
 
     // continuity
@@ -543,9 +591,10 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
         } else {
             const R r1 = (B.su + hu_b) * B.un;
             const R r2 = (bsu_w + bhu_w) * bun_w;
-            s_c = B.ss + SS(dv((r2 - r1 + st.r3 - nt.r3) * x.p.rdt, m.e12t));
+            s_c = B.ss + SS(gq<DM, QD_E12T>(x.rc, (r2 - r1 + st.r3 - nt.r3) * x.p.rdt, m.e12t, ck));
         }
     }
+    o.s_c = s_c;
 
     // west-face terms; the east ones come from lane+1
     const R un_c = B.un;
@@ -553,23 +602,23 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
     const R depw_u = ht_b + R(B.ss);
     const R sw_u = sgn_half(u_w);
     const R fw = ((h - sw_u) * un_c + (h + sw_u) * bun_w) * u_w * depw_u;
-    const R dudx_w = dv(un_c - bun_w, m.e1t) * (ht_b + R(B.ss));
+    const R dudx_w = gq<DM, QD_E1T>(x.rc, un_c - bun_w, m.e1t, ck) * (ht_b + R(B.ss));
     const R vn_c = B.vn;
     const R u_wc = h * (bun_w + cun_w);
     const R u_wv = h * u_wc * (m.e2u + m.e2u);
     const R depw_v = h * (bhu_w + bsu_w + chu_w + csu_w);
     const R sw_v = sgn_half(u_wv);
     const R gw = ((h - sw_v) * vn_c + (h + sw_v) * bvn_w) * u_wv * depw_v;
-    const R qdvdx_w = dv(vn_c - bvn_w, m.e1v + m.e1v);
+    const R qdvdx_w = gq<DM, QD_E1V2>(x.rc, vn_c - bvn_w, m.e1v + m.e1v, ck);
     const R fe = lane_next(fw);
     const R dudx_e = lane_next(dudx_w);
     const R ge = lane_next(gw);
     const R u_ec = lane_next(u_wc);
     const R qdvdx_e = lane_next(qdvdx_w);
 
+    R u = R(0), v = R(0);
     if (x.own) {
         const int i = x.i;
-        R u = R(0), v = R(0);
         const unsigned cu = ucls(f), cv = vcls(f);
         if (PLAIN || (j >= 1 && cu == CLS_COMP)) {
             const bool ws = !PLAIN && (f & U_WALL_S) != 0, wn = !PLAIN && (f & U_WALL_N) != 0;
@@ -579,14 +628,16 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
             const R adv = fw - fe + ts - tn;
             R dudy_s = R(0), dudy_n = R(0);
             if (!ws) dudy_s = st.qdudy * (hu_b + B.su + hu_a + A.su);
-            if (!wn) dudy_n = nt.qdudy * (hu_b + B.su + hu_c + C.su);
+            if (!wn) dudy_n = nt.qdudy * (hu_b + B.su + hu_c + csu);
             R vis = (dudx_e - dudx_w) * m.e2u + (dudy_n - dudy_s) * m.e1u * h;
             vis = x.p.visc * vis;
             const R hsu = hu_b + B.su;
             const R cor = h * (m.fcu * (st.vnc + nt.vnc)) * m.e12u * hsu;
             const R hpg = -R(GRAV) * hsu * m.e2u * (R(bss_e) - R(B.ss));
-            u = dv(dv(un_c * hsu + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12u), hu_b + R(0)),
-                   x.p.fric);
+            u = gq<DM, QD_FRIC>(x.rc,
+                                dv(un_c * hsu + gq<DM, QD_E12U>(x.rc, x.p.rdt * (adv + vis + cor + hpg), m.e12u, ck),
+                                   hu_b + R(0)),
+                                x.p.fric, ck);
         }
         if (PLAIN || (i >= 1 && cv == CLS_COMP)) {
             const bool ww = !PLAIN && (f & V_WALL_W) != 0, we = !PLAIN && (f & V_WALL_E) != 0;
@@ -594,49 +645,99 @@ __device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>&
             if (ww) tw = ((h - sw_v) * vn_c) * u_wv * depw_v;
             if (we) {
                 const R u_e = h * u_ec * (m.e2u + m.e2u);
-                const R depe = h * (hu_b + B.su + hu_c + C.su);
+                const R depe = h * (hu_b + B.su + hu_c + csu);
                 te = ((h + sgn_half(u_e)) * vn_c) * u_e * depe;
             }
             const R adv = tw - te + st.hn - nt.hn;
             R dvdx_w = R(0), dvdx_e = R(0);
-            if (!ww) dvdx_w = qdvdx_w * (hv_b + B.sv + bhv_w + bsv_w);
-            if (!we) dvdx_e = qdvdx_e * (hv_b + B.sv + bhv_e + bsv_e);
+            if (!ww) dvdx_w = qdvdx_w * (hv_b + bsv + bhv_w + bsv_w);
+            if (!we) dvdx_e = qdvdx_e * (hv_b + bsv + bhv_e + bsv_e);
             R vis = (nt.dvdyn - st.dvdyn) * m.e1v + (dvdx_e - dvdx_w) * m.e2v * h;
             vis = x.p.visc * vis;
-            const R hsv = hv_b + B.sv;
+            const R hsv = hv_b + bsv;
             const R cor = -h * (m.fcv * (u_ec + u_wc)) * m.e12v * hsv;
             const R hpg = -R(GRAV) * hsv * m.e1v * (R(C.ss) - R(B.ss));
-            v = dv(dv(vn_c * hsv + dv(x.p.rdt * (adv + vis + cor + hpg), m.e12v), hv_b + R(0)),
-                   x.p.fric);
+            v = gq<DM, QD_FRIC>(x.rc,
+                                dv(vn_c * hsv + gq<DM, QD_E12V>(x.rc, x.p.rdt * (adv + vis + cor + hpg), m.e12v, ck),
+                                   hv_b + R(0)),
+                                x.p.fric, ck);
         }
+    }
+    o.u = u;
+    o.v = v;
+    return o;
+}
+
+// Row j's stores and carries.
+template <class R, class SS, bool DARR, bool PLAIN>
+__device__ __forceinline__ void rcg_commit(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k, const ColG<R, SS>& A,
+                                           ColG<R, SS>& B, ColG<R, SS>& C, const RcgOut<R, SS>& o, int j) {
+    const unsigned kP = (unsigned)x.L.pitch;
+    const unsigned f = B.f;
+    const unsigned ko = x.rowk(j) + x.lk;
+    C.su = o.csu;
+    B.sv = o.bsv;
+    if (x.own) {
+        const int i = x.i;
+        const unsigned cu = ucls(f), cv = vcls(f);
         if (PLAIN || j >= 1) {
             if (PLAIN || i >= 1) {
-                stg_(x.a.sshn, ko, s_c);
-                k.ct += fabs((double)s_c);
+                stg_(x.a.sshn, ko, o.s_c);
+                k.ct += fabs((double)o.s_c);
             }
             if (PLAIN || cu != CLS_FLATHER) {
-                stg_(x.a.un, ko, u);
-                if (PLAIN || i >= 1) k.cu += fabs((double)u);
+                stg_(x.a.un, ko, o.u);
+                if (PLAIN || i >= 1) k.cu += fabs((double)o.u);
             }
         }
         if (PLAIN || i >= 1) {
             if (PLAIN || cv != CLS_FLATHER) {
-                stg_(x.a.vn, ko, v);
-                if (PLAIN || j >= 1) k.cv += fabs((double)v);
+                stg_(x.a.vn, ko, o.v);
+                if (PLAIN || j >= 1) k.cv += fabs((double)o.v);
             }
             // bc_flather_v at a row-0 point whose inner point is this one
             // (row 0 is outside the checksums)
             if (!PLAIN && j == 1 && vcls(A.f) == CLS_FLATHER && (A.f & V_FL_NORTH))
-                stg_(x.a.vn, ko - kP, v + R(ldg_(x.flv, ko - kP)) * (A.sv - B.sv));
+                stg_(x.a.vn, ko - kP, o.v + R(ldg_(x.flv, ko - kP)) * (A.sv - B.sv));
         }
     }
-    k.south = nt;
-    k.un_w = cun_w;
-    k.su_w = csu_w;
-    k.hu_w = chu_w;
+    k.south = o.nt;
+    k.un_w = o.cun_w;
+    k.su_w = o.csu_w;
+    k.hu_w = o.chu_w;
 }
 
-template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false>
+// One row j: A, B, C hold rows j-1, j, j+1 (C just arrived: its sshn_u is
+// formed here, and B's sshn_v); D receives row j+1+LA.  DM 3 divides by the
+// metrics' reciprocals with one guard for the row; if it fails on any lane,
+// the wave does the row again with IEEE divisions.
+template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false, int DM = 0>
+__device__ __forceinline__ void rcg_row(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k,
+                                        const ColG<R, SS>& A, ColG<R, SS>& B, ColG<R, SS>& C,
+                                        ColG<R, SS>& D, int j) {
+    const Layout& L = x.L;
+    {
+        const int jn = j + 1 + LA <= L.jpj + 2 ? j + 1 + LA : L.jpj + 2;
+        load_colg<R, SS, DARR, PLAIN, DV>(D, x.n, x.fl, x.rowk(jn) + x.lk);
+    }
+    if constexpr (DARR && PLAIN && DV) {
+        C.hu = hu_of(C);
+        B.hv = hv_of(B, C);
+    }
+    if constexpr (DM == 3) {
+        DivCheck ck;
+        div_check_reset(ck);
+        RcgOut<R, SS> o = rcg_compute<R, SS, DARR, PLAIN, 3>(x, k, A, B, C, j, &ck);
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(!div_check_ok(ck)) != 0, 0))
+            o = rcg_compute<R, SS, DARR, PLAIN, 0>(x, k, A, B, C, j, nullptr);
+        rcg_commit<R, SS, DARR, PLAIN>(x, k, A, B, C, o, j);
+    } else {
+        const RcgOut<R, SS> o = rcg_compute<R, SS, DARR, PLAIN, 0>(x, k, A, B, C, j, nullptr);
+        rcg_commit<R, SS, DARR, PLAIN>(x, k, A, B, C, o, j);
+    }
+}
+
+template <class R, class SS, bool DARR, int LA, bool PLAIN, bool DV = false, int DM = 0>
 __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R>& k, int jb,
                                           int je) {
     const Layout& L = x.L;
@@ -662,25 +763,25 @@ __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R
     k.hu_w = DARR ? lane_prev(R1.hu) : x.m.hu;
     if constexpr (LA == 1) {
         for (int j = jb; j <= je; j += 4) {
-            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R0, R1, R2, R3, j);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV, DM>(x, k, R0, R1, R2, R3, j);
             if (j + 1 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R1, R2, R3, R0, j + 1);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV, DM>(x, k, R1, R2, R3, R0, j + 1);
             if (j + 2 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R2, R3, R0, R1, j + 2);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV, DM>(x, k, R2, R3, R0, R1, j + 2);
             if (j + 3 > je) break;
-            rcg_row<R, SS, DARR, 1, PLAIN, DV>(x, k, R3, R0, R1, R2, j + 3);
+            rcg_row<R, SS, DARR, 1, PLAIN, DV, DM>(x, k, R3, R0, R1, R2, j + 3);
         }
     } else {
         for (int j = jb; j <= je; j += 5) {
-            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R0, R1, R2, R4, j);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV, DM>(x, k, R0, R1, R2, R4, j);
             if (j + 1 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R1, R2, R3, R0, j + 1);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV, DM>(x, k, R1, R2, R3, R0, j + 1);
             if (j + 2 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R2, R3, R4, R1, j + 2);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV, DM>(x, k, R2, R3, R4, R1, j + 2);
             if (j + 3 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R3, R4, R0, R2, j + 3);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV, DM>(x, k, R3, R4, R0, R2, j + 3);
             if (j + 4 > je) break;
-            rcg_row<R, SS, DARR, 2, PLAIN, DV>(x, k, R4, R0, R1, R3, j + 4);
+            rcg_row<R, SS, DARR, 2, PLAIN, DV, DM>(x, k, R4, R0, R1, R3, j + 4);
         }
     }
 }
@@ -690,12 +791,13 @@ __device__ __forceinline__ void rcg_march(const CtxGR<R, SS, DARR>& x, CarryGR<R
 // for none
 // DV: the grid's hu and hv are the means of adjacent T depths (plain tiles
 // form them from ht instead of loading them).
-template <class R, class SS, bool DARR, int NJ, int LA, bool DV = false>
+// DM 3 (double): plain tiles divide by the metrics' reciprocals rc.
+template <class R, class SS, bool DARR, int NJ, int LA, bool DV = false, int DM = 0>
 __global__ void __launch_bounds__(64 * GW)
 k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
                 const uint16_t* __restrict__ fl, const double* __restrict__ flv, Layout L,
                 double tide, double* __restrict__ partials, int nbx, int ntiles,
-                const uint8_t* __restrict__ plain) {
+                const uint8_t* __restrict__ plain, RcpG rc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
     const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -712,6 +814,7 @@ k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
         x.flv = flv;
         x.L = L;
         x.tide = tide;
+        x.rc = rc;
         // lane 0 is column i0 - 1 (>= -1, inside the left padding)
         x.i = strip * GROUT + lane - 1;
         x.il = x.i < L.jpi + 1 ? x.i : L.jpi + 1;  // clamped column for loads
@@ -723,7 +826,7 @@ k_step_march_gr(MetG<R> m, ParamsT<R> p, StateG<R, SS, DARR> n, OutG<R, SS> a,
         const int jb = seg * NJ;
         const int je = jb + NJ - 1 < L.jpj ? jb + NJ - 1 : L.jpj;
         if (plain && plain[(long)strip * (ntiles / nbx) + seg])
-            rcg_march<R, SS, DARR, LA, true, DV>(x, k, jb, je);
+            rcg_march<R, SS, DARR, LA, true, DV, DM>(x, k, jb, je);
         else
             rcg_march<R, SS, DARR, LA, false>(x, k, jb, je);
     }
