@@ -16,9 +16,11 @@ Prints one JSON line (metric/unit as the benchmark definition names them):
                  80 algorithmic bytes per cell-update (5 FP64 state arrays
                  read once and written once) over its average launch time
   cpu_baseline   the reference's serial Fortran model itself (built from its
-                 sources into oracle/_ref) on one host core over a bounded
-                 sample of the same workload (steps 1..16); the C oracle (a
-                 restatement of it) where that binary is absent
+                 sources into oracle/_ref with the reference's own -Ofast
+                 optimisation; the IEEE -O2 build beside it) on one host
+                 core over a bounded sample of the same workload (steps
+                 1..16); the C oracle (a restatement of it) where neither
+                 binary is there
 """
 import argparse
 import json
@@ -37,6 +39,11 @@ BYTES_PER_CELL = 80.0
 
 
 REF_EXE = os.path.join(ROOT, "oracle", "_ref", "nemolite2d_ref")
+REF_EXE_FAST = os.path.join(ROOT, "oracle", "_ref", "nemolite2d_ref_fast")
+REF_FLAGS = {REF_EXE: "flang -O2 -ffp-contract=off: IEEE double, subnormals kept (no FTZ/DAZ), no FMA "
+                      "contraction",
+             REF_EXE_FAST: "flang -O3 -ffast-math -march=x86-64-v3 (the reference's own -Ofast): "
+                           "flush-to-zero, FMA contraction, reassociation"}
 REF_NAMELIST = """&namctl
    jpiglo = {g}
    jpjglo = {g}
@@ -60,8 +67,9 @@ def progress(msg):
 
 
 def host_facts():
-    """The host the CPU baseline ran on: CPU model, logical CPUs, the CPUs
-    this process may use, and the floating-point mode of the timed code."""
+    """The host the CPU baseline ran on: CPU model, logical CPUs and the CPUs
+    this process may use (the timed binary's floating-point mode is in its
+    "flags")."""
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -75,12 +83,11 @@ def host_facts():
         usable = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
         usable = None
-    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
-            "fp_mode": "IEEE double, no fast-math: subnormals kept (no FTZ/DAZ), no FMA contraction"}
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
-def cpu_baseline_reference(n, steps):
-    """The reference's own serial model (the binary oracle/_ref builds from
+def cpu_baseline_reference(n, steps, exe=REF_EXE):
+    """The reference's own serial model (a binary oracle/_ref builds from
     original/nemolite2d.f90) on one core: steps 1..steps of the same run from
     rest, timed by its own dl_timer 'Time-stepping' region
     (original/nemolite2d.f90:70-79).  Its unconditional istp=0 snapshot
@@ -88,7 +95,7 @@ def cpu_baseline_reference(n, steps):
     import shutil
     import subprocess
     import tempfile
-    if not os.access(REF_EXE, os.X_OK):
+    if not os.access(exe, os.X_OK):
         return None
     d = tempfile.mkdtemp(prefix="nemo2d_ref_")
     try:
@@ -97,7 +104,7 @@ def cpu_baseline_reference(n, steps):
         os.symlink(os.devnull, os.path.join(d, "go2d_00000.dat"))
         core = sorted(os.sched_getaffinity(0))[0]
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        p = subprocess.run([REF_EXE], cwd=d, env=env, capture_output=True, text=True, timeout=600,
+        p = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=600,
                            preexec_fn=lambda: os.sched_setaffinity(0, {core}))
         if p.returncode != 0:
             print("bench.py: reference binary failed (%d): %s" % (p.returncode, p.stderr[-500:]),
@@ -112,9 +119,10 @@ def cpu_baseline_reference(n, steps):
         shutil.rmtree(d, ignore_errors=True)
     return {"value": n * n * steps / t, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
             "sample": "the reference's serial model (original/nemolite2d.f90 and the dl_esm_inf/"
-                      "dl_timer sources it links, built by flang -O2 -ffp-contract=off into "
-                      "oracle/_ref), steps 1..%d of the %dx%d run from rest, its dl_timer "
-                      "'Time-stepping' region: %.2f s, one core" % (steps, n, n, t)}
+                      "dl_timer sources it links, built into oracle/_ref by %s), steps 1..%d of "
+                      "the %dx%d run from rest, its dl_timer 'Time-stepping' region: %.2f s, one "
+                      "core" % (REF_FLAGS[exe].split(":")[0], steps, n, n, t),
+            "flags": REF_FLAGS[exe]}
 
 
 def cpu_baseline(n, steps):
@@ -507,9 +515,14 @@ def main():
         out["forms"] = run_forms(args.forms_scale, keep_going=True)
     if args.cpu_steps > 0:
         progress("CPU baseline")
-        # the reference binary when it is there and runs, else the oracle port
-        out["cpu_baseline"] = (cpu_baseline_reference(n, args.cpu_steps)
-                               or cpu_baseline(n, args.cpu_steps))
+        # the reference binary built with the reference's own optimisation
+        # (-Ofast), beside the IEEE build the parity fixtures come from; the
+        # oracle port where neither is there
+        fast = cpu_baseline_reference(n, args.cpu_steps, REF_EXE_FAST)
+        ieee = cpu_baseline_reference(n, args.cpu_steps, REF_EXE)
+        out["cpu_baseline"] = fast or ieee or cpu_baseline(n, args.cpu_steps)
+        if fast and ieee:
+            out["cpu_baseline"]["ieee_build"] = {k: ieee[k] for k in ("value", "sample", "flags")}
         out["cpu_baseline"]["host"] = host_facts()
     else:
         out["cpu_baseline"] = None

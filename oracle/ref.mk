@@ -8,7 +8,11 @@
 #
 # Flags: IEEE double arithmetic (-O2, no fast-math, no FMA contraction), so
 # that the binary's results are reproducible; the reference's own gnu flags
-# are -Ofast -march=core2 (compiler_setup/gnu.sh:10-11).
+# are -Ofast -march=core2 (compiler_setup/gnu.sh:10-11).  A second build of
+# the model, nemolite2d_ref_fast (objects in $(OUT)/fast), uses flang's
+# equivalent, -O3 -ffast-math -fstack-arrays (what flang's deprecated -Ofast
+# means; its startup sets flush-to-zero), for an AVX2/FMA x86-64 target: the
+# CPU baseline as the reference's users would build it.
 REF     ?= /root/reference
 OUT     ?= oracle/_ref
 FLANG   ?= flang
@@ -30,6 +34,9 @@ TIM_FOBJ = $(addprefix $(OUT)/tim_,$(addsuffix .o,$(basename $(TIM_F))))
 TIM_COBJ = $(addprefix $(OUT)/timc_,$(addsuffix .o,$(basename $(TIM_C))))
 
 EXE = $(OUT)/nemolite2d_ref
+FAST_DIR   = $(OUT)/fast
+FAST_EXE   = $(OUT)/nemolite2d_ref_fast
+FFLAGS_FAST ?= -O3 -ffast-math -fstack-arrays -march=x86-64-v3 -fopenmp
 
 # The same model with one added statement: after each step of the time loop
 # (original/nemolite2d.f90:74-77) it prints the step's checksums
@@ -40,7 +47,7 @@ EXE = $(OUT)/nemolite2d_ref
 TRACE_SRC = $(OUT)/nemolite2d_trace.f90
 TRACE_EXE = $(OUT)/nemolite2d_ref_trace
 
-all: $(EXE) $(TRACE_EXE)
+all: $(EXE) $(TRACE_EXE) $(FAST_EXE)
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -66,6 +73,16 @@ $(TRACE_EXE): $(EXE) $(ORIG)/nemolite2d.f90
 	mkdir -p $(OUT)/trace_mod
 	$(FLANG) $(FFLAGS) -module-dir $(OUT)/trace_mod -I$(OUT) -c $(TRACE_SRC) -o $(OUT)/nemolite2d_trace.o
 	$(FLANG) $(FFLAGS) -o $@ $(OUT)/nemolite2d_trace.o $(INF_OBJ) $(TIM_FOBJ) $(TIM_COBJ)
+
+$(FAST_EXE): $(EXE)
+	mkdir -p $(FAST_DIR)
+	set -e; for f in $(INF_SRC); do \
+	  $(FLANG) $(FFLAGS_FAST) -module-dir $(FAST_DIR) -I$(FAST_DIR) -c $(INF)/$$f -o $(FAST_DIR)/inf_$${f%.*}.o; done
+	set -e; for f in $(TIM_F); do \
+	  $(FLANG) $(FFLAGS_FAST) -module-dir $(FAST_DIR) -I$(FAST_DIR) -c $(TIM)/$$f -o $(FAST_DIR)/tim_$${f%.*}.o; done
+	$(FLANG) $(FFLAGS_FAST) -module-dir $(FAST_DIR) -I$(FAST_DIR) -c $(ORIG)/nemolite2d.f90 -o $(FAST_DIR)/nemolite2d.o
+	$(FLANG) $(FFLAGS_FAST) -o $@ $(FAST_DIR)/nemolite2d.o $(addprefix $(FAST_DIR)/,$(notdir $(INF_OBJ) $(TIM_FOBJ))) \
+	    $(TIM_COBJ)
 
 clean:
 	rm -rf $(OUT)
