@@ -789,9 +789,12 @@ def test_every_kernel_variant_is_bitwise_equal(variant):
     """Default grid (uniform statics: all variants apply; every divisor d
     has |d*RN(1/d) - 1| <= 2^-54, so the reciprocal divisions take one
     correction), the same basin 150 m deep (1/150 fails that test: two
-    corrections) and a general mask with uniform metrics (Flather points on
-    three edges)."""
-    for o in (Oracle(300, 70), Oracle(300, 70, dep_const=150.0),
+    corrections in the uniform kernel; the per-point-depth kernel divides
+    by the depths with IEEE division anyway), one with dx = dy = 1700 m
+    (1/1700 fails it: the per-point-depth kernel's metric divisions stay
+    IEEE) and a general mask with uniform metrics (Flather points on three
+    edges)."""
+    for o in (Oracle(300, 70), Oracle(300, 70, dep_const=150.0), Oracle(300, 70, dx=1700.0, dy=1700.0),
               random_grid_oracle(137, 45, 11, vary_metrics=False)):
         with model_like(o) as m:
             m.set_variant(variant)
