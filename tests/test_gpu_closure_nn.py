@@ -96,6 +96,25 @@ def test_gradients_match_reference_over_several_pixel_chunks():
         assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
 
 
+def test_forward_and_gradients_with_several_tiles_per_block():
+    """6 images of 130 x 260 pixels: 918 tiles of 8 x 32, more than the
+    convolution kernels' resident blocks (512 for 32 input channels, 768
+    for the first layer's 16), so each block walks several tiles through
+    its two LDS buffers; tiles cut by both image edges."""
+    net, params, x, t, xg, tg, (n, h, w) = setup(7, n=6, h=130, w=260)
+    y = net.forward(xg, n, h, w).download()[:, :3].reshape(n, h, w, 3)
+    ref, _ = cr.forward(params, x)
+    assert relerr(y, ref) < 2e-2, relerr(y, ref)
+    wch = net.upload(np.ones(3))
+    loss = float(net.backward(tg, wch))
+    gw, gb = net.grads()
+    ref_loss, ref_grads = cr.loss_and_grads(params, x, t, np.ones(3))
+    assert abs(loss - ref_loss) <= 3e-2 * ref_loss
+    for k in range(4):
+        assert relerr(gw[k], ref_grads[k][0]) < 5e-2, (k, relerr(gw[k], ref_grads[k][0]))
+        assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
+
+
 @pytest.mark.parametrize("fp32_preact", [False, True])
 def test_gradients_with_either_preactivation_storage(fp32_preact):
     """The GELU backward reads the layer below's pre-activation stored in
