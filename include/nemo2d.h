@@ -325,12 +325,15 @@ int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
  * with its east face U(i, j) and north face V(i, j).
  * features: x (device, BF16 bits [jpj*jpi][8]) gets, per pixel, BF16 of
  *   FP32((value - mean[c]) / scale[c]) for value = sshn, un, vn, ht of the
- *   cell (c = 0..3); channels 4..7 are 0.  Synchronous.
+ *   cell (c = 0..3); channels 4..7 are 0.  Asynchronous: work queued later
+ *   on the default stream (the network) waits for x; the host does not.
  * apply: adds (double)z[p][c] * scale[c] (z device FP32 [jpj*jpi][zstride])
  *   to sshn (every cell, c = 0), un on the east faces with i < jpi (c = 1)
  *   and vn on the north faces with j < jpj (c = 2), then re-derives sshn_u
- *   and sshn_v from the new sshn by the next-stage rules.  Waits for the
- *   default stream first (where z is produced); synchronous. */
+ *   and sshn_v from the new sshn by the next-stage rules.  Asynchronous:
+ *   the model's stream waits for the work queued so far on the default
+ *   stream (where z is produced), and work queued later on the default
+ *   stream waits for the read of z; the host does not wait. */
 int nemo2d_closure_features(nemo2d_model *m, const double *mean, const double *scale, void *x);
 int nemo2d_closure_apply(nemo2d_model *m, const float *z, int zstride, const double *scale);
 /* restrict: the coarse model's now-state becomes the restriction of the fine

@@ -130,6 +130,9 @@ struct nemo2d_model {
     double* h_pinned = nullptr;  // 3 doubles of pinned host memory
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // order the closure's hand-overs between this stream and the default
+    // stream (features out, increments in, increments read) without a host wait
+    hipEvent_t ev_feat = nullptr, ev_incr = nullptr, ev_read = nullptr;
     bool grid_set = false, state_set = false;
 
     // single-precision form: float state (sshn in double when compensated),
@@ -1771,7 +1774,10 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
         return bail(fail(NEMO2D_ERR_HIP, "nemo2d_create: zeroing the device arrays failed"));
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "pinned allocation failed"));
-    if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
+    if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_feat, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_incr, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_read, hipEventDisableTiming) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "event creation failed"));
     *out = m;
     return NEMO2D_OK;
@@ -2845,7 +2851,9 @@ int nemo2d_closure_features(nemo2d_model* m, const double* mean, const double* s
     WITH_STATICS(m, st, (k_closure_features<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(
                             st, s[F_SSHN], s[F_UN], s[F_VN], L, nm, (uint16_t*)x)));
     LAUNCHCHK("closure_features");
-    HIPCHK(hipStreamSynchronize(m->stream));
+    // the network reads x on the default stream: that stream waits for them
+    HIPCHK(hipEventRecord(m->ev_feat, m->stream));
+    HIPCHK(hipStreamWaitEvent(nullptr, m->ev_feat, 0));
     return NEMO2D_OK;
 }
 
@@ -2856,8 +2864,9 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
         const int frc = lz_flush(m);  // recorded stages act on the state before the increments
         if (frc) return frc;
     }
-    // the increments come from work on the default stream
-    HIPCHK(hipStreamSynchronize(nullptr));
+    // the increments come from work on the default stream: this stream waits for it
+    HIPCHK(hipEventRecord(m->ev_incr, nullptr));
+    HIPCHK(hipStreamWaitEvent(m->stream, m->ev_incr, 0));
     int grc = snap_guard(m, m->cur);
     if (grc) return grc;
     const Layout& L = m->L;
@@ -2869,7 +2878,9 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
     WITH_STATICS(m, st, (k_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
                             st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
     LAUNCHCHK("closure_interp");
-    HIPCHK(hipStreamSynchronize(m->stream));
+    // later default-stream work (the next network pass rewriting z) waits for the read of z
+    HIPCHK(hipEventRecord(m->ev_read, m->stream));
+    HIPCHK(hipStreamWaitEvent(nullptr, m->ev_read, 0));
     m->a_current = false;
     m->uv_derived = true;
     return NEMO2D_OK;
@@ -3239,6 +3250,9 @@ int nemo2d_destroy(nemo2d_model* m) {
         for (int f = 0; f < NF; ++f) dfree(m->X[k][f]);
     dfree(m->hf_t); dfree(m->hf_u); dfree(m->hf_v); dfree(m->diag);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
+    if (m->ev_feat) (void)hipEventDestroy(m->ev_feat);
+    if (m->ev_incr) (void)hipEventDestroy(m->ev_incr);
+    if (m->ev_read) (void)hipEventDestroy(m->ev_read);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->stream) (void)hipStreamDestroy(m->stream);

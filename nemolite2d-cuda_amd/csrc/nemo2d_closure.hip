@@ -3,15 +3,16 @@
 //
 // Activations are NHWC with the channel stride rounded up to 8; weights and
 // activations are BF16, accumulation is FP32 on the matrix cores
-// (v_mfma_f32_32x32x8_bf16).  Every convolution is an implicit GEMM: the
+// (v_mfma_f32_32x32x16_bf16).  Every convolution is an implicit GEMM: the
 // im2col operand is gathered on the fly from the NHWC tensor, never stored.
 //   forward       Z[p, co]  = b[co] + sum_{tap, ci} X[p + off(tap), ci] W[tap, ci, co]
 //   data grad     dX[p, ci] = sum_{tap, co} dZ[p - off(tap), co] W[tap, ci, co]
 //   weight grad   dW[tap, ci, co] = sum_p X[p + off(tap), ci] dZ[p, co]
 // One wave computes a 32x32 tile of the GEMM.  MFMA operand layout
-// (32x32x8): lane l holds A row l%32 and B column l%32, k = 4*(l/32)+0..3;
-// accumulator register r of lane l is row 8*(r/4) + 4*(l/32) + r%4, column
-// l%32.
+// (32x32x16): lane l holds A row l%32 and B column l%32 at the 8
+// consecutive k of half l/32; accumulator register r of lane l is row
+// 8*(r/4) + 4*(l/32) + r%4, column l%32.  An image's element offsets fit in
+// 32 bits (h w 32 < 2^31, checked at launch); the image's base is 64-bit.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -46,29 +47,33 @@ __device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
 
 // GELU(z) = z/2 (1 + erf(z/sqrt2)) and its derivative, with erf by
 // Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7; the values are rounded to
-// BF16, whose half ulp is 2e-3 relative): for a = |z|/sqrt2, t = 1/(1+p a),
-// 1 - erf(a) = P(t) exp(-a^2), so 1 + erf(z/sqrt2) is P e for z < 0 (no
-// cancellation) and 2 - P e for z >= 0, and the derivative's Gaussian is the
-// same e = exp(-z^2/2).
+// BF16, whose half ulp is 2e-3 relative): for a = |z|/sqrt2 and t = 1/(1 +
+// p a), 1 - erf(a) = t P(t) exp(-a^2).  With E = exp(-z^2/2)/2 and H = t P(t) E
+// that gives, without cancellation on either side of 0,
+//   GELU(z)  = max(z, 0) - |z| H
+//   GELU'(z) = (z >= 0 ? 1 - H : H) + 2 z E / sqrt(2 pi)
+// and E is one v_exp_f32: 2^(-z^2 log2(e)/2 - 1).
 struct GeluParts {
-    float ope;  // 1 + erf(z / sqrt2)
-    float e;    // exp(-z^2 / 2)
+    float tp;  // t P(t)
+    float e;   // exp(-z^2/2) / 2
 };
 __device__ __forceinline__ GeluParts gelu_parts(float z) {
-    const float a = fabsf(z) * 0.70710678118654752f;
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
-    const float e = __expf(-0.5f * z * z);
+    const float t = __builtin_amdgcn_rcpf(fmaf(fabsf(z), 0.3275911f * 0.70710678118654752f, 1.0f));
     float p = fmaf(1.061405429f, t, -1.453152027f);
     p = fmaf(p, t, 1.421413741f);
     p = fmaf(p, t, -0.284496736f);
     p = fmaf(p, t, 0.254829592f);
-    const float h = p * t * e;
-    return GeluParts{z >= 0.0f ? 2.0f - h : h, e};
+    const float e = __builtin_amdgcn_exp2f(fmaf(z * -0.72134752044448170f, z, -1.0f));
+    return GeluParts{p * t, e};
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * gelu_parts(z).ope; }
+__device__ __forceinline__ float gelu(float z) {
+    const GeluParts g = gelu_parts(z);
+    return fmaf(-fabsf(z) * g.e, g.tp, fmaxf(z, 0.0f));
+}
 __device__ __forceinline__ float gelu_grad(float z) {
     const GeluParts g = gelu_parts(z);
-    return fmaf(z * 0.39894228040143268f, g.e, 0.5f * g.ope);
+    const float h = g.tp * g.e;
+    return fmaf(z * 0.79788456080286536f, g.e, z >= 0.0f ? 1.0f - h : h);
 }
 
 struct Img {
@@ -130,6 +135,7 @@ template <int KT>
 __device__ __forceinline__ void tile_load(const uint16_t* __restrict__ src, const Img& im, int cp,
                                           const TilePos& tp, TileRegs<KT>& r) {
     constexpr int CH = TileRegs<KT>::CH;
+    const uint16_t* img = src + (long)tp.img * im.h * im.w * cp;
 #pragma unroll
     for (int s = 0; s < TileRegs<KT>::IT; ++s) {
         const int e = threadIdx.x + s * NBC;
@@ -137,7 +143,7 @@ __device__ __forceinline__ void tile_load(const uint16_t* __restrict__ src, cons
         const int yy = tp.y0 - 1 + q / (CT_W + 2), xx = tp.x0 - 1 + q % (CT_W + 2);
         r.v[s] = make_uint4(0, 0, 0, 0);
         if (e < CT_PIX * CH && c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
-            r.v[s] = *(const uint4*)(src + (((long)tp.img * im.h + yy) * im.w + xx) * cp + c8);
+            r.v[s] = *(const uint4*)(img + (unsigned)((yy * im.w + xx) * cp + c8));
     }
 }
 template <int KT>
@@ -252,14 +258,17 @@ template <int KT>
 constexpr int conv_lds_elems() { return (2 * CT_PIX + 9 * 32) * (KT + 8); }
 
 // Forward: z = b + sum_tap sum_ci X[p + off(tap)][ci] W[tap][ci][co];
-// act = gelu(z) (or z).  wt: [9][32][cinp] (cout-major, from pack).  The
-// epilogue: lane (c, half) of wave w holds, for row w of the tile, channels
-// co0 .. co0+3 (co0 = 8g + 4 half) of pixel c in accumulator registers
-// 4g .. 4g+3; padded channels co >= cout are zero.
+// act = gelu(z) (or z with GELU false).  wt: [9][32][cinp] (cout-major,
+// from pack).  The epilogue: lane (c, half) of wave w holds, for row w of
+// the tile, channels co0 .. co0+3 (co0 = 8g + 4 half) of pixel c in
+// accumulator registers 4g .. 4g+3.  The packed weights and the bias of
+// channels co >= cout are zero, so those accumulators, z, and GELU(z) are
+// zero too (for finite inputs) and need no masking.
+template <bool GELU>
 struct FwdEpi {
     Img im;
     const float* bias;
-    int cout, coutp, use_gelu;
+    int cout, coutp;
     float* z;
     uint16_t* zb;
     uint16_t* act;
@@ -283,12 +292,11 @@ struct FwdEpi {
         tp = t;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const int co0 = 8 * g + 4 * half;
             float v[4], va[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                v[r] = co0 + r < cout ? acc[4 * g + r] + bv[g][r] : 0.f;
-                va[r] = use_gelu ? gelu(v[r]) : v[r];  // GELU(0) = 0
+                v[r] = acc[4 * g + r] + bv[g][r];
+                va[r] = GELU ? gelu(v[r]) : v[r];
             }
             oz[g] = make_float4(v[0], v[1], v[2], v[3]);
             ob[g] = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
@@ -298,38 +306,39 @@ struct FwdEpi {
     __device__ __forceinline__ void store() const {
         const int xx = tp.x0 + c, y = tp.y0 + wave;  // this lane's pixel
         if (xx >= im.w || y >= im.h) return;
-        const long q0 = (((long)tp.img * im.h + y) * im.w + xx) * coutp;
+        const long ib = (long)tp.img * im.h * im.w * coutp;  // the image (uniform)
+        const unsigned q0 = (unsigned)((y * im.w + xx) * coutp);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int co0 = 8 * g + 4 * half;
             if (co0 >= coutp) break;
+            const unsigned q = q0 + co0;
             if (z) {
                 if (co0 + 4 <= cout) {
-                    *(float4*)(z + q0 + co0) = oz[g];
+                    *(float4*)(z + ib + q) = oz[g];
                 } else {
                     const float v[4] = {oz[g].x, oz[g].y, oz[g].z, oz[g].w};
-                    for (int r = 0; r < 4 && co0 + r < cout; ++r) z[q0 + co0 + r] = v[r];
+                    for (int r = 0; r < 4 && co0 + r < cout; ++r) z[ib + q + r] = v[r];
                 }
             }
-            if (zb) *(uint2*)(zb + q0 + co0) = ob[g];
-            if (act) *(uint2*)(act + q0 + co0) = oa[g];
+            if (zb) *(uint2*)(zb + ib + q) = ob[g];
+            if (act) *(uint2*)(act + ib + q) = oa[g];
         }
     }
 };
 
-template <int KT>
+template <int KT, bool GELU>
 __global__ void __launch_bounds__(NBC, 4)  // 4 waves per SIMD: two blocks per CU
 k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __restrict__ wt,
            const float* __restrict__ bias, int cout, int coutp, float* __restrict__ z,
-           uint16_t* __restrict__ zb, uint16_t* __restrict__ act, int use_gelu, int ntiles) {
+           uint16_t* __restrict__ zb, uint16_t* __restrict__ act, int ntiles) {
     __shared__ __attribute__((aligned(16))) uint16_t lds[conv_lds_elems<KT>()];
     const int lane = threadIdx.x & 63;
-    FwdEpi epi;
+    FwdEpi<GELU> epi;
     epi.im = im;
     epi.bias = bias;
     epi.cout = cout;
     epi.coutp = coutp;
-    epi.use_gelu = use_gelu;
     epi.z = z;
     epi.zb = zb;
     epi.act = act;
@@ -340,9 +349,13 @@ k_conv_fwd(const uint16_t* __restrict__ x, Img im, int cinp, const uint16_t* __r
 }
 
 // Data gradient: dX[p][ci] = sum_tap sum_co dZ[p - off(tap)][co] W[tap][ci][co];
-// with a pre-activation of the layer below (zp FP32 or zpb BF16) the output
-// is that layer's dz = BF16(dX * GELU'(z)) (GELU backward fused), else FP32
-// dX.  wt: [9][32][coutp] (cin-major, from pack).
+// with GL, the pre-activation of the layer below (zp FP32, or zpb BF16 with
+// ZB) makes the output that layer's dz = BF16(dX * GELU'(z)) (GELU backward
+// fused), else FP32 dX.  wt: [9][32][coutp] (cin-major, from pack).  The
+// packed weights of channels ci >= cin are zero, so those accumulators are
+// zero; their pre-activations are read as zero (GELU'(0) = 1/2), so the
+// padded channels of dz are zero whatever the layer below's buffer holds.
+template <bool GL, bool ZB>
 struct BwdEpi {
     Img im;
     int cin, cinp;
@@ -355,70 +368,80 @@ struct BwdEpi {
     uint2 od[4];
     float4 ox[4];
     float4 zr[4];  // the layer below's pre-activations at this lane's pixel (prefetched)
-    __device__ __forceinline__ long at(const TilePos& p) const {
+    // this lane's pixel of tile p: false past the image; else its offset
+    // within the image (q) and the image's (ib, uniform)
+    __device__ __forceinline__ bool at(const TilePos& p, long& ib, unsigned& q) const {
         const int xx = p.x0 + c, y = p.y0 + wave;
-        return (xx < im.w && y < im.h) ? (((long)p.img * im.h + y) * im.w + xx) * cinp : -1;
+        ib = (long)p.img * im.h * im.w * cinp;
+        q = (unsigned)((y * im.w + xx) * cinp);
+        return xx < im.w && y < im.h;
     }
     __device__ __forceinline__ void init() {}
     __device__ __forceinline__ void prefetch(const TilePos& p) {
-        if (!(zp || zpb)) return;
-        const long q0 = at(p);
+        if (!GL) return;
+        long ib;
+        unsigned q0;
+        const bool in = at(p, ib, q0);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int ci0 = 8 * g + 4 * half;
             zr[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (q0 >= 0 && ci0 < cinp) {
-                if (zpb) {
-                    const uint2 u = *(const uint2*)(zpb + q0 + ci0);
+            if (in && ci0 < cinp) {
+                if (ZB) {
+                    const uint2 u = *(const uint2*)(zpb + ib + q0 + ci0);
                     zr[g] = make_float4(bf2f(u.x & 0xffff), bf2f(u.x >> 16), bf2f(u.y & 0xffff), bf2f(u.y >> 16));
                 } else {
-                    zr[g] = *(const float4*)(zp + q0 + ci0);
+                    zr[g] = *(const float4*)(zp + ib + q0 + ci0);
+                }
+                if (ci0 + 4 > cin) {  // a group across cin (none when cin = cinp)
+                    zr[g].y = ci0 + 1 < cin ? zr[g].y : 0.f;
+                    zr[g].z = ci0 + 2 < cin ? zr[g].z : 0.f;
+                    zr[g].w = 0.f;
+                    if (ci0 >= cin) zr[g].x = 0.f;
                 }
             }
         }
     }
     __device__ __forceinline__ void compute(const TilePos& t, const f32x16& acc) {
         tp = t;
-        const bool gl = zp || zpb;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const int ci0 = 8 * g + 4 * half;
             ox[g] = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
-            if (gl) {
+            if (GL) {
                 const float zv[4] = {zr[g].x, zr[g].y, zr[g].z, zr[g].w};
                 float o[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) o[r] = ci0 + r < cin ? acc[4 * g + r] * gelu_grad(zv[r]) : 0.f;
+                for (int r = 0; r < 4; ++r) o[r] = acc[4 * g + r] * gelu_grad(zv[r]);
                 od[g] = make_uint2(pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]));
             }
         }
     }
     __device__ __forceinline__ void store() const {
-        const long q0 = at(tp);
-        if (q0 < 0) return;
-        const bool gl = zp || zpb;
+        long ib;
+        unsigned q0;
+        if (!at(tp, ib, q0)) return;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int ci0 = 8 * g + 4 * half;
             if (ci0 >= cinp) break;
-            if (gl) {
-                *(uint2*)(dzp + q0 + ci0) = od[g];
+            if (GL) {
+                *(uint2*)(dzp + ib + q0 + ci0) = od[g];
             } else {
                 const float v[4] = {ox[g].x, ox[g].y, ox[g].z, ox[g].w};
-                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[q0 + ci0 + r] = v[r];
+                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[ib + q0 + ci0 + r] = v[r];
             }
         }
     }
 };
 
-template <int KT>
+template <int KT, bool GL, bool ZB>
 __global__ void __launch_bounds__(NBC, 4)  // 4 waves per SIMD: two blocks per CU
 k_conv_bwd_data(const uint16_t* __restrict__ dz, Img im, int coutp, const uint16_t* __restrict__ wt,
                 int cin, int cinp, float* __restrict__ dx, const float* __restrict__ zp,
                 const uint16_t* __restrict__ zpb, uint16_t* __restrict__ dzp, int ntiles) {
     __shared__ __attribute__((aligned(16))) uint16_t lds[conv_lds_elems<KT>()];
     const int lane = threadIdx.x & 63;
-    BwdEpi epi;
+    BwdEpi<GL, ZB> epi;
     epi.im = im;
     epi.cin = cin;
     epi.cinp = cinp;
@@ -455,10 +478,21 @@ struct WgItems {
     uint2 zv[4];     // 1 dZ item, 4 images
 };
 
-__device__ __forceinline__ uint2 ld_px(const uint16_t* __restrict__ src, const Img& im, int cp, int img,
-                                      int y, int x, int c4) {
-    if (img >= im.n || y < 0 || y >= im.h || x < 0 || x >= im.w || c4 * 4 >= cp) return make_uint2(0, 0);
-    return *(const uint2*)(src + (((long)img * im.h + y) * im.w + x) * cp + c4 * 4);
+// One item: channels 4 c4 .. 4 c4 + 3 of pixel (y, x) in the 4 images of
+// group g (zero outside the image, past the channels or past the batch).
+// The pixel's offset within an image is one 32-bit value for all 4 images;
+// the images' bases are uniform.
+__device__ __forceinline__ void ld_px4(const uint16_t* __restrict__ src, const Img& im, int cp, int g, bool ok,
+                                       int y, int x, int c4, uint2* v) {
+    const bool in = ok && y >= 0 && y < im.h && x >= 0 && x < im.w && c4 * 4 < cp;
+    const long is = (long)im.h * im.w * cp;
+    const uint16_t* b = src + 4L * g * is;
+    const unsigned q = (unsigned)((y * im.w + x) * cp + c4 * 4);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        v[n] = make_uint2(0, 0);
+        if (in && 4 * g + n < im.n) v[n] = *(const uint2*)(b + n * is + q);
+    }
 }
 
 __device__ __forceinline__ void wg_load(const uint16_t* __restrict__ x, int cinp, const uint16_t* __restrict__ dz,
@@ -468,18 +502,14 @@ __device__ __forceinline__ void wg_load(const uint16_t* __restrict__ x, int cinp
     for (int s = 0; s < 2; ++s) {
         const int e = threadIdx.x + s * WG_NT;
         const int q = e / cq, c4 = e % cq;
-        const bool ok = q < WG_XP;
         const int yy = y0 - 1 + q / (WG_W + 2), xx = x0 - 1 + q % (WG_W + 2);
-#pragma unroll
-        for (int n = 0; n < 4; ++n) it.xv[s][n] = ok ? ld_px(x, im, cinp, 4 * g + n, yy, xx, c4) : make_uint2(0, 0);
+        ld_px4(x, im, cinp, g, q < WG_XP, yy, xx, c4, it.xv[s]);
     }
     {
         const int e = threadIdx.x;
         const int q = e / zq, c4 = e % zq;
-        const bool ok = q < WG_H * WG_W;
         const int yy = y0 + q / WG_W, xx = x0 + q % WG_W;
-#pragma unroll
-        for (int n = 0; n < 4; ++n) it.zv[n] = ok ? ld_px(dz, im, coutp, 4 * g + n, yy, xx, c4) : make_uint2(0, 0);
+        ld_px4(dz, im, coutp, g, q < WG_H * WG_W, yy, xx, c4, it.zv);
     }
 }
 
@@ -715,6 +745,8 @@ int conv_grid(K kern, long ntiles) {
 
 bool bad_ch(int c) { return c < 1 || c > 32; }
 int pad8(int c) { return (c + 7) / 8 * 8; }
+// An image's element offsets (32 channels at most) in 32 bits.
+bool too_big(int h, int w) { return (long)h * w * 32 > 0x7fffffffL; }
 
 }  // namespace
 
@@ -727,13 +759,21 @@ int nemo2d_nn_conv_fwd(const void* x, int n, int h, int w, int cin, const void* 
         return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: bad argument");
     const Img im{n, h, w};
     const long nt = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
-    if (nt > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: too many pixels");
+    if (nt > 0x7fffffffL || too_big(h, w))
+        return set_error(NEMO2D_ERR_ARG, "nemo2d_nn_conv_fwd: too many pixels");
     const bool k16 = pad8(cin) <= 16;
-    auto* k = k16 ? k_conv_fwd<16> : k_conv_fwd<32>;
-    const int g = k16 ? conv_grid<0>(k, nt) : conv_grid<1>(k, nt);
+    int g = 0;
+    auto* k = k_conv_fwd<32, true>;
+    if (use_gelu) {
+        k = k16 ? k_conv_fwd<16, true> : k_conv_fwd<32, true>;
+        g = k16 ? conv_grid<0>(k, nt) : conv_grid<1>(k, nt);
+    } else {
+        k = k16 ? k_conv_fwd<16, false> : k_conv_fwd<32, false>;
+        g = k16 ? conv_grid<2>(k, nt) : conv_grid<3>(k, nt);
+    }
     if (g < 1) return set_error(NEMO2D_ERR_HIP, "nemo2d_nn_conv_fwd: occupancy query failed");
     k<<<g, NBC, 0, (hipStream_t)stream>>>((const uint16_t*)x, im, pad8(cin), (const uint16_t*)wf, bias, cout,
-                                         pad8(cout), z, (uint16_t*)zb, (uint16_t*)act, use_gelu, (int)nt);
+                                         pad8(cout), z, (uint16_t*)zb, (uint16_t*)act, (int)nt);
     return launched("conv_fwd");
 }
 
@@ -741,10 +781,21 @@ static int bwd_data(const void* dz, int n, int h, int w, int cout, const void* w
                     const float* zp, const void* zpb, void* dzp, void* stream, const char* who) {
     const Img im{n, h, w};
     const long nt = (long)n * ((h + CT_H - 1) / CT_H) * ((w + CT_W - 1) / CT_W);
-    if (nt > 0x7fffffffL) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
+    if (nt > 0x7fffffffL || too_big(h, w))
+        return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
     const bool k16 = pad8(cout) <= 16;
-    auto* k = k16 ? k_conv_bwd_data<16> : k_conv_bwd_data<32>;
-    const int g = k16 ? conv_grid<2>(k, nt) : conv_grid<3>(k, nt);
+    int g = 0;
+    auto* k = k_conv_bwd_data<32, false, false>;
+    if (zpb) {
+        k = k16 ? k_conv_bwd_data<16, true, true> : k_conv_bwd_data<32, true, true>;
+        g = k16 ? conv_grid<4>(k, nt) : conv_grid<5>(k, nt);
+    } else if (zp) {
+        k = k16 ? k_conv_bwd_data<16, true, false> : k_conv_bwd_data<32, true, false>;
+        g = k16 ? conv_grid<6>(k, nt) : conv_grid<7>(k, nt);
+    } else {
+        k = k16 ? k_conv_bwd_data<16, false, false> : k_conv_bwd_data<32, false, false>;
+        g = k16 ? conv_grid<8>(k, nt) : conv_grid<9>(k, nt);
+    }
     if (g < 1) return set_error(NEMO2D_ERR_HIP, std::string(who) + ": occupancy query failed");
     k<<<g, NBC, 0, (hipStream_t)stream>>>((const uint16_t*)dz, im, pad8(cout), (const uint16_t*)wb, cin,
                                          pad8(cin), dx, zp, (const uint16_t*)zpb, (uint16_t*)dzp, (int)nt);
@@ -780,6 +831,7 @@ long nemo2d_nn_wgrad_work_floats(long pixels) {
 
 static int bwd_weight(const void* x, const void* dz, int n, int h, int w, int cin, int cout, float* dw,
                       float* db, float* work, void* stream, const char* who) {
+    if (too_big(h, w)) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
     const Img im{n, h, w};
     const int nblk = wgrad_blocks(n, h, w);
     hipStream_t s = (hipStream_t)stream;

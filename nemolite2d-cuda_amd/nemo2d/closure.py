@@ -202,9 +202,11 @@ class ConvNet:
         for k, v in d.items():
             setattr(self, k, v)
 
-    def forward(self, x_bf16, n, h, w):
+    def forward(self, x_bf16, n, h, w, train=True):
         """x_bf16: device [n*h*w, 8] BF16 bits (channels 0..3 used).  Returns
-        the FP32 output, a device [P, 8] array (channels 0..2)."""
+        the FP32 output, a device [P, 8] array (channels 0..2).  train=False
+        (inference) writes no hidden pre-activations: they are read only by
+        backward."""
         self._buffers(n, h, w)
         self.x = x_bf16
         a = x_bf16
@@ -212,9 +214,10 @@ class ConvNet:
         for k, (cin, cout, g) in enumerate(self.layers):
             last = k == nl - 1
             f32 = self.z[k].dtype == np.float32
+            keep = train or last
             _check(self.L.nemo2d_nn_conv_fwd(a.ptr, n, h, w, cin, self.wf[k].ptr, self.b[k].fptr(),
-                                             cout, self.z[k].fptr() if f32 else None,
-                                             None if f32 else self.z[k].ptr,
+                                             cout, self.z[k].fptr() if f32 and keep else None,
+                                             None if f32 or not keep else self.z[k].ptr,
                                              None if last else self.act[k].ptr, 1 if g else 0,
                                              self.STREAM))
             a = self.act[k]

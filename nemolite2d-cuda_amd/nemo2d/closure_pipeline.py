@@ -221,7 +221,7 @@ class Closure:
 
     def predict(self, X):
         n, h, w, _ = X.shape
-        y = self.net.forward(self._x(X), n, h, w)
+        y = self.net.forward(self._x(X), n, h, w, train=False)
         return y.download()[:, :3].reshape(n, h, w, 3) * self.ys
 
     def correct_model(self, model):
@@ -232,7 +232,7 @@ class Closure:
         if getattr(self, "_feat", None) is None or self._feat.shape[0] != P:
             self._feat = DevArray((P, 8), np.int16)
         model.closure_features(self.xm, self.xs, self._feat)
-        z = self.net.forward(self._feat, 1, model.jpj, model.jpi)
+        z = self.net.forward(self._feat, 1, model.jpj, model.jpi, train=False)
         model.closure_apply(z, self.ys * self.gain)
 
     def correct(self, state, grid):

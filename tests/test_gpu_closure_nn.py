@@ -47,6 +47,15 @@ def test_forward_matches_reference():
     assert relerr(y, ref) < 2e-2, relerr(y, ref)
 
 
+def test_inference_forward_equals_training_forward():
+    """forward(train=False) skips the hidden pre-activation stores only: the
+    output is bitwise the training forward's."""
+    net, params, x, t, xg, tg, (n, h, w) = setup(4, n=3, h=29, w=45)
+    y_train = net.forward(xg, n, h, w).download()
+    y_inf = net.forward(xg, n, h, w, train=False).download()
+    assert np.array_equal(y_train, y_inf)
+
+
 def test_first_layer_exact_accumulation():
     """One convolution with BF16 operands: FP32 accumulation of exactly
     representable products; agrees with FP64 to FP32 rounding."""
@@ -133,6 +142,48 @@ def test_gradients_with_either_preactivation_storage(fp32_preact):
     for k in range(4):
         assert relerr(gw[k], ref_grads[k][0]) < 5e-2, (k, relerr(gw[k], ref_grads[k][0]))
         assert relerr(gb[k], ref_grads[k][1]) < 5e-2, (k, relerr(gb[k], ref_grads[k][1]))
+
+
+def bf16_bits_to_f64(a):
+    return (np.asarray(a).astype(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def test_gelu_and_its_derivative_against_erf():
+    """The kernels' GELU (forward epilogue) and GELU' (nemo2d_nn_gelu_bwd)
+    over pre-activations in [-8, 8] against z/2 (1 + erf(z/sqrt2)) and
+    Phi(z) + z phi(z) in FP64: within BF16 rounding (2^-8 relative, one
+    rounding and the FP32 arithmetic) plus 1e-6 absolute (the erf
+    approximation's 1.5e-7 times |z|/2)."""
+    from math import erf, exp, pi, sqrt
+    from nemo2d import closure as C
+    rng = np.random.default_rng(11)
+    n, h, w, cin, cout = 1, 16, 40, 4, 32
+    net = C.ConvNet(seed=11, fp32_preact=True)
+    wgt = np.zeros((9, cin, cout))
+    wgt[4, 0, :] = 1.0  # z[p][co] = x[p][0] + b[co]
+    bias = np.arange(cout) * 0.375 - 6.0
+    net.set_weights([wgt] + [np.zeros((9, a, b)) for a, b, _ in C.LAYERS[1:]],
+                    [bias] + [np.zeros(b) for _, b, _ in C.LAYERS[1:]])
+    x0 = bf16(rng.uniform(-2.0, 2.0, (n * h * w)))
+    xp = np.zeros((n * h * w, 8), dtype=np.float32)
+    xp[:, 0] = x0
+    net.forward(net.to_bf16(xp), n, h, w)
+    z = net.z[0].download()[:, :cout].astype(np.float64)
+    assert np.array_equal(z, x0[:, None] + bias[None, :])  # exact sums
+    act = bf16_bits_to_f64(net.act[0].download()[:, :cout])
+    ref = np.vectorize(lambda v: 0.5 * v * (1.0 + erf(v / sqrt(2.0))))(z)
+    err = np.abs(act - ref) - (2.0 ** -8 * np.abs(ref) + 1e-6)
+    assert err.max() <= 0, (err.max(), z.ravel()[np.argmax(err)])
+    zf = C.DevArray.from_numpy(z.astype(np.float32).ravel())
+    one = C.DevArray.from_numpy(np.ones(z.size, dtype=np.float32))
+    dz = C.DevArray((z.size,), np.int16)
+    C._check(net.L.nemo2d_nn_gelu_bwd(one.fptr(), zf.fptr(), z.size, dz.ptr, None))
+    d = bf16_bits_to_f64(dz.download()).reshape(z.shape)
+    dref = np.vectorize(lambda v: 0.5 * (1.0 + erf(v / sqrt(2.0))) + v * exp(-0.5 * v * v) / sqrt(2 * pi))(z)
+    derr = np.abs(d - dref) - (2.0 ** -8 * np.abs(dref) + 1e-6)
+    assert derr.max() <= 0, (derr.max(), z.ravel()[np.argmax(derr)])
+    for a in (zf, one, dz):
+        a.close()
 
 
 def test_device_arrays_round_trip():
