@@ -228,10 +228,11 @@ def conv_kernel_rates(net, nc, reps=20):
     flop = CONV_FLOP_PER_PIXEL * P
     kernels = {}
     for name, t in kt.items():
-        # HBM bytes per pixel: forward reads X (64) and writes z, act (64 + 64);
-        # data gradient reads dZ, z below (64 + 64) and writes dz below (64);
-        # weight gradient reads X and dZ (64 + 64)
-        bpp = {"conv_fwd": 192, "conv_bwd_data_gelu": 192, "conv_bwd_weight": 128}[name]
+        # HBM bytes per pixel: forward reads X (64) and writes z, act (64 + 64)
+        # (in inference act only); data gradient reads dZ, z below (64 + 64)
+        # and writes dz below (64); weight gradient reads X and dZ (64 + 64)
+        bpp = {"conv_fwd": 192, "conv_fwd_inference": 128, "conv_bwd_data_gelu": 192,
+               "conv_bwd_weight": 128}[name]
         kernels[name] = {"us": t * 1e6, "tflops": flop / t / 1e12, "frac_of_bf16_mfma_peak": flop / t / MFMA_BF16_PEAK,
                          "hbm_gbs": bpp * P / t / 1e9, "frac_of_hbm": bpp * P / t / 1e9 / HBM_PEAK_GBS}
     return kernels

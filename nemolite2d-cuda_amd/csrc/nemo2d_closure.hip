@@ -12,7 +12,7 @@
 // (32x32x16): lane l holds A row l%32 and B column l%32 at the 8
 // consecutive k of half l/32; accumulator register r of lane l is row
 // 8*(r/4) + 4*(l/32) + r%4, column l%32.  An image's element offsets fit in
-// 32 bits (h w 32 < 2^31, checked at launch); the image's base is 64-bit.
+// 32 bits (h w <= 2^24 pixels, checked at launch); the image's base is 64-bit.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -113,6 +113,50 @@ __device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
     return r;
 }
 
+// Raw buffer loads within one image (base, bytes <= 2^31): an offset at or
+// past `bytes` reads as zero by the hardware's range check (0x00020000: the
+// gfx9 raw-buffer word 3), so lanes outside the image or past the channels
+// load without a branch, and as many loads are in flight on every path: the
+// waits can count them.
+constexpr unsigned OOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t img_rsrc(const void* base, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint4 ld16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ uint2 ld8(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// The accumulator layout gives lane (c, half) channels 8g + 4 half .. +3
+// of pixel c for g = 0..3: four 8-byte pieces in BF16.  One
+// v_permlane32_swap_b32 per dword exchanges lane c's pieces of group 2p + 1
+// with lane c + 32's of group 2p (it swaps the upper 32 lanes of its first
+// operand with the lower 32 lanes of its second), after which lane (c, half)
+// holds the 16 contiguous bytes of channels 8 (2p + half) .. +7: stores of
+// 16 bytes, half as many.  The same exchange maps loaded 16-byte pieces back
+// to the accumulator layout.  All 64 lanes must be active.
+__device__ __forceinline__ void pair_swap(uint32_t& a, uint32_t& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+}
+// lo, hi: groups 2p, 2p + 1 in the accumulator layout -> channels 8 (2p + half) .. +7
+__device__ __forceinline__ uint4 to_runs(uint2 lo, uint2 hi) {
+    pair_swap(lo.x, hi.x);
+    pair_swap(lo.y, hi.y);
+    return make_uint4(lo.x, lo.y, hi.x, hi.y);
+}
+// the inverse: channels 8 (2p + half) .. +7 -> groups 2p (lo), 2p + 1 (hi)
+__device__ __forceinline__ void from_runs(uint4 v, uint2& lo, uint2& hi) {
+    uint32_t a = v.x, b = v.y, c = v.z, d = v.w;
+    pair_swap(a, c);
+    pair_swap(b, d);
+    lo = make_uint2(a, b);
+    hi = make_uint2(c, d);
+}
+
 // Tile geometry of tile index t.
 struct TilePos {
     int img, y0, x0;
@@ -131,19 +175,20 @@ struct TileRegs {
     static constexpr int IT = (CT_PIX * CH + NBC - 1) / NBC;
     uint4 v[IT];
 };
+// (all zero with `any` false: no loads)
 template <int KT>
 __device__ __forceinline__ void tile_load(const uint16_t* __restrict__ src, const Img& im, int cp,
-                                          const TilePos& tp, TileRegs<KT>& r) {
+                                          const TilePos& tp, bool any, TileRegs<KT>& r) {
     constexpr int CH = TileRegs<KT>::CH;
-    const uint16_t* img = src + (long)tp.img * im.h * im.w * cp;
+    const long ie = (long)im.h * im.w * cp;  // elements of an image
+    const auto rs = img_rsrc(src + tp.img * ie, 2 * ie);
 #pragma unroll
     for (int s = 0; s < TileRegs<KT>::IT; ++s) {
         const int e = threadIdx.x + s * NBC;
         const int q = e / CH, c8 = (e % CH) * 8;
         const int yy = tp.y0 - 1 + q / (CT_W + 2), xx = tp.x0 - 1 + q % (CT_W + 2);
-        r.v[s] = make_uint4(0, 0, 0, 0);
-        if (e < CT_PIX * CH && c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w)
-            r.v[s] = *(const uint4*)(img + (unsigned)((yy * im.w + xx) * cp + c8));
+        const bool in = any && e < CT_PIX * CH && c8 < cp && yy >= 0 && yy < im.h && xx >= 0 && xx < im.w;
+        r.v[s] = ld16(rs, in ? 2u * (unsigned)((yy * im.w + xx) * cp + c8) : OOB);
     }
 }
 template <int KT>
@@ -215,13 +260,17 @@ static_assert(CT_H * 64 == NBC, "each of the 8 waves owns one row of the tile");
 // The persistent tile loop shared by the forward and data-gradient kernels:
 // epi.init() loads what every tile's epilogue needs, epi.prefetch(tile)
 // issues a tile's epilogue loads before its MFMAs, epi.compute(tile,
-// accumulators) forms its outputs in registers and epi.store() writes them.  Vector memory counters count
-// loads and stores together, in issue order, so a wait for a load also
-// waits for every store issued before it: each iteration therefore issues
-// the previous tile's stores first, then the next tile's loads, and only
-// after this tile's MFMAs and epilogue arithmetic moves the next tile from
-// registers to LDS (the one wait of the iteration, a whole compute phase
-// after those stores and loads were issued).
+// accumulators) forms its outputs in registers and epi.store() writes them.
+// Vector memory counters count loads and stores together, in issue order,
+// so a wait for a load also waits for every store issued before it.  Each
+// iteration therefore issues the previous tile's stores, this tile's
+// epilogue loads and the next tile's loads, in that order, and waits only
+// twice: in epi.compute for the epilogue loads (the next tile's still in
+// flight), and after the MFMAs and the epilogue's arithmetic for the next
+// tile, which then goes from registers to the other LDS buffer.  The
+// stores' counts are fixed (predicated, no data-dependent loops) and the
+// next tile's loads and LDS writes run on every iteration (no loads and
+// zeros after the last tile), so no wait is needed before the loads.
 template <int KT, int SIGN, class Epi>
 __device__ __forceinline__ void conv_tiles(const uint16_t* __restrict__ src, const Img& im, int cp,
                                            const uint16_t* __restrict__ wt, int ntiles, uint16_t* tiles,
@@ -231,7 +280,7 @@ __device__ __forceinline__ void conv_tiles(const uint16_t* __restrict__ src, con
     int t = blockIdx.x;
     if (t >= ntiles) return;  // (uniform per block)
     TileRegs<KT> r;
-    tile_load<KT>(src, im, cp, tile_pos(im, t), r);
+    tile_load<KT>(src, im, cp, tile_pos(im, t), true, r);
     epi.init();
     weights_store<KT>(wt, cp, wl);  // (the weights' K is the input's channels)
     tile_store<KT>(r, tiles);
@@ -239,14 +288,16 @@ __device__ __forceinline__ void conv_tiles(const uint16_t* __restrict__ src, con
     int cur = 0;
     for (bool prev = false; t < ntiles; t += gridDim.x, prev = true) {
         const int nxt = t + gridDim.x;
-        if (prev) epi.store();                                                // the previous tile's
-        if (nxt < ntiles) tile_load<KT>(src, im, cp, tile_pos(im, nxt), r);  // in flight meanwhile
+        if (prev) epi.store();           // the previous tile's
         epi.prefetch(tile_pos(im, t));  // this tile's epilogue operands, needed after the MFMAs
+        tile_load<KT>(src, im, cp, tile_pos(im, nxt < ntiles ? nxt : t), nxt < ntiles, r);  // in flight meanwhile
         f32x16 acc = {0};
         conv_row<KT, SIGN>(tiles + cur * TS, wl, wave, c, half, acc);
         epi.compute(tile_pos(im, t), acc);
-        // (that buffer's readers passed the last barrier)
-        if (nxt < ntiles) tile_store<KT>(r, tiles + (1 - cur) * TS);
+        // (keep the LDS writes, and so the wait for the next tile's loads,
+        // after the epilogue's arithmetic)
+        __builtin_amdgcn_sched_barrier(0);
+        tile_store<KT>(r, tiles + (1 - cur) * TS);  // (that buffer's readers passed the last barrier)
         __syncthreads();  // the next tile is in LDS, and no wave still reads this one
         cur ^= 1;
     }
@@ -274,7 +325,7 @@ struct FwdEpi {
     uint16_t* act;
     int wave, c, half;
     TilePos tp;
-    uint2 ob[4], oa[4];
+    uint4 ob[2], oa[2];  // BF16 z and act, channels 8 (2p + half) .. +7
     float4 oz[4];
     float bv[4][4];  // bias of channels 8g + 4 half + r (0 past cout)
     __device__ __forceinline__ void init() {
@@ -290,6 +341,7 @@ struct FwdEpi {
     __device__ __forceinline__ void prefetch(const TilePos&) {}
     __device__ __forceinline__ void compute(const TilePos& t, const f32x16& acc) {
         tp = t;
+        uint2 b[4], a[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             float v[4], va[4];
@@ -299,8 +351,20 @@ struct FwdEpi {
                 va[r] = GELU ? gelu(v[r]) : v[r];
             }
             oz[g] = make_float4(v[0], v[1], v[2], v[3]);
-            ob[g] = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
-            oa[g] = make_uint2(pk_bf16(va[0], va[1]), pk_bf16(va[2], va[3]));
+            b[g] = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
+            a[g] = make_uint2(pk_bf16(va[0], va[1]), pk_bf16(va[2], va[3]));
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            ob[p] = to_runs(b[2 * p], b[2 * p + 1]);
+            oa[p] = to_runs(a[2 * p], a[2 * p + 1]);
+        }
+        // computed here, ahead of the next tile's LDS writes and the wait for
+        // its loads, rather than sunk towards their stores
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            asm volatile("" : "+v"(ob[p].x), "+v"(ob[p].y), "+v"(ob[p].z), "+v"(ob[p].w) :: "memory");
+            asm volatile("" : "+v"(oa[p].x), "+v"(oa[p].y), "+v"(oa[p].z), "+v"(oa[p].w) :: "memory");
         }
     }
     __device__ __forceinline__ void store() const {
@@ -308,21 +372,28 @@ struct FwdEpi {
         if (xx >= im.w || y >= im.h) return;
         const long ib = (long)tp.img * im.h * im.w * coutp;  // the image (uniform)
         const unsigned q0 = (unsigned)((y * im.w + xx) * coutp);
+        if (z) {
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int co0 = 8 * g + 4 * half;
-            if (co0 >= coutp) break;
-            const unsigned q = q0 + co0;
-            if (z) {
+            for (int g = 0; g < 4; ++g) {
+                const int co0 = 8 * g + 4 * half;
+                if (co0 >= coutp) continue;  // (predicated: the loop is unrolled)
+                const unsigned q = q0 + co0;
                 if (co0 + 4 <= cout) {
                     *(float4*)(z + ib + q) = oz[g];
                 } else {
                     const float v[4] = {oz[g].x, oz[g].y, oz[g].z, oz[g].w};
-                    for (int r = 0; r < 4 && co0 + r < cout; ++r) z[ib + q + r] = v[r];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (co0 + r < cout) z[ib + q + r] = v[r];
                 }
             }
-            if (zb) *(uint2*)(zb + ib + q) = ob[g];
-            if (act) *(uint2*)(act + ib + q) = oa[g];
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int c8 = 8 * (2 * p + half);
+            if (c8 >= coutp) continue;
+            if (zb) *(uint4*)(zb + ib + q0 + c8) = ob[p];
+            if (act) *(uint4*)(act + ib + q0 + c8) = oa[p];
         }
     }
 };
@@ -365,9 +436,14 @@ struct BwdEpi {
     uint16_t* dzp;
     int wave, c, half;
     TilePos tp;
-    uint2 od[4];
+    uint4 od[2];  // BF16 dz below, channels 8 (2p + half) .. +7
     float4 ox[4];
-    float4 zr[4];  // the layer below's pre-activations at this lane's pixel (prefetched)
+    // the layer below's pre-activations at this lane's pixel, as loaded
+    // (with ZB BF16 channels 8 (2p + half) .. +7, else FP32 channels 8g + 4
+    // half .. +3; converted after the MFMAs, so that the loads' wait comes
+    // there)
+    uint4 zb8[2];
+    float4 zf4[4];
     // this lane's pixel of tile p: false past the image; else its offset
     // within the image (q) and the image's (ib, uniform)
     __device__ __forceinline__ bool at(const TilePos& p, long& ib, unsigned& q) const {
@@ -382,53 +458,87 @@ struct BwdEpi {
         long ib;
         unsigned q0;
         const bool in = at(p, ib, q0);
+        const long ie = (long)im.h * im.w * cinp;  // elements of an image
+        const auto rs = ZB ? img_rsrc(zpb + ib, 2 * ie) : img_rsrc(zp + ib, 4 * ie);
+        if (ZB) {
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int ci0 = 8 * g + 4 * half;
-            zr[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (in && ci0 < cinp) {
-                if (ZB) {
-                    const uint2 u = *(const uint2*)(zpb + ib + q0 + ci0);
-                    zr[g] = make_float4(bf2f(u.x & 0xffff), bf2f(u.x >> 16), bf2f(u.y & 0xffff), bf2f(u.y >> 16));
-                } else {
-                    zr[g] = *(const float4*)(zp + ib + q0 + ci0);
-                }
-                if (ci0 + 4 > cin) {  // a group across cin (none when cin = cinp)
-                    zr[g].y = ci0 + 1 < cin ? zr[g].y : 0.f;
-                    zr[g].z = ci0 + 2 < cin ? zr[g].z : 0.f;
-                    zr[g].w = 0.f;
-                    if (ci0 >= cin) zr[g].x = 0.f;
-                }
+            for (int p = 0; p < 2; ++p) {
+                const int c8 = 8 * (2 * p + half);
+                zb8[p] = ld16(rs, in && c8 < cinp ? 2u * (q0 + c8) : OOB);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int ci0 = 8 * g + 4 * half;
+                zf4[g] = __builtin_bit_cast(float4, ld16(rs, in && ci0 < cinp ? 4u * (q0 + ci0) : OOB));
             }
         }
     }
     __device__ __forceinline__ void compute(const TilePos& t, const f32x16& acc) {
         tp = t;
+        uint2 zb2[4], od2[4];
+        if (GL && ZB) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) from_runs(zb8[p], zb2[2 * p], zb2[2 * p + 1]);
+        }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             ox[g] = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
             if (GL) {
-                const float zv[4] = {zr[g].x, zr[g].y, zr[g].z, zr[g].w};
+                const int ci0 = 8 * g + 4 * half;
+                float zv[4];
+                if (ZB) {
+                    const uint2 u = zb2[g];
+                    zv[0] = bf2f(u.x & 0xffff); zv[1] = bf2f(u.x >> 16);
+                    zv[2] = bf2f(u.y & 0xffff); zv[3] = bf2f(u.y >> 16);
+                } else {
+                    zv[0] = zf4[g].x; zv[1] = zf4[g].y; zv[2] = zf4[g].z; zv[3] = zf4[g].w;
+                }
+                if (ci0 + 4 > cin) {  // a group across cin (none when cin = cinp)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) zv[r] = ci0 + r < cin ? zv[r] : 0.f;
+                }
                 float o[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) o[r] = acc[4 * g + r] * gelu_grad(zv[r]);
-                od[g] = make_uint2(pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]));
+                od2[g] = make_uint2(pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]));
             }
+        }
+        // computed here, ahead of the next tile's LDS writes and the wait for
+        // its loads, rather than sunk towards their stores
+        if (GL) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                od[p] = to_runs(od2[2 * p], od2[2 * p + 1]);
+                asm volatile("" : "+v"(od[p].x), "+v"(od[p].y), "+v"(od[p].z), "+v"(od[p].w) :: "memory");
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                asm volatile("" : "+v"(ox[g].x), "+v"(ox[g].y), "+v"(ox[g].z), "+v"(ox[g].w) :: "memory");
         }
     }
     __device__ __forceinline__ void store() const {
         long ib;
         unsigned q0;
         if (!at(tp, ib, q0)) return;
+        if (GL) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int c8 = 8 * (2 * p + half);
+                if (c8 < cinp) *(uint4*)(dzp + ib + q0 + c8) = od[p];
+            }
+            return;
+        }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int ci0 = 8 * g + 4 * half;
-            if (ci0 >= cinp) break;
-            if (GL) {
-                *(uint2*)(dzp + ib + q0 + ci0) = od[g];
-            } else {
+            if (ci0 >= cinp) continue;  // (predicated: the loop is unrolled)
+            {
                 const float v[4] = {ox[g].x, ox[g].y, ox[g].z, ox[g].w};
-                for (int r = 0; r < 4 && ci0 + r < cin; ++r) dx[ib + q0 + ci0 + r] = v[r];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (ci0 + r < cin) dx[ib + q0 + ci0 + r] = v[r];
             }
         }
     }
@@ -745,8 +855,9 @@ int conv_grid(K kern, long ntiles) {
 
 bool bad_ch(int c) { return c < 1 || c > 32; }
 int pad8(int c) { return (c + 7) / 8 * 8; }
-// An image's element offsets (32 channels at most) in 32 bits.
-bool too_big(int h, int w) { return (long)h * w * 32 > 0x7fffffffL; }
+// An image's bytes (32 channels at most, FP32 at most) within the 2^31 of
+// a buffer load's range: h w <= 2^24 pixels.
+bool too_big(int h, int w) { return (long)h * w > (1L << 24); }
 
 }  // namespace
 

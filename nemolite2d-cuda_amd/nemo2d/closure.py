@@ -261,16 +261,20 @@ class ConvNet:
 
     def time_kernels(self, x_bf16, n, h, w, reps=20, layer=1):
         """Seconds per launch of the convolution kernels of one 32 -> 32
-        layer (forward, data gradient with GELU backward, weight gradient
-        with the partial sum) on a batch of n images h x w, each launched reps
-        times back to back after a forward/backward pass (host clock around
-        the launches, device synchronised on both sides)."""
+        layer (forward as in training, forward as in inference (activations
+        only), data gradient with GELU backward, weight gradient with the
+        partial sum) on a batch of n images h x w, each launched reps times
+        back to back after a forward/backward pass (host clock around the
+        launches, device synchronised on both sides)."""
         import time
         self.forward(x_bf16, n, h, w)
         cin, cout, _ = self.layers[layer]
         k = layer
         fw = lambda: self.L.nemo2d_nn_conv_fwd(self.act[k - 1].ptr, n, h, w, cin, self.wf[k].ptr,  # noqa: E731
                                                self.b[k].fptr(), cout, None, self.z[k].ptr,
+                                               self.act[k].ptr, 1, self.STREAM)
+        fi = lambda: self.L.nemo2d_nn_conv_fwd(self.act[k - 1].ptr, n, h, w, cin, self.wf[k].ptr,  # noqa: E731
+                                               self.b[k].fptr(), cout, None, None,
                                                self.act[k].ptr, 1, self.STREAM)
         bd = lambda: self.L.nemo2d_nn_conv_bwd_data_gelu(self.dz[k].ptr, n, h, w, cout,  # noqa: E731
                                                          self.wb[k].ptr, cin, self.z[k - 1].ptr, 1,
@@ -279,7 +283,8 @@ class ConvNet:
                                                       cin, cout, self.gw[k].fptr(), self.gb[k].fptr(),
                                                       self.work.fptr(), self.STREAM)
         out = {}
-        for name, f in (("conv_fwd", fw), ("conv_bwd_data_gelu", bd), ("conv_bwd_weight", bw)):
+        for name, f in (("conv_fwd", fw), ("conv_fwd_inference", fi), ("conv_bwd_data_gelu", bd),
+                        ("conv_bwd_weight", bw)):
             _check(f())
             _check(self.L.nemo2d_dev_synchronize())
             t0 = time.perf_counter()
