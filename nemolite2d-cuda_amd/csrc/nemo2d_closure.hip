@@ -656,55 +656,75 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint2 a, uint2 b) {
     return r;
 }
 
+// The MFMAs of one staged tile for one wave: K steps k of pixels xq, xq + 1
+// (xq = x4 + 2 half) of row y, images 0..3.  A is X shifted by the wave's
+// tap, or ones for the bias wave; B is dZ.  (Channel planes past the
+// channels are zero, so the reads need no masks.)  Step k + 1's operands
+// are read while step k's MFMA runs.
+template <bool BIAS>
+__device__ __forceinline__ void wg_operands(const uint32_t* xs, const uint32_t* zs, int c, int half, int dy,
+                                            int dxo, int k, bf16x8& a, bf16x8& bz) {
+    constexpr int NX = WG_W / 4;
+    const int y = k / NX, xq = (k % NX) * 4 + 2 * half;
+    if (BIAS) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = (short)0x3f80;  // BF16 1.0
+    } else {
+        const uint32_t* pa = xs + c * WG_CSX + 2 * ((y + 1 + dy) * (WG_W + 2) + xq + 1 + dxo);
+        a = as_bf16x8(*(const uint2*)pa, *(const uint2*)(pa + 2));
+    }
+    const uint4 v = *(const uint4*)(zs + c * WG_CSZ + 2 * (y * WG_W + xq));
+    bz = as_bf16x8(make_uint2(v.x, v.y), make_uint2(v.z, v.w));
+}
+template <bool BIAS>
+__device__ __forceinline__ void wg_mfmas(const uint32_t* xs, const uint32_t* zs, int c, int half, int dy,
+                                         int dxo, f32x16& acc) {
+    constexpr int NK = WG_H * (WG_W / 4);
+    bf16x8 a[2], bz[2];
+    wg_operands<BIAS>(xs, zs, c, half, dy, dxo, 0, a[0], bz[0]);
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        if (k + 1 < NK) wg_operands<BIAS>(xs, zs, c, half, dy, dxo, k + 1, a[(k + 1) & 1], bz[(k + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k & 1], bz[k & 1], acc, 0, 0, 0);
+    }
+}
+
 __global__ void __launch_bounds__(WG_NT, 5)  // 5 waves per SIMD: 2 blocks per CU
 k_conv_bwd_weight(const uint16_t* __restrict__ x, int cinp, const uint16_t* __restrict__ dz,
                   int coutp, Img im, float* __restrict__ work) {
     __shared__ __attribute__((aligned(16))) uint32_t xs[32 * WG_CSX];
     __shared__ __attribute__((aligned(16))) uint32_t zs[32 * WG_CSZ];
-    const int lane = threadIdx.x & 63, tap = threadIdx.x >> 6, half = lane >> 5, c = lane & 31;
+    const int lane = threadIdx.x & 63, half = lane >> 5, c = lane & 31;
+    const int tap = threadIdx.x >> 6;
     const int dy = tap / 3 - 1, dxo = tap % 3 - 1;
     const int tx = (im.w + WG_W - 1) / WG_W, ty = (im.h + WG_H - 1) / WG_H, ng = (im.n + 3) / 4;
-    const long ntiles = (long)tx * ty * ng;
+    const int txy = tx * ty, ntiles = txy * ng;  // (< 2^31: checked at launch)
     f32x16 acc = {0};
-    bf16x8 ones;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ones[k] = (short)0x3f80;  // BF16 1.0
     const bool bias_wave = tap == 9;
+    // the channel planes past cinp / coutp stay zero: wg_store writes only
+    // the channels' planes
+    for (int e = threadIdx.x; e < 32 * WG_CSX; e += WG_NT) xs[e] = 0;
+    for (int e = threadIdx.x; e < 32 * WG_CSZ; e += WG_NT) zs[e] = 0;
     WgItems it;
-    long t = blockIdx.x;
+    int t = blockIdx.x;
     if (t < ntiles) {
-        const int g = (int)(t / ((long)tx * ty)), r = (int)(t % ((long)tx * ty));
+        const int g = t / txy, r = t % txy;
         wg_load(x, cinp, dz, coutp, im, g, r / tx * WG_H, r % tx * WG_W, it);
     }
     for (; t < ntiles; t += gridDim.x) {
-        __syncthreads();  // the previous tile's MFMAs have read the LDS
+        __syncthreads();  // the previous tile's MFMAs have read the LDS (and the zeroing is done)
         wg_store(cinp, coutp, it, xs, zs);
         __syncthreads();
-        const long tn = t + gridDim.x;
+        const int tn = t + gridDim.x;
         if (tn < ntiles) {
-            const int g = (int)(tn / ((long)tx * ty)), r = (int)(tn % ((long)tx * ty));
+            const int g = tn / txy, r = tn % txy;
             wg_load(x, cinp, dz, coutp, im, g, r / tx * WG_H, r % tx * WG_W, it);
         }
-        // K step: pixels xq, xq+1 (xq = x4 + 2*half) of row y, images 0..3
-#pragma unroll
-        for (int y = 0; y < WG_H; ++y)
-#pragma unroll
-            for (int x4 = 0; x4 < WG_W; x4 += 4) {
-                const int xq = x4 + 2 * half;
-                bf16x8 a = ones, bz = {0, 0, 0, 0, 0, 0, 0, 0};
-                if (!bias_wave) {
-                    a = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                    if (c < cinp) {
-                        const uint32_t* pa = xs + c * WG_CSX + 2 * ((y + 1 + dy) * (WG_W + 2) + xq + 1 + dxo);
-                        a = as_bf16x8(*(const uint2*)pa, *(const uint2*)(pa + 2));
-                    }
-                }
-                if (c < coutp) {
-                    const uint4 v = *(const uint4*)(zs + c * WG_CSZ + 2 * (y * WG_W + xq));
-                    bz = as_bf16x8(make_uint2(v.x, v.y), make_uint2(v.z, v.w));
-                }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bz, acc, 0, 0, 0);
-            }
+        if (bias_wave)
+            wg_mfmas<true>(xs, zs, c, half, dy, dxo, acc);
+        else
+            wg_mfmas<false>(xs, zs, c, half, dy, dxo, acc);
     }
     // register k of lane l: row ci = 8*(k/4) + 4*half + k%4, column co = c
     float* out = work + (long)blockIdx.x * WG_PART;
@@ -942,7 +962,8 @@ long nemo2d_nn_wgrad_work_floats(long pixels) {
 
 static int bwd_weight(const void* x, const void* dz, int n, int h, int w, int cin, int cout, float* dw,
                       float* db, float* work, void* stream, const char* who) {
-    if (too_big(h, w)) return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
+    if (too_big(h, w) || (long)n * h * w > 0x7fffffffL)
+        return set_error(NEMO2D_ERR_ARG, std::string(who) + ": too many pixels");
     const Img im{n, h, w};
     const int nblk = wgrad_blocks(n, h, w);
     hipStream_t s = (hipStream_t)stream;
