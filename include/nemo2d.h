@@ -325,8 +325,10 @@ int nemo2d_nn_to_bf16(const float *a, long n, void *b, void *stream);
  * with its east face U(i, j) and north face V(i, j).
  * features: x (device, BF16 bits [jpj*jpi][8]) gets, per pixel, BF16 of
  *   FP32((value - mean[c]) / scale[c]) for value = sshn, un, vn, ht of the
- *   cell (c = 0..3); channels 4..7 are 0.  Asynchronous: work queued later
- *   on the default stream (the network) waits for x; the host does not.
+ *   cell (c = 0..3); channels 4..7 are 0.  Asynchronous: the kernel writing x
+ *   waits for the work queued so far on the default stream (the network,
+ *   which may still read x), and work queued later on the default stream
+ *   waits for x; the host does not wait.
  * apply: adds (double)z[p][c] * scale[c] (z device FP32 [jpj*jpi][zstride])
  *   to sshn (every cell, c = 0), un on the east faces with i < jpi (c = 1)
  *   and vn on the north faces with j < jpj (c = 2), then re-derives sshn_u
@@ -346,7 +348,11 @@ int nemo2d_closure_apply(nemo2d_model *m, const float *z, int zstride, const dou
  *   As nemo2d_set_state, ua/va restart at zero.  Synchronous.
  * target: y (device FP32 [jpj*jpi][8]) gets, per pixel, FP32((truth - pred)
  *   / scale[c]) for the cell's sshn (c = 0), east-face un (1) and north-face
- *   vn (2), and 0 in channels 3..7 (two models of one size).  Synchronous. */
+ *   vn (2), and 0 in channels 3..7 (two models of one size).  Asynchronous,
+ *   on pred's stream: the kernel waits for truth's stream and for the work
+ *   queued so far on the default stream (which may still read y); work
+ *   queued later on the default stream waits for y, and on truth's stream
+ *   for the read of truth's state; the host does not wait. */
 int nemo2d_closure_restrict(nemo2d_model *fine, nemo2d_model *coarse);
 int nemo2d_closure_target(nemo2d_model *truth, nemo2d_model *pred, const double *scale, float *y);
 

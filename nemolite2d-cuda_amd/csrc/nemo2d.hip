@@ -131,8 +131,9 @@ struct nemo2d_model {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // order the closure's hand-overs between this stream and the default
-    // stream (features out, increments in, increments read) without a host wait
-    hipEvent_t ev_feat = nullptr, ev_incr = nullptr, ev_read = nullptr;
+    // stream (buffers free, features or targets out, increments in,
+    // increments read) without a host wait
+    hipEvent_t ev_free = nullptr, ev_feat = nullptr, ev_incr = nullptr, ev_read = nullptr, ev_tgt = nullptr;
     bool grid_set = false, state_set = false;
 
     // single-precision form: float state (sshn in double when compensated),
@@ -1775,6 +1776,8 @@ int nemo2d_create(nemo2d_model** out, int jpi, int jpj, double dx, double dy, do
     if (hipHostMalloc((void**)&m->h_pinned, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(NEMO2D_ERR_HIP, "pinned allocation failed"));
     if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_free, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_tgt, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&m->ev_feat, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&m->ev_incr, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&m->ev_read, hipEventDisableTiming) != hipSuccess)
@@ -2837,6 +2840,21 @@ int nemo2d_checksums(nemo2d_model* m, double out[3]) {
     return NEMO2D_OK;
 }
 
+// The network runs on the default stream.  A buffer it reads (features,
+// targets) is rewritten on a model's stream only after the default stream's
+// work queued so far (which may still read it): the model's stream waits.
+static int after_default_stream(nemo2d_model* m) {
+    HIPCHK(hipEventRecord(m->ev_free, nullptr));
+    HIPCHK(hipStreamWaitEvent(m->stream, m->ev_free, 0));
+    return NEMO2D_OK;
+}
+// ... and the default stream waits for what m's stream wrote.
+static int before_default_stream(nemo2d_model* m) {
+    HIPCHK(hipEventRecord(m->ev_feat, m->stream));
+    HIPCHK(hipStreamWaitEvent(nullptr, m->ev_feat, 0));
+    return NEMO2D_OK;
+}
+
 int nemo2d_closure_features(nemo2d_model* m, const double* mean, const double* scale, void* x) {
     READY_STAGE(m, "nemo2d_closure_features");
     if (!mean || !scale || !x) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_features: null argument");
@@ -2848,13 +2866,12 @@ int nemo2d_closure_features(nemo2d_model* m, const double* mean, const double* s
     }
     const Layout& L = m->L;
     double** s = m->S[m->cur];
+    int rc = after_default_stream(m);
+    if (rc) return rc;
     WITH_STATICS(m, st, (k_closure_features<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(
                             st, s[F_SSHN], s[F_UN], s[F_VN], L, nm, (uint16_t*)x)));
     LAUNCHCHK("closure_features");
-    // the network reads x on the default stream: that stream waits for them
-    HIPCHK(hipEventRecord(m->ev_feat, m->stream));
-    HIPCHK(hipStreamWaitEvent(nullptr, m->ev_feat, 0));
-    return NEMO2D_OK;
+    return before_default_stream(m);
 }
 
 int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const double* scale) {
@@ -2943,7 +2960,11 @@ int nemo2d_closure_target(nemo2d_model* truth, nemo2d_model* pred, const double*
         if (!(scale[k] != 0.0)) return fail(NEMO2D_ERR_ARG, "nemo2d_closure_target: zero scale");
     int rc = lz_flush(truth);
     if (rc || (rc = lz_flush(pred))) return rc;
-    HIPCHK(hipStreamSynchronize(truth->stream));
+    // pred's stream writes y after truth's state is final and after the
+    // default stream's work queued so far (which may still read y)
+    HIPCHK(hipEventRecord(truth->ev_feat, truth->stream));
+    HIPCHK(hipStreamWaitEvent(pred->stream, truth->ev_feat, 0));
+    if ((rc = after_default_stream(pred))) return rc;
     const Layout& L = pred->L;
     double** t = truth->S[truth->cur];
     double** q = pred->S[pred->cur];
@@ -2951,8 +2972,10 @@ int nemo2d_closure_target(nemo2d_model* truth, nemo2d_model* pred, const double*
         t[F_SSHN], t[F_UN], t[F_VN], q[F_SSHN], q[F_UN], q[F_VN], L, Scale3{{scale[0], scale[1], scale[2]}},
         y);
     LAUNCHCHK("closure_target");
-    HIPCHK(hipStreamSynchronize(pred->stream));
-    return NEMO2D_OK;
+    // later work on truth's stream (a new state) waits for that read
+    HIPCHK(hipEventRecord(pred->ev_tgt, pred->stream));
+    HIPCHK(hipStreamWaitEvent(truth->stream, pred->ev_tgt, 0));
+    return before_default_stream(pred);
 }
 
 int nemo2d_retrieve_results(nemo2d_model* m, double* sshn, double* un, double* vn, double* ua,
@@ -3250,6 +3273,8 @@ int nemo2d_destroy(nemo2d_model* m) {
         for (int f = 0; f < NF; ++f) dfree(m->X[k][f]);
     dfree(m->hf_t); dfree(m->hf_u); dfree(m->hf_v); dfree(m->diag);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
+    if (m->ev_free) (void)hipEventDestroy(m->ev_free);
+    if (m->ev_tgt) (void)hipEventDestroy(m->ev_tgt);
     if (m->ev_feat) (void)hipEventDestroy(m->ev_feat);
     if (m->ev_incr) (void)hipEventDestroy(m->ev_incr);
     if (m->ev_read) (void)hipEventDestroy(m->ev_read);

@@ -95,6 +95,22 @@ def test_device_restriction_and_target_equal_host_path_bitwise():
         m.close()
 
 
+def test_online_training_is_deterministic():
+    """Two trainings on the device from the same seeds give bitwise the same
+    weights: the pairs are written on the models' streams into the batch the
+    network reads on the default stream, and each rewrite waits for the
+    network's previous pass (a rewrite racing the pass it feeds would make
+    the result depend on timing)."""
+    bathy = dict(n_mounts=8, radius=(20e3, 100e3), height=(10.0, 60.0))
+    runs = []
+    for _ in range(2):
+        clos, info = cp.train_online(512, range(4), 8, group=2, bathy=bathy, lr=1e-3, rollout=2)
+        assert info["pairs"] == 32
+        runs.append(clos.net.weights())
+    for a, b in zip(runs[0][0] + runs[0][1], runs[1][0] + runs[1][1]):
+        assert np.array_equal(a, b)
+
+
 def test_closure_at_1024_beats_plain_coarse_and_fallback_fires():
     """The closure at the benchmark's size: 1024^2 coarse basins, pairs from
     2048^2 fine runs made and used on the device (16 basins x 40 windows),
