@@ -112,6 +112,7 @@ struct nemo2d_model {
     // per (wave strip, 12-row segment) of k_step_march_rc's tiling: 1 where
     // every point the wave touches has the plain interior class word
     uint8_t* rc_plain = nullptr;
+    uint8_t* rc_plain_short = nullptr;  // the same for RC_NJ_SHORT-row segments
     double* cs_now = nullptr;    // 3 doubles: last on-demand checksums
     double* chk = nullptr;       // one word: nemo2d_set_state's interpolation check
     double* cs_log = nullptr;    // 3 doubles per logged step
@@ -1876,29 +1877,33 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
     }
     HIPCHK(hipMemcpyAsync(m->flags, fl.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, m->stream));
     {
-        // plain tiles of k_step_march_rc (strips of ROUT columns with lanes
-        // at c0-1 .. c0+62, segments of RC_NJ rows with rows jb-1 .. je+1+RC_LA
-        // loaded): every class word there equal to RC_PLAIN
+        // plain tiles of k_step_march_rc and k_step_march_gr (strips of ROUT
+        // columns with lanes at c0-1 .. c0+62, segments of nj rows with rows
+        // jb-1 .. je+1+RC_LA loaded): every class word there equal to RC_PLAIN
         const int nbx = (jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
-        const int nstrip = nbx * MW, nseg = (jpj + 1 + RC_NJ - 1) / RC_NJ;
-        std::vector<uint8_t> tab((size_t)nstrip * nseg, 0);
-        for (int w = 0; w < nstrip; ++w) {
-            const int i0 = w * ROUT - 1, i1 = w * ROUT + 62;
-            if (i0 < 0 || i1 > jpi + 1) continue;
-            for (int q = 0; q < nseg; ++q) {
-                const int jb = q * RC_NJ, je = jb + RC_NJ - 1 < jpj ? jb + RC_NJ - 1 : jpj;
-                const int j0 = jb - 1, j1 = je + 1 + RC_LA;
-                if (j0 < 0 || j1 > jpj + 1) continue;
-                bool ok = true;
-                for (int j = j0; j <= j1 && ok; ++j)
-                    for (int i = i0; i <= i1 && ok; ++i) ok = fl[L.at(i, j)] == RC_PLAIN;
-                tab[(size_t)w * nseg + q] = ok ? 1 : 0;
+        const int nstrip = nbx * MW;
+        for (const int nj : {RC_NJ, RC_NJ_SHORT}) {
+            const int nseg = (jpj + 1 + nj - 1) / nj;
+            std::vector<uint8_t> tab((size_t)nstrip * nseg, 0);
+            for (int w = 0; w < nstrip; ++w) {
+                const int i0 = w * ROUT - 1, i1 = w * ROUT + 62;
+                if (i0 < 0 || i1 > jpi + 1) continue;
+                for (int q = 0; q < nseg; ++q) {
+                    const int jb = q * nj, je = jb + nj - 1 < jpj ? jb + nj - 1 : jpj;
+                    const int j0 = jb - 1, j1 = je + 1 + RC_LA;
+                    if (j0 < 0 || j1 > jpj + 1) continue;
+                    bool ok = true;
+                    for (int j = j0; j <= j1 && ok; ++j)
+                        for (int i = i0; i <= i1 && ok; ++i) ok = fl[L.at(i, j)] == RC_PLAIN;
+                    tab[(size_t)w * nseg + q] = ok ? 1 : 0;
+                }
             }
+            uint8_t*& dst = nj == RC_NJ ? m->rc_plain : m->rc_plain_short;
+            dfree(dst);
+            dst = nullptr;
+            HIPCHK(hipMalloc((void**)&dst, tab.size()));
+            HIPCHK(hipMemcpyAsync(dst, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream));
         }
-        dfree(m->rc_plain);
-        m->rc_plain = nullptr;
-        HIPCHK(hipMalloc((void**)&m->rc_plain, tab.size()));
-        HIPCHK(hipMemcpyAsync(m->rc_plain, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream));
     }
     HIPCHK(hipMemcpyAsync(m->flu, hflu.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(m->flv, hflv.data(), n * sizeof(double), hipMemcpyHostToDevice, m->stream));
@@ -2302,7 +2307,8 @@ enum {
                             // ahead, reciprocal divisions under one guard per row, class-free
                             // code on plain interior tiles
     VAR_MARCH12_GRD = 11,   // 5 reading sshn, un, vn (and depths) only, plain tiles
-    VAR_LAST = 11
+    VAR_MARCH8_GRD = 12,    // 11 with 8-row segments (more waves on mid-size grids)
+    VAR_LAST = 12
 };
 
 static int launch_march16(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
@@ -2410,7 +2416,9 @@ template <class R, class SS, bool DARR, int NJ, int LA, bool DV = false>
 static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>& p,
                            const StateG<R, SS, DARR>& n, const OutG<R, SS>& a, double tide,
                            int& nparts) {
-    static_assert(GROUT == ROUT && GW == MW && NJ == RC_NJ && LA <= RC_LA, "the plain-tile table");
+    static_assert(GROUT == ROUT && GW == MW && (NJ == RC_NJ || NJ == RC_NJ_SHORT) && LA <= RC_LA,
+                  "the plain-tile tables");
+    const uint8_t* plain = NJ == RC_NJ ? m->rc_plain : m->rc_plain_short;
     const Layout& L = m->L;
     const int nbx = (L.jpi + 1 + GW * GROUT - 1) / (GW * GROUT);
     const int nby = (L.jpj + 1 + NJ - 1) / NJ;
@@ -2424,24 +2432,25 @@ static int launch_march_gr(nemo2d_model* m, const MetG<R>& mt, const ParamsT<R>&
     if (sizeof(R) == sizeof(double) && m->scal.one_corr_met)
         k_step_march_gr<R, SS, DARR, NJ, LA, DV, sizeof(R) == sizeof(double) ? 3 : 0>
             <<<nblocks, 64 * GW, 0, m->stream>>>(mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx,
-                                                 ntiles, m->rc_plain, rc);
+                                                 ntiles, plain, rc);
     else
         k_step_march_gr<R, SS, DARR, NJ, LA, DV><<<nblocks, 64 * GW, 0, m->stream>>>(
-            mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, m->rc_plain, rc);
+            mt, p, n, a, m->flags, m->flv, L, tide, m->part_dst, nbx, ntiles, plain, rc);
     LAUNCHCHK("step_march_gr");
     nparts = nblocks;
     return NEMO2D_OK;
 }
 
+template <int NJ>
 static int launch_march_depths_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide,
                                   int& nparts) {
     StateG<double, double, true> ng{n.sshn, n.sshn_u, n.sshn_v, n.un, n.vn, m->st[S_HT], m->st[S_HU],
                                     m->st[S_HV]};
     OutG<double, double> ag{a.sshn, a.sshn_u, a.sshn_v, a.un, a.vn};
     if (m->dep_derived)
-        return launch_march_gr<double, double, true, 12, 2, true>(m, metg(m->scal), m->prm, ng, ag, tide,
+        return launch_march_gr<double, double, true, NJ, 2, true>(m, metg(m->scal), m->prm, ng, ag, tide,
                                                                   nparts);
-    return launch_march_gr<double, double, true, 12, 2>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
+    return launch_march_gr<double, double, true, NJ, 2>(m, metg(m->scal), m->prm, ng, ag, tide, nparts);
 }
 
 static int launch_march_depths(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
@@ -2473,6 +2482,11 @@ constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
 constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
 constexpr long AUTO_CELL_MAX_MIXED = 300000;
 constexpr long AUTO_SHN_MAX = 2000000;
+// Per-point depths: 8-row segments (more waves in flight) up to this many
+// cells, 12-row ones (less halo work) beyond.  Measured on MI355X (device us
+// per step, seamount basins, 12 / 8 rows): 512^2 23.1 / 17.5, 1024^2 33.5 /
+// 29.3, 2048^2 71.8 / 68.9, 3072^2 150.7 / 148.8, 4096^2 238.3 / 244.3.
+constexpr long AUTO_SHORT_SEG_MAX_DEPTHS = 10000000;
 
 extern "C" {
 
@@ -2504,6 +2518,9 @@ static int step_mixed(nemo2d_model* m, int istp) {
         }
         const MetG<float> mt{sf.e1t, sf.e2t, sf.e1u, sf.e2u, sf.e1v, sf.e2v, sf.e12t, sf.e12u, sf.e12v,
                              sf.fcu, sf.fcv, sf.ht, sf.hu, sf.hv};
+        // 8-row segments when chosen, or by default up to the FP64 form's size
+        const bool short_seg = m->variant == VAR_MARCH8_GRD ||
+                               (m->variant == VAR_AUTO && (long)L.jpi * L.jpj <= AUTO_SHORT_SEG_MAX_DEPTHS);
         int nparts = 0, rc;
         m->part_dst = m->partials;
         if (m->compensated) {
@@ -2513,6 +2530,10 @@ static int step_mixed(nemo2d_model* m, int istp) {
             OutG<float, double> a{(double*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                   (float*)y[F_UN], (float*)y[F_VN]};
             rc = !gr ? launch_march_g<float, double, true, 12>(m, mt, pf, n, a, tide, nparts)
+                 : short_seg
+                     ? (m->dep_derived
+                            ? launch_march_gr<float, double, true, RC_NJ_SHORT, 2, true>(m, mt, pf, n, a, tide, nparts)
+                            : launch_march_gr<float, double, true, RC_NJ_SHORT, 2>(m, mt, pf, n, a, tide, nparts))
                  : m->dep_derived ? launch_march_gr<float, double, true, 12, 2, true>(m, mt, pf, n, a, tide, nparts)
                                   : launch_march_gr<float, double, true, 12, 2>(m, mt, pf, n, a, tide, nparts);
         } else {
@@ -2522,6 +2543,10 @@ static int step_mixed(nemo2d_model* m, int istp) {
             OutG<float, float> a{(float*)y[F_SSHN], (float*)y[F_SSHN_U], (float*)y[F_SSHN_V],
                                  (float*)y[F_UN], (float*)y[F_VN]};
             rc = !gr ? launch_march_g<float, float, true, 12>(m, mt, pf, n, a, tide, nparts)
+                 : short_seg
+                     ? (m->dep_derived
+                            ? launch_march_gr<float, float, true, RC_NJ_SHORT, 2, true>(m, mt, pf, n, a, tide, nparts)
+                            : launch_march_gr<float, float, true, RC_NJ_SHORT, 2>(m, mt, pf, n, a, tide, nparts))
                  : m->dep_derived ? launch_march_gr<float, float, true, 12, 2, true>(m, mt, pf, n, a, tide, nparts)
                                   : launch_march_gr<float, float, true, 12, 2>(m, mt, pf, n, a, tide, nparts);
         }
@@ -2606,18 +2631,21 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
             var = (m->uniform && flather_row0_only(m)) ? VAR_CELL_FLIN : VAR_CELL;
         else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
             var = VAR_MARCH12_SHN;
+        else if (m->uniform)
+            var = VAR_MARCH12_RC;
         else
-            var = m->uniform ? VAR_MARCH12_RC : (depth_march_ok ? VAR_MARCH12_GRD : VAR_CELL);
+            var = !depth_march_ok ? VAR_CELL
+                                  : cells <= AUTO_SHORT_SEG_MAX_DEPTHS ? VAR_MARCH8_GRD : VAR_MARCH12_GRD;
     }
-    const bool depth_var = var == VAR_MARCH12_SHD || var == VAR_MARCH12_GRD;
+    const bool depth_var = var == VAR_MARCH12_SHD || var == VAR_MARCH12_GRD || var == VAR_MARCH8_GRD;
     if (depth_var && !depth_march_ok) var = VAR_CELL;
     // marching with scalar depths needs uniform statics
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
     // the kernels that recompute sshn_u/sshn_v need them to be sshn's
     // interpolations (true after any step) and no Flather points off row 0
     if (var == VAR_MARCH12_RC && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
-    if (var == VAR_MARCH12_GRD && !m->uv_derived) var = VAR_MARCH12_SHD;
-    const bool rc_kernel = var == VAR_MARCH12_RC || var == VAR_MARCH12_GRD;
+    if ((var == VAR_MARCH12_GRD || var == VAR_MARCH8_GRD) && !m->uv_derived) var = VAR_MARCH12_SHD;
+    const bool rc_kernel = var == VAR_MARCH12_RC || var == VAR_MARCH12_GRD || var == VAR_MARCH8_GRD;
     if (!rc_kernel && m->uv_stale[k]) {
         const int urc = uv_sync(m);
         if (urc) return urc;
@@ -2633,7 +2661,8 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
         case VAR_CELL_FLIN: rc = launch_cell<true>(m, n, a, tide, nparts); flin = true; break;
         case VAR_MARCH12_SHN: rc = launch_march_sh<12>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_SHD: rc = launch_march_depths(m, n, a, tide, nparts); break;
-        case VAR_MARCH12_GRD: rc = launch_march_depths_rc(m, n, a, tide, nparts); break;
+        case VAR_MARCH12_GRD: rc = launch_march_depths_rc<RC_NJ>(m, n, a, tide, nparts); break;
+        case VAR_MARCH8_GRD: rc = launch_march_depths_rc<RC_NJ_SHORT>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DP: rc = launch_march_dp<12, false, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPF: rc = launch_march_dp<12, true, false>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
@@ -2649,7 +2678,8 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
     m->a_current = false;
     // the shared-term kernels set the row-0 Flather v points themselves
     const bool sh = var >= VAR_MARCH12_SHN && var <= VAR_LAST && sh_fits(L) &&
-                    ((var != VAR_MARCH12_SHD && var != VAR_MARCH12_GRD) || depth_march_ok);
+                    ((var != VAR_MARCH12_SHD && var != VAR_MARCH12_GRD && var != VAR_MARCH8_GRD) ||
+                     depth_march_ok);
     const bool has_fl = (m->nflu + m->nflv - (sh ? m->nflv0 : 0)) > 0;
     if (!flin && has_fl) {
         // Flather points of the new state, then (if recording) the checksums
@@ -3266,7 +3296,7 @@ int nemo2d_destroy(nemo2d_model* m) {
         for (int f = 0; f < NF; ++f) dfree(m->S[k][f]);
     for (int s = 0; s < NS; ++s) dfree(m->st[s]);
     dfree(m->ssha); dfree(m->ua); dfree(m->va);
-    dfree(m->flags); dfree(m->flu); dfree(m->flv); dfree(m->rc_plain);
+    dfree(m->flags); dfree(m->flu); dfree(m->flv); dfree(m->rc_plain); dfree(m->rc_plain_short);
     dfree(m->flu_ptr); dfree(m->flu_pts); dfree(m->flv_ptr); dfree(m->flv_pts);
     dfree(m->partials); dfree(m->cs_now); dfree(m->chk); dfree(m->cs_log); dfree(m->cs_part); dfree(m->d_pend);
     for (int k = 0; k < 2; ++k)

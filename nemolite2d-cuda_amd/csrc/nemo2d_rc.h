@@ -120,6 +120,7 @@ constexpr int ROUT = 61;
 constexpr unsigned RC_PLAIN = (CLS_COMP << U_CLS_SHIFT) | (CLS_COMP << V_CLS_SHIFT) |
                               (NX_AVG << SU_SHIFT) | (NX_AVG << SV_SHIFT);
 constexpr int RC_NJ = 12, RC_LA = 2;  // the tiling the plain-tile table is made for
+constexpr int RC_NJ_SHORT = 8;       // the shorter segments of mid-size grids (a second table)
 
 // sshn_u / sshn_v at a point from next's class of it (NX_SKIP where next does
 // not set it): next_su / next_sv's expressions, or the stored value (loaded

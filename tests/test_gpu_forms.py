@@ -51,13 +51,13 @@ def test_fp32_marching_kernel_equals_per_cell_kernel(compensated):
     ni, nj, nsteps = 211, 97, 60
     grid, _ = nemo2d.seamount_grid(ni, nj, seed=5, radius=(5e3, 30e3))
     out = {}
-    for variant in ("march12_grd", "march12_shd", "auto", "cell"):
+    for variant in ("march12_grd", "march8_grd", "march12_shd", "auto", "cell"):
         with nemo2d.Model(ni, nj, grid=grid, precision="fp32", compensated=compensated,
                           cbfr=0.0015) as m:
             m.set_variant(variant)
             m.run(1, nsteps)
             out[variant] = m.retrieve()
-    for variant in ("march12_grd", "march12_shd", "auto"):
+    for variant in ("march12_grd", "march8_grd", "march12_shd", "auto"):
         for k in ("sshn", "un", "vn", "sshn_u", "sshn_v"):
             a, b = out[variant][k], out["cell"][k]
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (variant, k,

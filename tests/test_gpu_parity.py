@@ -652,7 +652,7 @@ def test_snapshots_behind_steps_equal_synchronous_state(tmp_path):
         assert_fields_equal(m, o, "after snapshots")
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_grd", "march12_shd", "cell"])
+@pytest.mark.parametrize("variant", ["auto", "march12_grd", "march8_grd", "march12_shd", "cell"])
 @pytest.mark.parametrize("islands", [False, True])
 @pytest.mark.parametrize("depths", [True, "derived"])
 def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
@@ -745,11 +745,13 @@ def test_default_kernel_by_grid_size():
     per cell (Flather inline) on the 128^2 basin, shared face terms at
     1024^2, and beyond 2M cells the sshn-only kernel (from the first step:
     the state from rest passes set_state's interpolation check); on a
-    seamount grid the per-point-depth marching kernel."""
+    seamount grid the per-point-depth marching kernel, with 8-row segments
+    up to 10M cells."""
     cases = [(128, 128, None, ["cell_flin", "cell_flin"]),
              (1024, 1024, None, ["march12_shn", "march12_shn"]),
              (1500, 1500, None, ["march12_rc", "march12_rc"]),
-             (1200, 1200, "seamount", ["march12_grd", "march12_grd"])]
+             (1200, 1200, "seamount", ["march8_grd", "march8_grd"]),
+             (3200, 3200, "seamount", ["march12_grd", "march12_grd"])]
     for ni, nj, kind, want in cases:
         grid = nemo2d.seamount_grid(ni, nj, seed=2)[0] if kind else None
         with nemo2d.Model(ni, nj, grid=grid) as m:
