@@ -315,7 +315,8 @@ def form_closure(sc, eval_windows=60, train=None):
 # the fused-step kernel each variant runs on the uniform benchmark grid
 # (march12_rc: <3> divides by reciprocals with one correction, as on the
 # benchmark grid, whose divisors all allow it; <2> with two elsewhere)
-KERNELS = {"march12_rc": "k_step_march_rc<3>", "march12_dpfb": "k_step_march_dp<12, true, true>",
+KERNELS = {"march12_rc": "k_step_march_rc<3>", "march8_rc": "k_step_march_rc<3>",
+           "march12_dpfb": "k_step_march_dp<12, true, true>",
            "march12_dpf": "k_step_march_dp<12, true, false>", "march12_dpb": "k_step_march_dp<12, false, true>",
            "march12_dp": "k_step_march_dp<12, false, false>", "march12_shn": "k_step_march_sh<12>",
            "march16": "k_step_march", "cell": "k_step_fused<nemo2d::StUniform, false>",

@@ -122,9 +122,9 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
  * choice): 0 default, by grid size (uniform statics: 2 up to 300,000 T cells
- * when the Flather points are all on row 0, else 1; 4 up to 2,000,000; then
- * 10.  Per-point depths with uniform metrics: 1 up to 100,000, 12 up to
- * 10,000,000, then 11.
+ * when the Flather points are all on row 0, else 1; 13 up to 3,000,000;
+ * then 10.  Per-point depths with uniform metrics: 1 up to 100,000, 12 up
+ * to 10,000,000, then 11.
  * Per-point metrics: 1).  1 one thread per cell (Flather points by an
  * epilogue), 2 the same with Flather points recomputed inline, 3 five-array
  * column-marching with 16-row segments (the kernel for arrays beyond 4 GiB,
@@ -138,10 +138,11 @@ int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
  * rows ahead, reciprocal divisions under one guard per row (the row is done
  * again with IEEE divisions when any numerator is out of range), class-free
  * code on tiles of plain interior points, 11 as 5 reading sshn, un, vn and
- * the depths only, 12 as 11 with 8-row segments (more waves on mid-size
- * grids).  Marching (3..12) needs uniform metrics, 3, 4 and 6..10 also
- * uniform depths, 5, 11 and 12 Flather points only on row 0 with the inner
- * point north; 10..12 need sshn_u/sshn_v to be sshn's interpolations
+ * the depths only, 12 and 13 as 11 and 10 with 8-row segments (more waves
+ * on mid-size grids).  Marching (3..13) needs uniform metrics, 3, 4, 6..10
+ * and 13 also uniform depths, 5 and 10..13 Flather points only on row 0 with
+ * the inner point north; 10..13 need sshn_u/sshn_v to be sshn's
+ * interpolations
  * (true after any step; nemo2d_set_state checks the given arrays on the
  * device, and when they are not, the first step after it runs 9 or 5).
  * Other grids use 1 (or 2).  The single-precision form marches where it

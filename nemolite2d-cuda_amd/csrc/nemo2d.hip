@@ -923,7 +923,7 @@ __global__ void __launch_bounds__(64 * MW)
 k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
                 const double* __restrict__ flv, Layout L, Params p, double tide,
                 double* __restrict__ partials, int nbx, int ntiles,
-                const uint8_t* __restrict__ plain) {
+                const uint8_t* __restrict__ plain, int nj) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     RcCarry k;
@@ -947,8 +947,8 @@ k_step_march_rc(Scal sc_, NState n, AOut a, const uint16_t* __restrict__ fl,
         x.tcol = x.i >= 1 && x.i <= L.jpi;
         x.ucol = x.i >= 0 && x.i <= L.jpi;
         x.vcol = x.i >= 1 && x.i <= L.jpi;
-        const int jb = by * RC_NJ;
-        const int je = jb + RC_NJ - 1 < L.jpj ? jb + RC_NJ - 1 : L.jpj;
+        const int jb = by * nj;  // (segments of nj rows: the plain table's)
+        const int je = jb + nj - 1 < L.jpj ? jb + nj - 1 : L.jpj;
         const int nseg = ntiles / nbx;
         // (tiles that are not plain divide by IEEE division: their general
         // code with the guard's bookkeeping would not fit three waves per SIMD)
@@ -2308,7 +2308,8 @@ enum {
                             // code on plain interior tiles
     VAR_MARCH12_GRD = 11,   // 5 reading sshn, un, vn (and depths) only, plain tiles
     VAR_MARCH8_GRD = 12,    // 11 with 8-row segments (more waves on mid-size grids)
-    VAR_LAST = 12
+    VAR_MARCH8_RC = 13,     // 10 with 8-row segments
+    VAR_LAST = 13
 };
 
 static int launch_march16(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
@@ -2362,19 +2363,21 @@ static int launch_march_dp(nemo2d_model* m, const NState& n, const AOut& a, doub
     return NEMO2D_OK;
 }
 
-template <int DM>
+template <int DM, int NJ = RC_NJ>
 static int launch_march_rc(nemo2d_model* m, const NState& n, const AOut& a, double tide, int& nparts) {
     const Layout& L = m->L;
-    if (DM != 0 && !m->scal.fast_div) return launch_march_rc<0>(m, n, a, tide, nparts);
-    if (DM == 2 && m->scal.one_corr) return launch_march_rc<3>(m, n, a, tide, nparts);
+    if (DM != 0 && !m->scal.fast_div) return launch_march_rc<0, NJ>(m, n, a, tide, nparts);
+    if (DM == 2 && m->scal.one_corr) return launch_march_rc<3, NJ>(m, n, a, tide, nparts);
     static_assert(RC_LA >= 2, "the plain-tile table covers the rows loaded ahead");
+    static_assert(NJ == RC_NJ || NJ == RC_NJ_SHORT, "the plain-tile tables");
     const int nbx = (L.jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
-    const int nby = (L.jpj + 1 + RC_NJ - 1) / RC_NJ;
+    const int nby = (L.jpj + 1 + NJ - 1) / NJ;
     const int ntiles = nbx * nby;
     const int nblocks = (ntiles + 7) / 8 * 8;
     if (nblocks > m->npart_max) return fail(NEMO2D_ERR_ARG, "partials buffer too small");
     k_step_march_rc<DM><<<nblocks, 64 * MW, 0, m->stream>>>(
-        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles, m->rc_plain);
+        m->scal, n, a, m->flags, m->flv, L, m->prm, tide, m->part_dst, nbx, ntiles,
+        NJ == RC_NJ ? m->rc_plain : m->rc_plain_short, NJ);
     LAUNCHCHK("step_march_rc");
     nparts = nblocks;
     return NEMO2D_OK;
@@ -2481,7 +2484,12 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
 constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
 constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
 constexpr long AUTO_CELL_MAX_MIXED = 300000;
-constexpr long AUTO_SHN_MAX = 2000000;
+// Uniform statics: the sshn-only kernel with 8-row segments up to this many
+// cells, 12-row ones beyond (device us per step, reference basin, 12 / 8
+// rows / the five-array shared-term kernel: 768^2 26.1 / 20.4 / 28.3,
+// 1024^2 27.1 / 25.4 / 29.2, 1536^2 42.0 / 38.7 / 49.3, 2048^2 55.8 / 55.5,
+// 3072^2 118.5 / 121.2, 4096^2 191.7 / 197.2).
+constexpr long AUTO_SHORT_SEG_MAX_UNIFORM = 3000000;
 // Per-point depths: 8-row segments (more waves in flight) up to this many
 // cells, 12-row ones (less halo work) beyond.  Measured on MI355X (device us
 // per step, seamount basins, 12 / 8 rows): 512^2 23.1 / 17.5, 1024^2 33.5 /
@@ -2619,20 +2627,18 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
     int var = m->variant;
     const bool depth_march_ok = m->uni_metrics && flather_row0_only(m) && sh_fits(L);
     if (var == VAR_AUTO) {
-        // A marching wave walks its 12 rows one after another, so on a small
-        // grid (few waves) the step is the latency of one wave's march; there
-        // one thread per cell is faster.  Crossovers measured on MI355X
-        // (device us per step, default / one thread per cell / five-array
-        // marching): uniform 512^2 31.2 / 16.0 / 20.4, 1024^2 38.9 / 42.5 /
-        // 29.1, 2048^2 64.9 / 166.7 / 94.4; per-point depths 256^2 22.3 /
-        // 12.5, 512^2 23.3 / 26.2.
+        // A marching wave walks its segment's rows one after another, so on a
+        // small grid (few waves) the step is the latency of one wave's march;
+        // there one thread per cell is faster, and on mid-size grids shorter
+        // segments (more waves).  Crossovers measured on MI355X (device us
+        // per step, marching / one thread per cell): uniform 512^2 20.4 /
+        // 16.0, 1024^2 25.4 / 42.5; per-point depths 256^2 22.3 / 12.5,
+        // 512^2 17.5 / 26.2.
         const long cells = (long)L.jpi * L.jpj;
         if (cells <= (m->uniform ? AUTO_CELL_MAX_UNIFORM : AUTO_CELL_MAX_DEPTHS))
             var = (m->uniform && flather_row0_only(m)) ? VAR_CELL_FLIN : VAR_CELL;
-        else if (m->uniform && cells <= AUTO_SHN_MAX && sh_fits(L))
-            var = VAR_MARCH12_SHN;
         else if (m->uniform)
-            var = VAR_MARCH12_RC;
+            var = cells <= AUTO_SHORT_SEG_MAX_UNIFORM ? VAR_MARCH8_RC : VAR_MARCH12_RC;
         else
             var = !depth_march_ok ? VAR_CELL
                                   : cells <= AUTO_SHORT_SEG_MAX_DEPTHS ? VAR_MARCH8_GRD : VAR_MARCH12_GRD;
@@ -2643,9 +2649,11 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
     if (!m->uniform && var != VAR_CELL_FLIN && !depth_var) var = VAR_CELL;
     // the kernels that recompute sshn_u/sshn_v need them to be sshn's
     // interpolations (true after any step) and no Flather points off row 0
-    if (var == VAR_MARCH12_RC && !(m->uv_derived && flather_row0_only(m) && sh_fits(L))) var = VAR_MARCH12_DPFB;
+    if ((var == VAR_MARCH12_RC || var == VAR_MARCH8_RC) && !(m->uv_derived && flather_row0_only(m) && sh_fits(L)))
+        var = VAR_MARCH12_DPFB;
     if ((var == VAR_MARCH12_GRD || var == VAR_MARCH8_GRD) && !m->uv_derived) var = VAR_MARCH12_SHD;
-    const bool rc_kernel = var == VAR_MARCH12_RC || var == VAR_MARCH12_GRD || var == VAR_MARCH8_GRD;
+    const bool rc_kernel = var == VAR_MARCH12_RC || var == VAR_MARCH8_RC || var == VAR_MARCH12_GRD ||
+                           var == VAR_MARCH8_GRD;
     if (!rc_kernel && m->uv_stale[k]) {
         const int urc = uv_sync(m);
         if (urc) return urc;
@@ -2668,6 +2676,7 @@ static int fused_step(nemo2d_model* m, double tide, int slot) {
         case VAR_MARCH12_DPB: rc = launch_march_dp<12, false, true>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_DPFB: rc = launch_march_dp<12, true, true>(m, n, a, tide, nparts); break;
         case VAR_MARCH12_RC: rc = launch_march_rc<2>(m, n, a, tide, nparts); break;
+        case VAR_MARCH8_RC: rc = launch_march_rc<2, RC_NJ_SHORT>(m, n, a, tide, nparts); break;
         default: rc = launch_march16(m, n, a, tide, nparts); break;
     }
     if (rc) return rc;

@@ -48,7 +48,7 @@ SYMBOLS = (
 # nemo2d_set_variant choices
 VARIANTS = {"auto": 0, "cell": 1, "cell_flin": 2, "march16": 3, "march12_shn": 4, "march12_shd": 5,
             "march12_dp": 6, "march12_dpf": 7, "march12_dpb": 8, "march12_dpfb": 9, "march12_rc": 10,
-            "march12_grd": 11, "march8_grd": 12}
+            "march12_grd": 11, "march8_grd": 12, "march8_rc": 13}
 
 
 class Nemo2dError(RuntimeError):

@@ -674,7 +674,7 @@ def test_per_point_depths_bitwise_vs_oracle(variant, islands, depths):
         assert np.all(np.isfinite(m.checksum_log(25)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march8_rc"])
 @pytest.mark.parametrize("islands", [False, True])
 def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
     """The fused kernel that reads sshn, un and vn only (sshn_u and sshn_v
@@ -696,7 +696,7 @@ def test_sshn_only_kernel_bitwise_vs_oracle(variant, islands):
         assert np.all(np.isfinite(m.checksum_log(40)))
 
 
-@pytest.mark.parametrize("variant", ["auto", "march12_rc"])
+@pytest.mark.parametrize("variant", ["auto", "march12_rc", "march8_rc"])
 def test_sshn_only_kernel_after_a_state_with_other_interpolations(variant):
     """A state whose sshn_u/sshn_v are not sshn's interpolations (set by the
     caller) must be stepped from the stored values: the kernel that recomputes
@@ -742,14 +742,14 @@ def test_tiny_values_take_the_ieee_divisions(variant):
 
 def test_default_kernel_by_grid_size():
     """The kernel the default picks, as the library reports it: one thread
-    per cell (Flather inline) on the 128^2 basin, shared face terms at
-    1024^2, and beyond 2M cells the sshn-only kernel (from the first step:
-    the state from rest passes set_state's interpolation check); on a
-    seamount grid the per-point-depth marching kernel, with 8-row segments
-    up to 10M cells."""
+    per cell (Flather inline) on the 128^2 basin, then the sshn-only kernel
+    (from the first step: the state from rest passes set_state's
+    interpolation check) with 8-row segments up to 3M cells and 12-row ones
+    beyond; on a seamount grid the per-point-depth marching kernel, with
+    8-row segments up to 10M cells."""
     cases = [(128, 128, None, ["cell_flin", "cell_flin"]),
-             (1024, 1024, None, ["march12_shn", "march12_shn"]),
-             (1500, 1500, None, ["march12_rc", "march12_rc"]),
+             (1024, 1024, None, ["march8_rc", "march8_rc"]),
+             (2000, 2000, None, ["march12_rc", "march12_rc"]),
              (1200, 1200, "seamount", ["march8_grd", "march8_grd"]),
              (3200, 3200, "seamount", ["march12_grd", "march12_grd"])]
     for ni, nj, kind, want in cases:
@@ -782,7 +782,7 @@ def test_first_step_kernel_follows_the_interpolation_check():
                 m.step(istp)
                 o.step(istp)
                 kernels.append(m.last_variant())
-            assert kernels == (["march12_dpfb", "march12_rc"] if perturb else ["march12_rc"] * 2), kernels
+            assert kernels == (["march12_dpfb", "march8_rc"] if perturb else ["march8_rc"] * 2), kernels
             assert_fields_equal(m, o, "perturbed" if perturb else "evolved")
 
 
