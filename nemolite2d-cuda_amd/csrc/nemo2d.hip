@@ -2931,9 +2931,10 @@ int nemo2d_closure_apply(nemo2d_model* m, const float* z, int zstride, const dou
     k_closure_add<<<grid2d(L.jpi, L.jpj), dim3(TBX, TBY), 0, m->stream>>>(z, zstride, sc, L, s[F_SSHN],
                                                                           s[F_UN], s[F_VN]);
     LAUNCHCHK("closure_add");
-    WITH_STATICS(m, st, (k_interp<<<grid2d(L.jpi + 1, L.jpj + 1), dim3(TBX, TBY), 0, m->stream>>>(
-                            st, m->flags, s[F_SSHN], L, s[F_SSHN_U], s[F_SSHN_V])));
-    LAUNCHCHK("closure_interp");
+    // sshn_u/sshn_v are now next's interpolations of the new sshn (where
+    // next sets them) and not yet written: the sshn-only step kernels form
+    // them from sshn, and uv_sync fills them in before anything reads them
+    m->uv_stale[m->cur] = true;
     // later default-stream work (the next network pass rewriting z) waits for the read of z
     HIPCHK(hipEventRecord(m->ev_read, m->stream));
     HIPCHK(hipStreamWaitEvent(nullptr, m->ev_read, 0));
