@@ -276,7 +276,8 @@ int nemo2d_ensemble_destroy(nemo2d_ensemble *e);
  * Device pointers, NHWC activations with the channel stride rounded up to
  * 8 (BF16 stored as 16-bit integers), 1 <= channels <= 32, stream a
  * hipStream_t (NULL = default stream).  Convolutions are 'same'-padded 3x3,
- * implicit-GEMM on BF16 matrix cores with FP32 accumulation.
+ * implicit-GEMM on BF16 matrix cores with FP32 accumulation; an image has
+ * at most 2^24 pixels (h * w; larger: NEMO2D_ERR_ARG).
  * conv_fwd writes the pre-activation b + conv(x) as FP32 into z and/or as
  *   BF16 into zb, and BF16 gelu(it) (use_gelu) or it into act; any of the
  *   three may be NULL (not all).
@@ -306,7 +307,8 @@ int nemo2d_nn_conv_bwd_weight(const void *x, const void *dz, int n, int h, int w
  * hipMalloc/hipFree and needs no workspace from the caller.  No counterpart
  * in the reference: for callers that cannot allocate (code under HIP graph
  * capture, serving loops, harnesses that forbid it).  Any batch up to
- * 2147483647 pixels (larger: NEMO2D_ERR_ARG); the results are bitwise those
+ * 2147483647 pixels of images of at most 2^24 pixels (larger:
+ * NEMO2D_ERR_ARG); the results are bitwise those
  * of nemo2d_nn_conv_bwd_weight.  Calls that overlap on one GPU share the
  * array and race. */
 int nemo2d_nn_conv_bwd_weight_noalloc(const void *x, const void *dz, int n, int h, int w, int cin,

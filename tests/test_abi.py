@@ -76,6 +76,24 @@ def test_null_arguments_return_status():
     assert L.nemo2d_destroy(None) == nemo2d.OK
 
 
+def test_convolutions_reject_images_over_2_24_pixels():
+    """The convolution kernels address an image with 32-bit offsets and
+    range-checked buffer loads: an image of more than 2^24 pixels is refused
+    on the host, before any HIP call (the pointers are never used)."""
+    from nemo2d.closure import _setup_nn
+    L = nemo2d.lib()
+    _setup_nn(L)
+    p = ctypes.c_void_p(256)
+    f = ctypes.cast(p, ctypes.POINTER(ctypes.c_float))
+    big = (1, 4097, 4096)  # 16,781,312 pixels
+    assert L.nemo2d_nn_conv_fwd(p, *big, 32, p, f, 32, None, p, p, 1, None) == nemo2d.ERR_ARG
+    assert b"too many pixels" in L.nemo2d_last_error()
+    assert L.nemo2d_nn_conv_bwd_data(p, *big, 32, p, 32, f, None) == nemo2d.ERR_ARG
+    assert L.nemo2d_nn_conv_bwd_data_gelu(p, *big, 32, p, 32, p, 1, p, None) == nemo2d.ERR_ARG
+    assert L.nemo2d_nn_conv_bwd_weight(p, p, *big, 32, 32, f, f, f, None) == nemo2d.ERR_ARG
+    assert b"too many pixels" in L.nemo2d_last_error()
+
+
 def test_driver_binary_built():
     exe = os.path.join(ROOT, "nemolite2d-cuda_amd", "bin", "nemolite2d_hip")
     assert os.access(exe, os.X_OK)
