@@ -381,6 +381,9 @@ def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, ite
     t_wait = 0.0
     trios = []
     pool = ThreadPoolExecutor(max_workers=group)
+    # the group's models take their grids side by side (the library's set-up
+    # runs on the host without the interpreter lock, one model per thread)
+    loader = ThreadPoolExecutor(max_workers=group)
 
     def make(gs):
         return [pool.submit(Setup, nc, sd, **bathy) for sd in gs]
@@ -395,8 +398,8 @@ def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, ite
         while len(trios) < len(setups):
             trios.append(Trio(setups[len(trios)]))
         active = trios[:len(setups)]
-        for tr, su in zip(active, setups):
-            tr.load(su)
+        for f in [loader.submit(tr.load, su) for tr, su in zip(active, setups)]:
+            f.result()
         for _ in range(spinup):
             for tr in active:
                 tr.advance()
@@ -425,6 +428,7 @@ def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, ite
                     clos.correct_model(tr.pred)
                     tr.step_pred()
     pool.shutdown()
+    loader.shutdown()
     for tr in trios:
         tr.close()
     t_train = _time.perf_counter() - t1
