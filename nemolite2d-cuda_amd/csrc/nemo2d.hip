@@ -1707,18 +1707,25 @@ int classify(const Layout& L, const int* pt, const double* hu, const double* hv,
         }
     }
     uptr[jpj] = (int)upts.size();
-    for (int i = 1; i <= jpi; ++i) {
-        vptr[i - 1] = (int)vpts.size();
-        for (int j = 0; j <= jpj; ++j) {
+    // V lines are columns (i outer, j inner in the serial loop); the class
+    // words are scanned row by row (a column walk strides a whole row per
+    // point), counting per column, then filled in at each column's offset
+    std::vector<int> cnt(jpi + 1, 0);
+    for (int j = 0; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i)
+            if (vcls(fl[L.at(i, j)]) == CLS_FLATHER) ++cnt[i];
+    for (int i = 1; i <= jpi; ++i) vptr[i] = vptr[i - 1] + cnt[i];
+    vpts.resize(vptr[jpi]);
+    std::vector<int> nxt(vptr.begin(), vptr.end() - 1);  // next slot of column i at nxt[i - 1]
+    for (int j = 0; j <= jpj; ++j)
+        for (int i = 1; i <= jpi; ++i) {
             const long o = L.at(i, j);
             if (vcls(fl[o]) == CLS_FLATHER) {
-                vpts.push_back(o);
+                vpts[nxt[i - 1]++] = o;
                 const long oi = (fl[o] & V_FL_NORTH) ? o + L.pitch : o - L.pitch;
                 if (vcls(fl[oi]) == CLS_FLATHER) chained = true;
             }
         }
-    }
-    vptr[jpi] = (int)vpts.size();
     return NEMO2D_OK;
 }
 
