@@ -302,10 +302,18 @@ class Trio:
 
     def advance(self):
         """Fine model on by one window; truth = R(fine)."""
+        self.advance_fine()
+        self.advance_truth()
+
+    def advance_fine(self):
+        """The fine model's window, queued on its stream (no wait)."""
         w = self.w
         self.fine.run(2 * K_STEPS * w + 1, 2 * K_STEPS * (w + 1))
+
+    def advance_truth(self):
+        """truth = R(fine) once the fine window has run."""
         self.truth.restrict_from(self.fine)
-        self.w = w + 1
+        self.w += 1
 
     def restart_pred(self):
         """Prediction: from R(fine) now, K coarse steps on."""
@@ -400,14 +408,20 @@ def train_online(nc, seeds, windows, group=4, lr=2e-3, spinup=2, bathy=None, ite
         active = trios[:len(setups)]
         for f in [loader.submit(tr.load, su) for tr, su in zip(active, setups)]:
             f.result()
+        # the group's fine windows are queued on their own streams before
+        # any is waited for, so the basins' steps overlap on the device
         for _ in range(spinup):
             for tr in active:
-                tr.advance()
+                tr.advance_fine()
+            for tr in active:
+                tr.advance_truth()
         for tr in active:
             tr.restart_pred()
         for w in range(windows):
             for tr in active:
-                tr.advance()
+                tr.advance_fine()
+            for tr in active:
+                tr.advance_truth()
             for k, tr in enumerate(active):
                 tr.pred.closure_features(clos.xm, clos.xs, xb, offset=k * P * 8 * 2)
                 tr.truth.closure_target(tr.pred, clos.ys, yb, offset=k * P * 8 * 4)
