@@ -1910,15 +1910,18 @@ int nemo2d_set_grid(nemo2d_model* m, const int* pt, const double* e1t, const dou
         if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync(m->flags, fl.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, m->stream));
+    // plain tiles of k_step_march_rc and k_step_march_gr (strips of ROUT
+    // columns with lanes at c0-1 .. c0+62, segments of nj rows with rows
+    // jb-1 .. je+1+RC_LA loaded): every class word there equal to RC_PLAIN
+    // (the host tables live until the stream is synchronised below)
+    std::vector<uint8_t> tabs[2];
     {
-        // plain tiles of k_step_march_rc and k_step_march_gr (strips of ROUT
-        // columns with lanes at c0-1 .. c0+62, segments of nj rows with rows
-        // jb-1 .. je+1+RC_LA loaded): every class word there equal to RC_PLAIN
         const int nbx = (jpi + 1 + MW * ROUT - 1) / (MW * ROUT);
         const int nstrip = nbx * MW;
         for (const int nj : {RC_NJ, RC_NJ_SHORT}) {
             const int nseg = (jpj + 1 + nj - 1) / nj;
-            std::vector<uint8_t> tab((size_t)nstrip * nseg, 0);
+            std::vector<uint8_t>& tab = tabs[nj == RC_NJ ? 0 : 1];
+            tab.assign((size_t)nstrip * nseg, 0);
             for (int w = 0; w < nstrip; ++w) {
                 const int i0 = w * ROUT - 1, i1 = w * ROUT + 62;
                 if (i0 < 0 || i1 > jpi + 1) continue;
