@@ -121,7 +121,7 @@ int nemo2d_set_tide(nemo2d_model *m, double amp, double period_hours);
 int nemo2d_set_stage_fusion(nemo2d_model *m, int on);
 
 /* Choose the kernel of nemo2d_step (results are identical for every
- * choice): 0 default, by grid size (uniform statics: 2 up to 300,000 T cells
+ * choice): 0 default, by grid size (uniform statics: 2 up to 230,000 T cells
  * when the Flather points are all on row 0, else 1; 13 up to 3,000,000;
  * then 10.  Per-point depths with uniform metrics: 1 up to 100,000, 12 up
  * to 10,000,000, then 11.

@@ -2518,9 +2518,12 @@ static int launch_cell(nemo2d_model* m, const NState& n, const AOut& a, double t
 // a uniform FP64 grid takes the five-array marching kernel.  On smaller grids
 // a marching step lasts as long as one wave's march of its segment, whatever
 // the grid size.
-constexpr long AUTO_CELL_MAX_UNIFORM = 300000;
+// (device us per step, one thread per cell / 8-row marching: uniform 448^2
+// 14.0 / 14.2, 512^2 16.2 / 14.5; seamount FP64 384^2 21.2 / 17.3; seamount
+// FP32 448^2 11.9 / 13.4, 512^2 13.9 / 13.4)
+constexpr long AUTO_CELL_MAX_UNIFORM = 230000;
 constexpr long AUTO_CELL_MAX_DEPTHS = 100000;
-constexpr long AUTO_CELL_MAX_MIXED = 300000;
+constexpr long AUTO_CELL_MAX_MIXED = 230000;
 // Uniform statics: the sshn-only kernel with 8-row segments up to this many
 // cells, 12-row ones beyond (device us per step, reference basin, 12 / 8
 // rows / the five-array shared-term kernel: 768^2 26.1 / 20.4 / 28.3,
