@@ -177,6 +177,7 @@ struct RcCtx : ShCtx {
 struct RcCarry {
     NorthTerms south;         // the row below's north terms
     double un_w, su_w;        // un and sshn_u one column west, this row
+    double ss_e;              // sshn one column east, this row
     double ct, cu, cv;        // per-lane checksum sums
 };
 
@@ -185,7 +186,7 @@ struct RcCarry {
 // carried to the next row, and the new sshn, u and v.
 struct RcOut {
     NorthTerms nt;
-    double csu, bsv, cun_w, csu_w;
+    double csu, bsv, cun_w, csu_w, css_e;
     double s_c, u, v;
 };
 
@@ -201,12 +202,13 @@ __device__ __forceinline__ RcOut rc_compute(const RcCtx& x, const RcCarry& k, co
     const unsigned f = B0.f;
     RcOut o;
     Col5F B = B0, C = C0;
-    C.su = o.csu = rc_su<DM, PLAIN>(s, x.ucls_at(C.f, j + 1), C.ss, lane_next(C.ss), C0.su, ck);
+    o.css_e = lane_next(C.ss);  // (the next row's east sshn too)
+    C.su = o.csu = rc_su<DM, PLAIN>(s, x.ucls_at(C.f, j + 1), C.ss, o.css_e, C0.su, ck);
     B.sv = o.bsv = rc_sv<DM, PLAIN>(s, x.vcls_at(f, j), B.ss, C.ss, B0.sv, ck);
     // neighbours by lane shifts (every lane active here)
-    const double bun_w = k.un_w, bsu_w = k.su_w;
+    const double bun_w = k.un_w, bsu_w = k.su_w, bss_e = k.ss_e;
     const double bvn_w = lane_prev(B.vn), bsv_w = lane_prev(B.sv);
-    const double bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv), bss_e = lane_next(B.ss);
+    const double bvn_e = lane_next(B.vn), bsv_e = lane_next(B.sv);
     o.cun_w = lane_prev(C.un);
     o.csu_w = lane_prev(C.su);
     const double cun_w = o.cun_w, csu_w = o.csu_w;
@@ -339,6 +341,7 @@ __device__ __forceinline__ void rc_commit(const RcCtx& x, RcCarry& k, const Col5
     k.south = o.nt;
     k.un_w = o.cun_w;
     k.su_w = o.csu_w;
+    k.ss_e = o.css_e;
     C.su = o.csu;
     B.sv = o.bsv;
 }
@@ -381,7 +384,8 @@ __device__ __forceinline__ void rc_march(const RcCtx& x, RcCarry& k, int jb, int
     load_col3<PLAIN>(R3, n, fl, x.rowb(jb + 2 <= L.jpj + 2 ? jb + 2 : L.jpj + 2) + x.lb8);
     // the segment's first interpolations and south terms, IEEE divisions
     R0.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R0.f, jb - 1), R0.ss, lane_next(R0.ss), R0.su, nullptr);
-    R1.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R1.f, jb), R1.ss, lane_next(R1.ss), R1.su, nullptr);
+    k.ss_e = lane_next(R1.ss);
+    R1.su = rc_su<0, PLAIN>(sc_, x.ucls_at(R1.f, jb), R1.ss, k.ss_e, R1.su, nullptr);
     R0.sv = rc_sv<0, PLAIN>(sc_, x.vcls_at(R0.f, jb - 1), R0.ss, R1.ss, R0.sv, nullptr);
     k.south = north_terms1<0>(sc_, R0, lane_next(R0.vn), lane_next(R0.sv), R1);
     k.un_w = lane_prev(R1.un);
